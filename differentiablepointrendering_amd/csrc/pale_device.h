@@ -1,0 +1,479 @@
+// pale_device.h -- device functions of the gfx950 renderer: closest-hit
+// traversal, surfel and triangle tests, sampling, Russian roulette, light
+// sampling and the photon-grid irradiance gather.
+//
+// Traversal keeps the reference's visiting order exactly (near-far child
+// push with ties to the left child, IntersectionKernels.h:21-32 and
+// :585-675), so the closest hit -- including which surfel wins an exact tie
+// in t -- is the reference's.  The stack lives in LDS: entry k of a lane is
+// at stk[k * stride], stride = blockDim.x (bank-conflict free per row).  A
+// stack entry is (node reference, entry distance); re-testing a popped box
+// against the current best distance is done with the stored entry distance,
+// which equals the reference's full slab re-test because the other two
+// rejection conditions do not depend on the best distance.
+#pragma once
+
+#include "pale_common.h"
+
+namespace pale {
+
+struct DevScene {
+    const SurfelGeo* geo;
+    const SurfelShade* shade;
+    const WideNode* nodes;
+    const TriGeo* tris;
+    const DevInstance* inst;
+    const TlasNode* tlas;
+    const DevMaterial* mats;
+    const DevLight* lights;
+    const DevEmTri* emTris;
+    uint32_t nInst;
+    uint32_t nTlas;
+    uint32_t nLights;
+    uint32_t stackDepth;
+};
+
+struct DevGrid {
+    const float4* posPow;  // sorted photons: position, power.x
+    const float4* dirPow;  // incoming direction, power.y
+    const float* powZ;     // power.z
+    const uint32_t* cellStart;  // exclusive prefix over cells, total+1 entries
+    float ox, oy, oz;
+    float cx, cy, cz;
+    int rx, ry, rz;
+    float radius;
+    const uint32_t* attempted;  // deposits attempted (T), device memory
+    uint32_t capacity;          // photon capacity
+    uint32_t valid;      // a photon map exists
+};
+
+struct Hit {
+    F3 pos;
+    float t;
+    float alpha;
+    uint32_t inst;  // kInvalid: miss
+    uint32_t prim;
+};
+
+__device__ __forceinline__ F3 ld3(const float4& v) { return mk3(v.x, v.y, v.z); }
+
+// safeInvDir (KernelHelpers.h:140-147)
+__device__ __forceinline__ F3 safeInvDir(F3 d) {
+    return mk3(fabsf(d.x) < 1e-6f ? 1e30f : 1.0f / d.x, fabsf(d.y) < 1e-6f ? 1e30f : 1.0f / d.y,
+               fabsf(d.z) < 1e-6f ? 1e30f : 1.0f / d.z);
+}
+
+// slabIntersectAABB (KernelHelpers.h:149-214); clampLo is 0 for TLAS nodes
+// and 1e-6 for BVH nodes.
+__device__ __forceinline__ bool slab(F3 o, F3 inv, float mnx, float mny, float mnz, float mxx, float mxy,
+                                     float mxz, float limit, float clampLo, float& tEntry) {
+    const float t0x = (mnx - o.x) * inv.x, t0y = (mny - o.y) * inv.y, t0z = (mnz - o.z) * inv.z;
+    const float t1x = (mxx - o.x) * inv.x, t1y = (mxy - o.y) * inv.y, t1z = (mxz - o.z) * inv.z;
+    const float tmin = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fminf(t0z, t1z));
+    const float tmax = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z));
+    if (tmin > tmax) return false;
+    if (tmax < 0.0f) return false;
+    if (tmin > limit) return false;
+    tEntry = fmaxf(tmin, clampLo);
+    return true;
+}
+
+struct LaneStack {
+    uint2* base;
+    uint32_t stride;
+    uint32_t cap;
+    // SmallStack::push (GPUDataTypes.h:398-402): a push beyond capacity is
+    // dropped, so sp never exceeds cap and every pop reads a written slot.
+    __device__ __forceinline__ void push(uint32_t& sp, uint32_t ref, float t) const {
+        if (sp < cap) {
+            base[sp * stride] = make_uint2(ref, __float_as_uint(t));
+            ++sp;
+        }
+    }
+    __device__ __forceinline__ uint2 get(uint32_t k) const { return base[k * stride]; }
+};
+
+struct LocalHit {
+    F3 pos;
+    float t;
+    float alpha;
+    uint32_t prim;
+};
+
+// intersectBLASPointCloudFirstHit (IntersectionKernels.h:585-675) with
+// intersectSurfel (KernelHelpers.h:637-669); alpha_geom is evaluated once for
+// the final hit (it is a pure function of the hit).
+__device__ bool traverseCloud(const DevScene& S, const DevInstance& I, F3 o, F3 d, const LaneStack& st,
+                              uint32_t base, LocalHit& out) {
+    const F3 inv = safeInvDir(d);
+    float best = __builtin_huge_valf();
+    float te;
+    if (!slab(o, inv, I.rmnx, I.rmny, I.rmnz, I.rmxx, I.rmxy, I.rmxz, best, 1e-6f, te)) return false;
+    uint32_t sp = base;
+    st.push(sp, I.rootRef, te);
+    uint32_t bestIdx = 0;
+    float bestR2 = 0.0f;
+    F3 bestX = mk3(0, 0, 0);
+    bool any = false;
+    while (sp > base) {
+        const uint2 e = st.get(--sp);
+        if (__uint_as_float(e.y) > best) continue;
+        const uint32_t ref = e.x;
+        if (ref & kLeafBit) {
+            const uint32_t first = leafFirst(ref), cnt = leafCount(ref);
+            for (uint32_t k = 0; k < cnt; ++k) {
+                const uint32_t si = first + k;
+                const float4* g = reinterpret_cast<const float4*>(S.geo + si);
+                const float4 g0 = g[0], g1 = g[1];
+                const F3 n = mk3(g1.x, g1.y, g1.z);
+                const float nd = dot(n, d);
+                if (fabsf(nd) < 1e-6f) continue;
+                const F3 p = mk3(g0.x, g0.y, g0.z);
+                const float t = dot(n, p - o) / nd;
+                if (t <= 1e-5f || t >= best) continue;
+                const float4 g2 = g[2], g3 = g[3];
+                const F3 x = o + d * t;
+                const F3 r = x - p;
+                const float uu = dot(mk3(g2.x, g2.y, g2.z), r) / g0.w;
+                const float vv = dot(mk3(g3.x, g3.y, g3.z), r) / g1.w;
+                const float r2 = uu * uu + vv * vv;
+                if (r2 >= 1.0f) continue;
+                any = true;
+                best = t;
+                bestIdx = si;
+                bestX = x;
+                bestR2 = r2;
+            }
+            continue;
+        }
+        const float4* nd4 = reinterpret_cast<const float4*>(S.nodes + ref);
+        const float4 a = nd4[0], b = nd4[1], c = nd4[2];
+        const uint4 refs = reinterpret_cast<const uint4*>(S.nodes + ref)[3];
+        float tl = __builtin_huge_valf(), tr = __builtin_huge_valf();
+        const bool hl = slab(o, inv, a.x, a.y, a.z, a.w, b.x, b.y, best, 1e-6f, tl);
+        const bool hr = slab(o, inv, b.z, b.w, c.x, c.y, c.z, c.w, best, 1e-6f, tr);
+        if (hl && hr) {
+            if (tl <= tr) {
+                st.push(sp, refs.y, tr);
+                st.push(sp, refs.x, tl);
+            } else {
+                st.push(sp, refs.x, tl);
+                st.push(sp, refs.y, tr);
+            }
+        } else if (hl) {
+            st.push(sp, refs.x, tl);
+        } else if (hr) {
+            st.push(sp, refs.y, tr);
+        }
+    }
+    if (!any) return false;
+    out.t = best;
+    out.prim = bestIdx;
+    out.pos = bestX;
+    out.alpha = pow_r(1.0f - bestR2, S.geo[bestIdx].e4b);
+    return true;
+}
+
+// intersectTriangle (KernelHelpers.h:248-277)
+__device__ __forceinline__ bool intersectTri(F3 o, F3 d, F3 v0, F3 v1, F3 v2, float tMin, float& tOut) {
+    const F3 e1 = v1 - v0, e2 = v2 - v0;
+    const F3 h = cross(d, e2);
+    const float a = dot(e1, h);
+    if (fabsf(a) < 1.0e-4f) return false;
+    const float f = 1.0f / a;
+    const F3 s = o - v0;
+    const float u = f * dot(s, h);
+    if (u < 0.0f || u > 1.0f) return false;
+    const F3 q = cross(s, e1);
+    const float v = f * dot(d, q);
+    if (v < 0.0f || u + v > 1.0f) return false;
+    const float t = f * dot(e2, q);
+    if (t <= tMin) return false;
+    tOut = t;
+    return true;
+}
+
+// intersectBLASMesh (IntersectionKernels.h:37-105)
+__device__ bool traverseMesh(const DevScene& S, const DevInstance& I, F3 o, F3 d, const LaneStack& st,
+                             uint32_t base, LocalHit& out) {
+    const F3 inv = safeInvDir(d);
+    float best = __builtin_huge_valf();
+    float te;
+    if (!slab(o, inv, I.rmnx, I.rmny, I.rmnz, I.rmxx, I.rmxy, I.rmxz, best, 1e-6f, te)) return false;
+    uint32_t sp = base;
+    st.push(sp, I.rootRef, te);
+    bool any = false;
+    uint32_t bestTri = 0;
+    while (sp > base) {
+        const uint2 e = st.get(--sp);
+        if (__uint_as_float(e.y) > best) continue;
+        const uint32_t ref = e.x;
+        if (ref & kLeafBit) {
+            const uint32_t first = leafFirst(ref), cnt = leafCount(ref);
+            for (uint32_t k = 0; k < cnt; ++k) {
+                const TriGeo tg = S.tris[first + k];
+                float t;
+                if (intersectTri(o, d, mk3(tg.v0x, tg.v0y, tg.v0z), mk3(tg.v1x, tg.v1y, tg.v1z),
+                                 mk3(tg.v2x, tg.v2y, tg.v2z), 1e-4f, t) &&
+                    t < best && t > 0.0f) {
+                    best = t;
+                    any = true;
+                    bestTri = first + k;
+                }
+            }
+            continue;
+        }
+        const float4* nd4 = reinterpret_cast<const float4*>(S.nodes + ref);
+        const float4 a = nd4[0], b = nd4[1], c = nd4[2];
+        const uint4 refs = reinterpret_cast<const uint4*>(S.nodes + ref)[3];
+        float tl = __builtin_huge_valf(), tr = __builtin_huge_valf();
+        const bool hl = slab(o, inv, a.x, a.y, a.z, a.w, b.x, b.y, best, 1e-6f, tl);
+        const bool hr = slab(o, inv, b.z, b.w, c.x, c.y, c.z, c.w, best, 1e-6f, tr);
+        if (hl && hr) {
+            if (tl <= tr) {
+                st.push(sp, refs.y, tr);
+                st.push(sp, refs.x, tl);
+            } else {
+                st.push(sp, refs.x, tl);
+                st.push(sp, refs.y, tr);
+            }
+        } else if (hl) {
+            st.push(sp, refs.x, tl);
+        } else if (hr) {
+            st.push(sp, refs.y, tr);
+        }
+    }
+    if (!any) return false;
+    out.t = best;
+    out.prim = bestTri;
+    out.pos = toWorldPoint(I.o2w, o + d * best);
+    out.alpha = 0.0f;
+    return true;
+}
+
+// intersectScene, FirstHit mode (IntersectionKernels.h:954-1097).  The TLAS
+// stack shares the lane's LDS stack below the BLAS entries.
+__device__ bool intersectScene(const DevScene& S, F3 o, F3 d, const LaneStack& st, Hit& h) {
+    h.inst = kInvalid;
+    if (S.nTlas == 0) return false;
+    const F3 inv = safeInvDir(d);
+    float bestW = __builtin_huge_valf();
+    bool found = false;
+    uint32_t sp = 0;
+    st.push(sp, 0u, 0.0f);
+    while (sp > 0) {
+        const uint32_t ni = st.get(--sp).x;
+        const TlasNode nd = S.tlas[ni];
+        float te;
+        if (!slab(o, inv, nd.mnx, nd.mny, nd.mnz, nd.mxx, nd.mxy, nd.mxz, bestW, 0.0f, te)) continue;
+        if (nd.count == 0) {
+            const TlasNode L = S.tlas[nd.left], R = S.tlas[nd.right];
+            float tl = __builtin_huge_valf(), tr = __builtin_huge_valf();
+            const bool hl = slab(o, inv, L.mnx, L.mny, L.mnz, L.mxx, L.mxy, L.mxz, bestW, 0.0f, tl);
+            const bool hr = slab(o, inv, R.mnx, R.mny, R.mnz, R.mxx, R.mxy, R.mxz, bestW, 0.0f, tr);
+            if (hl && hr) {
+                if (tl <= tr) {
+                    st.push(sp, nd.right, 0.0f);
+                    st.push(sp, nd.left, 0.0f);
+                } else {
+                    st.push(sp, nd.left, 0.0f);
+                    st.push(sp, nd.right, 0.0f);
+                }
+            } else if (hl) {
+                st.push(sp, nd.left, 0.0f);
+            } else if (hr) {
+                st.push(sp, nd.right, 0.0f);
+            }
+            continue;
+        }
+        const uint32_t ii = nd.left;
+        const DevInstance& I = S.inst[ii];
+        if (I.empty) continue;
+        // toObjectSpace (KernelHelpers.h:217-227)
+        float hv[4];
+        mulv4(I.w2o, o.x, o.y, o.z, 1.0f, hv);
+        const F3 oo = divs(mk3(hv[0], hv[1], hv[2]), hv[3]);
+        mulv4(I.w2o, d.x, d.y, d.z, 0.0f, hv);
+        const F3 od = normalize(mk3(hv[0], hv[1], hv[2]));
+        LocalHit lh;
+        const bool acc = (I.type == kGeoMesh) ? traverseMesh(S, I, oo, od, st, sp, lh)
+                                              : traverseCloud(S, I, oo, od, st, sp, lh);
+        if (acc) {
+            const float tw = dot(lh.pos - o, d);
+            if (tw > 0.0f && tw < bestW) {
+                bestW = tw;
+                found = true;
+                h.t = tw;
+                h.pos = lh.pos;
+                h.inst = ii;
+                h.prim = lh.prim;
+                h.alpha = lh.alpha;
+            }
+        }
+    }
+    return found;
+}
+
+// buildIntersectionNormal (KernelHelpers.h:480-503)
+__device__ __forceinline__ F3 hitNormal(const DevScene& S, const Hit& h) {
+    const DevInstance& I = S.inst[h.inst];
+    if (I.type == kGeoMesh) {
+        const TriGeo tg = S.tris[h.prim];
+        const F3 p0 = toWorldPoint(I.o2w, mk3(tg.v0x, tg.v0y, tg.v0z));
+        const F3 p1 = toWorldPoint(I.o2w, mk3(tg.v1x, tg.v1y, tg.v1z));
+        const F3 p2 = toWorldPoint(I.o2w, mk3(tg.v2x, tg.v2y, tg.v2z));
+        return normalize(cross(p1 - p0, p2 - p0));
+    }
+    const SurfelShade sh = S.shade[h.prim];
+    return mk3(sh.nsx, sh.nsy, sh.nsz);
+}
+
+// sampleCosineHemisphere(rng, n, dir, pdf) (KernelHelpers.h:317-337)
+__device__ __forceinline__ void sampleCosine(Rng& rng, F3 n, F3& dir, float& pdf) {
+    const float u1 = rng.nextFloat();
+    const float u2 = rng.nextFloat();
+    const float z = sqrtf(1.0f - u1);
+    const float r = sqrtf(1.0f - z * z);
+    const float phi = 2.0f * kPi * u2;
+    const float x = r * cos_r(phi);
+    const float y = r * sin_r(phi);
+    const F3 up = fabsf(n.z) < 0.999f ? mk3(0, 0, 1) : mk3(1, 0, 0);
+    const F3 tang = normalize(cross(up, n));
+    const F3 bit = cross(n, tang);
+    dir = normalize(tang * x + bit * y + n * z);
+    pdf = fmaxf(0.0f, dot(dir, n)) / kPi;
+}
+
+// sampleUniformHemisphereAroundNormal (KernelHelpers.h:551-596)
+__device__ __forceinline__ void sampleUniformHemi(Rng& rng, F3 normal, F3& dir, float& pdf) {
+    const F3 n = normalize(normal);
+    const float u1 = rng.nextFloat();
+    const float u2 = rng.nextFloat();
+    const float z = u1;
+    const float phi = 2.0f * kPi * u2;
+    const float r = sqrtf(fmaxf(0.0f, 1.0f - z * z));
+    const float x = r * cos_r(phi);
+    const float y = r * sin_r(phi);
+    const float sign = copysignf(1.0f, n.z);
+    const float a = -1.0f / (sign + n.z);
+    const float b = n.x * n.y * a;
+    const F3 t = mk3(1.0f + sign * n.x * n.x * a, sign * b, -sign * n.x);
+    const F3 bt = mk3(b, sign + n.y * n.y * a, -n.y);
+    dir = normalize(t * x + bt * y + n * z);
+    pdf = 1.0f / (2.0f * kPi);
+}
+
+// applyRussianRoulette (KernelHelpers.h:1612-1657)
+__device__ __forceinline__ bool russianRoulette(Rng& rng, uint32_t bounce, F3& T, uint32_t start) {
+    if (bounce < start) return true;
+    if (!isfinite(T.x) || !isfinite(T.y) || !isfinite(T.z)) return false;
+    const float lum = 0.2126f * T.x + 0.7152f * T.y + 0.0722f * T.z;
+    const float p = fminf(fmaxf(lum, 0.0f), 0.99f);
+    if (rng.nextFloat() > p) return false;
+    float c = 1.0f / p;
+    c = fminf(c, 10.0f);
+    T = T * c;
+    return true;
+}
+
+// sampleMeshAreaLight (KernelHelpers.h:340-449)
+__device__ bool sampleLight(const DevScene& S, Rng& rng, F3& pos, F3& dir, F3& power) {
+    const uint32_t lc = S.nLights;
+    if (lc == 0) return false;
+    const float ul = rng.nextFloat();
+    const uint32_t li = min((uint32_t)(ul * (float)lc), lc - 1u);
+    const DevLight L = S.lights[li];
+    if (L.triCount == 0u || L.area <= 0.0f) return false;
+    const float ut = rng.nextFloat();
+    uint32_t lo = 0u, hi = L.triCount - 1u;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1u;
+        if (ut <= S.emTris[L.triOffset + mid].cdf) hi = mid;
+        else lo = mid + 1u;
+    }
+    const TriGeo tg = S.tris[S.emTris[L.triOffset + lo].tri];
+    const F3 v0 = mk3(tg.v0x, tg.v0y, tg.v0z), v1 = mk3(tg.v1x, tg.v1y, tg.v1z), v2 = mk3(tg.v2x, tg.v2y, tg.v2z);
+    const float u1 = rng.nextFloat();
+    const float u2 = rng.nextFloat();
+    const float s1 = sqrtf(u1);
+    const float b0 = 1.0f - s1, b1 = s1 * (1.0f - u2), b2 = s1 * u2;
+    const F3 xo = v0 * b0 + v1 * b1 + v2 * b2;
+    const float* o2w = S.inst[L.instance].o2w;
+    const F3 p0 = toWorldPoint(o2w, v0), p1 = toWorldPoint(o2w, v1), p2 = toWorldPoint(o2w, v2);
+    const F3 e0 = p1 - p0, e1 = p2 - p0;
+    F3 n = mk3(e0.y * e1.z - e0.z * e1.y, e0.z * e1.x - e0.x * e1.z, e0.x * e1.y - e0.y * e1.x);
+    const float nl = sqrtf(dot(n, n));
+    if (nl <= 0.0f) return false;
+    n = divs(n, nl);
+    float pdf;
+    sampleCosine(rng, n, dir, pdf);
+    pos = toWorldPoint(o2w, xo);
+    power = mk3(L.cr, L.cg, L.cb) * L.power;
+    return true;
+}
+
+// worldToCellClamped (KernelHelpers.h:920-934)
+__device__ __forceinline__ void worldToCell(const DevGrid& g, F3 p, int& x, int& y, int& z) {
+    const F3 f = divv(p - mk3(g.ox, g.oy, g.oz), mk3(g.cx, g.cy, g.cz));
+    x = min(max((int)floorf(f.x), 0), g.rx - 1);
+    y = min(max((int)floorf(f.y), 0), g.ry - 1);
+    z = min(max((int)floorf(f.z), 0), g.rz - 1);
+}
+
+// gatherDiffuseIrradianceAtPoint (KernelHelpers.h:990-1037).  Photons are
+// stored sorted by (cell, deposit index), so the cells of one grid row form
+// one contiguous range; summation order equals the reference's cell-by-cell,
+// photon-by-photon order.
+__device__ F3 gatherIrradiance(const DevGrid& g, F3 q, F3 n) {
+    if (!g.valid) return mk3(0, 0, 0);
+    const float r = g.radius;
+    const float r2 = r * r;
+    const float invArea = 1.0f / (kPi * r2);
+    int lx, ly, lz, hx, hy, hz;
+    worldToCell(g, q - mk3(r, r, r), lx, ly, lz);
+    worldToCell(g, q + mk3(r, r, r), hx, hy, hz);
+    F3 E = mk3(0, 0, 0);
+    for (int cz = lz; cz <= hz; ++cz) {
+        for (int cy = ly; cy <= hy; ++cy) {
+            const uint32_t row = (uint32_t)((cz * g.ry + cy) * g.rx);
+            const uint32_t a = g.cellStart[row + lx];
+            const uint32_t b = g.cellStart[row + hx + 1];
+            for (uint32_t j = a; j < b; ++j) {
+                const float4 pp = g.posPow[j];
+                const float4 dp = g.dirPow[j];
+                const F3 dd = mk3(pp.x, pp.y, pp.z) - q;
+                const float cosine = dot(n, -mk3(dp.x, dp.y, dp.z));
+                if (cosine < 0.0f) continue;
+                const float d2 = dot(dd, dd);
+                if (d2 > r2) continue;
+                E = E + mk3(pp.w, dp.w, g.powZ[j]) * invArea;
+            }
+        }
+    }
+    const uint32_t T = *g.attempted;
+    const uint32_t D = min(T, g.capacity);
+    if (D == 0u) return mk3(0, 0, 0);
+    return E * ((float)T / (float)D);
+}
+
+// makePrimaryRayFromPixelJitteredFov (KernelHelpers.h:764-796)
+struct DevCamera {
+    float invView[16];
+    uint32_t W, H;
+    float fx, fy;
+};
+
+__device__ __forceinline__ void primaryRay(const DevCamera& c, float px, float py, float jx, float jy, F3& o,
+                                           F3& d) {
+    const float W = (float)c.W, H = (float)c.H;
+    const float u = px + 0.5f + jx;
+    const float v = py + 0.5f + jy;
+    const float vf = H - v;
+    const float ndcX = 2.0f * u / W - 1.0f;
+    const float ndcY = 2.0f * vf / H - 1.0f;
+    const F3 dc = normalize(mk3(ndcX * (0.5f * W) / c.fx, ndcY * (0.5f * H) / c.fy, -1.0f));
+    d = transformDirection(c.invView, dc);
+    o = transformPoint(c.invView, mk3(0, 0, 0));
+}
+
+}  // namespace pale

@@ -1,0 +1,106 @@
+// pale_kernels.h -- kernel launch interface between the orchestration
+// (pale_render.hip) and the gfx950 kernels (pale_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "pale_device.h"
+
+namespace pale {
+
+constexpr uint32_t kTile = 1024;       // compaction / scan tile (256 threads x 4)
+constexpr uint32_t kTraceBlock = 128;  // threads per traversal workgroup
+
+// Device-side progress of one render call.  Counts live on the device so the
+// whole forward or backward pass runs without host round trips.
+struct DevState {
+    uint32_t cursor;       // next entry of the seed table
+    uint32_t number;       // seed of the current launch
+    uint32_t live;         // current bounce runs (queue not empty)
+    uint32_t qCount;       // rays in the current queue
+    uint32_t photonCount;  // photon deposits so far (attempted = stored)
+    uint32_t extTotal;     // survivors of the last bounce
+    uint32_t depTotal;     // deposits of the last bounce
+    uint32_t evTotal;      // gradient events of the last bounce
+    unsigned long long photonQ, gatherQ, adjointQ, deposits, events, pad;
+};
+
+// Ray queue, structure of arrays: origin (w = pixel / path id bits),
+// direction (w = bounce bits), throughput (w unused).
+struct RayQueue {
+    float4* o;
+    float4* d;
+    float4* T;
+};
+
+// Photon deposits: position + power.x, direction + power.y, power.z.
+struct PhotonArrays {
+    float4* posPow;
+    float4* dirPow;
+    float* powZ;
+};
+
+// Adjoint gradient event: e0 = gather position + kind, e1 = gather normal +
+// target surfel, e2 = throughput + alpha, e3 = kind-specific words.
+struct EventArrays {
+    float4* e0;
+    float4* e1;
+    float4* e2;
+    float4* e3;
+};
+
+// Pending adjoint state per path: p0 = position + kind, p1 = throughput +
+// alpha, p2 = (primitive, instance) bits.
+struct PendingArrays {
+    float4* p0;
+    float4* p1;
+    uint2* p2;
+};
+
+struct TraceParams {
+    uint32_t rrStart;
+    float qNull, qReflect, qTransmit;
+};
+
+struct GridParams {
+    float ox, oy, oz, cell;
+    int rx, ry, rz;
+    uint32_t total;
+};
+
+// Launchers (all asynchronous on `s`).
+void launchFill(hipStream_t s, uint32_t* p, uint32_t n, uint32_t v);
+void launchFillF(hipStream_t s, float* p, uint32_t n, float v);
+void launchSeedNext(hipStream_t s, DevState* st, const uint32_t* table, uint32_t tableLen, int bounce);
+void launchEmit(hipStream_t s, const DevScene& S, DevState* st, uint32_t photons, float invEmitted,
+                RayQueue staged, uint8_t* flags);
+void launchPhotonTrace(hipStream_t s, const DevScene& S, DevState* st, uint32_t maxRays, RayQueue in,
+                       RayQueue staged, uint8_t* extFlags, PhotonArrays dep, uint8_t* depFlags, TraceParams tp);
+// Stable compaction of flagged items i < *count (items >= *count are ignored).
+void launchCompactRays(hipStream_t s, const uint8_t* flags, const uint32_t* count, uint32_t maxItems,
+                       uint32_t* tiles, uint32_t* total, RayQueue src, RayQueue dst, uint32_t dstCap);
+void launchCompactPhotons(hipStream_t s, const uint8_t* flags, const uint32_t* count, uint32_t maxItems,
+                          uint32_t* tiles, uint32_t* total, const uint32_t* base, PhotonArrays src,
+                          PhotonArrays dst, uint32_t dstCap);
+void launchCompactEvents(hipStream_t s, const uint8_t* flags, const uint32_t* count, uint32_t maxItems,
+                         uint32_t* tiles, uint32_t* total, EventArrays src, EventArrays dst, uint32_t dstCap);
+void launchFinishPhotonBounce(hipStream_t s, DevState* st, uint32_t cap);
+void launchSetQueue(hipStream_t s, DevState* st, const uint32_t* src);
+// Photon grid (counting sort by cell, stable in deposit order).
+void launchBuildGrid(hipStream_t s, const GridParams& g, DevState* st, uint32_t photonCap, PhotonArrays raw,
+                     PhotonArrays sorted, uint32_t* cellId, uint32_t* sortedIdx, uint32_t* cellCount,
+                     uint32_t* cellStart, uint32_t* tiles, uint32_t* scratchTotal);
+void launchGather(hipStream_t s, const DevScene& S, const DevGrid& G, const DevCamera& cam, DevState* st,
+                  uint32_t pass, int decorrelate, float invSpp, float4* fb);
+void launchPostprocess(hipStream_t s, const float4* fb, float4* ldr, uint32_t n);
+void launchAdjRaygen(hipStream_t s, const DevCamera& cam, const float* dloss, RayQueue q, PendingArrays pend,
+                     DevState* st);
+void launchAdjTrace(hipStream_t s, const DevScene& S, DevState* st, uint32_t maxRays, uint32_t bounce, RayQueue in,
+                    RayQueue staged, uint8_t* extFlags, EventArrays ev, uint8_t* evFlags, PendingArrays pend,
+                    TraceParams tp);
+void launchAdjTransport(hipStream_t s, const DevScene& S, const DevGrid& G, DevState* st, uint32_t maxEvents,
+                        EventArrays ev, float invSpp, float* gradOpacity);
+void launchTraceRays(hipStream_t s, const DevScene& S, uint32_t n, const float* o, const float* d, int32_t* inst,
+                     int32_t* prim, float* t, float* pos, float* alpha);
+
+}  // namespace pale

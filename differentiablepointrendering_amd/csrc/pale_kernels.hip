@@ -1,0 +1,819 @@
+// pale_kernels.hip -- gfx950 kernels of the beta-surfel photon-mapping
+// renderer and its opacity adjoint.
+//
+// Wavefront structure: every bounce is one traversal kernel over a ray queue
+// followed by a stable (order-preserving) compaction of its outputs.  Queue
+// slot = rank in the previous queue, so each ray meets the per-item random
+// stream the reference's sequential order gives it (the reference seeds every
+// work item from its queue slot, PrimalKernels.cpp:80-82,
+// AdjointKernels.cpp:99-101).  Counts stay on the device; no kernel waits for
+// the host.
+#include "pale_kernels.h"
+
+namespace pale {
+
+namespace {
+
+constexpr uint32_t kScanThreads = 256;
+constexpr uint32_t kTileSpan = kTile / kScanThreads;  // items per thread
+
+enum : uint32_t { kNone = 0, kNullT = 1, kReflectS = 2, kTransmitS = 3, kProjection = 4 };
+
+__device__ __forceinline__ uint32_t laneId() { return __lane_id(); }
+
+// Wave-level count of a predicate, one atomic per wave.
+__device__ __forceinline__ void waveCount(bool pred, unsigned long long* counter) {
+    const unsigned long long m = __ballot(pred);
+    if (laneId() == __ffsll((long long)__ballot(1)) - 1 && m) atomicAdd(counter, (unsigned long long)__popcll(m));
+}
+
+__device__ __forceinline__ uint32_t waveSum(uint32_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Exclusive scan over a 256-thread block; `total` receives the block sum.
+__device__ __forceinline__ uint32_t blockExclusiveScan(uint32_t v, uint32_t& total) {
+    __shared__ uint32_t waveTotals[kScanThreads / 64 + 1];
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= (uint32_t)o) x += y;
+    }
+    if (lane == 63u) waveTotals[wave] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t s = 0;
+        for (uint32_t w = 0; w < kScanThreads / 64; ++w) {
+            const uint32_t t = waveTotals[w];
+            waveTotals[w] = s;
+            s += t;
+        }
+        waveTotals[kScanThreads / 64] = s;
+    }
+    __syncthreads();
+    const uint32_t excl = x - v + waveTotals[wave];
+    total = waveTotals[kScanThreads / 64];
+    __syncthreads();
+    return excl;
+}
+
+__device__ __forceinline__ uint32_t loadCount(const uint32_t* count, uint32_t maxItems) {
+    const uint32_t n = *count;
+    return n < maxItems ? n : maxItems;
+}
+
+__device__ __forceinline__ LaneStack laneStack(uint2* lds, uint32_t depth) {
+    return LaneStack{lds + threadIdx.x, blockDim.x, depth};
+}
+
+__device__ __forceinline__ float4 f4(F3 v, float w) { return make_float4(v.x, v.y, v.z, w); }
+__device__ __forceinline__ float u2f(uint32_t u) { return __uint_as_float(u); }
+__device__ __forceinline__ uint32_t f2u(float f) { return __float_as_uint(f); }
+
+}  // namespace
+
+// ---------------------------------------------------------------- small --
+__global__ void k_fill(uint32_t* p, uint32_t n, uint32_t v) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
+}
+__global__ void k_fillf(float* p, uint32_t n, float v) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
+}
+
+// Seed schedule of submitPhotonMappingKernel (SyclBridge.cpp:83-143): one
+// draw per emitter launch, per executed bounce and per camera.  `bounce` < 0
+// always draws; a bounce draws only if its queue is not empty.
+__global__ void k_seed_next(DevState* st, const uint32_t* table, uint32_t len, int bounce) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    if (bounce >= 0 && st->qCount == 0) {
+        st->live = 0;
+        return;
+    }
+    const uint32_t c = st->cursor;
+    st->number = c < len ? table[c] : 0u;
+    st->cursor = c + 1;
+    st->live = 1;
+}
+
+__global__ void k_finish_photon_bounce(DevState* st, uint32_t cap) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    if (st->live) {
+        st->qCount = st->extTotal;
+        uint32_t pc = st->photonCount + st->depTotal;
+        st->photonCount = pc < cap ? pc : cap;
+    } else {
+        st->qCount = 0;
+    }
+}
+
+__global__ void k_set_queue(DevState* st, const uint32_t* src) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) st->qCount = *src;
+}
+
+// ---------------------------------------------------------- emission --
+// launchRayGenEmitterKernel (PrimalKernels.cpp:13-62)
+__global__ __launch_bounds__(256) void k_emit(DevScene S, const DevState* st, uint32_t P, float invEmitted,
+                                              RayQueue out, uint8_t* flags) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P) return;
+    Rng rng(perItemSeed(st->number, i));
+    F3 pos, dir, power;
+    const bool ok = sampleLight(S, rng, pos, dir, power);
+    flags[i] = ok ? 1 : 0;
+    if (!ok) return;
+    const F3 T = (power * (float)S.nLights) * invEmitted;
+    out.o[i] = f4(pos, u2f(kInvalid));
+    out.d[i] = f4(dir, u2f(0u));
+    out.T[i] = f4(T, 0.0f);
+}
+
+// ---------------------------------------------------- photon bounce --
+// launchIntersectKernel (PrimalKernels.cpp:65-321) for one queue.
+__global__ __launch_bounds__(kTraceBlock) void k_photon_trace(DevScene S, DevState* st, RayQueue in,
+                                                              RayQueue out, uint8_t* extFlags, PhotonArrays dep,
+                                                              uint8_t* depFlags, TraceParams tp) {
+    extern __shared__ uint2 s_stack[];
+    const LaneStack stk = laneStack(s_stack, S.stackDepth);
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n = st->live ? st->qCount : 0u;
+    const bool active = i < n;
+    if (active) {
+        uint8_t ef = 0, df = 0;
+        Rng rng(perItemSeed(st->number, i));
+        const float4 o4 = in.o[i], d4 = in.d[i], T4 = in.T[i];
+        const F3 o = ld3(o4), d = ld3(d4), T = ld3(T4);
+        const uint32_t bounce = f2u(d4.w);
+        Hit h;
+        if (intersectScene(S, o, d, stk, h)) {
+            const DevInstance& I = S.inst[h.inst];
+            F3 no = mk3(0, 0, 0), nT = mk3(0, 0, 0), ndir = d;
+            bool spawn = false;
+            if (I.type == kGeoMesh) {
+                const F3 ng = hitNormal(S, h);
+                df = 1;
+                dep.posPow[i] = f4(h.pos, T.x);
+                dep.dirPow[i] = f4(d, T.y);
+                dep.powZ[i] = T.z;
+                const DevMaterial m = S.mats[I.material];
+                float pdf;
+                sampleCosine(rng, ng, ndir, pdf);
+                no = h.pos + ng * 1e-6f;
+                nT = T * mk3(m.r, m.g, m.b);
+                spawn = true;
+            } else {
+                const float u = rng.nextFloat();
+                const SurfelShade sh = S.shade[h.prim];
+                if (u < tp.qNull) {
+                    const float att = 1.0f - h.alpha * sh.opacity;
+                    const float w = att / tp.qNull;
+                    no = h.pos + d * 1e-5f;
+                    nT = T * w;
+                    spawn = true;
+                } else if (u < tp.qNull + tp.qReflect) {
+                    const F3 nc = mk3(sh.nsx, sh.nsy, sh.nsz);
+                    const float sc = dot(nc, -d);
+                    const int sign = sc >= 0.0f ? 1 : -1;
+                    const F3 on = (float)sign * nc;
+                    const F3 flux = divs(T, tp.qReflect);
+                    df = 1;
+                    dep.posPow[i] = f4(h.pos, flux.x);
+                    dep.dirPow[i] = f4(d, flux.y);
+                    dep.powZ[i] = flux.z;
+                    float pdf;
+                    sampleUniformHemi(rng, on, ndir, pdf);
+                    const float cosT = fmaxf(0.0f, dot(ndir, on));
+                    const F3 fs = (sh.alpha_r * mk3(sh.ar, sh.ag, sh.ab)) * kInvPi;
+                    const float alpha = h.alpha * sh.opacity;
+                    const F3 w = divs(((alpha / tp.qReflect) * fs) * cosT, pdf);
+                    no = h.pos + on * 1e-5f;
+                    nT = T * w;
+                    spawn = true;
+                } else if (u < tp.qNull + tp.qReflect + tp.qTransmit) {
+                    const F3 nc = mk3(sh.nsx, sh.nsy, sh.nsz);
+                    const float sc = dot(nc, -d);
+                    const int sign = sc >= 0.0f ? 1 : -1;
+                    const F3 on = (float)sign * nc;
+                    float pdf;
+                    sampleUniformHemi(rng, on, ndir, pdf);
+                    const float cosT = fmaxf(0.0f, dot(ndir, on));
+                    const F3 fs = (0.0f * mk3(sh.ar, sh.ag, sh.ab)) * kInvPi;  // alpha_t = 0
+                    const float alpha = h.alpha * sh.opacity;
+                    const F3 w = divs(((alpha / tp.qReflect) * fs) * cosT, pdf);
+                    no = h.pos + (-on) * 1e-5f;
+                    nT = T * w;
+                    spawn = true;
+                }
+            }
+            if (spawn && russianRoulette(rng, bounce + 1, nT, tp.rrStart)) {
+                ef = 1;
+                out.o[i] = f4(no, o4.w);
+                out.d[i] = f4(ndir, u2f(bounce + 1));
+                out.T[i] = f4(nT, 0.0f);
+            }
+        }
+        extFlags[i] = ef;
+        depFlags[i] = df;
+    }
+    waveCount(active, &st->photonQ);
+}
+
+// -------------------------------------------------- stable compaction --
+__global__ __launch_bounds__(kScanThreads) void k_tile_count(const uint8_t* flags, const uint32_t* count,
+                                                             uint32_t maxItems, uint32_t* tiles) {
+    const uint32_t n = loadCount(count, maxItems);
+    const uint32_t base = blockIdx.x * kTile + threadIdx.x * kTileSpan;
+    uint32_t s = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kTileSpan; ++k)
+        if (base + k < n) s += flags[base + k];
+    uint32_t total;
+    blockExclusiveScan(s, total);
+    if (threadIdx.x == 0) tiles[blockIdx.x] = total;
+}
+
+// Exclusive scan of the tile sums (one workgroup); *total = sum, clamped to
+// `cap`.  The number of tiles is ceil(n / kTile) for n = *count (or
+// `fixedN` if count is null).
+__global__ __launch_bounds__(1024) void k_tile_scan(uint32_t* tiles, const uint32_t* count, uint32_t fixedN,
+                                                    uint32_t maxItems, uint32_t* total, uint32_t cap) {
+    __shared__ uint32_t sums[1024 / 64 + 1];
+    const uint32_t n = count ? loadCount(count, maxItems) : fixedN;
+    const uint32_t numTiles = (n + kTile - 1) / kTile;
+    uint32_t carry = 0;
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    for (uint32_t base = 0; base < numTiles; base += 1024) {
+        const uint32_t idx = base + threadIdx.x;
+        const uint32_t v = idx < numTiles ? tiles[idx] : 0u;
+        uint32_t x = v;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(x, o, 64);
+            if (lane >= (uint32_t)o) x += y;
+        }
+        if (lane == 63u) sums[wave] = x;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t s = 0;
+            for (uint32_t w = 0; w < 1024 / 64; ++w) {
+                const uint32_t t = sums[w];
+                sums[w] = s;
+                s += t;
+            }
+            sums[1024 / 64] = s;
+        }
+        __syncthreads();
+        if (idx < numTiles) tiles[idx] = carry + x - v + sums[wave];
+        carry += sums[1024 / 64];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *total = carry < cap ? carry : cap;
+}
+
+template <typename Copy>
+__device__ __forceinline__ void scatterTile(const uint8_t* flags, uint32_t n, const uint32_t* tiles, uint32_t base,
+                                            uint32_t cap, Copy copy) {
+    const uint32_t first = blockIdx.x * kTile + threadIdx.x * kTileSpan;
+    uint8_t f[kTileSpan];
+    uint32_t s = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kTileSpan; ++k) {
+        f[k] = (first + k < n) ? flags[first + k] : 0;
+        s += f[k];
+    }
+    uint32_t total;
+    uint32_t off = blockExclusiveScan(s, total) + tiles[blockIdx.x] + base;
+#pragma unroll
+    for (uint32_t k = 0; k < kTileSpan; ++k) {
+        if (f[k]) {
+            if (off < cap) copy(first + k, off);
+            ++off;
+        }
+    }
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scatter_rays(const uint8_t* flags, const uint32_t* count,
+                                                               uint32_t maxItems, const uint32_t* tiles,
+                                                               RayQueue src, RayQueue dst, uint32_t cap) {
+    const uint32_t n = loadCount(count, maxItems);
+    scatterTile(flags, n, tiles, 0u, cap, [&](uint32_t i, uint32_t j) {
+        dst.o[j] = src.o[i];
+        dst.d[j] = src.d[i];
+        dst.T[j] = src.T[i];
+    });
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scatter_photons(const uint8_t* flags, const uint32_t* count,
+                                                                  uint32_t maxItems, const uint32_t* tiles,
+                                                                  const uint32_t* base, PhotonArrays src,
+                                                                  PhotonArrays dst, uint32_t cap) {
+    const uint32_t n = loadCount(count, maxItems);
+    scatterTile(flags, n, tiles, *base, cap, [&](uint32_t i, uint32_t j) {
+        dst.posPow[j] = src.posPow[i];
+        dst.dirPow[j] = src.dirPow[i];
+        dst.powZ[j] = src.powZ[i];
+    });
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scatter_events(const uint8_t* flags, const uint32_t* count,
+                                                                 uint32_t maxItems, const uint32_t* tiles,
+                                                                 EventArrays src, EventArrays dst, uint32_t cap) {
+    const uint32_t n = loadCount(count, maxItems);
+    scatterTile(flags, n, tiles, 0u, cap, [&](uint32_t i, uint32_t j) {
+        dst.e0[j] = src.e0[i];
+        dst.e1[j] = src.e1[i];
+        dst.e2[j] = src.e2[i];
+        dst.e3[j] = src.e3[i];
+    });
+}
+
+// ------------------------------------------------------ photon grid --
+// computePhotonCellIdsAndPermutation + countPhotonsPerCell
+// (PrimalKernels.cpp:796-858).
+__global__ __launch_bounds__(256) void k_cell_count(GridParams g, const DevState* st, uint32_t cap,
+                                                    const float4* posPow, uint32_t* cellId, uint32_t* cellCount) {
+    const uint32_t n = min(st->photonCount, cap);
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 p = posPow[i];
+    const F3 f = divv(mk3(p.x, p.y, p.z) - mk3(g.ox, g.oy, g.oz), mk3(g.cell, g.cell, g.cell));
+    const int x = min(max((int)floorf(f.x), 0), g.rx - 1);
+    const int y = min(max((int)floorf(f.y), 0), g.ry - 1);
+    const int z = min(max((int)floorf(f.z), 0), g.rz - 1);
+    const uint32_t c = (uint32_t)((z * g.ry + y) * g.rx + x);
+    cellId[i] = c;
+    atomicAdd(&cellCount[c], 1u);
+}
+
+// Exclusive scan, pass 1: per-tile scan of `in` into `out`, tile sums.
+__global__ __launch_bounds__(kScanThreads) void k_scan_local(const uint32_t* in, uint32_t* out, uint32_t n,
+                                                             uint32_t* tiles) {
+    const uint32_t first = blockIdx.x * kTile + threadIdx.x * kTileSpan;
+    uint32_t v[kTileSpan];
+    uint32_t s = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kTileSpan; ++k) {
+        v[k] = (first + k < n) ? in[first + k] : 0u;
+        s += v[k];
+    }
+    uint32_t total;
+    uint32_t off = blockExclusiveScan(s, total);
+#pragma unroll
+    for (uint32_t k = 0; k < kTileSpan; ++k) {
+        if (first + k < n) out[first + k] = off;
+        off += v[k];
+    }
+    if (threadIdx.x == 0) tiles[blockIdx.x] = total;
+}
+
+// Pass 3: add the scanned tile offsets; out[n] = grand total.
+__global__ __launch_bounds__(256) void k_scan_add(uint32_t* out, uint32_t n, const uint32_t* tiles,
+                                                  const uint32_t* grand) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] += tiles[i / kTile];
+    if (i == 0) out[n] = *grand;
+}
+
+// scatterPhotonsIntoCells (PrimalKernels.cpp:861-893).  The decrement leaves
+// cellCount zeroed for the next build.
+__global__ __launch_bounds__(256) void k_cell_scatter(const DevState* st, uint32_t cap, const uint32_t* cellId,
+                                                      const uint32_t* cellStart, uint32_t* cellCount,
+                                                      uint32_t* sortedIdx) {
+    const uint32_t n = min(st->photonCount, cap);
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t c = cellId[i];
+    const uint32_t slot = cellStart[c] + atomicSub(&cellCount[c], 1u) - 1u;
+    if (slot < n) sortedIdx[slot] = i;
+}
+
+// Order each cell by deposit index: the reference's sequential scatter
+// order, which makes every gather sum in the same order.
+__global__ __launch_bounds__(256) void k_cell_sort(uint32_t total, const uint32_t* cellStart, uint32_t* sortedIdx) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= total) return;
+    const uint32_t a = cellStart[c], b = cellStart[c + 1];
+    for (uint32_t j = a + 1; j < b; ++j) {
+        const uint32_t v = sortedIdx[j];
+        uint32_t k = j;
+        while (k > a && sortedIdx[k - 1] > v) {
+            sortedIdx[k] = sortedIdx[k - 1];
+            --k;
+        }
+        sortedIdx[k] = v;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_cell_payload(const DevState* st, uint32_t cap, const uint32_t* sortedIdx,
+                                                      PhotonArrays raw, PhotonArrays sorted) {
+    const uint32_t n = min(st->photonCount, cap);
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const uint32_t k = sortedIdx[j];
+    sorted.posPow[j] = raw.posPow[k];
+    sorted.dirPow[j] = raw.dirPow[k];
+    sorted.powZ[j] = raw.powZ[k];
+}
+
+// ---------------------------------------------------- camera gather --
+// launchCameraGatherKernel (PrimalKernels.cpp:500-653): one pass.
+__global__ __launch_bounds__(kTraceBlock) void k_gather(DevScene S, DevGrid G, DevCamera cam, DevState* st,
+                                                        uint32_t pass, int decorrelate, float invSpp, float4* fb) {
+    extern __shared__ uint2 s_stack[];
+    const LaneStack stk = laneStack(s_stack, S.stackDepth);
+    const uint32_t npx = cam.W * cam.H;
+    const uint32_t pixel = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t layers = 0;
+    if (pixel < npx) {
+        const uint32_t seed = decorrelate ? st->number + pass : st->number;
+        Rng rng(perItemSeed(seed, pixel));
+        const float jx = rng.nextFloat() - 0.5f;
+        const float jy = rng.nextFloat() - 0.5f;
+        F3 o, d;
+        primaryRay(cam, (float)(pixel % cam.W), (float)(pixel / cam.W), jx, jy, o, d);
+        F3 C = mk3(0, 0, 0);
+        float tr = 1.0f;
+        while (true) {
+            Hit h;
+            ++layers;
+            if (!intersectScene(S, o, d, stk, h)) break;
+            const DevInstance& I = S.inst[h.inst];
+            if (I.type == kGeoCloud) {
+                const SurfelShade sh = S.shade[h.prim];
+                F3 n = mk3(sh.nsx, sh.nsy, sh.nsz);
+                if (dot(n, -d) < 0.0f) n = -n;
+                const F3 E = gatherIrradiance(G, h.pos, n);
+                const F3 Lo = E * ((sh.alpha_r * mk3(sh.ar, sh.ag, sh.ab)) * kInvPi);
+                const float ae = sh.opacity * h.alpha;
+                C = C + (tr * ae) * Lo;
+                tr *= (1.0f - ae);
+                o = h.pos + d * 1e-4f;
+                continue;
+            }
+            const DevMaterial m = S.mats[I.material];
+            const F3 base = mk3(m.r, m.g, m.b);
+            if (m.power > 0.0f) {
+                C = C + tr * vmin(base * m.power, mk3(1.0f, 1.0f, 1.0f));
+            } else {
+                const F3 E = gatherIrradiance(G, h.pos, hitNormal(S, h));
+                C = C + tr * ((base * kInvPi) * E);
+            }
+            break;
+        }
+        const float4 f = fb[pixel];
+        fb[pixel] = make_float4(C.x * invSpp + f.x, C.y * invSpp + f.y, C.z * invSpp + f.z, 1.0f * invSpp + f.w);
+    }
+    const uint32_t ws = waveSum(layers);
+    if (laneId() == 0 && ws) atomicAdd(&st->gatherQ, (unsigned long long)ws);
+}
+
+// launchPostProcessKernel (PostprocessKernel.cpp:11-106), float LDR output.
+__global__ __launch_bounds__(256) void k_post(const float4* fb, float4* ldr, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float invGamma = 1.0f / 2.2f;
+    const float4 v = fb[i];
+    float c[3] = {v.x, v.y, v.z};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        float x = c[k] * 1.0f;
+        x = fmaxf(x, 0.0f);
+        x = pow_r(x, invGamma);
+        x = fminf(fmaxf(x, 0.0f), 1.0f);
+        c[k] = x;
+    }
+    ldr[i] = make_float4(c[0], c[1], c[2], 1.0f);
+}
+
+// ---------------------------------------------------------- adjoint --
+// launchRayGenAdjointKernel (AdjointKernels.cpp:15-84).  The adjoint seeds
+// from PathTracerSettings::random.number, which keeps its default 42.
+__global__ __launch_bounds__(256) void k_adj_raygen(DevCamera cam, const float* dloss, RayQueue q,
+                                                    PendingArrays pend, DevState* st) {
+    const uint32_t npx = cam.W * cam.H;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) st->qCount = npx;
+    if (i >= npx) return;
+    Rng rng(perItemSeed(42ull, i));
+    const float jx = rng.nextFloat() - 0.5f;
+    const float jy = rng.nextFloat() - 0.5f;
+    F3 o, d;
+    primaryRay(cam, (float)(i % cam.W), (float)(i / cam.W), jx, jy, o, d);
+    q.o[i] = f4(o, u2f(i));
+    q.d[i] = f4(d, u2f(0u));
+    q.T[i] = make_float4(dloss[3 * (size_t)i], dloss[3 * (size_t)i + 1], dloss[3 * (size_t)i + 2], 0.0f);
+    pend.p0[i] = make_float4(0.0f, 0.0f, 0.0f, u2f(kNone));
+}
+
+// launchAdjointIntersectKernel (AdjointKernels.cpp:87-386) for one queue.
+__global__ __launch_bounds__(kTraceBlock) void k_adj_trace(DevScene S, DevState* st, uint32_t bounceIndex,
+                                                           RayQueue in, RayQueue out, uint8_t* extFlags,
+                                                           EventArrays ev, uint8_t* evFlags, PendingArrays pend,
+                                                           uint32_t pendCap, TraceParams tp) {
+    extern __shared__ uint2 s_stack[];
+    const LaneStack stk = laneStack(s_stack, S.stackDepth);
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n = st->qCount;
+    const bool active = i < n;
+    if (active) {
+        uint8_t ef = 0, vf = 0;
+        Rng rng(perItemSeed(42ull, i));
+        const float4 o4 = in.o[i], d4 = in.d[i], T4 = in.T[i];
+        const F3 o = ld3(o4), d = ld3(d4), T = ld3(T4);
+        const uint32_t path = f2u(o4.w);
+        const uint32_t bounce = f2u(d4.w);
+        Hit h;
+        if (intersectScene(S, o, d, stk, h)) {
+            const DevInstance& I = S.inst[h.inst];
+            const bool cloud = I.type == kGeoCloud;
+            F3 on, ng = mk3(0, 0, 0), dir;
+            float pdf;
+            SurfelShade sh{};
+            if (!cloud) {
+                ng = hitNormal(S, h);
+                sampleCosine(rng, ng, dir, pdf);
+                on = ng;
+            } else {
+                sh = S.shade[h.prim];
+                const F3 nc = mk3(sh.nsx, sh.nsy, sh.nsz);
+                const float sc = dot(nc, -d);
+                const int sign = sc >= 0.0f ? 1 : -1;
+                on = (float)sign * nc;
+                sampleUniformHemi(rng, on, dir, pdf);
+            }
+            // Complete the pending interaction with this hit as its endpoint.
+            if (path < pendCap) {
+                const float4 p0 = pend.p0[path];
+                const uint32_t kind = f2u(p0.w);
+                const bool reflectDone = kind == kReflectS && cloud;
+                const bool nullDone = kind == kNullT;
+                if (reflectDone || nullDone) {
+                    const float4 p1 = pend.p1[path];
+                    const uint2 p2 = pend.p2[path];
+                    const F3 Tp = ld3(p1) * (reflectDone ? tp.qReflect : tp.qNull);
+                    const float endCos = fmaxf(0.0f, dot(dir, on));
+                    ev.e0[i] = f4(h.pos, u2f(kind));
+                    if (reflectDone) {
+                        ev.e1[i] = f4(on, u2f(h.prim));
+                        ev.e2[i] = f4(Tp, h.alpha);
+                        ev.e3[i] = make_float4(u2f(S.inst[p2.y].material), pdf, endCos, 0.0f);
+                    } else {
+                        ev.e1[i] = f4(on, u2f(p2.x));
+                        ev.e2[i] = f4(Tp, p1.w);
+                        ev.e3[i] = make_float4(u2f(cloud ? kGeoCloud : kGeoMesh), u2f(cloud ? h.prim : I.material),
+                                               endCos, 0.0f);
+                    }
+                    vf = 1;
+                    pend.p0[path] = make_float4(0.0f, 0.0f, 0.0f, u2f(kNone));
+                }
+            }
+            F3 no, nT;
+            bool spawn = false;
+            uint32_t newKind = kNone;
+            F3 pT = mk3(0, 0, 0);
+            float pAlpha = 0.0f;
+            uint2 pIds = make_uint2(kInvalid, kInvalid);
+            F3 ndir = dir;
+            bool projection = false;
+            if (!cloud) {
+                const DevMaterial m = S.mats[I.material];
+                const F3 base = mk3(m.r, m.g, m.b);
+                no = h.pos + ng * 1e-6f;
+                nT = T * base;
+                spawn = true;
+                newKind = kReflectS;
+                pT = T * base;
+                pIds = make_uint2(kInvalid, h.inst);
+            } else {
+                const float u = rng.nextFloat();
+                if (u < tp.qNull) {
+                    const float att = 1.0f - h.alpha * sh.opacity;
+                    const float w = att / tp.qNull;
+                    no = h.pos + d * 1e-5f;
+                    ndir = d;
+                    nT = T * w;
+                    spawn = true;
+                    newKind = kNullT;
+                    pT = divs(T, tp.qNull);
+                    pAlpha = h.alpha;
+                    pIds = make_uint2(h.prim, kInvalid);
+                } else if (u < tp.qNull + tp.qReflect) {
+                    const F3 fs = (sh.alpha_r * mk3(sh.ar, sh.ag, sh.ab)) * kInvPi;
+                    const float cosT = fmaxf(0.0f, dot(dir, on));
+                    const float alpha = h.alpha * sh.opacity;
+                    const F3 mult = divs((alpha / tp.qReflect) * (fs * cosT), pdf);
+                    no = h.pos + on * 1e-5f;
+                    nT = T * mult;
+                    spawn = true;
+                    projection = bounceIndex == 0;
+                } else if (u < tp.qNull + tp.qReflect + tp.qTransmit) {
+                    const float alpha = h.alpha * sh.opacity;
+                    const float w = (alpha * 0.0f) / tp.qTransmit;  // alpha_t = 0
+                    const F3 nc = mk3(sh.nsx, sh.nsy, sh.nsz);
+                    const float sc = dot(nc, -d);
+                    const int sign = sc >= 0.0f ? 1 : -1;
+                    const F3 ot = (float)sign * nc;
+                    float p2df;
+                    sampleUniformHemi(rng, ot, ndir, p2df);
+                    no = h.pos + (-ot) * 1e-5f;
+                    nT = (T * w) * mk3(sh.ar, sh.ag, sh.ab);
+                    spawn = true;
+                }
+            }
+            if (spawn && russianRoulette(rng, bounce + 1, nT, tp.rrStart)) {
+                if (newKind != kNone && path < pendCap) {
+                    pend.p0[path] = f4(h.pos, u2f(newKind));
+                    pend.p1[path] = f4(pT, pAlpha);
+                    pend.p2[path] = pIds;
+                }
+                if (projection && path < pendCap) {
+                    ev.e0[i] = f4(h.pos, u2f(kProjection));
+                    ev.e1[i] = f4(on, u2f(h.prim));
+                    ev.e2[i] = f4(divs(T, tp.qReflect), h.alpha);
+                    ev.e3[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                    vf = 1;
+                }
+                ef = 1;
+                out.o[i] = f4(no, o4.w);
+                out.d[i] = f4(ndir, u2f(bounce + 1));
+                out.T[i] = f4(nT, 0.0f);
+            }
+        }
+        extFlags[i] = ef;
+        evFlags[i] = vf;
+    }
+    waveCount(active, &st->adjointQ);
+}
+
+// launchAdjointTransportKernel (AdjointKernels.cpp:389-509).
+__global__ __launch_bounds__(256) void k_adj_transport(DevScene S, DevGrid G, DevState* st, uint32_t maxEvents,
+                                                       EventArrays ev, float invSpp, float* grad) {
+    const uint32_t n = min(st->evTotal, maxEvents);
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool active = j < n;
+    if (active) {
+        const float4 e0 = ev.e0[j], e1 = ev.e1[j], e2 = ev.e2[j], e3 = ev.e3[j];
+        const uint32_t kind = f2u(e0.w);
+        const uint32_t target = f2u(e1.w);
+        const F3 T = ld3(e2);
+        const float a = e2.w;
+        const F3 E = gatherIrradiance(G, ld3(e0), ld3(e1));
+        float val = 0.0f;
+        if (kind == kProjection) {
+            const SurfelShade sh = S.shade[target];
+            const F3 fr = (sh.alpha_r * mk3(sh.ar, sh.ag, sh.ab)) * kInvPi;
+            const F3 Lr = fr * E;
+            const F3 g = (a * T) * Lr;
+            val = fsum(g) * invSpp;
+        } else if (kind == kNullT) {
+            F3 base;
+            float power;
+            if (f2u(e3.x) == kGeoMesh) {
+                const DevMaterial m = S.mats[f2u(e3.y)];
+                base = mk3(m.r, m.g, m.b);
+                power = m.power;
+            } else {
+                const SurfelShade es = S.shade[f2u(e3.y)];
+                base = es.alpha_r * mk3(es.ar, es.ag, es.ab);
+                power = 0.0f;
+            }
+            const F3 Lr = (E * base) * kInvPi;
+            F3 Le = mk3(0, 0, 0);
+            if (power > 0.0f && S.nLights > 0) Le = (base * (power / (kPi * S.lights[0].area))) * e3.z;
+            const F3 Lo = Le + Lr;
+            const F3 g = ((-a) * T) * Lo;
+            val = fsum(g) * invSpp;
+        } else if (kind == kReflectS) {
+            const DevMaterial m = S.mats[f2u(e3.x)];
+            const F3 Lo = (E * mk3(m.r, m.g, m.b)) * kInvPi;
+            const F3 p = divs(T, e3.y);
+            const F3 g = (p * Lo) * a;
+            val = fsum(g) * invSpp;
+        }
+        atomicAdd(&grad[target], val);
+    }
+    waveCount(active, &st->events);
+}
+
+// ------------------------------------------------------ diagnostics --
+__global__ __launch_bounds__(kTraceBlock) void k_trace_rays(DevScene S, uint32_t n, const float* o, const float* d,
+                                                            int32_t* inst, int32_t* prim, float* t, float* pos,
+                                                            float* alpha) {
+    extern __shared__ uint2 s_stack[];
+    const LaneStack stk = laneStack(s_stack, S.stackDepth);
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Hit h;
+    const bool hit = intersectScene(S, mk3(o[3 * i], o[3 * i + 1], o[3 * i + 2]),
+                                    mk3(d[3 * i], d[3 * i + 1], d[3 * i + 2]), stk, h);
+    inst[i] = hit ? (int32_t)h.inst : -1;
+    prim[i] = hit ? (int32_t)h.prim : -1;
+    t[i] = hit ? h.t : 0.0f;
+    pos[3 * i] = hit ? h.pos.x : 0.0f;
+    pos[3 * i + 1] = hit ? h.pos.y : 0.0f;
+    pos[3 * i + 2] = hit ? h.pos.z : 0.0f;
+    alpha[i] = hit ? h.alpha : 0.0f;
+}
+
+// ------------------------------------------------------------ launchers --
+static inline uint32_t blocks(uint32_t n, uint32_t b) { return (n + b - 1) / b; }
+static inline size_t stackBytes(const DevScene& S) { return (size_t)S.stackDepth * sizeof(uint2) * kTraceBlock; }
+
+void launchFill(hipStream_t s, uint32_t* p, uint32_t n, uint32_t v) {
+    if (n) k_fill<<<min(blocks(n, 256), 65535u), 256, 0, s>>>(p, n, v);
+}
+void launchFillF(hipStream_t s, float* p, uint32_t n, float v) {
+    if (n) k_fillf<<<min(blocks(n, 256), 65535u), 256, 0, s>>>(p, n, v);
+}
+void launchSeedNext(hipStream_t s, DevState* st, const uint32_t* table, uint32_t len, int bounce) {
+    k_seed_next<<<1, 64, 0, s>>>(st, table, len, bounce);
+}
+void launchEmit(hipStream_t s, const DevScene& S, DevState* st, uint32_t P, float invEmitted, RayQueue staged,
+                uint8_t* flags) {
+    if (P) k_emit<<<blocks(P, 256), 256, 0, s>>>(S, st, P, invEmitted, staged, flags);
+}
+void launchPhotonTrace(hipStream_t s, const DevScene& S, DevState* st, uint32_t maxRays, RayQueue in,
+                       RayQueue staged, uint8_t* extFlags, PhotonArrays dep, uint8_t* depFlags, TraceParams tp) {
+    if (maxRays)
+        k_photon_trace<<<blocks(maxRays, kTraceBlock), kTraceBlock, stackBytes(S), s>>>(S, st, in, staged, extFlags,
+                                                                                      dep, depFlags, tp);
+}
+void launchCompactRays(hipStream_t s, const uint8_t* flags, const uint32_t* count, uint32_t maxItems,
+                       uint32_t* tiles, uint32_t* total, RayQueue src, RayQueue dst, uint32_t dstCap) {
+    const uint32_t nt = blocks(maxItems, kTile);
+    k_tile_count<<<nt, kScanThreads, 0, s>>>(flags, count, maxItems, tiles);
+    k_tile_scan<<<1, 1024, 0, s>>>(tiles, count, 0u, maxItems, total, dstCap);
+    k_scatter_rays<<<nt, kScanThreads, 0, s>>>(flags, count, maxItems, tiles, src, dst, dstCap);
+}
+void launchCompactPhotons(hipStream_t s, const uint8_t* flags, const uint32_t* count, uint32_t maxItems,
+                          uint32_t* tiles, uint32_t* total, const uint32_t* base, PhotonArrays src,
+                          PhotonArrays dst, uint32_t dstCap) {
+    const uint32_t nt = blocks(maxItems, kTile);
+    k_tile_count<<<nt, kScanThreads, 0, s>>>(flags, count, maxItems, tiles);
+    k_tile_scan<<<1, 1024, 0, s>>>(tiles, count, 0u, maxItems, total, 0xFFFFFFFFu);
+    k_scatter_photons<<<nt, kScanThreads, 0, s>>>(flags, count, maxItems, tiles, base, src, dst, dstCap);
+}
+void launchCompactEvents(hipStream_t s, const uint8_t* flags, const uint32_t* count, uint32_t maxItems,
+                         uint32_t* tiles, uint32_t* total, EventArrays src, EventArrays dst, uint32_t dstCap) {
+    const uint32_t nt = blocks(maxItems, kTile);
+    k_tile_count<<<nt, kScanThreads, 0, s>>>(flags, count, maxItems, tiles);
+    k_tile_scan<<<1, 1024, 0, s>>>(tiles, count, 0u, maxItems, total, dstCap);
+    k_scatter_events<<<nt, kScanThreads, 0, s>>>(flags, count, maxItems, tiles, src, dst, dstCap);
+}
+void launchFinishPhotonBounce(hipStream_t s, DevState* st, uint32_t cap) {
+    k_finish_photon_bounce<<<1, 64, 0, s>>>(st, cap);
+}
+void launchSetQueue(hipStream_t s, DevState* st, const uint32_t* src) { k_set_queue<<<1, 64, 0, s>>>(st, src); }
+
+void launchBuildGrid(hipStream_t s, const GridParams& g, DevState* st, uint32_t cap, PhotonArrays raw,
+                     PhotonArrays sorted, uint32_t* cellId, uint32_t* sortedIdx, uint32_t* cellCount,
+                     uint32_t* cellStart, uint32_t* tiles, uint32_t* scratchTotal) {
+    const uint32_t pb = blocks(cap, 256);
+    k_cell_count<<<pb, 256, 0, s>>>(g, st, cap, raw.posPow, cellId, cellCount);
+    const uint32_t nt = blocks(g.total, kTile);
+    k_scan_local<<<nt, kScanThreads, 0, s>>>(cellCount, cellStart, g.total, tiles);
+    k_tile_scan<<<1, 1024, 0, s>>>(tiles, nullptr, g.total, g.total, scratchTotal, 0xFFFFFFFFu);
+    k_scan_add<<<blocks(g.total, 256), 256, 0, s>>>(cellStart, g.total, tiles, scratchTotal);
+    k_cell_scatter<<<pb, 256, 0, s>>>(st, cap, cellId, cellStart, cellCount, sortedIdx);
+    k_cell_sort<<<blocks(g.total, 256), 256, 0, s>>>(g.total, cellStart, sortedIdx);
+    k_cell_payload<<<pb, 256, 0, s>>>(st, cap, sortedIdx, raw, sorted);
+}
+
+void launchGather(hipStream_t s, const DevScene& S, const DevGrid& G, const DevCamera& cam, DevState* st,
+                  uint32_t pass, int decorrelate, float invSpp, float4* fb) {
+    const uint32_t npx = cam.W * cam.H;
+    if (npx)
+        k_gather<<<blocks(npx, kTraceBlock), kTraceBlock, stackBytes(S), s>>>(S, G, cam, st, pass, decorrelate,
+                                                                             invSpp, fb);
+}
+void launchPostprocess(hipStream_t s, const float4* fb, float4* ldr, uint32_t n) {
+    if (n) k_post<<<blocks(n, 256), 256, 0, s>>>(fb, ldr, n);
+}
+void launchAdjRaygen(hipStream_t s, const DevCamera& cam, const float* dloss, RayQueue q, PendingArrays pend,
+                     DevState* st) {
+    const uint32_t npx = cam.W * cam.H;
+    k_adj_raygen<<<blocks(npx > 0 ? npx : 1, 256), 256, 0, s>>>(cam, dloss, q, pend, st);
+}
+void launchAdjTrace(hipStream_t s, const DevScene& S, DevState* st, uint32_t maxRays, uint32_t bounce, RayQueue in,
+                    RayQueue staged, uint8_t* extFlags, EventArrays ev, uint8_t* evFlags, PendingArrays pend,
+                    TraceParams tp) {
+    if (maxRays)
+        k_adj_trace<<<blocks(maxRays, kTraceBlock), kTraceBlock, stackBytes(S), s>>>(
+            S, st, bounce, in, staged, extFlags, ev, evFlags, pend, maxRays, tp);
+}
+void launchAdjTransport(hipStream_t s, const DevScene& S, const DevGrid& G, DevState* st, uint32_t maxEvents,
+                        EventArrays ev, float invSpp, float* gradOpacity) {
+    if (maxEvents)
+        k_adj_transport<<<blocks(maxEvents, 256), 256, 0, s>>>(S, G, st, maxEvents, ev, invSpp, gradOpacity);
+}
+void launchTraceRays(hipStream_t s, const DevScene& S, uint32_t n, const float* o, const float* d, int32_t* inst,
+                     int32_t* prim, float* t, float* pos, float* alpha) {
+    if (n)
+        k_trace_rays<<<blocks(n, kTraceBlock), kTraceBlock, stackBytes(S), s>>>(S, n, o, d, inst, prim, t, pos,
+                                                                               alpha);
+}
+
+}  // namespace pale

@@ -1,0 +1,910 @@
+// pale_render.hip -- renderer handle, device memory, pass orchestration and
+// the extern "C" boundary declared in include/pale_abi.h.
+//
+// Orchestration restates the reference's PathTracer / SyclBridge
+// (PathTracer.cpp:22-356, SyclBridge.cpp:82-294) as one HIP stream per
+// handle: every launch of a forward or backward pass is queued back to back;
+// queue counts, seeds and deposit counts stay in device memory (DevState), so
+// the host synchronises once per call, when results are copied out.
+#include <algorithm>
+#include <cstring>
+#include <random>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "pale_abi.h"
+#include "pale_kernels.h"
+#include "pale_scene.h"
+
+using namespace pale;
+
+namespace {
+
+thread_local std::string g_err;
+
+struct StatusError : std::runtime_error {
+    int code;
+    StatusError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPCHECK(x)                                                                                   \
+    do {                                                                                              \
+        hipError_t e_ = (x);                                                                          \
+        if (e_ != hipSuccess) throw StatusError(PALE_ERR_DEVICE, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+struct DevMem {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevMem() = default;
+    DevMem(const DevMem&) = delete;
+    DevMem& operator=(const DevMem&) = delete;
+    DevMem(DevMem&& o) noexcept : p(o.p), bytes(o.bytes) {
+        o.p = nullptr;
+        o.bytes = 0;
+    }
+    ~DevMem() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    void reserve(size_t b) {
+        if (b <= bytes && p) return;
+        release();
+        HIPCHECK(hipMalloc(&p, std::max<size_t>(b, 16)));
+        bytes = std::max<size_t>(b, 16);
+    }
+    template <typename T>
+    T* as() const {
+        return static_cast<T*>(p);
+    }
+};
+
+template <typename T>
+void upload(hipStream_t s, DevMem& m, const std::vector<T>& v) {
+    m.reserve(v.size() * sizeof(T));
+    if (!v.empty()) HIPCHECK(hipMemcpyAsync(m.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+}
+
+uint32_t nextPow2(uint32_t v) {
+    uint32_t c = 1;
+    while (c < v) c <<= 1u;
+    return c;
+}
+
+F3 ld3h(const float* p) { return mk3(p[0], p[1], p[2]); }
+
+}  // namespace
+
+struct pale_handle {
+    pale_settings S{};
+    std::vector<HostMesh> meshes;
+    std::vector<HostCamera> cams;
+    std::vector<AssetPoint> asset;
+    BuildProducts bp;
+    DeviceImage img;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+    DevMem geo, shade, nodes, tris, inst, tlas, mats, lights, emTris;
+    DevScene scene{};
+
+    // Photon map (persistent between forward and backward passes, as the
+    // reference's RenderIntermediatesGPU::map is).
+    bool gridConfigured = false;
+    GridParams grid{};
+    float radius = 0.0f;
+    uint32_t photonCap = 0;
+    DevMem rawPosPow, rawDirPow, rawPowZ, srtPosPow, srtDirPow, srtPowZ, cellId, sortedIdx, cellCount, cellStart,
+        gridTiles, gridTotal;
+    bool mapValid = false;
+
+    // Ray queues and staging.
+    uint32_t queueCap = 0;
+    DevMem qA[3], qB[3], stg[3], extFlags, depStg[3], depFlags, tiles, counts;
+    // Adjoint.
+    uint32_t adjCap = 0;
+    DevMem pend0, pend1, pend2, evStg[4], evCmp[4], evFlags, dloss, gradOpacity;
+    // Device state and seed table.
+    DevMem state, seedTable;
+    // Per-camera framebuffers.
+    std::vector<DevMem> fb, ldr;
+    pale_stats stats{};
+
+    ~pale_handle() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    DevState* st() const { return state.as<DevState>(); }
+    uint32_t* stField(size_t off) const { return reinterpret_cast<uint32_t*>(state.as<char>() + off); }
+
+    static RayQueue rq(DevMem* m) { return RayQueue{m[0].as<float4>(), m[1].as<float4>(), m[2].as<float4>()}; }
+    static PhotonArrays pa(DevMem& a, DevMem& b, DevMem& c) {
+        return PhotonArrays{a.as<float4>(), b.as<float4>(), c.as<float>()};
+    }
+    static EventArrays ea(DevMem* m) {
+        return EventArrays{m[0].as<float4>(), m[1].as<float4>(), m[2].as<float4>(), m[3].as<float4>()};
+    }
+
+    TraceParams tp() const { return TraceParams{S.rr_start, S.q_null, S.q_reflect, S.q_transmit}; }
+
+    DevGrid devGrid() const {
+        DevGrid g{};
+        g.posPow = srtPosPow.as<float4>();
+        g.dirPow = srtDirPow.as<float4>();
+        g.powZ = srtPowZ.as<float>();
+        g.cellStart = cellStart.as<uint32_t>();
+        g.ox = grid.ox;
+        g.oy = grid.oy;
+        g.oz = grid.oz;
+        g.cx = g.cy = g.cz = grid.cell;
+        g.rx = grid.rx;
+        g.ry = grid.ry;
+        g.rz = grid.rz;
+        g.radius = radius;
+        g.attempted = reinterpret_cast<const uint32_t*>(state.as<char>() + offsetof(DevState, photonCount));
+        g.capacity = photonCap;
+        g.valid = mapValid ? 1u : 0u;
+        return g;
+    }
+
+    DevCamera devCam(const HostCamera& c) const {
+        DevCamera d{};
+        std::copy(c.invView, c.invView + 16, d.invView);
+        d.W = c.W;
+        d.H = c.H;
+        d.fx = c.fx;
+        d.fy = c.fy;
+        return d;
+    }
+
+    // Upload of the build products (SceneUpload::allocateAndUpload,
+    // SceneUpload.ixx:321-476).
+    void uploadScene() {
+        upload(stream, geo, img.surfGeo);
+        upload(stream, shade, img.surfShade);
+        upload(stream, nodes, img.nodes);
+        upload(stream, tris, img.tris);
+        upload(stream, inst, img.instances);
+        upload(stream, tlas, img.tlas);
+        upload(stream, mats, img.materials);
+        upload(stream, lights, img.lights);
+        upload(stream, emTris, img.emTris);
+        scene.geo = geo.as<SurfelGeo>();
+        scene.shade = shade.as<SurfelShade>();
+        scene.nodes = nodes.as<WideNode>();
+        scene.tris = tris.as<TriGeo>();
+        scene.inst = inst.as<DevInstance>();
+        scene.tlas = tlas.as<TlasNode>();
+        scene.mats = mats.as<DevMaterial>();
+        scene.lights = lights.as<DevLight>();
+        scene.emTris = emTris.as<DevEmTri>();
+        scene.nInst = (uint32_t)img.instances.size();
+        scene.nTlas = (uint32_t)img.tlas.size();
+        scene.nLights = (uint32_t)img.lights.size();
+        scene.stackDepth = img.stackDepth;
+        if ((size_t)img.stackDepth * sizeof(uint2) * kTraceBlock > 65536)
+            throw StatusError(PALE_ERR_INVALID_ARG, "BVH too deep for the LDS traversal stack");
+        gradOpacity.reserve(std::max<size_t>(1, bp.points.size()) * sizeof(float));
+        HIPCHECK(hipStreamSynchronize(stream));
+    }
+
+    void rebuild() {
+        bp = buildScene(meshes, asset);
+        img = makeDeviceImage(bp, meshes);
+        uploadScene();
+        setScene();
+    }
+
+    // PathTracer::setScene (PathTracer.cpp:22-53): queue capacity, photon map
+    // and grid (configured once, on the first call).
+    void setScene() {
+        ensureQueues(nextPow2(std::max(S.photons, 1u)));
+        if (!gridConfigured) {
+            const uint64_t requested = (uint64_t)S.photons * (uint64_t)(S.forward_passes * S.bounces);
+            const uint64_t maxPhotons = (10ull * 1024ull * 1024ull * 1024ull) / 64ull;
+            if (requested > maxPhotons)
+                throw StatusError(PALE_ERR_INVALID_ARG,
+                                  "photons * forward_passes * bounces exceeds the 10 GiB photon-map limit");
+            photonCap = (uint32_t)requested;
+            // configurePhotonGrid (PathTracer.cpp:184-221)
+            radius = S.gather_radius;
+            const float cell = 0.5f * radius;
+            const TlasNode& root = bp.tlas[0];
+            const F3 pad = mk3(radius, radius, radius);
+            const F3 origin = mk3(root.mnx, root.mny, root.mnz) - pad;
+            const F3 gmax = mk3(root.mxx, root.mxy, root.mxz) + pad;
+            const F3 ext = gmax - origin;
+            auto axisCount = [](float e, float c) { return (int)std::ceil(e / std::fmax(c, 1e-6f)); };
+            grid.ox = origin.x;
+            grid.oy = origin.y;
+            grid.oz = origin.z;
+            grid.cell = cell;
+            grid.rx = axisCount(ext.x, cell);
+            grid.ry = axisCount(ext.y, cell);
+            grid.rz = axisCount(ext.z, cell);
+            const uint64_t total = (uint64_t)grid.rx * (uint64_t)grid.ry * (uint64_t)grid.rz;
+            if (total == 0 || total >= 0xFFFFFFFFull)
+                throw StatusError(PALE_ERR_INVALID_ARG, "Photon grid resolution too high; increase r or tighten AABB.");
+            grid.total = (uint32_t)total;
+            const size_t pc = std::max<uint32_t>(photonCap, 1u);
+            rawPosPow.reserve(pc * sizeof(float4));
+            rawDirPow.reserve(pc * sizeof(float4));
+            rawPowZ.reserve(pc * sizeof(float));
+            srtPosPow.reserve(pc * sizeof(float4));
+            srtDirPow.reserve(pc * sizeof(float4));
+            srtPowZ.reserve(pc * sizeof(float));
+            cellId.reserve(pc * sizeof(uint32_t));
+            sortedIdx.reserve(pc * sizeof(uint32_t));
+            cellCount.reserve((size_t)grid.total * sizeof(uint32_t));
+            cellStart.reserve(((size_t)grid.total + 1) * sizeof(uint32_t));
+            gridTiles.reserve(((size_t)grid.total / kTile + 2) * sizeof(uint32_t));
+            gridTotal.reserve(sizeof(uint32_t) * 4);
+            launchFill(stream, cellCount.as<uint32_t>(), grid.total, 0u);
+            HIPCHECK(hipStreamSynchronize(stream));
+            gridConfigured = true;
+        }
+    }
+
+    void ensureQueues(uint32_t cap) {
+        if (cap <= queueCap) return;
+        for (int k = 0; k < 3; ++k) {
+            qA[k].reserve((size_t)cap * sizeof(float4));
+            qB[k].reserve((size_t)cap * sizeof(float4));
+            stg[k].reserve((size_t)cap * sizeof(float4));
+        }
+        extFlags.reserve(cap);
+        depStg[0].reserve((size_t)cap * sizeof(float4));
+        depStg[1].reserve((size_t)cap * sizeof(float4));
+        depStg[2].reserve((size_t)cap * sizeof(float));
+        depFlags.reserve(cap);
+        tiles.reserve(((size_t)cap / kTile + 2) * sizeof(uint32_t));
+        counts.reserve(16 * sizeof(uint32_t));
+        queueCap = cap;
+    }
+
+    void ensureAdjoint(uint32_t npx) {
+        ensureQueues(nextPow2(npx));
+        if (npx <= adjCap) return;
+        pend0.reserve((size_t)npx * sizeof(float4));
+        pend1.reserve((size_t)npx * sizeof(float4));
+        pend2.reserve((size_t)npx * sizeof(uint2));
+        for (int k = 0; k < 4; ++k) {
+            evStg[k].reserve((size_t)npx * sizeof(float4));
+            evCmp[k].reserve((size_t)npx * sizeof(float4));
+        }
+        evFlags.reserve(npx);
+        dloss.reserve((size_t)npx * 3 * sizeof(float));
+        adjCap = npx;
+    }
+
+    void resetState() {
+        launchFill(stream, state.as<uint32_t>(), sizeof(DevState) / 4, 0u);
+    }
+
+    void readStats(bool forward) {
+        DevState h{};
+        HIPCHECK(hipMemcpy(&h, state.p, sizeof(DevState), hipMemcpyDeviceToHost));
+        if (forward) {
+            stats.photon_queries = h.photonQ;
+            stats.gather_queries = h.gatherQ;
+            stats.photons_deposited = h.photonCount;
+        } else {
+            stats.adjoint_queries = h.adjointQ;
+            stats.gradient_events = h.events;
+        }
+        stats.grid_cells = grid.total;
+    }
+
+    // submitPhotonMappingKernel (SyclBridge.cpp:82-166) for the given cameras.
+    void renderForward(const std::vector<uint32_t>& camIdx) {
+        const uint32_t P = S.photons;
+        if (P > queueCap) ensureQueues(nextPow2(P));
+        // Seed schedule: mt19937_64(seed), truncated to 32 bits
+        // (SyclBridge.cpp:83-143, Random::number is uint32_t).
+        std::mt19937_64 gen(S.seed);
+        std::vector<uint32_t> table((size_t)S.forward_passes * (1 + S.bounces) + camIdx.size() + 1);
+        for (uint32_t& v : table) v = (uint32_t)gen();
+        upload(stream, seedTable, table);
+        HIPCHECK(hipStreamSynchronize(stream));
+        resetState();
+        HIPCHECK(hipEventRecord(ev0, stream));
+        DevState* st = this->st();
+        const uint32_t emitted = S.photons * S.forward_passes;
+        const float invEmitted = 1.0f / (float)emitted;
+        uint32_t* emitCount = counts.as<uint32_t>();
+        launchFill(stream, emitCount, 1, P);
+        uint32_t* qCount = stField(offsetof(DevState, qCount));
+        uint32_t* extTotal = stField(offsetof(DevState, extTotal));
+        uint32_t* depTotal = stField(offsetof(DevState, depTotal));
+        uint32_t* photonCount = stField(offsetof(DevState, photonCount));
+        const uint32_t tlen = (uint32_t)table.size();
+        DevMem* in = qA;
+        DevMem* out = qB;
+        for (uint32_t pass = 0; pass < S.forward_passes; ++pass) {
+            launchSeedNext(stream, st, seedTable.as<uint32_t>(), tlen, -1);
+            launchEmit(stream, scene, st, P, invEmitted, rq(stg), extFlags.as<uint8_t>());
+            launchCompactRays(stream, extFlags.as<uint8_t>(), emitCount, P, tiles.as<uint32_t>(), extTotal, rq(stg),
+                              rq(in), queueCap);
+            launchSetQueue(stream, st, extTotal);
+            for (uint32_t b = 0; b < S.bounces; ++b) {
+                launchSeedNext(stream, st, seedTable.as<uint32_t>(), tlen, (int)b);
+                launchPhotonTrace(stream, scene, st, P, rq(in), rq(stg), extFlags.as<uint8_t>(),
+                                  pa(depStg[0], depStg[1], depStg[2]), depFlags.as<uint8_t>(), tp());
+                launchCompactRays(stream, extFlags.as<uint8_t>(), qCount, P, tiles.as<uint32_t>(), extTotal, rq(stg),
+                                  rq(out), queueCap);
+                launchCompactPhotons(stream, depFlags.as<uint8_t>(), qCount, P, tiles.as<uint32_t>(), depTotal,
+                                     photonCount, pa(depStg[0], depStg[1], depStg[2]),
+                                     pa(rawPosPow, rawDirPow, rawPowZ), photonCap);
+                launchFinishPhotonBounce(stream, st, photonCap);
+                std::swap(in, out);
+            }
+        }
+        // buildPhotonCellRangesAndOrdering (PrimalKernels.cpp:1007-1018)
+        launchBuildGrid(stream, grid, st, photonCap, pa(rawPosPow, rawDirPow, rawPowZ),
+                        pa(srtPosPow, srtDirPow, srtPowZ), cellId.as<uint32_t>(), sortedIdx.as<uint32_t>(),
+                        cellCount.as<uint32_t>(), cellStart.as<uint32_t>(), gridTiles.as<uint32_t>(),
+                        gridTotal.as<uint32_t>());
+        mapValid = true;
+        const DevGrid G = devGrid();
+        const float invSpp = 1.0f / (float)S.gather_passes;
+        for (size_t k = 0; k < camIdx.size(); ++k) {
+            const HostCamera& c = cams[camIdx[k]];
+            const uint32_t npx = c.W * c.H;
+            launchSeedNext(stream, st, seedTable.as<uint32_t>(), tlen, -1);
+            launchFillF(stream, fb[camIdx[k]].as<float>(), npx * 4, 0.0f);
+            const DevCamera dc = devCam(c);
+            for (uint32_t pass = 0; pass < S.gather_passes; ++pass)
+                launchGather(stream, scene, G, dc, st, pass, S.spp_decorrelate, invSpp, fb[camIdx[k]].as<float4>());
+            launchPostprocess(stream, fb[camIdx[k]].as<float4>(), ldr[camIdx[k]].as<float4>(), npx);
+        }
+        HIPCHECK(hipEventRecord(ev1, stream));
+        HIPCHECK(hipStreamSynchronize(stream));
+        HIPCHECK(hipGetLastError());
+        float ms = 0.0f;
+        HIPCHECK(hipEventElapsedTime(&ms, ev0, ev1));
+        stats.forward_ms = ms;
+        readStats(true);
+    }
+
+    // submitAdjointKernel (SyclBridge.cpp:169-294).
+    void renderBackward(const std::vector<uint32_t>& camIdx, const std::vector<const float*>& src) {
+        const uint32_t N = (uint32_t)bp.points.size();
+        uint32_t maxNpx = 1;
+        for (uint32_t c : camIdx) maxNpx = std::max(maxNpx, cams[c].W * cams[c].H);
+        ensureAdjoint(maxNpx);
+        const DevGrid G = devGrid();
+        // Keep the forward's deposit count (the gather normalisation T) and
+        // reset only the adjoint counters.
+        DevState h{};
+        HIPCHECK(hipMemcpyAsync(&h, state.p, sizeof(DevState), hipMemcpyDeviceToHost, stream));
+        HIPCHECK(hipStreamSynchronize(stream));
+        h.adjointQ = 0;
+        h.events = 0;
+        h.qCount = 0;
+        HIPCHECK(hipMemcpyAsync(state.p, &h, sizeof(DevState), hipMemcpyHostToDevice, stream));
+        HIPCHECK(hipStreamSynchronize(stream));
+        launchFillF(stream, gradOpacity.as<float>(), N, 0.0f);
+        HIPCHECK(hipEventRecord(ev0, stream));
+        DevState* st = this->st();
+        uint32_t* qCount = stField(offsetof(DevState, qCount));
+        uint32_t* extTotal = stField(offsetof(DevState, extTotal));
+        uint32_t* evTotal = stField(offsetof(DevState, evTotal));
+        const float invSpp = 1.0f / (float)S.adjoint_passes;
+        for (size_t k = 0; k < camIdx.size(); ++k) {
+            const HostCamera& c = cams[camIdx[k]];
+            const uint32_t npx = c.W * c.H;
+            HIPCHECK(hipMemcpyAsync(dloss.p, src[k], (size_t)npx * 3 * sizeof(float), hipMemcpyHostToDevice, stream));
+            const DevCamera dc = devCam(c);
+            const PendingArrays pend{pend0.as<float4>(), pend1.as<float4>(), pend2.as<uint2>()};
+            for (uint32_t spp = 0; spp < S.adjoint_passes; ++spp) {
+                DevMem* in = qA;
+                DevMem* out = qB;
+                launchAdjRaygen(stream, dc, dloss.as<float>(), rq(in), pend, st);
+                for (uint32_t b = 0; b < S.adjoint_bounces; ++b) {
+                    launchAdjTrace(stream, scene, st, npx, b, rq(in), rq(stg), extFlags.as<uint8_t>(), ea(evStg),
+                                   evFlags.as<uint8_t>(), pend, tp());
+                    launchCompactEvents(stream, evFlags.as<uint8_t>(), qCount, npx, tiles.as<uint32_t>(), evTotal,
+                                        ea(evStg), ea(evCmp), adjCap);
+                    launchCompactRays(stream, extFlags.as<uint8_t>(), qCount, npx, tiles.as<uint32_t>(), extTotal,
+                                      rq(stg), rq(out), queueCap);
+                    launchAdjTransport(stream, scene, G, st, npx, ea(evCmp), invSpp, gradOpacity.as<float>());
+                    launchSetQueue(stream, st, extTotal);
+                    std::swap(in, out);
+                }
+            }
+        }
+        HIPCHECK(hipEventRecord(ev1, stream));
+        HIPCHECK(hipStreamSynchronize(stream));
+        HIPCHECK(hipGetLastError());
+        float ms = 0.0f;
+        HIPCHECK(hipEventElapsedTime(&ms, ev0, ev1));
+        stats.backward_ms = ms;
+        readStats(false);
+    }
+};
+
+namespace {
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+template <typename F>
+int guarded(F&& f) {
+    try {
+        g_err.clear();
+        f();
+        return PALE_OK;
+    } catch (const StatusError& e) {
+        return fail(e.code, e.what());
+    } catch (const std::exception& e) {
+        return fail(PALE_ERR_INTERNAL, e.what());
+    }
+}
+
+std::vector<AssetPoint> toAsset(const pale_points_desc& p) {
+    if (p.count && (!p.position || !p.tangent_u || !p.tangent_v || !p.scale || !p.albedo || !p.opacity || !p.beta))
+        throw StatusError(PALE_ERR_INVALID_ARG, "points: missing array");
+    std::vector<AssetPoint> out(p.count);
+    for (uint32_t i = 0; i < p.count; ++i) {
+        AssetPoint a{};
+        a.p = ld3h(p.position + 3 * i);
+        a.tu = ld3h(p.tangent_u + 3 * i);
+        a.tv = ld3h(p.tangent_v + 3 * i);
+        a.su = p.scale[2 * i];
+        a.sv = p.scale[2 * i + 1];
+        a.albedo = ld3h(p.albedo + 3 * i);
+        a.opacity = p.opacity[i];
+        a.beta = p.beta[i];
+        a.shape = p.shape ? p.shape[i] : 0.0f;
+        out[i] = a;
+    }
+    return out;
+}
+
+std::vector<uint32_t> selectCameras(const pale_handle* h, const char* name) {
+    std::vector<uint32_t> idx;
+    const std::string nm = name ? name : "";
+    for (uint32_t i = 0; i < h->cams.size(); ++i)
+        if (nm.empty() || h->cams[i].name == nm) idx.push_back(i);
+    return idx;
+}
+
+// glm's quat * vec3 (v + ((uv * w) + uuv) * 2 with uv = q x v, uuv = q x uv).
+F3 rotate(const float q[4], F3 v) {
+    const F3 qv = mk3(q[0], q[1], q[2]);
+    const F3 uv = cross(qv, v);
+    const F3 uuv = cross(qv, uv);
+    return v + ((uv * q[3]) + uuv) * 2.0f;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* pale_last_error(void) { return g_err.c_str(); }
+
+void pale_default_settings(pale_settings* out) {
+    if (!out) return;
+    pale_settings s{};
+    s.photons = 1000000u;
+    s.bounces = 6;
+    s.forward_passes = 6;
+    s.gather_passes = 6;
+    s.adjoint_bounces = 6;
+    s.adjoint_passes = 6;
+    s.seed = 42;
+    s.gather_radius = 0.01f;
+    s.rr_start = 6;
+    s.q_null = 0.5f;
+    s.q_reflect = 0.5f;
+    s.q_transmit = 0.0f;
+    s.spp_decorrelate = 0;
+    s.debug_images = 0;
+    *out = s;
+}
+
+int pale_create(const pale_scene_desc* scene, const pale_settings* settings, pale_handle** out) {
+    if (!scene || !settings || !out) return fail(PALE_ERR_INVALID_ARG, "pale_create: null argument");
+    *out = nullptr;
+    pale_handle* h = new pale_handle();
+    const int rc = guarded([&] {
+        int ndev = 0;
+        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+            throw StatusError(PALE_ERR_DEVICE, "no HIP device available");
+        h->S = *settings;
+        if (h->S.gather_passes == 0 || h->S.adjoint_passes == 0 || h->S.forward_passes == 0)
+            throw StatusError(PALE_ERR_INVALID_ARG, "passes must be >= 1");
+        if (!(h->S.gather_radius > 0.0f)) throw StatusError(PALE_ERR_INVALID_ARG, "gather_radius must be > 0");
+        for (uint32_t i = 0; i < scene->mesh_count; ++i) {
+            const pale_mesh_desc& md = scene->meshes[i];
+            HostMesh m;
+            for (uint32_t v = 0; v < md.vertex_count; ++v) m.verts.push_back(ld3h(md.positions + 3 * v));
+            m.idx.assign(md.indices, md.indices + 3 * (size_t)md.triangle_count);
+            for (uint32_t x : m.idx)
+                if (x >= md.vertex_count) throw StatusError(PALE_ERR_INVALID_ARG, "mesh index out of range");
+            std::copy(md.object_to_world, md.object_to_world + 16, m.o2w);
+            std::copy(md.world_to_object, md.world_to_object + 16, m.w2o);
+            m.base = ld3h(md.base_color);
+            m.power = md.power;
+            m.name = std::string(md.name, strnlen(md.name, PALE_NAME_LEN));
+            h->meshes.push_back(std::move(m));
+        }
+        for (uint32_t i = 0; i < scene->camera_count; ++i) {
+            const pale_camera_desc& cd = scene->cameras[i];
+            HostCamera c;
+            c.name = std::string(cd.name, strnlen(cd.name, PALE_NAME_LEN));
+            c.W = cd.width;
+            c.H = cd.height;
+            c.fovy = cd.fovy_deg;
+            std::copy(cd.view, cd.view + 16, c.view);
+            std::copy(cd.inv_view, cd.inv_view + 16, c.invView);
+            c.pos = ld3h(cd.position);
+            c.fwd = ld3h(cd.forward);
+            c.adjoint = cd.use_for_adjoint != 0;
+            cameraFocal(c);
+            h->cams.push_back(c);
+        }
+        h->asset = toAsset(scene->points);
+        HIPCHECK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+        HIPCHECK(hipEventCreate(&h->ev0));
+        HIPCHECK(hipEventCreate(&h->ev1));
+        h->state.reserve(sizeof(DevState));
+        h->resetState();
+        for (const HostCamera& c : h->cams) {
+            DevMem f, l;
+            f.reserve((size_t)c.W * c.H * sizeof(float4));
+            l.reserve((size_t)c.W * c.H * sizeof(float4));
+            h->fb.push_back(std::move(f));
+            h->ldr.push_back(std::move(l));
+        }
+        h->rebuild();
+    });
+    if (rc != PALE_OK) {
+        delete h;
+        return rc;
+    }
+    *out = h;
+    return PALE_OK;
+}
+
+int pale_destroy(pale_handle* h) {
+    delete h;
+    return PALE_OK;
+}
+
+int pale_camera_count(const pale_handle* h, uint32_t* count) {
+    if (!h || !count) return fail(PALE_ERR_INVALID_ARG, "null argument");
+    *count = (uint32_t)h->cams.size();
+    return PALE_OK;
+}
+
+int pale_camera_info(const pale_handle* h, uint32_t index, char name_out[PALE_NAME_LEN], uint32_t* width,
+                     uint32_t* height, int32_t* use_for_adjoint) {
+    if (!h) return fail(PALE_ERR_INVALID_ARG, "null handle");
+    if (index >= h->cams.size()) return fail(PALE_ERR_RANGE, "camera index out of range");
+    const HostCamera& c = h->cams[index];
+    if (name_out) {
+        std::memset(name_out, 0, PALE_NAME_LEN);
+        std::memcpy(name_out, c.name.data(), std::min<size_t>(c.name.size(), PALE_NAME_LEN - 1));
+    }
+    if (width) *width = c.W;
+    if (height) *height = c.H;
+    if (use_for_adjoint) *use_for_adjoint = c.adjoint ? 1 : 0;
+    return PALE_OK;
+}
+
+int pale_render_forward(pale_handle* h, const char* camera_name, float* const* ldr_out, float* const* raw_out,
+                        uint32_t* rendered) {
+    if (!h) return fail(PALE_ERR_INVALID_ARG, "null handle");
+    return guarded([&] {
+        const std::vector<uint32_t> idx = selectCameras(h, camera_name);
+        h->renderForward(idx);
+        for (size_t k = 0; k < idx.size(); ++k) {
+            const HostCamera& c = h->cams[idx[k]];
+            const size_t bytes = (size_t)c.W * c.H * 4 * sizeof(float);
+            if (raw_out && raw_out[k]) HIPCHECK(hipMemcpy(raw_out[k], h->fb[idx[k]].p, bytes, hipMemcpyDeviceToHost));
+            if (ldr_out && ldr_out[k]) HIPCHECK(hipMemcpy(ldr_out[k], h->ldr[idx[k]].p, bytes, hipMemcpyDeviceToHost));
+        }
+        if (rendered) *rendered = (uint32_t)idx.size();
+    });
+}
+
+int pale_render_backward(pale_handle* h, uint32_t n, const char* const* camera_names, const float* const* dloss_rgb,
+                         float* const* grads, float* const* adjoint_source_out) {
+    if (!h) return fail(PALE_ERR_INVALID_ARG, "null handle");
+    if (n && (!camera_names || !dloss_rgb)) return fail(PALE_ERR_INVALID_ARG, "null camera names or images");
+    return guarded([&] {
+        // Adjoint cameras in camera order that appear in the input
+        // (pale_bindings.cpp:301-376).
+        std::vector<uint32_t> idx;
+        std::vector<const float*> src;
+        std::vector<uint32_t> which;
+        for (uint32_t c = 0; c < h->cams.size(); ++c) {
+            if (!h->cams[c].adjoint) continue;
+            for (uint32_t k = 0; k < n; ++k) {
+                if (camera_names[k] && h->cams[c].name == camera_names[k]) {
+                    if (!dloss_rgb[k]) throw StatusError(PALE_ERR_INVALID_ARG, "null image for " + h->cams[c].name);
+                    idx.push_back(c);
+                    src.push_back(dloss_rgb[k]);
+                    which.push_back(k);
+                    break;
+                }
+            }
+        }
+        h->renderBackward(idx, src);
+        const size_t N = h->bp.points.size();
+        const size_t widths[8] = {3, 3, 3, 2, 3, 1, 1, 1};
+        if (grads) {
+            for (int g = 0; g < 8; ++g) {
+                if (!grads[g]) continue;
+                if (g == 5) {
+                    if (N) HIPCHECK(hipMemcpy(grads[g], h->gradOpacity.p, N * sizeof(float), hipMemcpyDeviceToHost));
+                } else {
+                    std::memset(grads[g], 0, N * widths[g] * sizeof(float));
+                }
+            }
+        }
+        if (adjoint_source_out) {
+            // The adjoint-source image is the uploaded dL/dI with alpha 1
+            // (pale_bindings.cpp:352-392, :470-482), in input order.
+            for (size_t k = 0; k < idx.size(); ++k) {
+                float* dst = adjoint_source_out[which[k]];
+                if (!dst) continue;
+                const HostCamera& c = h->cams[idx[k]];
+                const size_t npx = (size_t)c.W * c.H;
+                for (size_t i = 0; i < npx; ++i) {
+                    dst[4 * i + 0] = src[k][3 * i + 0];
+                    dst[4 * i + 1] = src[k][3 * i + 1];
+                    dst[4 * i + 2] = src[k][3 * i + 2];
+                    dst[4 * i + 3] = 1.0f;
+                }
+            }
+        }
+    });
+}
+
+int pale_point_count(const pale_handle* h, uint32_t* count) {
+    if (!h || !count) return fail(PALE_ERR_INVALID_ARG, "null argument");
+    *count = (uint32_t)h->bp.points.size();
+    return PALE_OK;
+}
+
+int pale_get_points(const pale_handle* h, float* position, float* tangent_u, float* tangent_v, float* scale,
+                    float* albedo, float* opacity, float* beta, float* shape) {
+    if (!h) return fail(PALE_ERR_INVALID_ARG, "null handle");
+    const auto& P = h->bp.points;
+    for (size_t i = 0; i < P.size(); ++i) {
+        const HostPoint& p = P[i];
+        if (position) { position[3 * i] = p.p.x; position[3 * i + 1] = p.p.y; position[3 * i + 2] = p.p.z; }
+        if (tangent_u) { tangent_u[3 * i] = p.tu.x; tangent_u[3 * i + 1] = p.tu.y; tangent_u[3 * i + 2] = p.tu.z; }
+        if (tangent_v) { tangent_v[3 * i] = p.tv.x; tangent_v[3 * i + 1] = p.tv.y; tangent_v[3 * i + 2] = p.tv.z; }
+        if (scale) { scale[2 * i] = p.su; scale[2 * i + 1] = p.sv; }
+        if (albedo) { albedo[3 * i] = p.albedo.x; albedo[3 * i + 1] = p.albedo.y; albedo[3 * i + 2] = p.albedo.z; }
+        if (opacity) opacity[i] = p.opacity;
+        if (beta) beta[i] = p.beta;
+        if (shape) shape[i] = p.shape;
+    }
+    return PALE_OK;
+}
+
+int pale_apply_points(pale_handle* h, const pale_points_desc* points) {
+    if (!h || !points) return fail(PALE_ERR_INVALID_ARG, "null argument");
+    return guarded([&] {
+        if (points->count == 0) return;  // nothing to render; ignored as in the reference
+        if (points->count != h->bp.points.size())
+            throw StatusError(PALE_ERR_SIZE,
+                              "apply_point_optimization expects consistent point count; use densification API for "
+                              "adding/removing points.");
+        const std::vector<AssetPoint> in = toAsset(*points);
+        for (size_t i = 0; i < in.size(); ++i) {
+            HostPoint& q = h->bp.points[i];
+            q.p = in[i].p;
+            q.tu = in[i].tu;
+            q.tv = in[i].tv;
+            q.su = in[i].su;
+            q.sv = in[i].sv;
+            q.albedo = in[i].albedo;
+            q.opacity = in[i].opacity;
+            q.beta = in[i].beta;
+            // Mirror into the asset (shape is not mirrored, as in the
+            // reference, pale_bindings.cpp:1025-1040).
+            AssetPoint& a = h->asset[i];
+            a.p = q.p;
+            a.tu = q.tu;
+            a.tv = q.tv;
+            a.su = q.su;
+            a.sv = q.sv;
+            a.albedo = q.albedo;
+            a.opacity = q.opacity;
+            a.beta = q.beta;
+        }
+        refreshSurfels(h->bp.points, h->img);
+        upload(h->stream, h->geo, h->img.surfGeo);
+        upload(h->stream, h->shade, h->img.surfShade);
+        h->scene.geo = h->geo.as<SurfelGeo>();
+        h->scene.shade = h->shade.as<SurfelShade>();
+        HIPCHECK(hipStreamSynchronize(h->stream));
+        h->setScene();
+    });
+}
+
+int pale_add_points(pale_handle* h, const pale_points_desc* points) {
+    if (!h || !points) return fail(PALE_ERR_INVALID_ARG, "null argument");
+    return guarded([&] {
+        if (points->count == 0) return;
+        std::vector<AssetPoint> in = toAsset(*points);
+        for (AssetPoint& a : in) a.shape = 0.0f;  // new points get shape 0
+        h->asset.insert(h->asset.end(), in.begin(), in.end());
+        h->rebuild();
+    });
+}
+
+int pale_remove_points(pale_handle* h, const int64_t* indices, size_t n) {
+    if (!h) return fail(PALE_ERR_INVALID_ARG, "null handle");
+    if (n && !indices) return fail(PALE_ERR_INVALID_ARG, "null indices");
+    return guarded([&] {
+        if (n == 0 || h->asset.empty()) return;
+        std::vector<char> keep(h->asset.size(), 1);
+        for (size_t k = 0; k < n; ++k) {
+            if (indices[k] < 0) throw StatusError(PALE_ERR_RANGE, "remove_points: negative index is not allowed");
+            if ((uint64_t)indices[k] >= h->asset.size())
+                throw StatusError(PALE_ERR_RANGE, "remove_points: index out of range");
+            keep[(size_t)indices[k]] = 0;
+        }
+        std::vector<AssetPoint> kept;
+        for (size_t i = 0; i < h->asset.size(); ++i)
+            if (keep[i]) kept.push_back(h->asset[i]);
+        h->asset.swap(kept);
+        h->rebuild();
+    });
+}
+
+int pale_rebuild_bvh(pale_handle* h) {
+    if (!h) return fail(PALE_ERR_INVALID_ARG, "null handle");
+    return guarded([&] { h->rebuild(); });
+}
+
+int pale_set_point_properties(pale_handle* h, const float translation[3], const float rotation_xyzw[4],
+                              const float scale[3], const float albedo[3], float opacity, float beta, int32_t index) {
+    if (!h || !translation || !rotation_xyzw || !scale || !albedo) return fail(PALE_ERR_INVALID_ARG, "null argument");
+    return guarded([&] {
+        auto& A = h->asset;
+        const F3 t = ld3h(translation), c = ld3h(albedo);
+        if (index < 0) {
+            // The reference advances i twice per iteration here
+            // (pale_bindings.cpp:1544-1561): every second point is changed.
+            for (size_t i = 0; i < A.size(); ++i) {
+                A[i].p = A[i].p + t;
+                A[i].tu = rotate(rotation_xyzw, A[i].tu);
+                A[i].tv = rotate(rotation_xyzw, A[i].tv);
+                A[i].albedo = A[i].albedo + c;
+                A[i].opacity += opacity;
+                A[i].beta += beta;
+                A[i].su *= scale[0];
+                A[i].sv *= scale[1];
+                ++i;
+            }
+        } else {
+            if ((size_t)index > A.size() - 1 || A.empty())
+                throw StatusError(PALE_ERR_INVALID_ARG, "set_point_properties: index out of range");
+            AssetPoint& a = A[(size_t)index];
+            a.p = a.p + t;
+            a.tu = rotate(rotation_xyzw, a.tu);
+            a.tv = rotate(rotation_xyzw, a.tv);
+            a.albedo = c;
+            a.opacity = opacity;
+            a.beta = beta;
+            a.su = scale[0];
+            a.sv = scale[1];
+        }
+        h->rebuild();
+    });
+}
+
+int pale_set_point_opacity(pale_handle* h, float opacity, int32_t index) {
+    if (!h) return fail(PALE_ERR_INVALID_ARG, "null handle");
+    return guarded([&] {
+        if (index < 0 || (size_t)index >= h->asset.size())
+            throw StatusError(PALE_ERR_INVALID_ARG, "set_gaussian_opacity: index out of range");
+        h->asset[(size_t)index].opacity = opacity;
+        h->rebuild();
+    });
+}
+
+int pale_set_point_beta(pale_handle* h, float beta, int32_t index) {
+    if (!h) return fail(PALE_ERR_INVALID_ARG, "null handle");
+    return guarded([&] {
+        if (index < 0 || (size_t)index >= h->asset.size())
+            throw StatusError(PALE_ERR_INVALID_ARG, "set_point_beta: index out of range");
+        h->asset[(size_t)index].beta = beta;
+        h->rebuild();
+    });
+}
+
+int pale_trace_rays(pale_handle* h, uint32_t n, const float* origins, const float* directions, int32_t* instance_out,
+                    int32_t* primitive_out, float* t_out, float* position_out, float* alpha_out) {
+    if (!h) return fail(PALE_ERR_INVALID_ARG, "null handle");
+    if (n == 0) return PALE_OK;
+    if (!origins || !directions || !instance_out || !primitive_out || !t_out || !position_out || !alpha_out)
+        return fail(PALE_ERR_INVALID_ARG, "null argument");
+    return guarded([&] {
+        DevMem o, d, ii, pp, tt, ps, al;
+        o.reserve((size_t)n * 12);
+        d.reserve((size_t)n * 12);
+        ii.reserve((size_t)n * 4);
+        pp.reserve((size_t)n * 4);
+        tt.reserve((size_t)n * 4);
+        ps.reserve((size_t)n * 12);
+        al.reserve((size_t)n * 4);
+        HIPCHECK(hipMemcpy(o.p, origins, (size_t)n * 12, hipMemcpyHostToDevice));
+        HIPCHECK(hipMemcpy(d.p, directions, (size_t)n * 12, hipMemcpyHostToDevice));
+        launchTraceRays(h->stream, h->scene, n, o.as<float>(), d.as<float>(), ii.as<int32_t>(), pp.as<int32_t>(),
+                        tt.as<float>(), ps.as<float>(), al.as<float>());
+        HIPCHECK(hipStreamSynchronize(h->stream));
+        HIPCHECK(hipGetLastError());
+        HIPCHECK(hipMemcpy(instance_out, ii.p, (size_t)n * 4, hipMemcpyDeviceToHost));
+        HIPCHECK(hipMemcpy(primitive_out, pp.p, (size_t)n * 4, hipMemcpyDeviceToHost));
+        HIPCHECK(hipMemcpy(t_out, tt.p, (size_t)n * 4, hipMemcpyDeviceToHost));
+        HIPCHECK(hipMemcpy(position_out, ps.p, (size_t)n * 12, hipMemcpyDeviceToHost));
+        HIPCHECK(hipMemcpy(alpha_out, al.p, (size_t)n * 4, hipMemcpyDeviceToHost));
+    });
+}
+
+int pale_get_surfel_bvh(const pale_handle* h, uint32_t* node_count, float* bounds_out, uint32_t* left_first_out,
+                        uint32_t* count_out, uint32_t* permutation_out) {
+    if (!h || !node_count) return fail(PALE_ERR_INVALID_ARG, "null argument");
+    const Range r = h->bp.blasRanges.back();
+    *node_count = r.count;
+    for (uint32_t i = 0; i < r.count; ++i) {
+        const RefNode& n = h->bp.blas[r.first + i];
+        if (bounds_out) {
+            float* b = bounds_out + 6 * i;
+            b[0] = n.mn.x; b[1] = n.mn.y; b[2] = n.mn.z;
+            b[3] = n.mx.x; b[4] = n.mx.y; b[5] = n.mx.z;
+        }
+        if (left_first_out) left_first_out[i] = n.leftFirst;
+        if (count_out) count_out[i] = n.count;
+    }
+    if (permutation_out && !h->bp.perm.empty())
+        std::memcpy(permutation_out, h->bp.perm.data(), h->bp.perm.size() * sizeof(uint32_t));
+    return PALE_OK;
+}
+
+int pale_get_photons(const pale_handle* h, uint32_t* count, uint64_t* attempted, float* position, float* power,
+                     float* direction) {
+    if (!h || !count || !attempted) return fail(PALE_ERR_INVALID_ARG, "null argument");
+    return guarded([&] {
+        DevState s{};
+        HIPCHECK(hipMemcpy(&s, h->state.p, sizeof(DevState), hipMemcpyDeviceToHost));
+        const uint32_t n = h->mapValid ? std::min(s.photonCount, h->photonCap) : 0u;
+        *count = n;
+        *attempted = h->mapValid ? s.photonCount : 0u;
+        if (!n || !(position || power || direction)) return;
+        std::vector<float4> a(n), b(n);
+        std::vector<float> c(n);
+        HIPCHECK(hipMemcpy(a.data(), h->rawPosPow.p, n * sizeof(float4), hipMemcpyDeviceToHost));
+        HIPCHECK(hipMemcpy(b.data(), h->rawDirPow.p, n * sizeof(float4), hipMemcpyDeviceToHost));
+        HIPCHECK(hipMemcpy(c.data(), h->rawPowZ.p, n * sizeof(float), hipMemcpyDeviceToHost));
+        for (uint32_t i = 0; i < n; ++i) {
+            if (position) { position[3 * i] = a[i].x; position[3 * i + 1] = a[i].y; position[3 * i + 2] = a[i].z; }
+            if (power) { power[3 * i] = a[i].w; power[3 * i + 1] = b[i].w; power[3 * i + 2] = c[i]; }
+            if (direction) { direction[3 * i] = b[i].x; direction[3 * i + 1] = b[i].y; direction[3 * i + 2] = b[i].z; }
+        }
+    });
+}
+
+int pale_get_stats(const pale_handle* h, pale_stats* out) {
+    if (!h || !out) return fail(PALE_ERR_INVALID_ARG, "null argument");
+    *out = h->stats;
+    return PALE_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,433 @@
+// pale_scene.hip -- host scene build: surfel collection, median-split BVHs,
+// TLAS, light CDF, and conversion to the device layouts.
+//
+// The BVH topology and the point permutation must equal the reference's
+// (BVH.ixx:49-133 with std::nth_element at count/2 on the centroid of the
+// longest node axis, children allocated as an adjacent pair, depth-first
+// left-then-right), so this build uses the same libstdc++ selection
+// algorithm and recursion order.
+#include <algorithm>
+#include <cmath>
+#include <functional>
+#include <limits>
+#include <stdexcept>
+
+#include "pale_scene.h"
+
+namespace pale {
+namespace {
+
+const float kInf = std::numeric_limits<float>::infinity();
+const float kFltMax = std::numeric_limits<float>::max();
+
+struct Box {
+    F3 mn, mx;
+};
+
+inline float clampf(float v, float lo, float hi) { return std::min(std::max(v, lo), hi); }
+
+// surfelObjectAabbBeta (SceneBuild.cpp:395-444)
+Box surfelBox(const HostPoint& s) {
+    const F3 tu = normalize(s.tu);
+    const F3 tv = normalize(s.tv);
+    const F3 n = normalize(cross(tu, tv));
+    const float rU = 1.0001f * s.su;
+    const float rV = 1.0001f * s.sv;
+    F3 h;
+    float* hp = &h.x;
+    for (int a = 0; a < 3; ++a) {
+        const float cu = comp(tu, a), cv = comp(tv, a), cn = std::fabs(comp(n, a));
+        hp[a] = std::sqrt((rU * cu) * (rU * cu) + (rV * cv) * (rV * cv)) + 0.001f * cn;
+    }
+    return Box{s.p - h, s.p + h};
+}
+
+// Generic median-split builder (BVH.ixx:49-133 for boxes, :26-45/:141-220
+// for triangles).  `bounds(i)` is the box of primitive i, `key(i, axis)` its
+// centroid coordinate.
+struct MedianSplit {
+    std::vector<RefNode>& nodes;
+    std::vector<uint32_t>& order;
+    std::function<Box(uint32_t)> bounds;
+    std::function<float(uint32_t, int)> key;
+    uint32_t maxLeaf;
+
+    void fit(RefNode& nd) {
+        F3 bmin = mk3(kInf, kInf, kInf), bmax = mk3(-kInf, -kInf, -kInf);
+        for (uint32_t i = 0; i < nd.count; ++i) {
+            const Box b = bounds(order[nd.leftFirst + i]);
+            bmin = vmin(bmin, b.mn);
+            bmax = vmax(bmax, b.mx);
+        }
+        nd.mn = bmin;
+        nd.mx = bmax;
+    }
+
+    void subdivide(uint32_t ni) {
+        const RefNode nd = nodes[ni];
+        if (nd.count <= maxLeaf) return;
+        const F3 e = nd.mx - nd.mn;
+        const int axis = (e.y > e.x && e.y > e.z) ? 1 : (e.z > e.x && e.z > e.y) ? 2 : 0;
+        auto b = order.begin() + nd.leftFirst;
+        auto en = b + nd.count;
+        auto mid = b + (nd.count >> 1);
+        std::nth_element(b, mid, en, [&](uint32_t p, uint32_t q) { return key(p, axis) < key(q, axis); });
+        const uint32_t leftCount = (uint32_t)(mid - b);
+        if (leftCount == 0 || leftCount == nd.count) return;
+        const uint32_t L = (uint32_t)nodes.size(), R = L + 1;
+        nodes.push_back(RefNode{});
+        nodes.push_back(RefNode{});
+        nodes[L].leftFirst = nd.leftFirst;
+        nodes[L].count = leftCount;
+        nodes[R].leftFirst = nd.leftFirst + leftCount;
+        nodes[R].count = nd.count - leftCount;
+        nodes[ni].leftFirst = L;
+        nodes[ni].count = 0;
+        fit(nodes[L]);
+        fit(nodes[R]);
+        subdivide(L);
+        subdivide(R);
+    }
+
+    void build(uint32_t n) {
+        nodes.clear();
+        nodes.reserve(std::max<uint32_t>(1u, n * 2));
+        order.resize(n);
+        for (uint32_t i = 0; i < n; ++i) order[i] = i;
+        nodes.push_back(RefNode{});
+        nodes[0].leftFirst = 0;
+        nodes[0].count = n;
+        fit(nodes[0]);
+        subdivide(0);
+    }
+};
+
+uint32_t maxDepth(const std::vector<RefNode>& nodes, uint32_t first, uint32_t count) {
+    if (count == 0) return 0;
+    uint32_t best = 0;
+    std::vector<std::pair<uint32_t, uint32_t>> st{{0u, 0u}};
+    while (!st.empty()) {
+        auto [ni, d] = st.back();
+        st.pop_back();
+        best = std::max(best, d);
+        const RefNode& n = nodes[first + ni];
+        if (n.count == 0 && count > 1) {
+            st.push_back({n.leftFirst, d + 1});
+            st.push_back({n.leftFirst + 1, d + 1});
+        }
+    }
+    return best;
+}
+
+}  // namespace
+
+void cameraFocal(HostCamera& c) {
+    // makePrimaryRayFromPixelJitteredFov (KernelHelpers.h:764-796): the focal
+    // lengths only depend on the camera, so they are host constants.
+    const float W = (float)c.W, H = (float)c.H;
+    const float rad = c.fovy * 0.01745329251994329576923690768489f;  // glm::radians
+    c.fy = 0.5f * H / (float)std::tan((double)(0.5f * rad));
+    c.fx = c.fy * (W / H);
+}
+
+BuildProducts buildScene(const std::vector<HostMesh>& meshes, const std::vector<AssetPoint>& asset) {
+    BuildProducts bp;
+    // collectGeometry (SceneBuild.cpp:72-127)
+    const float oneThird = 1.0f / 3.0f;
+    std::vector<uint32_t> vertBase;
+    for (const HostMesh& m : meshes) {
+        const uint32_t vb = (uint32_t)bp.verts.size(), tb = (uint32_t)bp.tris.size();
+        for (const F3& v : m.verts) bp.verts.push_back(v);
+        for (size_t i = 0; i + 2 < m.idx.size(); i += 3) {
+            HostTri t{m.idx[i] + vb, m.idx[i + 1] + vb, m.idx[i + 2] + vb, {0, 0, 0}};
+            t.centroid = (bp.verts[t.v0] + bp.verts[t.v1] + bp.verts[t.v2]) * oneThird;
+            bp.tris.push_back(t);
+        }
+        bp.meshRanges.push_back(Range{tb, (uint32_t)(m.idx.size() / 3)});
+        vertBase.push_back(vb);
+    }
+    // collectInstances (SceneBuild.cpp:129-177): one instance per mesh.
+    for (uint32_t i = 0; i < meshes.size(); ++i) {
+        HostInstance in{};
+        in.type = kGeoMesh;
+        in.geom = i;
+        in.material = i;
+        std::copy(meshes[i].o2w, meshes[i].o2w + 16, in.o2w);
+        std::copy(meshes[i].w2o, meshes[i].w2o + 16, in.w2o);
+        bp.instances.push_back(in);
+    }
+    // collectPointCloudGeometry (SceneBuild.cpp:181-227)
+    std::vector<HostPoint> pts(asset.size());
+    for (size_t i = 0; i < asset.size(); ++i) {
+        const AssetPoint& a = asset[i];
+        HostPoint p{};
+        p.p = a.p;
+        p.tu = normalize(a.tu);
+        p.tv = normalize(a.tv);
+        p.su = a.su;
+        p.sv = a.sv;
+        p.albedo = mk3(clampf(a.albedo.x, 0, 1), clampf(a.albedo.y, 0, 1), clampf(a.albedo.z, 0, 1));
+        p.opacity = clampf(a.opacity, 0, 1);
+        p.alpha_r = 1.0f;
+        p.alpha_t = 0.0f;
+        p.beta = a.beta;
+        p.shape = clampf(a.shape, -5.0f, 5.0f);
+        pts[i] = p;
+    }
+    // The "Gaussian" point-cloud instance with identity transform
+    // (pale_bindings.cpp:90-101, SceneBuild.cpp:229-251).
+    {
+        HostInstance in{};
+        in.type = kGeoCloud;
+        in.geom = 0;
+        in.material = kInvalid;
+        for (int k = 0; k < 16; ++k) in.o2w[k] = in.w2o[k] = (k % 5 == 0) ? 1.0f : 0.0f;
+        bp.instances.push_back(in);
+    }
+    // collectLights (SceneBuild.cpp:263-343), before the BLAS build reorders
+    // triangles: emissive-triangle indices refer to the pre-build order.
+    for (uint32_t ii = 0; ii < bp.instances.size(); ++ii) {
+        const HostInstance& in = bp.instances[ii];
+        if (in.type == kGeoCloud) continue;
+        const HostMesh& m = meshes[in.geom];
+        if (!(m.power > 0.0f)) continue;
+        const Range mr = bp.meshRanges[in.geom];
+        if (mr.count == 0) continue;
+        const uint32_t off = (uint32_t)bp.emTris.size();
+        float total = 0.0f;
+        for (uint32_t t = 0; t < mr.count; ++t) {
+            const HostTri& T = bp.tris[mr.first + t];
+            const F3 p0 = toWorldPoint(in.o2w, bp.verts[T.v0]);
+            const F3 p1 = toWorldPoint(in.o2w, bp.verts[T.v1]);
+            const F3 p2 = toWorldPoint(in.o2w, bp.verts[T.v2]);
+            const float a = 0.5f * length(cross(p1 - p0, p2 - p0));
+            total += a;
+            bp.emTris.push_back(HostEmTri{mr.first + t, a, 0.0f});
+        }
+        if (total <= 0.0f) {
+            bp.emTris.resize(off);
+            continue;
+        }
+        float run = 0.0f;
+        for (uint32_t t = 0; t < mr.count; ++t) {
+            run += bp.emTris[off + t].area;
+            bp.emTris[off + t].cdf = run / total;
+        }
+        bp.emTris[off + mr.count - 1].cdf = 1.0f;
+        bp.lights.push_back(HostLight{ii, off, mr.count, m.base, m.power, total});
+    }
+    // buildBottomLevelBVHs (SceneBuild.cpp:642-772): meshes, then the cloud.
+    std::vector<uint32_t> meshToBlas(meshes.size());
+    for (uint32_t mi = 0; mi < meshes.size(); ++mi) {
+        const Range mr = bp.meshRanges[mi];
+        const uint32_t vb = vertBase[mi];
+        std::vector<HostTri> lt(bp.tris.begin() + mr.first, bp.tris.begin() + mr.first + mr.count);
+        std::vector<RefNode> nodes;
+        std::vector<uint32_t> order;
+        MedianSplit ms{nodes, order,
+                       [&](uint32_t i) {
+                           const HostTri& T = lt[i];
+                           Box b{mk3(kInf, kInf, kInf), mk3(-kInf, -kInf, -kInf)};
+                           const F3 v[3] = {bp.verts[T.v0], bp.verts[T.v1], bp.verts[T.v2]};
+                           for (const F3& p : v) {
+                               b.mn = vmin(b.mn, p);
+                               b.mx = vmax(b.mx, p);
+                           }
+                           return b;
+                       },
+                       [&](uint32_t i, int a) { return comp(lt[i].centroid, a); }, kMaxLeafPrims};
+        (void)vb;
+        ms.build(mr.count);
+        for (uint32_t k = 0; k < order.size(); ++k) bp.tris[mr.first + k] = lt[order[k]];
+        const uint32_t first = (uint32_t)bp.blas.size();
+        for (RefNode& nd : nodes)
+            if (nd.count > 0) nd.leftFirst += mr.first;
+        bp.blas.insert(bp.blas.end(), nodes.begin(), nodes.end());
+        meshToBlas[mi] = (uint32_t)bp.blasRanges.size();
+        bp.blasRanges.push_back(Range{first, (uint32_t)nodes.size()});
+    }
+    {
+        std::vector<Box> boxes(pts.size());
+        for (size_t i = 0; i < pts.size(); ++i) boxes[i] = surfelBox(pts[i]);
+        std::vector<RefNode> nodes;
+        std::vector<uint32_t> order;
+        MedianSplit ms{nodes, order, [&](uint32_t i) { return boxes[i]; },
+                       [&](uint32_t i, int a) { return comp(pts[i].p, a); }, kMaxLeafPrims};
+        ms.build((uint32_t)pts.size());
+        bp.points.resize(pts.size());
+        for (size_t k = 0; k < order.size(); ++k) bp.points[k] = pts[order[k]];
+        bp.perm = order;
+        const uint32_t first = (uint32_t)bp.blas.size();
+        bp.blas.insert(bp.blas.end(), nodes.begin(), nodes.end());
+        const uint32_t bi = (uint32_t)bp.blasRanges.size();
+        bp.blasRanges.push_back(Range{first, (uint32_t)nodes.size()});
+        for (HostInstance& in : bp.instances) in.blas = (in.type == kGeoMesh) ? meshToBlas[in.geom] : bi;
+    }
+    for (const Range& r : bp.blasRanges) bp.blasMaxDepth = std::max(bp.blasMaxDepth, maxDepth(bp.blas, r.first, r.count));
+    // buildTLAS (SceneBuild.cpp:555-640)
+    struct IBox {
+        F3 mn, mx;
+        uint32_t inst;
+    };
+    std::vector<IBox> boxes;
+    for (uint32_t i = 0; i < bp.instances.size(); ++i) {
+        const HostInstance& in = bp.instances[i];
+        const RefNode& root = bp.blas[bp.blasRanges[in.blas].first];
+        F3 wmin = mk3(kFltMax, kFltMax, kFltMax), wmax = mk3(-kFltMax, -kFltMax, -kFltMax);
+        for (int c = 0; c < 8; ++c) {
+            const bool bx = c & 4, by = c & 2, bz = c & 1;
+            const F3 po = mk3(bx ? root.mx.x : root.mn.x, by ? root.mx.y : root.mn.y, bz ? root.mx.z : root.mn.z);
+            const F3 pw = toWorldPoint(in.o2w, po);
+            wmin = vmin(wmin, pw);
+            wmax = vmax(wmax, pw);
+        }
+        boxes.push_back(IBox{wmin, wmax, i});
+    }
+    uint32_t tdepth = 0;
+    std::function<uint32_t(uint32_t, uint32_t, uint32_t)> build = [&](uint32_t start, uint32_t end,
+                                                                      uint32_t depth) -> uint32_t {
+        tdepth = std::max(tdepth, depth);
+        const uint32_t n = (uint32_t)bp.tlas.size();
+        bp.tlas.push_back(TlasNode{});
+        F3 bmin = mk3(kFltMax, kFltMax, kFltMax), bmax = mk3(-kFltMax, -kFltMax, -kFltMax);
+        for (uint32_t i = start; i < end; ++i) {
+            bmin = vmin(bmin, boxes[i].mn);
+            bmax = vmax(bmax, boxes[i].mx);
+        }
+        TlasNode& N0 = bp.tlas[n];
+        N0.mnx = bmin.x; N0.mny = bmin.y; N0.mnz = bmin.z;
+        N0.mxx = bmax.x; N0.mxy = bmax.y; N0.mxz = bmax.z;
+        const uint32_t count = end - start;
+        if (count == 1) {
+            bp.tlas[n].count = 1;
+            bp.tlas[n].left = boxes[start].inst;
+            bp.tlas[n].right = 0;
+        } else {
+            bp.tlas[n].count = 0;
+            F3 cmin = mk3(kFltMax, kFltMax, kFltMax), cmax = mk3(-kFltMax, -kFltMax, -kFltMax);
+            for (uint32_t i = start; i < end; ++i) {
+                const F3 c = (boxes[i].mn + boxes[i].mx) * 0.5f;
+                cmin = vmin(cmin, c);
+                cmax = vmax(cmax, c);
+            }
+            const F3 ext = cmax - cmin;
+            const int axis = (ext.x > ext.y && ext.x > ext.z) ? 0 : (ext.y > ext.z) ? 1 : 2;
+            const float pivot = (comp(cmin, axis) + comp(cmax, axis)) * 0.5f;
+            auto it = std::partition(boxes.begin() + start, boxes.begin() + end, [&](const IBox& b) {
+                const F3 c = (b.mn + b.mx) * 0.5f;
+                return comp(c, axis) < pivot;
+            });
+            uint32_t mid = (uint32_t)(it - boxes.begin());
+            if (mid == start || mid == end) mid = start + count / 2;
+            const uint32_t l = build(start, mid, depth + 1);
+            const uint32_t r = build(mid, end, depth + 1);
+            bp.tlas[n].left = l;
+            bp.tlas[n].right = r;
+        }
+        return n;
+    };
+    if (!boxes.empty()) build(0, (uint32_t)boxes.size(), 0);
+    bp.tlasMaxDepth = tdepth;
+    return bp;
+}
+
+void refreshSurfels(const std::vector<HostPoint>& pts, DeviceImage& img) {
+    img.surfGeo.resize(pts.size());
+    img.surfShade.resize(pts.size());
+    for (size_t i = 0; i < pts.size(); ++i) {
+        const HostPoint& s = pts[i];
+        // intersectSurfel's frame (KernelHelpers.h:646-648)
+        const F3 u = normalize(s.tu);
+        const F3 v = normalize(s.tv - u * dot(u, s.tv));
+        const F3 n = normalize(cross(u, v));
+        SurfelGeo g{};
+        g.px = s.p.x; g.py = s.p.y; g.pz = s.p.z; g.su = s.su;
+        g.nx = n.x; g.ny = n.y; g.nz = n.z; g.sv = s.sv;
+        g.tux = s.tu.x; g.tuy = s.tu.y; g.tuz = s.tu.z; g.e4b = 4.0f * exp_r(s.beta);
+        g.tvx = s.tv.x; g.tvy = s.tv.y; g.tvz = s.tv.z; g.pad = 0.0f;
+        img.surfGeo[i] = g;
+        const F3 ns = normalize(cross(s.tu, s.tv));
+        SurfelShade h{};
+        h.nsx = ns.x; h.nsy = ns.y; h.nsz = ns.z; h.opacity = s.opacity;
+        h.ar = s.albedo.x; h.ag = s.albedo.y; h.ab = s.albedo.z; h.alpha_r = s.alpha_r;
+        img.surfShade[i] = h;
+    }
+}
+
+DeviceImage makeDeviceImage(const BuildProducts& bp, const std::vector<HostMesh>& meshes) {
+    DeviceImage img;
+    if (bp.points.size() >= (1u << 29)) throw std::runtime_error("more than 2^29 surfels");
+    refreshSurfels(bp.points, img);
+    img.tris.resize(bp.tris.size());
+    for (size_t i = 0; i < bp.tris.size(); ++i) {
+        const HostTri& t = bp.tris[i];
+        const F3 a = bp.verts[t.v0], b = bp.verts[t.v1], c = bp.verts[t.v2];
+        img.tris[i] = TriGeo{a.x, a.y, a.z, 0.f, b.x, b.y, b.z, 0.f, c.x, c.y, c.z, 0.f};
+    }
+    // Wide nodes: every internal reference node becomes one WideNode holding
+    // both child boxes; leaves become references.
+    std::vector<uint32_t> compact(bp.blas.size(), kInvalid);
+    uint32_t next = 0;
+    for (const Range& r : bp.blasRanges) {
+        for (uint32_t i = 0; i < r.count; ++i) {
+            const RefNode& n = bp.blas[r.first + i];
+            const bool internal = n.count == 0 && r.count > 1;
+            if (internal) compact[r.first + i] = next++;
+        }
+    }
+    img.nodes.resize(next);
+    auto refOf = [&](const Range& r, uint32_t local) -> uint32_t {
+        const RefNode& n = bp.blas[r.first + local];
+        if (n.count > 0) return leafRef(n.leftFirst, n.count);
+        return compact[r.first + local];
+    };
+    for (const Range& r : bp.blasRanges) {
+        for (uint32_t i = 0; i < r.count; ++i) {
+            const uint32_t ci = compact[r.first + i];
+            if (ci == kInvalid) continue;
+            const RefNode& n = bp.blas[r.first + i];
+            const RefNode& L = bp.blas[r.first + n.leftFirst];
+            const RefNode& R = bp.blas[r.first + n.leftFirst + 1];
+            WideNode w{};
+            w.lminx = L.mn.x; w.lminy = L.mn.y; w.lminz = L.mn.z; w.lmaxx = L.mx.x;
+            w.lmaxy = L.mx.y; w.lmaxz = L.mx.z; w.rminx = R.mn.x; w.rminy = R.mn.y;
+            w.rminz = R.mn.z; w.rmaxx = R.mx.x; w.rmaxy = R.mx.y; w.rmaxz = R.mx.z;
+            w.left = refOf(r, n.leftFirst);
+            w.right = refOf(r, n.leftFirst + 1);
+            img.nodes[ci] = w;
+        }
+    }
+    for (const HostInstance& in : bp.instances) {
+        const Range& r = bp.blasRanges[in.blas];
+        const RefNode& root = bp.blas[r.first];
+        DevInstance d{};
+        d.type = in.type;
+        d.empty = root.count == 0 && r.count <= 1 ? 1u : 0u;
+        d.rootRef = d.empty ? kInvalid : refOf(r, 0);
+        d.material = in.material;
+        d.rmnx = root.mn.x; d.rmny = root.mn.y; d.rmnz = root.mn.z;
+        d.rmxx = root.mx.x; d.rmxy = root.mx.y; d.rmxz = root.mx.z;
+        std::copy(in.o2w, in.o2w + 16, d.o2w);
+        std::copy(in.w2o, in.w2o + 16, d.w2o);
+        img.instances.push_back(d);
+    }
+    img.tlas = bp.tlas;
+    for (const HostMesh& m : meshes) img.materials.push_back(DevMaterial{m.base.x, m.base.y, m.base.z, m.power});
+    for (const HostLight& l : bp.lights) {
+        DevLight d{};
+        d.instance = l.instance;
+        d.triOffset = l.triOffset;
+        d.triCount = l.triCount;
+        d.cr = l.color.x; d.cg = l.color.y; d.cb = l.color.z;
+        d.power = l.power;
+        d.area = l.area;
+        img.lights.push_back(d);
+    }
+    for (const HostEmTri& e : bp.emTris) img.emTris.push_back(DevEmTri{e.tri, e.cdf});
+    // Near-far depth-first traversal keeps at most one pending sibling per
+    // level: depth + 1 entries per BLAS, tlasDepth + 1 below it.
+    img.stackDepth = (bp.tlasMaxDepth + 1) + (bp.blasMaxDepth + 1);
+    return img;
+}
+
+}  // namespace pale

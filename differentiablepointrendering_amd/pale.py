@@ -1,0 +1,289 @@
+"""Drop-in for the reference's pybind11 module ``pale`` (Pale/pale_bindings.cpp).
+
+``Renderer`` has the reference's constructor and methods with the same
+argument names, return shapes and errors (RuntimeError for bad shape / dtype
+/ key / size, IndexError for out-of-range indices), backed by the HIP library
+through the C-ABI in include/pale_abi.h.  The reference drivers
+(python/training.py, render_hooks.py, render_adjoint.py, finite_difference/*)
+can use it by making it importable as ``pale``::
+
+    import sys, differentiablepointrendering_amd.pale as pale
+    sys.modules["pale"] = pale
+
+Compatibility extensions (supersets of the reference):
+  * render_backward accepts a bare ndarray (every adjoint camera of that
+    size) and (H,W,4) images (alpha ignored);
+  * the gradient dict also carries "color" (alias of "albedo");
+  * render_forward returns a dict whose ``np.asarray`` is the first camera's
+    LDR RGB image.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Dict, List, Optional, Tuple
+
+import numpy as np
+
+from . import _abi
+from ._lib import check, load
+
+_fp = C.POINTER(C.c_float)
+
+
+def _p(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data_as(_fp)
+
+
+class ForwardImages(dict):
+    """render_forward result: {name: (H,W,4) LDR, name + "_raw": (H,W,4) HDR}."""
+
+    def __array__(self, dtype=None, copy=None):
+        for k, v in self.items():
+            if not k.endswith("_raw"):
+                a = v[..., :3]
+                return a.astype(dtype) if dtype is not None else a
+        return np.zeros((0, 0, 3), np.float32)
+
+
+class Renderer:
+    """pale.Renderer (pale_bindings.cpp:59-1650) on the MI355X."""
+
+    def __init__(self, assetRootDir: str, sceneXml: str = "cbox_custom.xml", pointCloudFile: str = "initial.ply",
+                 settings: Optional[dict] = None):
+        from . import assets
+
+        # The reference changes the working directory to the asset root
+        # (pale_bindings.cpp:66); paths are resolved against it here instead.
+        meshes, cameras, points = assets.load_scene(assetRootDir, sceneXml, pointCloudFile)
+        self._init(meshes, cameras, points, settings or {})
+
+    @classmethod
+    def from_scene(cls, meshes, cameras, points: Dict[str, np.ndarray], settings: Optional[dict] = None):
+        """Renderer over an in-memory scene (meshes/cameras of ``_abi``)."""
+        self = cls.__new__(cls)
+        self._init(meshes, cameras, points, settings or {})
+        return self
+
+    @classmethod
+    def from_spec(cls, spec, settings: Optional[dict] = None):
+        s = dict(spec.settings)
+        s.update(settings or {})
+        return cls.from_scene(spec.meshes, spec.cameras, spec.points, s)
+
+    def _init(self, meshes, cameras, points, settings):
+        self._L = load()
+        self._settings_dict = dict(settings)
+        holder = _abi.SceneHolder(list(meshes), list(cameras), points)
+        self._settings = _abi.make_settings(settings)
+        h = C.c_void_p()
+        check(self._L.pale_create(C.byref(holder.desc), C.byref(self._settings), C.byref(h)))
+        self._h = h
+        self._cams = []
+        n = C.c_uint32()
+        check(self._L.pale_camera_count(self._h, C.byref(n)))
+        for i in range(n.value):
+            name = C.create_string_buffer(_abi.PALE_NAME_LEN)
+            w, hh, adj = C.c_uint32(), C.c_uint32(), C.c_int32()
+            check(self._L.pale_camera_info(self._h, i, name, C.byref(w), C.byref(hh), C.byref(adj)))
+            self._cams.append((name.value.decode(), w.value, hh.value, bool(adj.value)))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.pale_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ------------------------------------------------------------ cameras --
+    def get_camera_names(self) -> List[str]:
+        return [c[0] for c in self._cams]
+
+    # ------------------------------------------------------------ forward --
+    def render_forward(self, camera_name: str = "") -> ForwardImages:
+        cams = [c for c in self._cams if camera_name == "" or c[0] == camera_name]
+        ldr = [np.empty((c[2], c[1], 4), np.float32) for c in cams]
+        raw = [np.empty((c[2], c[1], 4), np.float32) for c in cams]
+        k = max(1, len(cams))
+        lp = (_fp * k)(*[_p(a) for a in ldr])
+        rp = (_fp * k)(*[_p(a) for a in raw])
+        n = C.c_uint32()
+        check(self._L.pale_render_forward(self._h, camera_name.encode(), lp, rp, C.byref(n)))
+        out = ForwardImages()
+        for c, a, b in zip(cams, ldr, raw):
+            out[c[0]] = a
+            out[c[0] + "_raw"] = b
+        return out
+
+    # ----------------------------------------------------------- backward --
+    def _adjoint_inputs(self, target) -> Tuple[List[str], List[np.ndarray]]:
+        adj = [c for c in self._cams if c[3]]
+        if isinstance(target, np.ndarray):
+            target = {c[0]: target for c in adj if target.ndim == 3 and target.shape[:2] == (c[2], c[1])}
+        names, arrs = [], []
+        for name, w, h, _ in adj:
+            if name not in target:
+                continue
+            a = np.asarray(target[name])
+            if a.ndim != 3 or a.shape[2] not in (3, 4):
+                raise RuntimeError(f"render_backward: target image for camera '{name}' must be HxWx3 float32")
+            if a.dtype.itemsize != 4 or a.dtype.kind != "f":
+                raise RuntimeError(f"render_backward: target image for camera '{name}' must have dtype float32")
+            if a.shape[0] != h or a.shape[1] != w:
+                raise RuntimeError(f"render_backward: resolution mismatch for camera '{name}': target image is "
+                                   f"{a.shape[1]}x{a.shape[0]}, but sensor is {w}x{h}")
+            names.append(name)
+            arrs.append(np.ascontiguousarray(a[..., :3], dtype=np.float32))
+        return names, arrs
+
+    def render_backward(self, targetRgb32f) -> Tuple[dict, dict]:
+        names, arrs = self._adjoint_inputs(targetRgb32f)
+        n = self.point_count
+        grads = {k: np.empty((n, w) if w else (n,), np.float32) for k, w in _abi.POINT_SHAPES.items()}
+        gp = (_fp * 8)(*[_p(grads[k]) for k in _abi.POINT_KEYS])
+        srcs = [np.empty((a.shape[0], a.shape[1], 4), np.float32) for a in arrs]
+        k = max(1, len(names))
+        cn = (C.c_char_p * k)(*[s.encode() for s in names])
+        ap = (_fp * k)(*[_p(a) for a in arrs])
+        sp = (_fp * k)(*[_p(a) for a in srcs])
+        check(self._L.pale_render_backward(self._h, len(names), cn, ap, gp, sp))
+        grads["color"] = grads["albedo"]
+        images = {"adjoint_source": {nm: s for nm, s in zip(names, srcs)}}
+        return grads, images
+
+    # ------------------------------------------------------------- points --
+    @property
+    def point_count(self) -> int:
+        n = C.c_uint32()
+        check(self._L.pale_point_count(self._h, C.byref(n)))
+        return n.value
+
+    def get_point_parameters(self) -> Dict[str, np.ndarray]:
+        n = self.point_count
+        out = {k: np.empty((n, w) if w else (n,), np.float32) for k, w in _abi.POINT_SHAPES.items()}
+        check(self._L.pale_get_points(self._h, *[_p(out[k]) for k in _abi.POINT_KEYS]))
+        return out
+
+    @staticmethod
+    def _points_from_dict(d: dict, n: Optional[int], ctx: str) -> _abi.PointsHolder:
+        for key in ("position", "tangent_u", "tangent_v", "scale", "albedo", "opacity", "beta"):
+            if key not in d:
+                raise RuntimeError(f"{ctx}: missing key '{key}'")
+        arrays = {}
+        for key in ("position", "tangent_u", "tangent_v", "scale", "albedo", "opacity", "beta"):
+            a = np.asarray(d[key])
+            w = _abi.POINT_SHAPES[key]
+            m = a.shape[0] if a.ndim >= 1 else -1
+            want = (m, w) if w else (m,)
+            if a.shape != want or (n is not None and m != n):
+                shape = f"(N,{w})" if w else "(N,)"
+                raise RuntimeError(f"Expected '{key}' to have shape {shape}")
+            if a.dtype.itemsize != 4 or a.dtype.kind != "f":
+                raise RuntimeError(f"Expected '{key}' to be float32")
+            arrays[key] = a
+        return _abi.PointsHolder(arrays)
+
+    def apply_point_optimization(self, parameters: dict) -> None:
+        if "position" not in parameters:
+            return
+        pos = np.asarray(parameters["position"])
+        if pos.ndim != 2 or pos.shape[1] != 3:
+            raise RuntimeError("Expected 'position' to have shape (N,3)")
+        if pos.shape[0] == 0:
+            return
+        if pos.shape[0] != self.point_count:
+            raise RuntimeError("apply_point_optimization expects consistent point count; use densification API "
+                               "for adding/removing points.")
+        holder = self._points_from_dict(parameters, pos.shape[0], "apply_point_optimization")
+        check(self._L.pale_apply_points(self._h, C.byref(holder.desc)))
+
+    def add_points(self, parameters: dict) -> None:
+        if "new" not in parameters:
+            return
+        new = parameters["new"]
+        pos = np.asarray(new.get("position", np.zeros((0, 3), np.float32)))
+        if pos.ndim != 2 or pos.shape[1] != 3:
+            raise RuntimeError("add_new_points: 'new.position' must have shape (N,3)")
+        if pos.shape[0] == 0:
+            return
+        holder = self._points_from_dict(new, pos.shape[0], "add_new_points")
+        check(self._L.pale_add_points(self._h, C.byref(holder.desc)))
+
+    def remove_points(self, parameters: dict) -> None:
+        if "indices" not in parameters:
+            raise RuntimeError("remove_points: expected key 'indices' (1D int32 / int64 array)")
+        idx = np.asarray(parameters["indices"])
+        if idx.ndim != 1:
+            raise RuntimeError("remove_points: 'indices' must be a 1D array")
+        if idx.size == 0:
+            return
+        if idx.dtype.kind not in "iu" or idx.dtype.itemsize not in (4, 8):
+            raise RuntimeError("remove_points: 'indices' must have dtype int32 or int64")
+        idx64 = np.ascontiguousarray(idx, dtype=np.int64)
+        check(self._L.pale_remove_points(self._h, idx64.ctypes.data_as(C.POINTER(C.c_int64)), idx64.size))
+
+    def rebuild_bvh(self) -> None:
+        check(self._L.pale_rebuild_bvh(self._h))
+
+    def set_point_properties(self, translation3, rotation_quat4, scale3, albedo3, opacity, beta, index=-1) -> None:
+        if len(translation3) != 3 or len(rotation_quat4) != 4 or len(scale3) != 3:
+            raise RuntimeError("Expected translation(3), rotation_quat(4), scale(3)")
+        t = np.asarray(translation3, np.float32)
+        q = np.asarray(rotation_quat4, np.float32)
+        s = np.asarray(scale3, np.float32)
+        c = np.asarray(albedo3, np.float32)
+        check(self._L.pale_set_point_properties(self._h, _p(t), _p(q), _p(s), _p(c), float(opacity), float(beta),
+                                                int(index)))
+
+    def set_point_opacity(self, opacity, index) -> None:
+        check(self._L.pale_set_point_opacity(self._h, float(opacity), int(index)))
+
+    def set_point_beta(self, beta, index) -> None:
+        check(self._L.pale_set_point_beta(self._h, float(beta), int(index)))
+
+    # -------------------------------------------------------- diagnostics --
+    def trace_rays(self, origins: np.ndarray, directions: np.ndarray) -> dict:
+        o = _abi.f32(origins).reshape(-1, 3)
+        d = _abi.f32(directions).reshape(-1, 3)
+        n = o.shape[0]
+        inst = np.empty(n, np.int32)
+        prim = np.empty(n, np.int32)
+        t = np.empty(n, np.float32)
+        pos = np.empty((n, 3), np.float32)
+        alpha = np.empty(n, np.float32)
+        check(self._L.pale_trace_rays(self._h, n, _p(o), _p(d), inst.ctypes.data_as(C.POINTER(C.c_int32)),
+                                      prim.ctypes.data_as(C.POINTER(C.c_int32)), _p(t), _p(pos), _p(alpha)))
+        return dict(instance=inst, primitive=prim, t=t, position=pos, alpha=alpha)
+
+    def surfel_bvh(self) -> dict:
+        cnt = C.c_uint32()
+        check(self._L.pale_get_surfel_bvh(self._h, C.byref(cnt), None, None, None, None))
+        n = cnt.value
+        bounds = np.empty((n, 6), np.float32)
+        lf = np.empty(n, np.uint32)
+        c = np.empty(n, np.uint32)
+        perm = np.empty(self.point_count, np.uint32)
+        u = C.POINTER(C.c_uint32)
+        check(self._L.pale_get_surfel_bvh(self._h, C.byref(cnt), _p(bounds), lf.ctypes.data_as(u),
+                                          c.ctypes.data_as(u), perm.ctypes.data_as(u)))
+        return dict(bounds=bounds, left_first=lf, count=c, permutation=perm)
+
+    def photons(self) -> dict:
+        cnt = C.c_uint32()
+        att = C.c_uint64()
+        check(self._L.pale_get_photons(self._h, C.byref(cnt), C.byref(att), None, None, None))
+        n = cnt.value
+        pos = np.empty((n, 3), np.float32)
+        pw = np.empty((n, 3), np.float32)
+        dr = np.empty((n, 3), np.float32)
+        check(self._L.pale_get_photons(self._h, C.byref(cnt), C.byref(att), _p(pos), _p(pw), _p(dr)))
+        return dict(count=n, attempted=att.value, position=pos, power=pw, direction=dr)
+
+    def stats(self) -> dict:
+        s = _abi.Stats()
+        check(self._L.pale_get_stats(self._h, C.byref(s)))
+        return s.as_dict()

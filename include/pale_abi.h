@@ -1,0 +1,247 @@
+/*
+ * pale_abi.h -- C ABI of the MI355X beta-surfel adjoint renderer.
+ *
+ * This is the drop-in boundary for the reference's pybind11 module `pale`
+ * (Pale/pale_bindings.cpp:1613-1650, class `Renderer`).  Every entry point
+ * below replaces one method of that class; the comment on each names the
+ * reference method it stands in for.  The Python shim
+ * differentiablepointrendering_amd/pale.py binds these with ctypes and
+ * restores the reference's Python call shapes (dicts of numpy arrays).
+ *
+ * Conventions
+ *  - Only plain C types cross the boundary.  All arrays are float32 / int
+ *    C-contiguous host buffers owned by the caller; the library never keeps a
+ *    caller pointer after a call returns.  All device memory belongs to the
+ *    handle.
+ *  - Every function returns an int status: PALE_OK (0) or a negative code.
+ *    pale_last_error() returns a thread-local message for the last failure.
+ *    The Python shim maps PALE_ERR_RANGE to IndexError and every other code to
+ *    RuntimeError, as pybind11 maps std::out_of_range / std::runtime_error in
+ *    the reference.
+ *  - Point arrays are in BVH order (the order the reference's
+ *    get_point_parameters returns, pale_bindings.cpp:734-850).
+ *  - Images are row-major (H, W, 4) float32 with row 0 at the top of the
+ *    image (ray v = H - (py + 0.5 + jy), KernelHelpers.h:776).
+ */
+#ifndef PALE_ABI_H
+#define PALE_ABI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PALE_NAME_LEN 16
+
+enum {
+    PALE_OK = 0,
+    PALE_ERR_INVALID_ARG = -1, /* bad pointer, shape, key or dtype      */
+    PALE_ERR_SIZE = -2,        /* size / resolution mismatch             */
+    PALE_ERR_RANGE = -3,       /* index out of range (IndexError)        */
+    PALE_ERR_DEVICE = -4,      /* HIP runtime or allocation failure      */
+    PALE_ERR_INTERNAL = -5     /* anything else                          */
+};
+
+/* One triangle-mesh instance (the reference's MeshComponent + Material +
+ * Transform, SceneBuild.cpp:72-177).  A mesh with power > 0 is an area
+ * emitter (GPUMaterial::isEmissive, GPUDataStructures.h:93-95); its emission
+ * colour is base_color (MaterialBakerMitsuba.ixx:42-66). */
+typedef struct pale_mesh_desc {
+    const float* positions;      /* [vertex_count*3] object-space vertices */
+    uint32_t vertex_count;
+    const uint32_t* indices;     /* [triangle_count*3]                     */
+    uint32_t triangle_count;
+    float object_to_world[16];   /* row-major 4x4                          */
+    float world_to_object[16];   /* row-major 4x4 (inverse of the above)   */
+    float base_color[3];
+    float power;
+    char name[PALE_NAME_LEN];
+} pale_mesh_desc;
+
+/* One pinhole camera (CameraGPU, GPUDataStructures.h:126-153). */
+typedef struct pale_camera_desc {
+    char name[PALE_NAME_LEN];
+    uint32_t width;
+    uint32_t height;
+    float fovy_deg;
+    float view[16];              /* world -> camera, row-major            */
+    float inv_view[16];          /* camera -> world, row-major            */
+    float position[3];
+    float forward[3];            /* unit world-space viewing direction    */
+    int32_t use_for_adjoint;     /* CameraGPU::useForAdjointPass          */
+} pale_camera_desc;
+
+/* The surfel cloud (PointGeometry of the reference's PLY asset,
+ * SceneBuild.cpp:181-227).  `shape` may be NULL (zeros). */
+typedef struct pale_points_desc {
+    uint32_t count;
+    const float* position;   /* [count*3] */
+    const float* tangent_u;  /* [count*3] */
+    const float* tangent_v;  /* [count*3] */
+    const float* scale;      /* [count*2] */
+    const float* albedo;     /* [count*3] */
+    const float* opacity;    /* [count]   */
+    const float* beta;       /* [count]   */
+    const float* shape;      /* [count] or NULL */
+} pale_points_desc;
+
+/* Whole scene: meshes become instances first (in order), the surfel cloud
+ * is the last instance with an identity transform, as the reference's
+ * "Gaussian" entity is (pale_bindings.cpp:90-101, SceneBuild.cpp:26-46). */
+typedef struct pale_scene_desc {
+    const pale_mesh_desc* meshes;
+    uint32_t mesh_count;
+    const pale_camera_desc* cameras;
+    uint32_t camera_count;
+    pale_points_desc points;
+} pale_scene_desc;
+
+/* Renderer settings: the reference's settings-dict keys
+ * (pale_bindings.cpp:123-139) with the PathTracerSettings defaults
+ * (GPUDataStructures.h:400-422), plus keys the reference hard-codes. */
+typedef struct pale_settings {
+    uint32_t photons;          /* "photons"          photonsPerLaunch (1e6)   */
+    uint32_t bounces;          /* "bounces"          maxBounces (6)           */
+    uint32_t forward_passes;   /* "forward_passes"   numForwardPasses (6)     */
+    uint32_t gather_passes;    /* "gather_passes"    numGatherPasses (6)      */
+    uint32_t adjoint_bounces;  /* "adjoint_bounces"  maxAdjointBounces (6)    */
+    uint32_t adjoint_passes;   /* "adjoint_passes"   adjointSamplesPerPixel(6)*/
+    uint64_t seed;             /* "seed"             Random::seed (42)        */
+    float gather_radius;       /* "gather_radius"    PathTracer.cpp:187 (0.01)*/
+    uint32_t rr_start;         /* "rr_start"         russianRouletteStart (6) */
+    float q_null;              /* AdjointSampleSettings (0.5)                */
+    float q_reflect;           /* (0.5)                                      */
+    float q_transmit;          /* (0.0)                                      */
+    int32_t spp_decorrelate;   /* "spp_decorrelate": 0 = every gather pass
+                                  reuses the camera seed, as the reference
+                                  does (PrimalKernels.cpp:516-520); 1 = pass
+                                  k uses seed + k (independent samples)     */
+    int32_t debug_images;      /* "debug_images"                             */
+} pale_settings;
+
+/* Work counters of the last render calls (no reference counterpart; the
+ * reference only logs wall-clock, ScopedTimer.h:15-56). */
+typedef struct pale_stats {
+    uint64_t photon_queries;    /* closest-hit queries of the photon pass    */
+    uint64_t gather_queries;    /* closest-hit queries of the camera gather  */
+    uint64_t adjoint_queries;   /* closest-hit queries of the adjoint pass   */
+    uint64_t photons_deposited; /* attempted photon deposits                 */
+    uint64_t gradient_events;   /* completed adjoint events                  */
+    uint64_t grid_cells;        /* cells of the photon grid                  */
+    double forward_ms;          /* device time of the last render_forward    */
+    double backward_ms;         /* device time of the last render_backward   */
+} pale_stats;
+
+typedef struct pale_handle pale_handle;
+
+/* Thread-local text of the last failure in this thread ("" if none). */
+const char* pale_last_error(void);
+
+/* Fill `out` with the reference defaults (GPUDataStructures.h:400-422,
+ * PathTracer.cpp:187). */
+void pale_default_settings(pale_settings* out);
+
+/* Renderer(assetRootDir, sceneXml, pointCloudFile, settings)
+ * (pale_bindings.cpp:61-170).  Scene/PLY parsing happens in the Python
+ * shim; this call receives the parsed scene. */
+int pale_create(const pale_scene_desc* scene, const pale_settings* settings,
+                pale_handle** out);
+
+/* ~PythonRenderer (pale_bindings.cpp:172-177). */
+int pale_destroy(pale_handle* h);
+
+/* get_camera_names (pale_bindings.cpp:1414-1420) -- count + per-camera info. */
+int pale_camera_count(const pale_handle* h, uint32_t* count);
+int pale_camera_info(const pale_handle* h, uint32_t index,
+                     char name_out[PALE_NAME_LEN], uint32_t* width,
+                     uint32_t* height, int32_t* use_for_adjoint);
+
+/* render_forward(camera_name="") (pale_bindings.cpp:180-277).
+ * camera_name NULL or "" renders every camera, else the cameras with that
+ * name.  For the k-th rendered camera (camera order) the LDR image goes to
+ * ldr_out[k] and the HDR ("_raw") image to raw_out[k], H*W*4 floats each;
+ * either array of pointers may be NULL.  *rendered = number of cameras. */
+int pale_render_forward(pale_handle* h, const char* camera_name,
+                        float* const* ldr_out, float* const* raw_out,
+                        uint32_t* rendered);
+
+/* render_backward(targetRgb32f) (pale_bindings.cpp:279-731).
+ * dloss_rgb[k] is the (H,W,3) dL/dI image of camera camera_names[k]; names
+ * that are not adjoint cameras are skipped, as the reference skips them.
+ * grads[0..7] receive position (N*3), tangent_u (N*3), tangent_v (N*3),
+ * scale (N*2), albedo (N*3), opacity (N), beta (N), shape (N); any may be
+ * NULL.  adjoint_source_out[k] (may be NULL) receives the (H,W,4) adjoint
+ * source image of camera k. */
+int pale_render_backward(pale_handle* h, uint32_t n,
+                         const char* const* camera_names,
+                         const float* const* dloss_rgb, float* const* grads,
+                         float* const* adjoint_source_out);
+
+/* get_point_parameters (pale_bindings.cpp:734-850). */
+int pale_point_count(const pale_handle* h, uint32_t* count);
+int pale_get_points(const pale_handle* h, float* position, float* tangent_u,
+                    float* tangent_v, float* scale, float* albedo,
+                    float* opacity, float* beta, float* shape);
+
+/* apply_point_optimization(parameters) (pale_bindings.cpp:853-1056):
+ * same N required; writes parameters in BVH order and uploads them without
+ * rebuilding the BVH. */
+int pale_apply_points(pale_handle* h, const pale_points_desc* points);
+
+/* add_points({"new": {...}}) (pale_bindings.cpp:1221-1412). */
+int pale_add_points(pale_handle* h, const pale_points_desc* points);
+
+/* remove_points({"indices": ...}) (pale_bindings.cpp:1081-1218). */
+int pale_remove_points(pale_handle* h, const int64_t* indices, size_t n);
+
+/* rebuild_bvh() (pale_bindings.cpp:1058-1079). */
+int pale_rebuild_bvh(pale_handle* h);
+
+/* set_point_properties(translation3, rotation_quat4 (x,y,z,w), scale3,
+ * albedo3, opacity, beta, index=-1) (pale_bindings.cpp:1486-1587). */
+int pale_set_point_properties(pale_handle* h, const float translation[3],
+                              const float rotation_xyzw[4],
+                              const float scale[3], const float albedo[3],
+                              float opacity, float beta, int32_t index);
+
+/* set_point_opacity / set_point_beta (pale_bindings.cpp:1422-1483). */
+int pale_set_point_opacity(pale_handle* h, float opacity, int32_t index);
+int pale_set_point_beta(pale_handle* h, float beta, int32_t index);
+
+/* ---- diagnostics: no reference counterpart; used by the parity tests ---- */
+
+/* Closest-hit query (the reference's intersectScene in FirstHit mode,
+ * IntersectionKernels.h:954-1097) for n world rays.  Outputs per ray:
+ * instance (-1 = miss), primitive (BVH-order surfel or triangle index),
+ * world t, world hit position (3 floats) and alpha_geom. */
+int pale_trace_rays(pale_handle* h, uint32_t n, const float* origins,
+                    const float* directions, int32_t* instance_out,
+                    int32_t* primitive_out, float* t_out, float* position_out,
+                    float* alpha_out);
+
+/* Surfel BLAS topology in the reference's BVHNode form
+ * (GPUDataStructures.h:53-63): per node aabb min (3), aabb max (3),
+ * leftFirst, triCount.  nodes_out may be NULL to query *node_count only.
+ * permutation_out (may be NULL) receives, per BVH slot, the index of the
+ * surfel in the order the cloud had before this build. */
+int pale_get_surfel_bvh(const pale_handle* h, uint32_t* node_count,
+                        float* bounds_out, uint32_t* left_first_out,
+                        uint32_t* count_out, uint32_t* permutation_out);
+
+/* Photon map of the last render_forward: *count stored photons (deposit
+ * order); positions/powers/directions (3 floats each) may be NULL.
+ * *attempted = the deposit counter T of the reference. */
+int pale_get_photons(const pale_handle* h, uint32_t* count,
+                     uint64_t* attempted, float* position, float* power,
+                     float* direction);
+
+/* Work counters and device times of the last render calls. */
+int pale_get_stats(const pale_handle* h, pale_stats* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PALE_ABI_H */

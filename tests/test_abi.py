@@ -1,0 +1,62 @@
+"""The C-ABI library loads, exports every symbol include/pale_abi.h declares,
+and the ctypes mirror matches the C layout (CPU only: no compute calls)."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import pytest
+
+from differentiablepointrendering_amd import _abi, _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "pale_abi.h")
+
+
+def declared_functions():
+    src = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?[a-z_]+\s*\*?\s*(pale_[a-z_]+)\s*\(", src, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib.load()
+    names = declared_functions()
+    assert len(names) >= 20
+    assert set(names) == set(_lib.EXPORTED)
+    for n in names:
+        assert hasattr(L, n), n
+
+
+def test_ctypes_layout_matches_c(tmp_path):
+    prog = tmp_path / "sizes.c"
+    prog.write_text(
+        '#include <stdio.h>\n#include <stddef.h>\n#include "pale_abi.h"\n'
+        "int main(void){printf(\"%zu %zu %zu %zu %zu %zu %zu\\n\", sizeof(pale_mesh_desc), sizeof(pale_camera_desc),"
+        " sizeof(pale_points_desc), sizeof(pale_scene_desc), sizeof(pale_settings), sizeof(pale_stats),"
+        " offsetof(pale_settings, gather_radius));return 0;}\n")
+    exe = tmp_path / "sizes"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(prog), "-o", str(exe)], check=True)
+    got = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    want = [C.sizeof(_abi.MeshDesc), C.sizeof(_abi.CameraDesc), C.sizeof(_abi.PointsDesc),
+            C.sizeof(_abi.SceneDesc), C.sizeof(_abi.Settings), C.sizeof(_abi.Stats),
+            _abi.Settings.gather_radius.offset]
+    assert got == want
+
+
+def test_default_settings_are_the_reference_defaults():
+    L = _lib.load()
+    s = _abi.Settings()
+    L.pale_default_settings(C.byref(s))
+    ref = _abi.make_settings({})
+    for name, _ in _abi.Settings._fields_:
+        assert getattr(s, name) == pytest.approx(getattr(ref, name)), name
+
+
+def test_error_path_without_device_or_with_bad_args():
+    L = _lib.load()
+    h = C.c_void_p()
+    rc = L.pale_create(None, None, C.byref(h))
+    assert rc == _abi.PALE_ERR_INVALID_ARG
+    assert L.pale_last_error()
+    with pytest.raises(RuntimeError):
+        _lib.check(rc)
