@@ -1,0 +1,240 @@
+#!/usr/bin/env python
+"""Benchmark of the beta-surfel adjoint renderer on one MI355X.
+
+One step = one forward pass (photon tracing, photon grid, camera gather,
+post-processing) plus one adjoint pass of the L2 loss against a target image
+of the same scene with jittered centers (the reference's training-step pair,
+python/training.py:621-662), on the plain single-view workload:
+
+  1,048,576 clustered surfels (64 Gaussian clusters, sigma 0.1), 800x800,
+  16 gather passes and 16 adjoint samples per pixel, depth 6, 2M photons,
+  gather radius 0.02, fp32.
+
+Metric: Mrays/s = closest-hit scene queries (photon bounces + camera-gather
+surfel layers + adjoint bounces) per second of device time of the step.  The
+step's inputs are resident in device memory when the timed region starts;
+the wall-clock rate, which includes the host<->device image copies and the
+loss on the host, is reported beside it (mrays_per_s_wall).
+
+Also reported: the roofline of the dominant kernel (the camera gather) from
+the reference's algorithmic byte model per query, and the CPU oracle timed
+on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Mrays/s fwd+adjoint at 1M surfels 800² 16spp vs plain; served p50/p99 ms"
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--surfels", type=int, default=None, help="override the surfel count (tests)")
+    ap.add_argument("--width", type=int, default=None)
+    ap.add_argument("--height", type=int, default=None)
+    ap.add_argument("--photons", type=int, default=None)
+    ap.add_argument("--spp", type=int, default=None)
+    ap.add_argument("--decorrelate", action="store_true", help="independent seed per gather pass")
+    ap.add_argument("--cpu-photon-stride", type=int, default=16)
+    ap.add_argument("--cpu-pixel-stride", type=int, default=256)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-count", action="store_true", help="skip the work-counting step (roofline)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only a device copy of 1 GiB (read) + 1 GiB (write): the reference kernel that scales "
+                         "rocprof FETCH_SIZE / WRITE_SIZE to bytes")
+    return ap.parse_args()
+
+
+def copy_calibration():
+    import torch
+
+    n = 1 << 28  # 2^28 fp32 = 1 GiB
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n}))
+    return 0
+
+
+# Algorithmic bytes of the reference layouts per work unit (per query: ray
+# in + hit/next ray out 48 + 48 B, 24 B per AABB test, 8 B per popped node,
+# 52 B per surfel test; per gather: 4 B per cell start, 4 B per non-empty
+# cell end, 40 B per photon scanned).
+def algorithmic_bytes(queries: int, w: dict) -> float:
+    return (96.0 * queries + 24.0 * w["aabb"] + 8.0 * w["pops"] + 52.0 * w["prims"] + 4.0 * w["cells"]
+            + 4.0 * w["cells_nonempty"] + 40.0 * w["photons"])
+
+
+def load_traffic(path):
+    if os.path.exists(path):
+        with open(path) as f:
+            return json.load(f)
+    return None
+
+
+def main():
+    a = parse()
+    if a.copy_calibration:
+        return copy_calibration()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {a.gpus}).",
+              file=sys.stderr)
+        return 2
+
+    from differentiablepointrendering_amd import scene
+    from differentiablepointrendering_amd.pale import Renderer
+
+    t0 = time.time()
+    spec = scene.config_scene(1, n=a.surfels, width=a.width, height=a.height)
+    settings = dict(spec.settings)
+    if a.photons:
+        settings["photons"] = a.photons
+    if a.spp:
+        settings["gather_passes"] = settings["adjoint_passes"] = a.spp
+    settings["spp_decorrelate"] = 1 if a.decorrelate else 0
+    cam = spec.cameras[0]
+    W, H = cam.width, cam.height
+    n = len(spec.points["position"])
+    print(f"[bench] scene {n} surfels {W}x{H} spp {settings['gather_passes']} photons {settings['photons']}",
+          file=sys.stderr, flush=True)
+
+    # Target image: the same scene with jittered centers (untimed).
+    tgt = Renderer.from_scene(spec.meshes, spec.cameras, spec.target_points, settings)
+    target = tgt.render_forward()[cam.name + "_raw"][..., :3].copy()
+    tgt.close()
+    del tgt
+    t_build = time.time()
+    r = Renderer.from_spec(spec, settings)
+    build_s = time.time() - t_build
+    print(f"[bench] renderer ready (host BVH build {build_s:.2f}s)", file=sys.stderr, flush=True)
+
+    def step():
+        img = r.render_forward()
+        raw = img[cam.name + "_raw"][..., :3]
+        dl = ((raw - target) / raw.size).astype(np.float32)
+        r.render_backward({cam.name: dl})
+        return r.stats()
+
+    for i in range(a.warmup):
+        step()
+        print(f"[bench] warmup {i + 1}/{a.warmup} done", file=sys.stderr, flush=True)
+
+    dev_ms, wall_ms, queries = [], [], []
+    gather_ms, gather_launches = 0.0, 0
+    tw0 = time.time()
+    for i in range(a.steps):
+        ts = time.time()
+        st = step()
+        wall_ms.append((time.time() - ts) * 1e3)
+        dev_ms.append(st["forward_ms"] + st["backward_ms"])
+        queries.append(st["photon_queries"] + st["gather_queries"] + st["adjoint_queries"])
+        gather_ms += st["gather_ms"]
+        gather_launches += st["gather_launches"]
+        print(f"[bench] step {i + 1}/{a.steps}: device {dev_ms[-1]:.1f} ms, wall {wall_ms[-1]:.1f} ms, "
+              f"{queries[-1] / 1e6:.1f} M queries", file=sys.stderr, flush=True)
+    total_wall = time.time() - tw0
+    last = st
+    ms_step = float(np.mean(dev_ms))
+    q_step = float(np.mean(queries))
+    value = q_step / (ms_step * 1e-3) / 1e6
+
+    roofline = None
+    work = None
+    if not a.no_count:
+        r.set_count_work(True)
+        cst = step()
+        r.set_count_work(False)
+        work = {k: cst[f"work_{k}"] for k in ("photon", "gather", "adjoint")}
+        same = cst["gather_queries"] == last["gather_queries"]
+        bytes_gather = algorithmic_bytes(cst["gather_queries"], work["gather"]) + 16.0 * W * H * settings[
+            "gather_passes"]
+        per_launch = bytes_gather / max(1, cst["gather_launches"])
+        avg_launch_s = gather_ms / max(1, gather_launches) * 1e-3
+        achieved = per_launch / avg_launch_s / 1e9
+        traffic = load_traffic(os.path.join(ROOT, "profiles", "k_gather_c1_traffic.json"))
+        roofline = {
+            "kernel": "k_gather (camera gather, one pass)",
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic.get("bytes_per_launch") if traffic else None,
+            "algorithmic_bytes_per_launch": per_launch,
+            "avg_launch_ms": avg_launch_s * 1e3,
+            "counted_step_matches_timed": bool(same),
+        }
+
+    cpu = None
+    if not a.no_cpu:
+        from oracle.oracle import Oracle
+
+        o = Oracle.from_spec(spec, settings)
+        o.set_sample(a.cpu_photon_stride, a.cpu_pixel_stride)
+        tc = time.time()
+        img = o.render_forward()
+        raw = img[cam.name + "_raw"][..., :3]
+        o.render_backward({cam.name: ((raw - target) / raw.size).astype(np.float32)})
+        cpu_s = time.time() - tc
+        cs = o.stats()
+        cq = cs["photon_queries"] + cs["gather_queries"] + cs["adjoint_queries"]
+        cpu = {"value": round(cq / cpu_s / 1e6, 4), "unit": "Mrays/s", "cores": 1, "kind": "port",
+               "sample": f"oracle (single thread) on the same scene: photons/{a.cpu_photon_stride}, every "
+                         f"{a.cpu_pixel_stride}th pixel, {settings['gather_passes']} spp, depth 6; "
+                         f"{cq} queries in {cpu_s:.1f}s"}
+        print(f"[bench] cpu baseline {cpu['value']} Mrays/s ({cpu_s:.1f}s)", file=sys.stderr, flush=True)
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (seeded generator of the benchmark configuration)",
+        "config": {
+            "workload": f"plain single view: {n} clustered surfels, {W}x{H}, {settings['gather_passes']} spp, "
+                        f"depth {settings['bounces']}, {settings['photons']} photons, radius "
+                        f"{settings['gather_radius']}, forward + opacity adjoint",
+            "surfels": n, "width": W, "height": H, "spp": settings["gather_passes"],
+            "photons": settings["photons"], "depth": settings["bounces"],
+            "gather_passes_share_seed": not a.decorrelate,
+        },
+        "queries_per_step": int(q_step),
+        "ms_per_step_wall": round(float(np.mean(wall_ms)), 3),
+        "mrays_per_s_wall": round(q_step / (np.mean(wall_ms) * 1e-3) / 1e6, 2),
+        "phase_ms": {k: round(last[k], 3) for k in ("photon_trace_ms", "grid_ms", "gather_ms", "adjoint_trace_ms",
+                                                    "adjoint_transport_ms", "forward_ms", "backward_ms")},
+        "host_bvh_build_s": round(build_s, 3),
+        "timed_wall_s": round(total_wall, 3),
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    if work is not None:
+        out["work_per_step"] = work
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -105,10 +105,27 @@ class Stats(C.Structure):
         ("grid_cells", C.c_uint64),
         ("forward_ms", C.c_double),
         ("backward_ms", C.c_double),
+        ("photon_trace_ms", C.c_double),
+        ("gather_ms", C.c_double),
+        ("adjoint_trace_ms", C.c_double),
+        ("adjoint_transport_ms", C.c_double),
+        ("grid_ms", C.c_double),
+        ("photon_trace_launches", C.c_uint32),
+        ("gather_launches", C.c_uint32),
+        ("adjoint_trace_launches", C.c_uint32),
+        ("adjoint_transport_launches", C.c_uint32),
+        ("work", (C.c_uint64 * 7) * 3),
     ]
 
     def as_dict(self) -> dict:
-        return {name: getattr(self, name) for name, _ in self._fields_}
+        d = {name: getattr(self, name) for name, _ in self._fields_ if name != "work"}
+        for ci, cname in enumerate(WORK_CLASSES):
+            d[f"work_{cname}"] = {u: int(self.work[ci][k]) for k, u in enumerate(WORK_UNITS)}
+        return d
+
+
+WORK_CLASSES = ("photon", "gather", "adjoint")
+WORK_UNITS = ("pops", "aabb", "prims", "gathers", "cells", "cells_nonempty", "photons")
 
 
 # Reference defaults: PathTracerSettings (GPUDataStructures.h:400-422),

@@ -55,6 +55,13 @@ struct Hit {
     uint32_t prim;
 };
 
+// Per-thread work counters of the counting mode (the per-query units of the
+// algorithmic byte count: popped nodes, AABB slab tests, primitive tests,
+// gather cells / non-empty cells / photons).
+struct Work {
+    uint32_t pops, aabb, prims, cells, cellsNE, photons, gathers;
+};
+
 __device__ __forceinline__ F3 ld3(const float4& v) { return mk3(v.x, v.y, v.z); }
 
 // safeInvDir (KernelHelpers.h:140-147)
@@ -103,12 +110,18 @@ struct LocalHit {
 // intersectBLASPointCloudFirstHit (IntersectionKernels.h:585-675) with
 // intersectSurfel (KernelHelpers.h:637-669); alpha_geom is evaluated once for
 // the final hit (it is a pure function of the hit).
+template <bool kCount>
 __device__ bool traverseCloud(const DevScene& S, const DevInstance& I, F3 o, F3 d, const LaneStack& st,
-                              uint32_t base, LocalHit& out) {
+                              uint32_t base, LocalHit& out, Work& w) {
     const F3 inv = safeInvDir(d);
     float best = __builtin_huge_valf();
     float te;
-    if (!slab(o, inv, I.rmnx, I.rmny, I.rmnz, I.rmxx, I.rmxy, I.rmxz, best, 1e-6f, te)) return false;
+    if (kCount) w.aabb++;  // the reference slab-tests the root when it pops it
+    if (!slab(o, inv, I.rmnx, I.rmny, I.rmnz, I.rmxx, I.rmxy, I.rmxz, best, 1e-6f, te)) {
+        if (kCount) w.pops++;
+        return false;
+    }
+    bool firstPop = true;
     uint32_t sp = base;
     st.push(sp, I.rootRef, te);
     uint32_t bestIdx = 0;
@@ -117,10 +130,16 @@ __device__ bool traverseCloud(const DevScene& S, const DevInstance& I, F3 o, F3 
     bool any = false;
     while (sp > base) {
         const uint2 e = st.get(--sp);
+        if (kCount) {
+            w.pops++;
+            if (!firstPop) w.aabb++;  // re-test of a popped box (root already counted)
+            firstPop = false;
+        }
         if (__uint_as_float(e.y) > best) continue;
         const uint32_t ref = e.x;
         if (ref & kLeafBit) {
             const uint32_t first = leafFirst(ref), cnt = leafCount(ref);
+            if (kCount) w.prims += cnt;
             for (uint32_t k = 0; k < cnt; ++k) {
                 const uint32_t si = first + k;
                 const float4* g = reinterpret_cast<const float4*>(S.geo + si);
@@ -149,6 +168,7 @@ __device__ bool traverseCloud(const DevScene& S, const DevInstance& I, F3 o, F3 
         const float4* nd4 = reinterpret_cast<const float4*>(S.nodes + ref);
         const float4 a = nd4[0], b = nd4[1], c = nd4[2];
         const uint4 refs = reinterpret_cast<const uint4*>(S.nodes + ref)[3];
+        if (kCount) w.aabb += 2;
         float tl = __builtin_huge_valf(), tr = __builtin_huge_valf();
         const bool hl = slab(o, inv, a.x, a.y, a.z, a.w, b.x, b.y, best, 1e-6f, tl);
         const bool hr = slab(o, inv, b.z, b.w, c.x, c.y, c.z, c.w, best, 1e-6f, tr);
@@ -194,22 +214,34 @@ __device__ __forceinline__ bool intersectTri(F3 o, F3 d, F3 v0, F3 v1, F3 v2, fl
 }
 
 // intersectBLASMesh (IntersectionKernels.h:37-105)
+template <bool kCount>
 __device__ bool traverseMesh(const DevScene& S, const DevInstance& I, F3 o, F3 d, const LaneStack& st,
-                             uint32_t base, LocalHit& out) {
+                             uint32_t base, LocalHit& out, Work& w) {
     const F3 inv = safeInvDir(d);
     float best = __builtin_huge_valf();
     float te;
-    if (!slab(o, inv, I.rmnx, I.rmny, I.rmnz, I.rmxx, I.rmxy, I.rmxz, best, 1e-6f, te)) return false;
+    if (kCount) w.aabb++;  // the reference slab-tests the root when it pops it
+    if (!slab(o, inv, I.rmnx, I.rmny, I.rmnz, I.rmxx, I.rmxy, I.rmxz, best, 1e-6f, te)) {
+        if (kCount) w.pops++;
+        return false;
+    }
+    bool firstPop = true;
     uint32_t sp = base;
     st.push(sp, I.rootRef, te);
     bool any = false;
     uint32_t bestTri = 0;
     while (sp > base) {
         const uint2 e = st.get(--sp);
+        if (kCount) {
+            w.pops++;
+            if (!firstPop) w.aabb++;  // re-test of a popped box (root already counted)
+            firstPop = false;
+        }
         if (__uint_as_float(e.y) > best) continue;
         const uint32_t ref = e.x;
         if (ref & kLeafBit) {
             const uint32_t first = leafFirst(ref), cnt = leafCount(ref);
+            if (kCount) w.prims += cnt;
             for (uint32_t k = 0; k < cnt; ++k) {
                 const TriGeo tg = S.tris[first + k];
                 float t;
@@ -226,6 +258,7 @@ __device__ bool traverseMesh(const DevScene& S, const DevInstance& I, F3 o, F3 d
         const float4* nd4 = reinterpret_cast<const float4*>(S.nodes + ref);
         const float4 a = nd4[0], b = nd4[1], c = nd4[2];
         const uint4 refs = reinterpret_cast<const uint4*>(S.nodes + ref)[3];
+        if (kCount) w.aabb += 2;
         float tl = __builtin_huge_valf(), tr = __builtin_huge_valf();
         const bool hl = slab(o, inv, a.x, a.y, a.z, a.w, b.x, b.y, best, 1e-6f, tl);
         const bool hr = slab(o, inv, b.z, b.w, c.x, c.y, c.z, c.w, best, 1e-6f, tr);
@@ -253,7 +286,8 @@ __device__ bool traverseMesh(const DevScene& S, const DevInstance& I, F3 o, F3 d
 
 // intersectScene, FirstHit mode (IntersectionKernels.h:954-1097).  The TLAS
 // stack shares the lane's LDS stack below the BLAS entries.
-__device__ bool intersectScene(const DevScene& S, F3 o, F3 d, const LaneStack& st, Hit& h) {
+template <bool kCount>
+__device__ bool intersectScene(const DevScene& S, F3 o, F3 d, const LaneStack& st, Hit& h, Work& w) {
     h.inst = kInvalid;
     if (S.nTlas == 0) return false;
     const F3 inv = safeInvDir(d);
@@ -264,10 +298,15 @@ __device__ bool intersectScene(const DevScene& S, F3 o, F3 d, const LaneStack& s
     while (sp > 0) {
         const uint32_t ni = st.get(--sp).x;
         const TlasNode nd = S.tlas[ni];
+        if (kCount) {
+            w.pops++;
+            w.aabb++;
+        }
         float te;
         if (!slab(o, inv, nd.mnx, nd.mny, nd.mnz, nd.mxx, nd.mxy, nd.mxz, bestW, 0.0f, te)) continue;
         if (nd.count == 0) {
             const TlasNode L = S.tlas[nd.left], R = S.tlas[nd.right];
+            if (kCount) w.aabb += 2;
             float tl = __builtin_huge_valf(), tr = __builtin_huge_valf();
             const bool hl = slab(o, inv, L.mnx, L.mny, L.mnz, L.mxx, L.mxy, L.mxz, bestW, 0.0f, tl);
             const bool hr = slab(o, inv, R.mnx, R.mny, R.mnz, R.mxx, R.mxy, R.mxz, bestW, 0.0f, tr);
@@ -296,8 +335,8 @@ __device__ bool intersectScene(const DevScene& S, F3 o, F3 d, const LaneStack& s
         mulv4(I.w2o, d.x, d.y, d.z, 0.0f, hv);
         const F3 od = normalize(mk3(hv[0], hv[1], hv[2]));
         LocalHit lh;
-        const bool acc = (I.type == kGeoMesh) ? traverseMesh(S, I, oo, od, st, sp, lh)
-                                              : traverseCloud(S, I, oo, od, st, sp, lh);
+        const bool acc = (I.type == kGeoMesh) ? traverseMesh<kCount>(S, I, oo, od, st, sp, lh, w)
+                                              : traverseCloud<kCount>(S, I, oo, od, st, sp, lh, w);
         if (acc) {
             const float tw = dot(lh.pos - o, d);
             if (tw > 0.0f && tw < bestW) {
@@ -424,7 +463,9 @@ __device__ __forceinline__ void worldToCell(const DevGrid& g, F3 p, int& x, int&
 // stored sorted by (cell, deposit index), so the cells of one grid row form
 // one contiguous range; summation order equals the reference's cell-by-cell,
 // photon-by-photon order.
-__device__ F3 gatherIrradiance(const DevGrid& g, F3 q, F3 n) {
+template <bool kCount>
+__device__ F3 gatherIrradiance(const DevGrid& g, F3 q, F3 n, Work& w) {
+    if (kCount) w.gathers++;
     if (!g.valid) return mk3(0, 0, 0);
     const float r = g.radius;
     const float r2 = r * r;
@@ -438,6 +479,11 @@ __device__ F3 gatherIrradiance(const DevGrid& g, F3 q, F3 n) {
             const uint32_t row = (uint32_t)((cz * g.ry + cy) * g.rx);
             const uint32_t a = g.cellStart[row + lx];
             const uint32_t b = g.cellStart[row + hx + 1];
+            if (kCount) {
+                w.cells += (uint32_t)(hx - lx + 1);
+                w.photons += b - a;
+                for (int cx = lx; cx <= hx; ++cx) w.cellsNE += g.cellStart[row + cx + 1] > g.cellStart[row + cx];
+            }
             for (uint32_t j = a; j < b; ++j) {
                 const float4 pp = g.posPow[j];
                 const float4 dp = g.dirPow[j];

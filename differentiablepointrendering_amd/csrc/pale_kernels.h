@@ -23,7 +23,13 @@ struct DevState {
     uint32_t depTotal;     // deposits of the last bounce
     uint32_t evTotal;      // gradient events of the last bounce
     unsigned long long photonQ, gatherQ, adjointQ, deposits, events, pad;
+    // Counting mode, per kernel class (photon, gather, adjoint): popped
+    // nodes, AABB tests, primitive tests, gathers, gather cells, non-empty
+    // cells, photons scanned.
+    unsigned long long work[3][8];
 };
+
+enum : int { kWorkPhoton = 0, kWorkGather = 1, kWorkAdjoint = 2 };
 
 // Ray queue, structure of arrays: origin (w = pixel / path id bits),
 // direction (w = bounce bits), throughput (w unused).
@@ -75,7 +81,8 @@ void launchSeedNext(hipStream_t s, DevState* st, const uint32_t* table, uint32_t
 void launchEmit(hipStream_t s, const DevScene& S, DevState* st, uint32_t photons, float invEmitted,
                 RayQueue staged, uint8_t* flags);
 void launchPhotonTrace(hipStream_t s, const DevScene& S, DevState* st, uint32_t maxRays, RayQueue in,
-                       RayQueue staged, uint8_t* extFlags, PhotonArrays dep, uint8_t* depFlags, TraceParams tp);
+                       RayQueue staged, uint8_t* extFlags, PhotonArrays dep, uint8_t* depFlags, TraceParams tp,
+                       bool count);
 // Stable compaction of flagged items i < *count (items >= *count are ignored).
 void launchCompactRays(hipStream_t s, const uint8_t* flags, const uint32_t* count, uint32_t maxItems,
                        uint32_t* tiles, uint32_t* total, RayQueue src, RayQueue dst, uint32_t dstCap);
@@ -91,15 +98,15 @@ void launchBuildGrid(hipStream_t s, const GridParams& g, DevState* st, uint32_t 
                      PhotonArrays sorted, uint32_t* cellId, uint32_t* sortedIdx, uint32_t* cellCount,
                      uint32_t* cellStart, uint32_t* tiles, uint32_t* scratchTotal);
 void launchGather(hipStream_t s, const DevScene& S, const DevGrid& G, const DevCamera& cam, DevState* st,
-                  uint32_t pass, int decorrelate, float invSpp, float4* fb);
+                  uint32_t pass, int decorrelate, float invSpp, float4* fb, bool count);
 void launchPostprocess(hipStream_t s, const float4* fb, float4* ldr, uint32_t n);
 void launchAdjRaygen(hipStream_t s, const DevCamera& cam, const float* dloss, RayQueue q, PendingArrays pend,
                      DevState* st);
 void launchAdjTrace(hipStream_t s, const DevScene& S, DevState* st, uint32_t maxRays, uint32_t bounce, RayQueue in,
                     RayQueue staged, uint8_t* extFlags, EventArrays ev, uint8_t* evFlags, PendingArrays pend,
-                    TraceParams tp);
+                    TraceParams tp, bool count);
 void launchAdjTransport(hipStream_t s, const DevScene& S, const DevGrid& G, DevState* st, uint32_t maxEvents,
-                        EventArrays ev, float invSpp, float* gradOpacity);
+                        EventArrays ev, float invSpp, float* gradOpacity, bool count);
 void launchTraceRays(hipStream_t s, const DevScene& S, uint32_t n, const float* o, const float* d, int32_t* inst,
                      int32_t* prim, float* t, float* pos, float* alpha);
 

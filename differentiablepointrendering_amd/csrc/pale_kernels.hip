@@ -33,6 +33,18 @@ __device__ __forceinline__ uint32_t waveSum(uint32_t v) {
     return v;
 }
 
+// Flush a thread's work counters: one atomic per counter per wave.  Every
+// lane of the wave must call this.
+__device__ __forceinline__ void flushWork(const Work& w, DevState* st, int cls) {
+    const uint32_t v[7] = {waveSum(w.pops), waveSum(w.aabb), waveSum(w.prims), waveSum(w.gathers),
+                           waveSum(w.cells), waveSum(w.cellsNE), waveSum(w.photons)};
+    if (__lane_id() == 0) {
+#pragma unroll
+        for (int k = 0; k < 7; ++k)
+            if (v[k]) atomicAdd(&st->work[cls][k], (unsigned long long)v[k]);
+    }
+}
+
 // Exclusive scan over a 256-thread block; `total` receives the block sum.
 __device__ __forceinline__ uint32_t blockExclusiveScan(uint32_t v, uint32_t& total) {
     __shared__ uint32_t waveTotals[kScanThreads / 64 + 1];
@@ -133,6 +145,7 @@ __global__ __launch_bounds__(256) void k_emit(DevScene S, const DevState* st, ui
 
 // ---------------------------------------------------- photon bounce --
 // launchIntersectKernel (PrimalKernels.cpp:65-321) for one queue.
+template <bool kCount>
 __global__ __launch_bounds__(kTraceBlock) void k_photon_trace(DevScene S, DevState* st, RayQueue in,
                                                               RayQueue out, uint8_t* extFlags, PhotonArrays dep,
                                                               uint8_t* depFlags, TraceParams tp) {
@@ -141,6 +154,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_photon_trace(DevScene S, DevSta
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t n = st->live ? st->qCount : 0u;
     const bool active = i < n;
+    Work w{};
     if (active) {
         uint8_t ef = 0, df = 0;
         Rng rng(perItemSeed(st->number, i));
@@ -148,7 +162,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_photon_trace(DevScene S, DevSta
         const F3 o = ld3(o4), d = ld3(d4), T = ld3(T4);
         const uint32_t bounce = f2u(d4.w);
         Hit h;
-        if (intersectScene(S, o, d, stk, h)) {
+        if (intersectScene<kCount>(S, o, d, stk, h, w)) {
             const DevInstance& I = S.inst[h.inst];
             F3 no = mk3(0, 0, 0), nT = mk3(0, 0, 0), ndir = d;
             bool spawn = false;
@@ -219,6 +233,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_photon_trace(DevScene S, DevSta
         depFlags[i] = df;
     }
     waveCount(active, &st->photonQ);
+    if (kCount) flushWork(w, st, kWorkPhoton);
 }
 
 // -------------------------------------------------- stable compaction --
@@ -420,6 +435,7 @@ __global__ __launch_bounds__(256) void k_cell_payload(const DevState* st, uint32
 
 // ---------------------------------------------------- camera gather --
 // launchCameraGatherKernel (PrimalKernels.cpp:500-653): one pass.
+template <bool kCount>
 __global__ __launch_bounds__(kTraceBlock) void k_gather(DevScene S, DevGrid G, DevCamera cam, DevState* st,
                                                         uint32_t pass, int decorrelate, float invSpp, float4* fb) {
     extern __shared__ uint2 s_stack[];
@@ -427,6 +443,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_gather(DevScene S, DevGrid G, D
     const uint32_t npx = cam.W * cam.H;
     const uint32_t pixel = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t layers = 0;
+    Work w{};
     if (pixel < npx) {
         const uint32_t seed = decorrelate ? st->number + pass : st->number;
         Rng rng(perItemSeed(seed, pixel));
@@ -439,13 +456,13 @@ __global__ __launch_bounds__(kTraceBlock) void k_gather(DevScene S, DevGrid G, D
         while (true) {
             Hit h;
             ++layers;
-            if (!intersectScene(S, o, d, stk, h)) break;
+            if (!intersectScene<kCount>(S, o, d, stk, h, w)) break;
             const DevInstance& I = S.inst[h.inst];
             if (I.type == kGeoCloud) {
                 const SurfelShade sh = S.shade[h.prim];
                 F3 n = mk3(sh.nsx, sh.nsy, sh.nsz);
                 if (dot(n, -d) < 0.0f) n = -n;
-                const F3 E = gatherIrradiance(G, h.pos, n);
+                const F3 E = gatherIrradiance<kCount>(G, h.pos, n, w);
                 const F3 Lo = E * ((sh.alpha_r * mk3(sh.ar, sh.ag, sh.ab)) * kInvPi);
                 const float ae = sh.opacity * h.alpha;
                 C = C + (tr * ae) * Lo;
@@ -458,7 +475,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_gather(DevScene S, DevGrid G, D
             if (m.power > 0.0f) {
                 C = C + tr * vmin(base * m.power, mk3(1.0f, 1.0f, 1.0f));
             } else {
-                const F3 E = gatherIrradiance(G, h.pos, hitNormal(S, h));
+                const F3 E = gatherIrradiance<kCount>(G, h.pos, hitNormal(S, h), w);
                 C = C + tr * ((base * kInvPi) * E);
             }
             break;
@@ -468,6 +485,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_gather(DevScene S, DevGrid G, D
     }
     const uint32_t ws = waveSum(layers);
     if (laneId() == 0 && ws) atomicAdd(&st->gatherQ, (unsigned long long)ws);
+    if (kCount) flushWork(w, st, kWorkGather);
 }
 
 // launchPostProcessKernel (PostprocessKernel.cpp:11-106), float LDR output.
@@ -509,6 +527,7 @@ __global__ __launch_bounds__(256) void k_adj_raygen(DevCamera cam, const float* 
 }
 
 // launchAdjointIntersectKernel (AdjointKernels.cpp:87-386) for one queue.
+template <bool kCount>
 __global__ __launch_bounds__(kTraceBlock) void k_adj_trace(DevScene S, DevState* st, uint32_t bounceIndex,
                                                            RayQueue in, RayQueue out, uint8_t* extFlags,
                                                            EventArrays ev, uint8_t* evFlags, PendingArrays pend,
@@ -518,6 +537,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_adj_trace(DevScene S, DevState*
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t n = st->qCount;
     const bool active = i < n;
+    Work w{};
     if (active) {
         uint8_t ef = 0, vf = 0;
         Rng rng(perItemSeed(42ull, i));
@@ -526,7 +546,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_adj_trace(DevScene S, DevState*
         const uint32_t path = f2u(o4.w);
         const uint32_t bounce = f2u(d4.w);
         Hit h;
-        if (intersectScene(S, o, d, stk, h)) {
+        if (intersectScene<kCount>(S, o, d, stk, h, w)) {
             const DevInstance& I = S.inst[h.inst];
             const bool cloud = I.type == kGeoCloud;
             F3 on, ng = mk3(0, 0, 0), dir;
@@ -646,21 +666,24 @@ __global__ __launch_bounds__(kTraceBlock) void k_adj_trace(DevScene S, DevState*
         evFlags[i] = vf;
     }
     waveCount(active, &st->adjointQ);
+    if (kCount) flushWork(w, st, kWorkAdjoint);
 }
 
 // launchAdjointTransportKernel (AdjointKernels.cpp:389-509).
+template <bool kCount>
 __global__ __launch_bounds__(256) void k_adj_transport(DevScene S, DevGrid G, DevState* st, uint32_t maxEvents,
                                                        EventArrays ev, float invSpp, float* grad) {
     const uint32_t n = min(st->evTotal, maxEvents);
     const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     const bool active = j < n;
+    Work w{};
     if (active) {
         const float4 e0 = ev.e0[j], e1 = ev.e1[j], e2 = ev.e2[j], e3 = ev.e3[j];
         const uint32_t kind = f2u(e0.w);
         const uint32_t target = f2u(e1.w);
         const F3 T = ld3(e2);
         const float a = e2.w;
-        const F3 E = gatherIrradiance(G, ld3(e0), ld3(e1));
+        const F3 E = gatherIrradiance<kCount>(G, ld3(e0), ld3(e1), w);
         float val = 0.0f;
         if (kind == kProjection) {
             const SurfelShade sh = S.shade[target];
@@ -696,6 +719,7 @@ __global__ __launch_bounds__(256) void k_adj_transport(DevScene S, DevGrid G, De
         atomicAdd(&grad[target], val);
     }
     waveCount(active, &st->events);
+    if (kCount) flushWork(w, st, kWorkAdjoint);
 }
 
 // ------------------------------------------------------ diagnostics --
@@ -707,8 +731,9 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_rays(DevScene S, uint32_t
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     Hit h;
-    const bool hit = intersectScene(S, mk3(o[3 * i], o[3 * i + 1], o[3 * i + 2]),
-                                    mk3(d[3 * i], d[3 * i + 1], d[3 * i + 2]), stk, h);
+    Work w{};
+    const bool hit = intersectScene<false>(S, mk3(o[3 * i], o[3 * i + 1], o[3 * i + 2]),
+                                           mk3(d[3 * i], d[3 * i + 1], d[3 * i + 2]), stk, h, w);
     inst[i] = hit ? (int32_t)h.inst : -1;
     prim[i] = hit ? (int32_t)h.prim : -1;
     t[i] = hit ? h.t : 0.0f;
@@ -736,10 +761,14 @@ void launchEmit(hipStream_t s, const DevScene& S, DevState* st, uint32_t P, floa
     if (P) k_emit<<<blocks(P, 256), 256, 0, s>>>(S, st, P, invEmitted, staged, flags);
 }
 void launchPhotonTrace(hipStream_t s, const DevScene& S, DevState* st, uint32_t maxRays, RayQueue in,
-                       RayQueue staged, uint8_t* extFlags, PhotonArrays dep, uint8_t* depFlags, TraceParams tp) {
-    if (maxRays)
-        k_photon_trace<<<blocks(maxRays, kTraceBlock), kTraceBlock, stackBytes(S), s>>>(S, st, in, staged, extFlags,
-                                                                                      dep, depFlags, tp);
+                       RayQueue staged, uint8_t* extFlags, PhotonArrays dep, uint8_t* depFlags, TraceParams tp,
+                       bool count) {
+    if (!maxRays) return;
+    const dim3 g(blocks(maxRays, kTraceBlock));
+    if (count)
+        k_photon_trace<true><<<g, kTraceBlock, stackBytes(S), s>>>(S, st, in, staged, extFlags, dep, depFlags, tp);
+    else
+        k_photon_trace<false><<<g, kTraceBlock, stackBytes(S), s>>>(S, st, in, staged, extFlags, dep, depFlags, tp);
 }
 void launchCompactRays(hipStream_t s, const uint8_t* flags, const uint32_t* count, uint32_t maxItems,
                        uint32_t* tiles, uint32_t* total, RayQueue src, RayQueue dst, uint32_t dstCap) {
@@ -783,11 +812,14 @@ void launchBuildGrid(hipStream_t s, const GridParams& g, DevState* st, uint32_t 
 }
 
 void launchGather(hipStream_t s, const DevScene& S, const DevGrid& G, const DevCamera& cam, DevState* st,
-                  uint32_t pass, int decorrelate, float invSpp, float4* fb) {
+                  uint32_t pass, int decorrelate, float invSpp, float4* fb, bool count) {
     const uint32_t npx = cam.W * cam.H;
-    if (npx)
-        k_gather<<<blocks(npx, kTraceBlock), kTraceBlock, stackBytes(S), s>>>(S, G, cam, st, pass, decorrelate,
-                                                                             invSpp, fb);
+    if (!npx) return;
+    const dim3 g(blocks(npx, kTraceBlock));
+    if (count)
+        k_gather<true><<<g, kTraceBlock, stackBytes(S), s>>>(S, G, cam, st, pass, decorrelate, invSpp, fb);
+    else
+        k_gather<false><<<g, kTraceBlock, stackBytes(S), s>>>(S, G, cam, st, pass, decorrelate, invSpp, fb);
 }
 void launchPostprocess(hipStream_t s, const float4* fb, float4* ldr, uint32_t n) {
     if (n) k_post<<<blocks(n, 256), 256, 0, s>>>(fb, ldr, n);
@@ -799,15 +831,23 @@ void launchAdjRaygen(hipStream_t s, const DevCamera& cam, const float* dloss, Ra
 }
 void launchAdjTrace(hipStream_t s, const DevScene& S, DevState* st, uint32_t maxRays, uint32_t bounce, RayQueue in,
                     RayQueue staged, uint8_t* extFlags, EventArrays ev, uint8_t* evFlags, PendingArrays pend,
-                    TraceParams tp) {
-    if (maxRays)
-        k_adj_trace<<<blocks(maxRays, kTraceBlock), kTraceBlock, stackBytes(S), s>>>(
-            S, st, bounce, in, staged, extFlags, ev, evFlags, pend, maxRays, tp);
+                    TraceParams tp, bool count) {
+    if (!maxRays) return;
+    const dim3 g(blocks(maxRays, kTraceBlock));
+    if (count)
+        k_adj_trace<true><<<g, kTraceBlock, stackBytes(S), s>>>(S, st, bounce, in, staged, extFlags, ev, evFlags,
+                                                                pend, maxRays, tp);
+    else
+        k_adj_trace<false><<<g, kTraceBlock, stackBytes(S), s>>>(S, st, bounce, in, staged, extFlags, ev, evFlags,
+                                                                 pend, maxRays, tp);
 }
 void launchAdjTransport(hipStream_t s, const DevScene& S, const DevGrid& G, DevState* st, uint32_t maxEvents,
-                        EventArrays ev, float invSpp, float* gradOpacity) {
-    if (maxEvents)
-        k_adj_transport<<<blocks(maxEvents, 256), 256, 0, s>>>(S, G, st, maxEvents, ev, invSpp, gradOpacity);
+                        EventArrays ev, float invSpp, float* gradOpacity, bool count) {
+    if (!maxEvents) return;
+    if (count)
+        k_adj_transport<true><<<blocks(maxEvents, 256), 256, 0, s>>>(S, G, st, maxEvents, ev, invSpp, gradOpacity);
+    else
+        k_adj_transport<false><<<blocks(maxEvents, 256), 256, 0, s>>>(S, G, st, maxEvents, ev, invSpp, gradOpacity);
 }
 void launchTraceRays(hipStream_t s, const DevScene& S, uint32_t n, const float* o, const float* d, int32_t* inst,
                      int32_t* prim, float* t, float* pos, float* alpha) {

@@ -112,9 +112,53 @@ struct pale_handle {
     // Per-camera framebuffers.
     std::vector<DevMem> fb, ldr;
     pale_stats stats{};
+    bool countWork = false;
+
+    // Per-kernel-class device timing: event pairs recorded around launches on
+    // the handle's stream, read after the call's final synchronisation.
+    enum TimedClass { kTPhoton = 0, kTGather, kTAdjTrace, kTTransport, kTGrid, kTClasses };
+    std::vector<hipEvent_t> evPool;
+    std::vector<int> evClass;
+    size_t evUsed = 0;
+    double classMs[kTClasses] = {};
+    uint32_t classLaunches[kTClasses] = {};
+
+    size_t tBegin(int cls) {
+        if (evUsed + 2 > evPool.size()) {
+            for (int k = 0; k < 64; ++k) {
+                hipEvent_t e;
+                HIPCHECK(hipEventCreate(&e));
+                evPool.push_back(e);
+            }
+        }
+        const size_t i = evUsed;
+        evUsed += 2;
+        if (evClass.size() < evUsed / 2) evClass.resize(evUsed / 2);
+        evClass[i / 2] = cls;
+        HIPCHECK(hipEventRecord(evPool[i], stream));
+        return i;
+    }
+    void tEnd(size_t i) { HIPCHECK(hipEventRecord(evPool[i + 1], stream)); }
+    void tReset() {
+        evUsed = 0;
+        for (int k = 0; k < kTClasses; ++k) {
+            classMs[k] = 0.0;
+            classLaunches[k] = 0;
+        }
+    }
+    void tCollect() {
+        for (size_t i = 0; i < evUsed; i += 2) {
+            float ms = 0.0f;
+            HIPCHECK(hipEventElapsedTime(&ms, evPool[i], evPool[i + 1]));
+            classMs[evClass[i / 2]] += ms;
+            classLaunches[evClass[i / 2]] += 1;
+        }
+        evUsed = 0;
+    }
 
     ~pale_handle() {
         if (stream) (void)hipStreamSynchronize(stream);
+        for (hipEvent_t e : evPool) (void)hipEventDestroy(e);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
         if (stream) (void)hipStreamDestroy(stream);
@@ -279,7 +323,6 @@ struct pale_handle {
             evCmp[k].reserve((size_t)npx * sizeof(float4));
         }
         evFlags.reserve(npx);
-        dloss.reserve((size_t)npx * 3 * sizeof(float));
         adjCap = npx;
     }
 
@@ -290,14 +333,26 @@ struct pale_handle {
     void readStats(bool forward) {
         DevState h{};
         HIPCHECK(hipMemcpy(&h, state.p, sizeof(DevState), hipMemcpyDeviceToHost));
+        tCollect();
         if (forward) {
             stats.photon_queries = h.photonQ;
             stats.gather_queries = h.gatherQ;
             stats.photons_deposited = h.photonCount;
+            stats.photon_trace_ms = classMs[kTPhoton];
+            stats.gather_ms = classMs[kTGather];
+            stats.grid_ms = classMs[kTGrid];
+            stats.photon_trace_launches = classLaunches[kTPhoton];
+            stats.gather_launches = classLaunches[kTGather];
         } else {
             stats.adjoint_queries = h.adjointQ;
             stats.gradient_events = h.events;
+            stats.adjoint_trace_ms = classMs[kTAdjTrace];
+            stats.adjoint_transport_ms = classMs[kTTransport];
+            stats.adjoint_trace_launches = classLaunches[kTAdjTrace];
+            stats.adjoint_transport_launches = classLaunches[kTTransport];
         }
+        for (int c = 0; c < 3; ++c)
+            for (int u = 0; u < 7; ++u) stats.work[c][u] = h.work[c][u];
         stats.grid_cells = grid.total;
     }
 
@@ -313,6 +368,7 @@ struct pale_handle {
         upload(stream, seedTable, table);
         HIPCHECK(hipStreamSynchronize(stream));
         resetState();
+        tReset();
         HIPCHECK(hipEventRecord(ev0, stream));
         DevState* st = this->st();
         const uint32_t emitted = S.photons * S.forward_passes;
@@ -334,8 +390,10 @@ struct pale_handle {
             launchSetQueue(stream, st, extTotal);
             for (uint32_t b = 0; b < S.bounces; ++b) {
                 launchSeedNext(stream, st, seedTable.as<uint32_t>(), tlen, (int)b);
+                const size_t te = tBegin(kTPhoton);
                 launchPhotonTrace(stream, scene, st, P, rq(in), rq(stg), extFlags.as<uint8_t>(),
-                                  pa(depStg[0], depStg[1], depStg[2]), depFlags.as<uint8_t>(), tp());
+                                  pa(depStg[0], depStg[1], depStg[2]), depFlags.as<uint8_t>(), tp(), countWork);
+                tEnd(te);
                 launchCompactRays(stream, extFlags.as<uint8_t>(), qCount, P, tiles.as<uint32_t>(), extTotal, rq(stg),
                                   rq(out), queueCap);
                 launchCompactPhotons(stream, depFlags.as<uint8_t>(), qCount, P, tiles.as<uint32_t>(), depTotal,
@@ -346,10 +404,12 @@ struct pale_handle {
             }
         }
         // buildPhotonCellRangesAndOrdering (PrimalKernels.cpp:1007-1018)
+        const size_t tg = tBegin(kTGrid);
         launchBuildGrid(stream, grid, st, photonCap, pa(rawPosPow, rawDirPow, rawPowZ),
                         pa(srtPosPow, srtDirPow, srtPowZ), cellId.as<uint32_t>(), sortedIdx.as<uint32_t>(),
                         cellCount.as<uint32_t>(), cellStart.as<uint32_t>(), gridTiles.as<uint32_t>(),
                         gridTotal.as<uint32_t>());
+        tEnd(tg);
         mapValid = true;
         const DevGrid G = devGrid();
         const float invSpp = 1.0f / (float)S.gather_passes;
@@ -359,8 +419,12 @@ struct pale_handle {
             launchSeedNext(stream, st, seedTable.as<uint32_t>(), tlen, -1);
             launchFillF(stream, fb[camIdx[k]].as<float>(), npx * 4, 0.0f);
             const DevCamera dc = devCam(c);
-            for (uint32_t pass = 0; pass < S.gather_passes; ++pass)
-                launchGather(stream, scene, G, dc, st, pass, S.spp_decorrelate, invSpp, fb[camIdx[k]].as<float4>());
+            for (uint32_t pass = 0; pass < S.gather_passes; ++pass) {
+                const size_t tk = tBegin(kTGather);
+                launchGather(stream, scene, G, dc, st, pass, S.spp_decorrelate, invSpp, fb[camIdx[k]].as<float4>(),
+                             countWork);
+                tEnd(tk);
+            }
             launchPostprocess(stream, fb[camIdx[k]].as<float4>(), ldr[camIdx[k]].as<float4>(), npx);
         }
         HIPCHECK(hipEventRecord(ev1, stream));
@@ -389,7 +453,16 @@ struct pale_handle {
         h.qCount = 0;
         HIPCHECK(hipMemcpyAsync(state.p, &h, sizeof(DevState), hipMemcpyHostToDevice, stream));
         HIPCHECK(hipStreamSynchronize(stream));
+        // dL/dI of every camera goes to the device before the timed region.
+        std::vector<size_t> off(camIdx.size() + 1, 0);
+        for (size_t k = 0; k < camIdx.size(); ++k) off[k + 1] = off[k] + (size_t)cams[camIdx[k]].W * cams[camIdx[k]].H * 3;
+        dloss.reserve(std::max<size_t>(off.back(), 1) * sizeof(float));
+        for (size_t k = 0; k < camIdx.size(); ++k)
+            HIPCHECK(hipMemcpyAsync(dloss.as<float>() + off[k], src[k], (off[k + 1] - off[k]) * sizeof(float),
+                                    hipMemcpyHostToDevice, stream));
         launchFillF(stream, gradOpacity.as<float>(), N, 0.0f);
+        HIPCHECK(hipStreamSynchronize(stream));
+        tReset();
         HIPCHECK(hipEventRecord(ev0, stream));
         DevState* st = this->st();
         uint32_t* qCount = stField(offsetof(DevState, qCount));
@@ -399,21 +472,25 @@ struct pale_handle {
         for (size_t k = 0; k < camIdx.size(); ++k) {
             const HostCamera& c = cams[camIdx[k]];
             const uint32_t npx = c.W * c.H;
-            HIPCHECK(hipMemcpyAsync(dloss.p, src[k], (size_t)npx * 3 * sizeof(float), hipMemcpyHostToDevice, stream));
             const DevCamera dc = devCam(c);
             const PendingArrays pend{pend0.as<float4>(), pend1.as<float4>(), pend2.as<uint2>()};
             for (uint32_t spp = 0; spp < S.adjoint_passes; ++spp) {
                 DevMem* in = qA;
                 DevMem* out = qB;
-                launchAdjRaygen(stream, dc, dloss.as<float>(), rq(in), pend, st);
+                launchAdjRaygen(stream, dc, dloss.as<float>() + off[k], rq(in), pend, st);
                 for (uint32_t b = 0; b < S.adjoint_bounces; ++b) {
+                    const size_t ta = tBegin(kTAdjTrace);
                     launchAdjTrace(stream, scene, st, npx, b, rq(in), rq(stg), extFlags.as<uint8_t>(), ea(evStg),
-                                   evFlags.as<uint8_t>(), pend, tp());
+                                   evFlags.as<uint8_t>(), pend, tp(), countWork);
+                    tEnd(ta);
                     launchCompactEvents(stream, evFlags.as<uint8_t>(), qCount, npx, tiles.as<uint32_t>(), evTotal,
                                         ea(evStg), ea(evCmp), adjCap);
                     launchCompactRays(stream, extFlags.as<uint8_t>(), qCount, npx, tiles.as<uint32_t>(), extTotal,
                                       rq(stg), rq(out), queueCap);
-                    launchAdjTransport(stream, scene, G, st, npx, ea(evCmp), invSpp, gradOpacity.as<float>());
+                    const size_t tt = tBegin(kTTransport);
+                    launchAdjTransport(stream, scene, G, st, npx, ea(evCmp), invSpp, gradOpacity.as<float>(),
+                                       countWork);
+                    tEnd(tt);
                     launchSetQueue(stream, st, extTotal);
                     std::swap(in, out);
                 }
@@ -899,6 +976,12 @@ int pale_get_photons(const pale_handle* h, uint32_t* count, uint64_t* attempted,
             if (direction) { direction[3 * i] = b[i].x; direction[3 * i + 1] = b[i].y; direction[3 * i + 2] = b[i].z; }
         }
     });
+}
+
+int pale_set_count_work(pale_handle* h, int32_t on) {
+    if (!h) return fail(PALE_ERR_INVALID_ARG, "null handle");
+    h->countWork = on != 0;
+    return PALE_OK;
 }
 
 int pale_get_stats(const pale_handle* h, pale_stats* out) {

@@ -283,6 +283,9 @@ class Renderer:
         check(self._L.pale_get_photons(self._h, C.byref(cnt), C.byref(att), _p(pos), _p(pw), _p(dr)))
         return dict(count=n, attempted=att.value, position=pos, power=pw, direction=dr)
 
+    def set_count_work(self, on: bool) -> None:
+        check(self._L.pale_set_count_work(self._h, 1 if on else 0))
+
     def stats(self) -> dict:
         s = _abi.Stats()
         check(self._L.pale_get_stats(self._h, C.byref(s)))

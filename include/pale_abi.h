@@ -132,7 +132,23 @@ typedef struct pale_stats {
     uint64_t grid_cells;        /* cells of the photon grid                  */
     double forward_ms;          /* device time of the last render_forward    */
     double backward_ms;         /* device time of the last render_backward   */
+    /* Device time per kernel class (sum over launches) and launch counts. */
+    double photon_trace_ms, gather_ms, adjoint_trace_ms, adjoint_transport_ms, grid_ms;
+    uint32_t photon_trace_launches, gather_launches, adjoint_trace_launches, adjoint_transport_launches;
+    /* Counting mode only (pale_set_count_work): work units of the
+     * reference algorithm per kernel class, summed over the last
+     * render_forward and any render_backward after it.  work[c][u] with
+     * class c = PALE_WORK_PHOTON / _GATHER / _ADJOINT and unit u = popped
+     * nodes, AABB slab tests, primitive tests, irradiance gathers, gather
+     * cells visited, non-empty cells, photons scanned. */
+    uint64_t work[3][7];
 } pale_stats;
+
+enum { PALE_WORK_PHOTON = 0, PALE_WORK_GATHER = 1, PALE_WORK_ADJOINT = 2 };
+enum {
+    PALE_WORK_POPS = 0, PALE_WORK_AABB = 1, PALE_WORK_PRIMS = 2, PALE_WORK_GATHERS = 3,
+    PALE_WORK_CELLS = 4, PALE_WORK_CELLS_NONEMPTY = 5, PALE_WORK_PHOTONS = 6
+};
 
 typedef struct pale_handle pale_handle;
 
@@ -239,6 +255,10 @@ int pale_get_photons(const pale_handle* h, uint32_t* count,
 
 /* Work counters and device times of the last render calls. */
 int pale_get_stats(const pale_handle* h, pale_stats* out);
+
+/* Counting mode on (1) / off (0): render calls run kernel variants that also
+ * count the work units of pale_stats (slower; results are identical). */
+int pale_set_count_work(pale_handle* h, int32_t on);
 
 #ifdef __cplusplus
 }

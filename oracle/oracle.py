@@ -57,6 +57,7 @@ def lib():
         L.orc_get_stats.argtypes = [C.c_void_p, C.POINTER(_abi.Stats)]
         L.orc_apply_points.argtypes = [C.c_void_p, C.POINTER(_abi.PointsDesc)]
         L.orc_rebuild_bvh.argtypes = [C.c_void_p]
+        L.orc_set_sample.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32]
         _lib = L
     return _lib
 
@@ -195,3 +196,8 @@ class Oracle:
 
     def rebuild_bvh(self):
         lib().orc_rebuild_bvh(self.h)
+
+    def set_sample(self, photon_stride: int, pixel_stride: int):
+        """Bounded-sample mode (CPU baseline): photons/photon_stride photons,
+        pixels with index % pixel_stride == 0."""
+        lib().orc_set_sample(self.h, int(photon_stride), int(pixel_stride))

@@ -295,9 +295,23 @@ struct Renderer {
     // Gradients.
     std::vector<float> gradOpacity;
     pale_stats stats{};
+    // Bounded-sample mode of the CPU baseline: trace photons/photonStride
+    // photons and only pixels with index % pixelStride == 0.
+    uint32_t photonStride = 1, pixelStride = 1;
 };
 
 static thread_local std::string g_err;
+
+// Work units of the reference algorithm (per-query counts of the
+// algorithmic byte model): popped nodes, slab tests, primitive tests,
+// gathers, gather cells, non-empty cells, photons scanned.
+struct Work {
+    uint64_t pops = 0, aabb = 0, prims = 0, gathers = 0, cells = 0, cellsNE = 0, photons = 0;
+};
+static thread_local Work g_workClass[3];
+static thread_local Work* g_workPtr = &g_workClass[0];
+#define g_work (*g_workPtr)
+static void workClass(int c) { g_workPtr = &g_workClass[c]; }
 
 // ------------------------------------------------------- BVH (BVH.ixx) --
 struct AABB {
@@ -793,8 +807,11 @@ static bool blasMesh(const Renderer& R, const Ray& ro, uint32_t bi, LocalHit& ou
         const uint32_t ni = st.pop();
         const Node& nd = nodes[ni];
         float te = 0.0f;
+        g_work.pops++;
+        g_work.aabb++;
         if (!slab(ro, nd.mn, nd.mx, inv, best, 1e-6f, te)) continue;
         if (nd.count == 0) {
+            g_work.aabb += 2;
             const uint32_t l = nd.leftFirst, r = nd.leftFirst + 1;
             float tl = kInf, tr = kInf;
             const bool hl = slab(ro, nodes[l].mn, nodes[l].mx, inv, best, 1e-6f, tl);
@@ -804,6 +821,7 @@ static bool blasMesh(const Renderer& R, const Ray& ro, uint32_t bi, LocalHit& ou
             else if (hr) st.push(r);
             continue;
         }
+        g_work.prims += nd.count;
         for (uint32_t i = 0; i < nd.count; ++i) {
             const uint32_t ti = nd.leftFirst + i;
             const Tri& T = R.tris[ti];
@@ -837,8 +855,11 @@ static bool blasCloud(const Renderer& R, const Ray& ro, uint32_t bi, LocalHit& o
         const uint32_t ni = st.pop();
         const Node& nd = nodes[ni];
         float te = 0.0f;
+        g_work.pops++;
+        g_work.aabb++;
         if (!slab(ro, nd.mn, nd.mx, inv, best, 1e-6f, te)) continue;
         if (nd.count == 0) {
+            g_work.aabb += 2;
             const uint32_t l = nd.leftFirst, r = nd.leftFirst + 1;
             float tl = kInf, tr = kInf;
             const bool hl = slab(ro, nodes[l].mn, nodes[l].mx, inv, best, 1e-6f, tl);
@@ -848,6 +869,7 @@ static bool blasCloud(const Renderer& R, const Ray& ro, uint32_t bi, LocalHit& o
             else if (hr) st.push(r);
             continue;
         }
+        g_work.prims += nd.count;
         for (uint32_t k = 0; k < nd.count; ++k) {
             const uint32_t si = nd.leftFirst + k;
             float t = 0.0f, a = 0.0f;
@@ -893,8 +915,11 @@ static bool intersectScene(const Renderer& R, const Ray& rw, WorldHit& out) {
         const uint32_t ni = st.pop();
         const TNode& nd = R.tlas[ni];
         float te = 0.0f;
+        g_work.pops++;
+        g_work.aabb++;
         if (!slab(rw, nd.mn, nd.mx, inv, bestW, 0.0f, te)) continue;
         if (nd.count == 0) {
+            g_work.aabb += 2;
             const uint32_t l = nd.left, r = nd.right;
             float tl = kInf, tr = kInf;
             const bool hl = slab(rw, R.tlas[l].mn, R.tlas[l].mx, inv, bestW, 0.0f, tl);
@@ -1181,6 +1206,7 @@ static void buildGrid(Renderer& R) {
 
 // gatherDiffuseIrradianceAtPoint (KernelHelpers.h:990-1037)
 static F3 gather(const Renderer& R, F3 q, F3 n) {
+    g_work.gathers++;
     if (!R.mapValid) return mk(0, 0, 0);
     const Grid& g = R.grid;
     const float r = g.radius;
@@ -1196,8 +1222,11 @@ static F3 gather(const Renderer& R, F3 q, F3 n) {
                 const int c[3] = {cx, cy, cz};
                 const uint32_t id = linearCell(g, c);
                 const uint32_t st = g.cellStart[id];
+                g_work.cells++;
                 if (st == kInvalid) continue;
                 const uint32_t en = g.cellEnd[id];
+                g_work.cellsNE++;
+                g_work.photons += en - st;
                 for (uint32_t j = st; j < en; ++j) {
                     const Photon& ph = R.photons[g.sorted[j]];
                     const F3 d = ph.pos - q;
@@ -1249,6 +1278,7 @@ static void cameraGather(Renderer& R, const Camera& c, uint32_t baseSeed, std::v
     for (uint32_t pass = 0; pass < R.S.gatherPasses; ++pass) {
         const uint64_t seed = R.S.decorrelate ? (uint64_t)(uint32_t)(baseSeed + pass) : (uint64_t)baseSeed;
         for (uint32_t pi = 0; pi < npx; ++pi) {
+            if (pi % R.pixelStride) continue;
             const uint32_t px = pi % W, py = pi / W;
             Xorshift128 rng(perItemSeed(seed, pi));
             const float jx = rng.nextFloat() - 0.5f;
@@ -1319,14 +1349,16 @@ static void renderForward(Renderer& R, const std::vector<uint32_t>& camIdx, floa
     std::mt19937_64 seedGen(S.seed);
     R.photonCounter = 0;
     R.photons.assign(R.photonCapacity, Photon{});
-    const uint32_t emitted = S.photons * S.forwardPasses;
+    const uint32_t P = std::max(1u, S.photons / R.photonStride);
+    const uint32_t emitted = P * S.forwardPasses;
     const float invEmitted = 1.0f / (float)emitted;
+    workClass(0);
     for (uint32_t pass = 0; pass < S.forwardPasses; ++pass) {
         uint32_t number = (uint32_t)seedGen();
         std::vector<RayState> q, nq;
-        q.reserve(S.photons);
+        q.reserve(P);
         // launchRayGenEmitterKernel (PrimalKernels.cpp:13-62)
-        for (uint32_t i = 0; i < S.photons; ++i) {
+        for (uint32_t i = 0; i < P; ++i) {
             Xorshift128 rng(perItemSeed(number, i));
             F3 pos, n, dir, power;
             if (!sampleLight(R, rng, pos, n, dir, power)) continue;
@@ -1350,6 +1382,7 @@ static void renderForward(Renderer& R, const std::vector<uint32_t>& camIdx, floa
     R.mapValid = true;
     R.stats.grid_cells = R.grid.total;
     std::vector<float> fb;
+    workClass(1);
     for (size_t k = 0; k < camIdx.size(); ++k) {
         const uint32_t number = (uint32_t)seedGen();
         const Camera& c = R.cams[camIdx[k]];
@@ -1432,6 +1465,7 @@ static void renderBackward(Renderer& R, const std::vector<uint32_t>& camIdx,
                            const std::vector<const float*>& dloss) {
     const Settings& S = R.S;
     const uint64_t number = 42;
+    workClass(2);
     R.gradOpacity.assign(R.points.size(), 0.0f);
     const float invSpp = 1.0f / (float)S.adjointPasses;
     for (size_t k = 0; k < camIdx.size(); ++k) {
@@ -1440,8 +1474,10 @@ static void renderBackward(Renderer& R, const std::vector<uint32_t>& camIdx,
         const uint32_t W = c.W, H = c.H, npx = W * H;
         for (uint32_t spp = 0; spp < S.adjointPasses; ++spp) {
             // launchRayGenAdjointKernel (AdjointKernels.cpp:15-84)
-            std::vector<ARay> q(npx), nq;
+            std::vector<ARay> q, nq;
+            q.reserve(npx / R.pixelStride + 1);
             for (uint32_t pi = 0; pi < npx; ++pi) {
+                if (pi % R.pixelStride) continue;
                 Xorshift128 rng(perItemSeed(number, pi));
                 const float jx = rng.nextFloat() - 0.5f;
                 const float jy = rng.nextFloat() - 0.5f;
@@ -1451,7 +1487,7 @@ static void renderBackward(Renderer& R, const std::vector<uint32_t>& camIdx,
                 a.bounce = 0;
                 a.pixel = pi;
                 a.pathId = pi;
-                q[pi] = a;
+                q.push_back(a);
             }
             std::vector<Pending> pend(npx);
             for (uint32_t b = 0; b < S.adjointBounces && !q.empty(); ++b) {
@@ -1753,6 +1789,14 @@ int orc_trace_rays(void* h, uint32_t n, const float* o, const float* d, int32_t*
     return 0;
 }
 
+static void copyWork(Renderer& R) {
+    for (int c = 0; c < 3; ++c) {
+        const Work& w = g_workClass[c];
+        const uint64_t v[7] = {w.pops, w.aabb, w.prims, w.gathers, w.cells, w.cellsNE, w.photons};
+        for (int u = 0; u < 7; ++u) R.stats.work[c][u] = v[u];
+    }
+}
+
 static std::vector<uint32_t> selectCams(Renderer& R, const char* name) {
     std::vector<uint32_t> idx;
     const std::string nm = name ? name : "";
@@ -1765,10 +1809,12 @@ int orc_render_forward(void* h, const char* cam, float* const* ldr, float* const
     try {
         Renderer& R = *static_cast<Renderer*>(h);
         R.stats = pale_stats{};
+        for (Work& w : g_workClass) w = Work{};
         const std::vector<uint32_t> idx = selectCams(R, cam);
         const auto t0 = std::chrono::steady_clock::now();
         renderForward(R, idx, ldr, raw);
         R.stats.forward_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        copyWork(R);
         if (rendered) *rendered = (uint32_t)idx.size();
         return 0;
     } catch (const std::exception& e) {
@@ -1802,6 +1848,7 @@ int orc_render_backward(void* h, uint32_t n, const char* const* names, const flo
         const auto t0 = std::chrono::steady_clock::now();
         renderBackward(R, idx, src);
         R.stats.backward_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        copyWork(R);
         if (opacityGrad) std::memcpy(opacityGrad, R.gradOpacity.data(), R.gradOpacity.size() * sizeof(float));
         return 0;
     } catch (const std::exception& e) {
@@ -1877,6 +1924,13 @@ int orc_apply_points(void* h, const pale_points_desc* p) {
         a.beta = q.beta;
     }
     setScene(R);
+    return 0;
+}
+
+int orc_set_sample(void* h, uint32_t photonStride, uint32_t pixelStride) {
+    Renderer& R = *static_cast<Renderer*>(h);
+    R.photonStride = std::max(1u, photonStride);
+    R.pixelStride = std::max(1u, pixelStride);
     return 0;
 }
 

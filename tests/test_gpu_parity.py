@@ -140,3 +140,25 @@ def test_single_surfel_forward_and_adjoint():
     g, _ = gpu.render_backward({"camera1": dl})
     go = orc.render_backward({"camera1": dl})
     np.testing.assert_allclose(g["opacity"], go, rtol=1e-4, atol=1e-9)
+
+
+def test_work_counters_equal_reference_algorithm(c0_small):
+    # Same traversal order and culling => identical counts of popped nodes,
+    # slab tests, surfel tests and gather cells/photons, per kernel class.
+    _, gpu, orc = c0_small
+    gpu.set_count_work(True)
+    try:
+        ib = orc.render_forward()
+        gpu.render_forward()
+        raw = ib["camera1_raw"][..., :3]
+        dl = (raw / raw.size).astype(np.float32)
+        gpu.render_backward({"camera1": dl})
+        orc.render_backward({"camera1": dl})
+    finally:
+        gpu.set_count_work(False)
+    a, b = gpu.stats(), orc.stats()
+    for k in ("photon_queries", "gather_queries", "adjoint_queries", "gradient_events"):
+        assert a[k] == b[k], (k, a[k], b[k])
+    for cls in ("work_photon", "work_gather", "work_adjoint"):
+        assert a[cls] == b[cls], (cls, a[cls], b[cls])
+    assert a["work_gather"]["pops"] > 0 and a["work_gather"]["photons"] > 0
