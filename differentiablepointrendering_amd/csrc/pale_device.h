@@ -462,7 +462,11 @@ __device__ __forceinline__ void worldToCell(const DevGrid& g, F3 p, int& x, int&
 // gatherDiffuseIrradianceAtPoint (KernelHelpers.h:990-1037).  Photons are
 // stored sorted by (cell, deposit index), so the cells of one grid row form
 // one contiguous range; summation order equals the reference's cell-by-cell,
-// photon-by-photon order.
+// photon-by-photon order.  Memory-level parallelism: the row bounds of a
+// whole z-slice are loaded before any photon of that slice, and photons are
+// loaded four at a time (added in order).
+constexpr int kSliceRows = 6;  // cells per axis: 4 + 1, +1 for rounding
+
 template <bool kCount>
 __device__ F3 gatherIrradiance(const DevGrid& g, F3 q, F3 n, Work& w) {
     if (kCount) w.gathers++;
@@ -474,25 +478,48 @@ __device__ F3 gatherIrradiance(const DevGrid& g, F3 q, F3 n, Work& w) {
     worldToCell(g, q - mk3(r, r, r), lx, ly, lz);
     worldToCell(g, q + mk3(r, r, r), hx, hy, hz);
     F3 E = mk3(0, 0, 0);
+    auto addPhoton = [&](const float4& pp, const float4& dp, float pz) {
+        const F3 dd = mk3(pp.x, pp.y, pp.z) - q;
+        // dot(n, -dir) == dot(-n, dir) exactly (negation is exact).
+        const float cosine = dot(n, -mk3(dp.x, dp.y, dp.z));
+        if (cosine < 0.0f) return;
+        const float d2 = dot(dd, dd);
+        if (d2 > r2) return;
+        E = E + mk3(pp.w, dp.w, pz) * invArea;
+    };
     for (int cz = lz; cz <= hz; ++cz) {
-        for (int cy = ly; cy <= hy; ++cy) {
-            const uint32_t row = (uint32_t)((cz * g.ry + cy) * g.rx);
-            const uint32_t a = g.cellStart[row + lx];
-            const uint32_t b = g.cellStart[row + hx + 1];
-            if (kCount) {
-                w.cells += (uint32_t)(hx - lx + 1);
-                w.photons += b - a;
-                for (int cx = lx; cx <= hx; ++cx) w.cellsNE += g.cellStart[row + cx + 1] > g.cellStart[row + cx];
+        for (int cy0 = ly; cy0 <= hy; cy0 += kSliceRows) {
+            uint32_t ra[kSliceRows], rb[kSliceRows];
+#pragma unroll
+            for (int k = 0; k < kSliceRows; ++k) {
+                const int cy = cy0 + k;
+                ra[k] = rb[k] = 0u;
+                if (cy <= hy) {
+                    const uint32_t row = (uint32_t)((cz * g.ry + cy) * g.rx);
+                    ra[k] = g.cellStart[row + lx];
+                    rb[k] = g.cellStart[row + hx + 1];
+                }
             }
-            for (uint32_t j = a; j < b; ++j) {
-                const float4 pp = g.posPow[j];
-                const float4 dp = g.dirPow[j];
-                const F3 dd = mk3(pp.x, pp.y, pp.z) - q;
-                const float cosine = dot(n, -mk3(dp.x, dp.y, dp.z));
-                if (cosine < 0.0f) continue;
-                const float d2 = dot(dd, dd);
-                if (d2 > r2) continue;
-                E = E + mk3(pp.w, dp.w, g.powZ[j]) * invArea;
+#pragma unroll
+            for (int k = 0; k < kSliceRows; ++k) {
+                if (kCount && cy0 + k <= hy) {
+                    const uint32_t row = (uint32_t)((cz * g.ry + cy0 + k) * g.rx);
+                    w.cells += (uint32_t)(hx - lx + 1);
+                    w.photons += rb[k] - ra[k];
+                    for (int cx = lx; cx <= hx; ++cx) w.cellsNE += g.cellStart[row + cx + 1] > g.cellStart[row + cx];
+                }
+                uint32_t j = ra[k];
+                const uint32_t b = rb[k];
+                for (; j + 4 <= b; j += 4) {
+                    const float4 p0 = g.posPow[j], p1 = g.posPow[j + 1], p2 = g.posPow[j + 2], p3 = g.posPow[j + 3];
+                    const float4 d0 = g.dirPow[j], d1 = g.dirPow[j + 1], d2 = g.dirPow[j + 2], d3 = g.dirPow[j + 3];
+                    const float z0 = g.powZ[j], z1 = g.powZ[j + 1], z2 = g.powZ[j + 2], z3 = g.powZ[j + 3];
+                    addPhoton(p0, d0, z0);
+                    addPhoton(p1, d1, z1);
+                    addPhoton(p2, d2, z2);
+                    addPhoton(p3, d3, z3);
+                }
+                for (; j < b; ++j) addPhoton(g.posPow[j], g.dirPow[j], g.powZ[j]);
             }
         }
     }

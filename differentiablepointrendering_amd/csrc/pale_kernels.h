@@ -98,7 +98,7 @@ void launchBuildGrid(hipStream_t s, const GridParams& g, DevState* st, uint32_t 
                      PhotonArrays sorted, uint32_t* cellId, uint32_t* sortedIdx, uint32_t* cellCount,
                      uint32_t* cellStart, uint32_t* tiles, uint32_t* scratchTotal);
 void launchGather(hipStream_t s, const DevScene& S, const DevGrid& G, const DevCamera& cam, DevState* st,
-                  uint32_t pass, int decorrelate, float invSpp, float4* fb, bool count);
+                  uint32_t pass, int decorrelate, float invSpp, float4* fb, uint32_t* cursor, bool count);
 void launchPostprocess(hipStream_t s, const float4* fb, float4* ldr, uint32_t n);
 void launchAdjRaygen(hipStream_t s, const DevCamera& cam, const float* dloss, RayQueue q, PendingArrays pend,
                      DevState* st);

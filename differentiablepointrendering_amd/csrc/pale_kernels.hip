@@ -8,6 +8,8 @@
 // work item from its queue slot, PrimalKernels.cpp:80-82,
 // AdjointKernels.cpp:99-101).  Counts stay on the device; no kernel waits for
 // the host.
+#include <algorithm>
+
 #include "pale_kernels.h"
 
 namespace pale {
@@ -434,57 +436,90 @@ __global__ __launch_bounds__(256) void k_cell_payload(const DevState* st, uint32
 }
 
 // ---------------------------------------------------- camera gather --
-// launchCameraGatherKernel (PrimalKernels.cpp:500-653): one pass.
+// launchCameraGatherKernel (PrimalKernels.cpp:500-653): one pass, run by a
+// persistent grid.  A lane walks one pixel's surfel layers (one closest-hit
+// query + one irradiance gather per layer); when its pixel is finished it
+// takes the next pixel from the pass's cursor (one atomic per wave refill),
+// so no lane waits for the wave's deepest pixel.  Pixel arithmetic and the
+// framebuffer update are the reference's, so the image is unchanged.
 template <bool kCount>
 __global__ __launch_bounds__(kTraceBlock) void k_gather(DevScene S, DevGrid G, DevCamera cam, DevState* st,
-                                                        uint32_t pass, int decorrelate, float invSpp, float4* fb) {
+                                                        uint32_t pass, int decorrelate, float invSpp, float4* fb,
+                                                        uint32_t* cursor) {
     extern __shared__ uint2 s_stack[];
     const LaneStack stk = laneStack(s_stack, S.stackDepth);
     const uint32_t npx = cam.W * cam.H;
-    const uint32_t pixel = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t seed = decorrelate ? st->number + pass : st->number;
+    const uint32_t lane = laneId();
     uint32_t layers = 0;
     Work w{};
-    if (pixel < npx) {
-        const uint32_t seed = decorrelate ? st->number + pass : st->number;
-        Rng rng(perItemSeed(seed, pixel));
-        const float jx = rng.nextFloat() - 0.5f;
-        const float jy = rng.nextFloat() - 0.5f;
-        F3 o, d;
-        primaryRay(cam, (float)(pixel % cam.W), (float)(pixel / cam.W), jx, jy, o, d);
-        F3 C = mk3(0, 0, 0);
-        float tr = 1.0f;
-        while (true) {
+    bool active = false, exhausted = false;
+    uint32_t pixel = 0;
+    F3 o = mk3(0, 0, 0), d = mk3(0, 0, 0), C = mk3(0, 0, 0);
+    float tr = 1.0f;
+    while (true) {
+        if (!exhausted) {
+            const unsigned long long need = __ballot(!active);
+            if (need) {
+                const uint32_t leader = (uint32_t)__ffsll((long long)need) - 1u;
+                uint32_t base = 0;
+                if (lane == leader) base = atomicAdd(cursor, (uint32_t)__popcll(need));
+                base = __shfl(base, (int)leader, 64);
+                if (base + (uint32_t)__popcll(need) >= npx) exhausted = true;
+                if (!active) {
+                    const uint32_t p = base + (uint32_t)__popcll(need & ((1ull << lane) - 1ull));
+                    if (p < npx) {
+                        pixel = p;
+                        active = true;
+                        Rng rng(perItemSeed(seed, pixel));
+                        const float jx = rng.nextFloat() - 0.5f;
+                        const float jy = rng.nextFloat() - 0.5f;
+                        primaryRay(cam, (float)(pixel % cam.W), (float)(pixel / cam.W), jx, jy, o, d);
+                        C = mk3(0, 0, 0);
+                        tr = 1.0f;
+                    }
+                }
+            }
+        }
+        if (!__any(active)) break;
+        if (active) {
             Hit h;
             ++layers;
-            if (!intersectScene<kCount>(S, o, d, stk, h, w)) break;
-            const DevInstance& I = S.inst[h.inst];
-            if (I.type == kGeoCloud) {
-                const SurfelShade sh = S.shade[h.prim];
-                F3 n = mk3(sh.nsx, sh.nsy, sh.nsz);
-                if (dot(n, -d) < 0.0f) n = -n;
-                const F3 E = gatherIrradiance<kCount>(G, h.pos, n, w);
-                const F3 Lo = E * ((sh.alpha_r * mk3(sh.ar, sh.ag, sh.ab)) * kInvPi);
-                const float ae = sh.opacity * h.alpha;
-                C = C + (tr * ae) * Lo;
-                tr *= (1.0f - ae);
-                o = h.pos + d * 1e-4f;
-                continue;
+            bool done = true;
+            if (intersectScene<kCount>(S, o, d, stk, h, w)) {
+                const DevInstance& I = S.inst[h.inst];
+                if (I.type == kGeoCloud) {
+                    const SurfelShade sh = S.shade[h.prim];
+                    F3 n = mk3(sh.nsx, sh.nsy, sh.nsz);
+                    if (dot(n, -d) < 0.0f) n = -n;
+                    const F3 E = gatherIrradiance<kCount>(G, h.pos, n, w);
+                    const F3 Lo = E * ((sh.alpha_r * mk3(sh.ar, sh.ag, sh.ab)) * kInvPi);
+                    const float ae = sh.opacity * h.alpha;
+                    C = C + (tr * ae) * Lo;
+                    tr *= (1.0f - ae);
+                    o = h.pos + d * 1e-4f;
+                    done = false;
+                } else {
+                    const DevMaterial m = S.mats[I.material];
+                    const F3 base = mk3(m.r, m.g, m.b);
+                    if (m.power > 0.0f) {
+                        C = C + tr * vmin(base * m.power, mk3(1.0f, 1.0f, 1.0f));
+                    } else {
+                        const F3 E = gatherIrradiance<kCount>(G, h.pos, hitNormal(S, h), w);
+                        C = C + tr * ((base * kInvPi) * E);
+                    }
+                }
             }
-            const DevMaterial m = S.mats[I.material];
-            const F3 base = mk3(m.r, m.g, m.b);
-            if (m.power > 0.0f) {
-                C = C + tr * vmin(base * m.power, mk3(1.0f, 1.0f, 1.0f));
-            } else {
-                const F3 E = gatherIrradiance<kCount>(G, h.pos, hitNormal(S, h), w);
-                C = C + tr * ((base * kInvPi) * E);
+            if (done) {
+                const float4 f = fb[pixel];
+                fb[pixel] = make_float4(C.x * invSpp + f.x, C.y * invSpp + f.y, C.z * invSpp + f.z,
+                                        1.0f * invSpp + f.w);
+                active = false;
             }
-            break;
         }
-        const float4 f = fb[pixel];
-        fb[pixel] = make_float4(C.x * invSpp + f.x, C.y * invSpp + f.y, C.z * invSpp + f.z, 1.0f * invSpp + f.w);
     }
     const uint32_t ws = waveSum(layers);
-    if (laneId() == 0 && ws) atomicAdd(&st->gatherQ, (unsigned long long)ws);
+    if (lane == 0 && ws) atomicAdd(&st->gatherQ, (unsigned long long)ws);
     if (kCount) flushWork(w, st, kWorkGather);
 }
 
@@ -811,15 +846,34 @@ void launchBuildGrid(hipStream_t s, const GridParams& g, DevState* st, uint32_t 
     k_cell_payload<<<pb, 256, 0, s>>>(st, cap, sortedIdx, raw, sorted);
 }
 
+// Persistent grid of the gather: as many workgroups as fit on the device.
+template <bool kCount>
+static uint32_t gatherGrid(size_t lds) {
+    static uint32_t cached[2] = {0, 0};
+    static size_t cachedLds[2] = {0, 0};
+    const int k = kCount ? 1 : 0;
+    if (cached[k] && cachedLds[k] == lds) return cached[k];
+    int dev = 0, cus = 0, perCu = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, k_gather<kCount>, kTraceBlock, lds);
+    cached[k] = (uint32_t)std::max(1, cus) * (uint32_t)std::max(1, perCu);
+    cachedLds[k] = lds;
+    return cached[k];
+}
+
 void launchGather(hipStream_t s, const DevScene& S, const DevGrid& G, const DevCamera& cam, DevState* st,
-                  uint32_t pass, int decorrelate, float invSpp, float4* fb, bool count) {
+                  uint32_t pass, int decorrelate, float invSpp, float4* fb, uint32_t* cursor, bool count) {
     const uint32_t npx = cam.W * cam.H;
     if (!npx) return;
-    const dim3 g(blocks(npx, kTraceBlock));
-    if (count)
-        k_gather<true><<<g, kTraceBlock, stackBytes(S), s>>>(S, G, cam, st, pass, decorrelate, invSpp, fb);
-    else
-        k_gather<false><<<g, kTraceBlock, stackBytes(S), s>>>(S, G, cam, st, pass, decorrelate, invSpp, fb);
+    const size_t lds = stackBytes(S);
+    if (count) {
+        const uint32_t g = min(gatherGrid<true>(lds), blocks(npx, kTraceBlock));
+        k_gather<true><<<g, kTraceBlock, lds, s>>>(S, G, cam, st, pass, decorrelate, invSpp, fb, cursor);
+    } else {
+        const uint32_t g = min(gatherGrid<false>(lds), blocks(npx, kTraceBlock));
+        k_gather<false><<<g, kTraceBlock, lds, s>>>(S, G, cam, st, pass, decorrelate, invSpp, fb, cursor);
+    }
 }
 void launchPostprocess(hipStream_t s, const float4* fb, float4* ldr, uint32_t n) {
     if (n) k_post<<<blocks(n, 256), 256, 0, s>>>(fb, ldr, n);

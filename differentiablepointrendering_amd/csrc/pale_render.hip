@@ -108,7 +108,7 @@ struct pale_handle {
     uint32_t adjCap = 0;
     DevMem pend0, pend1, pend2, evStg[4], evCmp[4], evFlags, dloss, gradOpacity;
     // Device state and seed table.
-    DevMem state, seedTable;
+    DevMem state, seedTable, cursors;
     // Per-camera framebuffers.
     std::vector<DevMem> fb, ldr;
     pale_stats stats{};
@@ -413,6 +413,10 @@ struct pale_handle {
         mapValid = true;
         const DevGrid G = devGrid();
         const float invSpp = 1.0f / (float)S.gather_passes;
+        // One pixel cursor per (camera, pass) of the persistent gather.
+        const uint32_t nCursor = (uint32_t)std::max<size_t>(1, camIdx.size() * S.gather_passes);
+        cursors.reserve((size_t)nCursor * sizeof(uint32_t));
+        launchFill(stream, cursors.as<uint32_t>(), nCursor, 0u);
         for (size_t k = 0; k < camIdx.size(); ++k) {
             const HostCamera& c = cams[camIdx[k]];
             const uint32_t npx = c.W * c.H;
@@ -422,7 +426,7 @@ struct pale_handle {
             for (uint32_t pass = 0; pass < S.gather_passes; ++pass) {
                 const size_t tk = tBegin(kTGather);
                 launchGather(stream, scene, G, dc, st, pass, S.spp_decorrelate, invSpp, fb[camIdx[k]].as<float4>(),
-                             countWork);
+                             cursors.as<uint32_t>() + k * S.gather_passes + pass, countWork);
                 tEnd(tk);
             }
             launchPostprocess(stream, fb[camIdx[k]].as<float4>(), ldr[camIdx[k]].as<float4>(), npx);
