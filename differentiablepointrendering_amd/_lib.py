@@ -23,7 +23,7 @@ EXPORTED = (
     "pale_last_error", "pale_default_settings", "pale_create", "pale_destroy", "pale_camera_count",
     "pale_camera_info", "pale_render_forward", "pale_render_backward", "pale_point_count", "pale_get_points",
     "pale_apply_points", "pale_add_points", "pale_remove_points", "pale_rebuild_bvh", "pale_set_point_properties",
-    "pale_set_point_opacity", "pale_set_point_beta", "pale_trace_rays", "pale_get_surfel_bvh", "pale_get_photons",
+    "pale_set_point_opacity", "pale_set_point_beta", "pale_trace_rays", "pale_gather_irradiance", "pale_get_surfel_bvh", "pale_get_photons",
     "pale_get_stats", "pale_set_count_work",
 )
 
@@ -62,6 +62,7 @@ def load(path: str = LIB_PATH):
     L.pale_set_point_opacity.argtypes = [H, C.c_float, C.c_int32]
     L.pale_set_point_beta.argtypes = [H, C.c_float, C.c_int32]
     L.pale_trace_rays.argtypes = [H, C.c_uint32, _fp, _fp, _i32p, _i32p, _fp, _fp, _fp]
+    L.pale_gather_irradiance.argtypes = [H, C.c_uint32, _fp, _fp, _fp]
     L.pale_get_surfel_bvh.argtypes = [H, _u32p, _fp, _u32p, _u32p, _u32p]
     L.pale_get_photons.argtypes = [H, _u32p, C.POINTER(C.c_uint64), _fp, _fp, _fp]
     L.pale_get_stats.argtypes = [H, C.POINTER(_abi.Stats)]

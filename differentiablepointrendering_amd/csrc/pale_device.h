@@ -459,12 +459,45 @@ __device__ __forceinline__ void worldToCell(const DevGrid& g, F3 p, int& x, int&
     z = min(max((int)floorf(f.z), 0), g.rz - 1);
 }
 
+// Cells of one grid row (cy, cz) that can hold a photon within radius r of
+// q: the row is skipped when its y-z extent lies farther than r from q, and
+// otherwise the x range shrinks to the chord of the sphere.  Every bound is
+// widened by a margin (1e-3 of a cell plus 2e-6 of the coordinates) that
+// exceeds the binary32 rounding of worldToCell and of the distance test by
+// orders of magnitude, and boundary cells, which also hold the photons that
+// worldToCell clamps into them, extend to infinity.  A photon in a skipped
+// cell therefore fails the reference's d2 > r2 test, so skipping it leaves
+// the sum unchanged.
+__device__ __forceinline__ bool rowSpan(const DevGrid& g, F3 q, float r2, int cy, int cz, int lx, int hx, int& ax,
+                                        int& bx) {
+    auto axisGap = [](float qc, float o, float c, int i, int res, float m) {
+        const float lo = (i == 0) ? -INFINITY : o + (float)i * c - m;
+        const float hi = (i == res - 1) ? INFINITY : o + (float)(i + 1) * c + m;
+        return fmaxf(fmaxf(lo - qc, qc - hi), 0.0f);
+    };
+    const float my = 1e-3f * g.cy + 2e-6f * (fabsf(q.y) + fabsf(g.oy) + g.radius);
+    const float mz = 1e-3f * g.cz + 2e-6f * (fabsf(q.z) + fabsf(g.oz) + g.radius);
+    const float dy = axisGap(q.y, g.oy, g.cy, cy, g.ry, my);
+    const float dz = axisGap(q.z, g.oz, g.cz, cz, g.rz, mz);
+    const float room = r2 * 1.001f - (dy * dy + dz * dz);
+    if (!(room >= 0.0f)) return false;
+    const float hw = sqrtf(room);
+    const float mx = 1e-3f * g.cx + 2e-6f * (fabsf(q.x) + fabsf(g.ox) + g.radius);
+    const float fa = floorf((q.x - hw - mx - g.ox) / g.cx);
+    const float fb = floorf((q.x + hw + mx - g.ox) / g.cx);
+    ax = (fa > (float)lx) ? (int)fa : lx;
+    bx = (fb < (float)hx) ? (int)fb : hx;
+    return ax <= bx;
+}
+
 // gatherDiffuseIrradianceAtPoint (KernelHelpers.h:990-1037).  Photons are
 // stored sorted by (cell, deposit index), so the cells of one grid row form
 // one contiguous range; summation order equals the reference's cell-by-cell,
-// photon-by-photon order.  Memory-level parallelism: the row bounds of a
-// whole z-slice are loaded before any photon of that slice, and photons are
-// loaded four at a time (added in order).
+// photon-by-photon order.  Cells that cannot hold an accepted photon are not
+// scanned (rowSpan).  Memory-level parallelism: the row bounds of a whole
+// z-slice are loaded before any photon of that slice, and photons are loaded
+// four at a time (added in order).  Counting mode counts the reference's
+// cells and photons (the whole box), not the scanned ones.
 constexpr int kSliceRows = 6;  // cells per axis: 4 + 1, +1 for rounding
 
 template <bool kCount>
@@ -494,10 +527,11 @@ __device__ F3 gatherIrradiance(const DevGrid& g, F3 q, F3 n, Work& w) {
             for (int k = 0; k < kSliceRows; ++k) {
                 const int cy = cy0 + k;
                 ra[k] = rb[k] = 0u;
-                if (cy <= hy) {
+                int ax, bx;
+                if (cy <= hy && rowSpan(g, q, r2, cy, cz, lx, hx, ax, bx)) {
                     const uint32_t row = (uint32_t)((cz * g.ry + cy) * g.rx);
-                    ra[k] = g.cellStart[row + lx];
-                    rb[k] = g.cellStart[row + hx + 1];
+                    ra[k] = g.cellStart[row + ax];
+                    rb[k] = g.cellStart[row + bx + 1];
                 }
             }
 #pragma unroll
@@ -505,7 +539,7 @@ __device__ F3 gatherIrradiance(const DevGrid& g, F3 q, F3 n, Work& w) {
                 if (kCount && cy0 + k <= hy) {
                     const uint32_t row = (uint32_t)((cz * g.ry + cy0 + k) * g.rx);
                     w.cells += (uint32_t)(hx - lx + 1);
-                    w.photons += rb[k] - ra[k];
+                    w.photons += g.cellStart[row + hx + 1] - g.cellStart[row + lx];
                     for (int cx = lx; cx <= hx; ++cx) w.cellsNE += g.cellStart[row + cx + 1] > g.cellStart[row + cx];
                 }
                 uint32_t j = ra[k];

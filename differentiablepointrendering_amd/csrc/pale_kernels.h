@@ -107,6 +107,7 @@ void launchAdjTrace(hipStream_t s, const DevScene& S, DevState* st, uint32_t max
                     TraceParams tp, bool count);
 void launchAdjTransport(hipStream_t s, const DevScene& S, const DevGrid& G, DevState* st, uint32_t maxEvents,
                         EventArrays ev, float invSpp, float* gradOpacity, bool count);
+void launchGatherPoints(hipStream_t s, const DevGrid& G, uint32_t n, const float* q, const float* nrm, float* out);
 void launchTraceRays(hipStream_t s, const DevScene& S, uint32_t n, const float* o, const float* d, int32_t* inst,
                      int32_t* prim, float* t, float* pos, float* alpha);
 

@@ -778,6 +778,20 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_rays(DevScene S, uint32_t
     alpha[i] = hit ? h.alpha : 0.0f;
 }
 
+// Irradiance estimate at given points (gatherDiffuseIrradianceAtPoint,
+// KernelHelpers.h:990-1037) for tests of the gather on its own.
+__global__ __launch_bounds__(256) void k_gather_points(DevGrid G, uint32_t n, const float* q, const float* nrm,
+                                                       float* out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Work w{};
+    const F3 E = gatherIrradiance<false>(G, mk3(q[3 * i], q[3 * i + 1], q[3 * i + 2]),
+                                         mk3(nrm[3 * i], nrm[3 * i + 1], nrm[3 * i + 2]), w);
+    out[3 * i] = E.x;
+    out[3 * i + 1] = E.y;
+    out[3 * i + 2] = E.z;
+}
+
 // ------------------------------------------------------------ launchers --
 static inline uint32_t blocks(uint32_t n, uint32_t b) { return (n + b - 1) / b; }
 static inline size_t stackBytes(const DevScene& S) { return (size_t)S.stackDepth * sizeof(uint2) * kTraceBlock; }
@@ -908,6 +922,10 @@ void launchTraceRays(hipStream_t s, const DevScene& S, uint32_t n, const float* 
     if (n)
         k_trace_rays<<<blocks(n, kTraceBlock), kTraceBlock, stackBytes(S), s>>>(S, n, o, d, inst, prim, t, pos,
                                                                                alpha);
+}
+
+void launchGatherPoints(hipStream_t s, const DevGrid& G, uint32_t n, const float* q, const float* nrm, float* out) {
+    if (n) k_gather_points<<<blocks(n, 256), 256, 0, s>>>(G, n, q, nrm, out);
 }
 
 }  // namespace pale

@@ -939,6 +939,25 @@ int pale_trace_rays(pale_handle* h, uint32_t n, const float* origins, const floa
     });
 }
 
+int pale_gather_irradiance(pale_handle* h, uint32_t n, const float* positions, const float* normals,
+                           float* irradiance_out) {
+    if (!h) return fail(PALE_ERR_INVALID_ARG, "null handle");
+    if (n == 0) return PALE_OK;
+    if (!positions || !normals || !irradiance_out) return fail(PALE_ERR_INVALID_ARG, "null argument");
+    return guarded([&] {
+        DevMem q, nr, e;
+        q.reserve((size_t)n * 12);
+        nr.reserve((size_t)n * 12);
+        e.reserve((size_t)n * 12);
+        HIPCHECK(hipMemcpy(q.p, positions, (size_t)n * 12, hipMemcpyHostToDevice));
+        HIPCHECK(hipMemcpy(nr.p, normals, (size_t)n * 12, hipMemcpyHostToDevice));
+        launchGatherPoints(h->stream, h->devGrid(), n, q.as<float>(), nr.as<float>(), e.as<float>());
+        HIPCHECK(hipStreamSynchronize(h->stream));
+        HIPCHECK(hipGetLastError());
+        HIPCHECK(hipMemcpy(irradiance_out, e.p, (size_t)n * 12, hipMemcpyDeviceToHost));
+    });
+}
+
 int pale_get_surfel_bvh(const pale_handle* h, uint32_t* node_count, float* bounds_out, uint32_t* left_first_out,
                         uint32_t* count_out, uint32_t* permutation_out) {
     if (!h || !node_count) return fail(PALE_ERR_INVALID_ARG, "null argument");

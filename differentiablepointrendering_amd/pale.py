@@ -259,6 +259,16 @@ class Renderer:
                                       prim.ctypes.data_as(C.POINTER(C.c_int32)), _p(t), _p(pos), _p(alpha)))
         return dict(instance=inst, primitive=prim, t=t, position=pos, alpha=alpha)
 
+    def gather_irradiance(self, positions: np.ndarray, normals: np.ndarray) -> np.ndarray:
+        """Irradiance estimate of the current photon map at points (n, 3)."""
+        q = _abi.f32(positions).reshape(-1, 3)
+        nr = _abi.f32(normals).reshape(-1, 3)
+        if nr.shape != q.shape:
+            raise RuntimeError(f"positions {q.shape} and normals {nr.shape} differ")
+        out = np.empty_like(q)
+        check(self._L.pale_gather_irradiance(self._h, q.shape[0], _p(q), _p(nr), _p(out)))
+        return out
+
     def surfel_bvh(self) -> dict:
         cnt = C.c_uint32()
         check(self._L.pale_get_surfel_bvh(self._h, C.byref(cnt), None, None, None, None))

@@ -237,6 +237,13 @@ int pale_trace_rays(pale_handle* h, uint32_t n, const float* origins,
                     int32_t* primitive_out, float* t_out, float* position_out,
                     float* alpha_out);
 
+/* Irradiance estimate of the photon map of the last render_forward at n
+ * points with n normals (3 floats each): the reference's device function
+ * gatherDiffuseIrradianceAtPoint (KernelHelpers.h:990-1037), which no
+ * binding exposes; a diagnostic for tests of the gather on its own. */
+int pale_gather_irradiance(pale_handle* h, uint32_t n, const float* positions,
+                           const float* normals, float* irradiance_out);
+
 /* Surfel BLAS topology in the reference's BVHNode form
  * (GPUDataStructures.h:53-63): per node aabb min (3), aabb max (3),
  * leftFirst, triCount.  nodes_out may be NULL to query *node_count only.

@@ -162,3 +162,50 @@ def test_work_counters_equal_reference_algorithm(c0_small):
     for cls in ("work_photon", "work_gather", "work_adjoint"):
         assert a[cls] == b[cls], (cls, a[cls], b[cls])
     assert a["work_gather"]["pops"] > 0 and a["work_gather"]["photons"] > 0
+
+
+def _gather_probe_points(orc, rng):
+    """Points that stress the gather's cell ranges: photon positions and
+    points at distance r from them along each axis, points on cell faces,
+    points outside the grid and random points inside it."""
+    ph = orc.photons()
+    g = orc.grid_info()
+    r = float(orc_radius(orc))
+    pos = ph["position"][rng.choice(ph["count"], 2000)]
+    pts = [pos, pos + rng.normal(scale=0.3 * r, size=pos.shape)]
+    for ax in range(3):
+        for s in (-1.0, 1.0):
+            e = np.zeros(3)
+            e[ax] = s * r
+            pts.append(pos[:200] + e)
+            pts.append(np.nextafter((pos[:200] + e).astype(np.float32), np.float32(s * np.inf)))
+    ijk = rng.integers(0, g["res"], size=(2000, 3))
+    faces = g["origin"] + ijk * g["cell"]
+    pts.append(faces)
+    pts.append(np.nextafter(faces.astype(np.float32), np.float32(-np.inf)))
+    lo = g["origin"]
+    hi = g["origin"] + g["res"] * g["cell"]
+    pts.append(rng.uniform(lo - 4 * r, hi + 4 * r, size=(2000, 3)))
+    pts.append(np.stack([np.full(200, lo[0] - 0.5 * r), rng.uniform(lo[1], hi[1], 200),
+                         rng.uniform(lo[2], hi[2], 200)], 1))
+    pts.append(np.stack([rng.uniform(lo[0], hi[0], 200), np.full(200, hi[1] + 0.5 * r),
+                         rng.uniform(lo[2], hi[2], 200)], 1))
+    q = np.concatenate(pts).astype(np.float32)
+    n = rng.normal(size=q.shape)
+    n /= np.linalg.norm(n, axis=1, keepdims=True)
+    return q, n.astype(np.float32)
+
+
+def orc_radius(orc):
+    return orc._settings.gather_radius
+
+
+def test_gather_points_bit_exact(c0_small):
+    _, gpu, orc = c0_small
+    gpu.render_forward()
+    orc.render_forward()
+    q, n = _gather_probe_points(orc, np.random.default_rng(5))
+    a = gpu.gather_irradiance(q, n)
+    b = orc.gather(q, n)
+    assert np.count_nonzero(b[:, 0]) > 1000
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
