@@ -73,13 +73,16 @@ def copy_calibration():
     return 0
 
 
-# Algorithmic bytes of the reference layouts per work unit (per query: ray
-# in + hit/next ray out 48 + 48 B, 24 B per AABB test, 8 B per popped node,
-# 52 B per surfel test; per gather: 4 B per cell start, 4 B per non-empty
-# cell end, 40 B per photon scanned).
-def algorithmic_bytes(queries: int, w: dict) -> float:
-    return (96.0 * queries + 24.0 * w["aabb"] + 8.0 * w["pops"] + 52.0 * w["prims"] + 4.0 * w["cells"]
-            + 4.0 * w["cells_nonempty"] + 40.0 * w["photons"])
+# Algorithmic bytes of the reference layouts per work unit.  Per closest-hit
+# query: ray in + hit/next ray out 48 + 48 B, 24 B per AABB test, 8 B per
+# popped node, 52 B per surfel test.  Per irradiance gather: 4 B per cell
+# start, 4 B per non-empty cell end, 40 B per photon scanned.
+def query_bytes(queries: int, w: dict) -> float:
+    return 96.0 * queries + 24.0 * w["aabb"] + 8.0 * w["pops"] + 52.0 * w["prims"]
+
+
+def gather_bytes(w: dict) -> float:
+    return 4.0 * w["cells"] + 4.0 * w["cells_nonempty"] + 40.0 * w["photons"]
 
 
 def load_traffic(path):
@@ -137,7 +140,7 @@ def main():
         print(f"[bench] warmup {i + 1}/{a.warmup} done", file=sys.stderr, flush=True)
 
     dev_ms, wall_ms, queries = [], [], []
-    gather_ms, gather_launches = 0.0, 0
+    gather_ms, gather_launches, trace_ms, trace_launches = 0.0, 0, 0.0, 0
     tw0 = time.time()
     for i in range(a.steps):
         ts = time.time()
@@ -147,6 +150,8 @@ def main():
         queries.append(st["photon_queries"] + st["gather_queries"] + st["adjoint_queries"])
         gather_ms += st["gather_ms"]
         gather_launches += st["gather_launches"]
+        trace_ms += st["camera_trace_ms"]
+        trace_launches += st["camera_trace_launches"]
         print(f"[bench] step {i + 1}/{a.steps}: device {dev_ms[-1]:.1f} ms, wall {wall_ms[-1]:.1f} ms, "
               f"{queries[-1] / 1e6:.1f} M queries", file=sys.stderr, flush=True)
     total_wall = time.time() - tw0
@@ -157,28 +162,38 @@ def main():
 
     roofline = None
     work = None
+    kernels = None
     if not a.no_count:
         r.set_count_work(True)
         cst = step()
         r.set_count_work(False)
         work = {k: cst[f"work_{k}"] for k in ("photon", "gather", "adjoint")}
         same = cst["gather_queries"] == last["gather_queries"]
-        bytes_gather = algorithmic_bytes(cst["gather_queries"], work["gather"]) + 16.0 * W * H * settings[
-            "gather_passes"]
-        per_launch = bytes_gather / max(1, cst["gather_launches"])
-        avg_launch_s = gather_ms / max(1, gather_launches) * 1e-3
-        achieved = per_launch / avg_launch_s / 1e9
-        traffic = load_traffic(os.path.join(ROOT, "profiles", "k_gather_c1_traffic.json"))
+        wg = work["gather"]
+        # Camera gather stages of the timed steps: the irradiance gathers over
+        # cell-sorted entries and the closest-hit layer walk.
+        kernels = {}
+        for name, ms, launches, nbytes, counted_launches in (
+                ("k_entry_gather", gather_ms, gather_launches, gather_bytes(wg), cst["gather_launches"]),
+                ("k_cam_trace", trace_ms, trace_launches, query_bytes(cst["gather_queries"], wg),
+                 cst["camera_trace_launches"])):
+            per_launch = nbytes / max(1, counted_launches)
+            avg_s = ms / max(1, launches) * 1e-3
+            kernels[name] = {"algorithmic_bytes_per_launch": per_launch, "avg_launch_ms": avg_s * 1e3,
+                             "achieved_gbs": per_launch / avg_s / 1e9, "total_ms_per_step": ms / a.steps}
+        dom = max(kernels, key=lambda k: kernels[k]["total_ms_per_step"])
+        kd = kernels[dom]
+        traffic = load_traffic(os.path.join(ROOT, "profiles", f"{dom}_c1_traffic.json"))
         roofline = {
-            "kernel": "k_gather (camera gather, one pass)",
+            "kernel": dom,
             "bound": "hbm",
-            "achieved": round(achieved, 1),
+            "achieved": round(kd["achieved_gbs"], 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "frac": round(kd["achieved_gbs"] / HBM_PEAK_GBS, 4),
             "traffic": traffic.get("bytes_per_launch") if traffic else None,
-            "algorithmic_bytes_per_launch": per_launch,
-            "avg_launch_ms": avg_launch_s * 1e3,
+            "algorithmic_bytes_per_launch": kd["algorithmic_bytes_per_launch"],
+            "avg_launch_ms": kd["avg_launch_ms"],
             "counted_step_matches_timed": bool(same),
         }
 
@@ -225,8 +240,10 @@ def main():
         "queries_per_step": int(q_step),
         "ms_per_step_wall": round(float(np.mean(wall_ms)), 3),
         "mrays_per_s_wall": round(q_step / (np.mean(wall_ms) * 1e-3) / 1e6, 2),
-        "phase_ms": {k: round(last[k], 3) for k in ("photon_trace_ms", "grid_ms", "gather_ms", "adjoint_trace_ms",
-                                                    "adjoint_transport_ms", "forward_ms", "backward_ms")},
+        "phase_ms": {k: round(last[k], 3) for k in ("photon_trace_ms", "grid_ms", "camera_trace_ms", "camera_sort_ms",
+                                                    "gather_ms", "adjoint_trace_ms", "adjoint_transport_ms",
+                                                    "forward_ms", "backward_ms")},
+        "camera_rounds": last["camera_rounds"],
         "host_bvh_build_s": round(build_s, 3),
         "timed_wall_s": round(total_wall, 3),
         "roofline": roofline,
@@ -234,6 +251,7 @@ def main():
     }
     if work is not None:
         out["work_per_step"] = work
+        out["kernels"] = kernels
     print(json.dumps(out))
     return 0
 

@@ -92,6 +92,7 @@ class Settings(C.Structure):
         ("q_transmit", C.c_float),
         ("spp_decorrelate", C.c_int32),
         ("debug_images", C.c_int32),
+        ("gather_entry_budget", C.c_uint32),
     ]
 
 
@@ -114,6 +115,10 @@ class Stats(C.Structure):
         ("gather_launches", C.c_uint32),
         ("adjoint_trace_launches", C.c_uint32),
         ("adjoint_transport_launches", C.c_uint32),
+        ("camera_trace_ms", C.c_double),
+        ("camera_sort_ms", C.c_double),
+        ("camera_trace_launches", C.c_uint32),
+        ("camera_rounds", C.c_uint32),
         ("work", (C.c_uint64 * 7) * 3),
     ]
 
@@ -145,6 +150,7 @@ DEFAULT_SETTINGS = dict(
     q_transmit=0.0,
     spp_decorrelate=0,
     debug_images=0,
+    gather_entry_budget=0,
 )
 
 

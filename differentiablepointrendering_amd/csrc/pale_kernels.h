@@ -22,6 +22,10 @@ struct DevState {
     uint32_t extTotal;     // survivors of the last bounce
     uint32_t depTotal;     // deposits of the last bounce
     uint32_t evTotal;      // gradient events of the last bounce
+    uint32_t entries;      // camera-gather entries appended this round
+    uint32_t overflow;     // the entry buffer filled this round
+    uint32_t evPool;       // adjoint events waiting for the transport
+    uint32_t pad0;
     unsigned long long photonQ, gatherQ, adjointQ, deposits, events, pad;
     // Counting mode, per kernel class (photon, gather, adjoint): popped
     // nodes, AABB tests, primitive tests, gathers, gather cells, non-empty
@@ -63,6 +67,32 @@ struct PendingArrays {
     uint2* p2;
 };
 
+// Camera gather, per pixel: ot = ray origin + throughput tr, dl = direction +
+// next layer (bits), cs = accumulated colour + status (bits), cnt = entries
+// of this round, off = their first slot in the ordered list.
+struct PixelArrays {
+    float4* ot;
+    float4* dl;
+    float4* cs;
+    uint32_t* cnt;
+    uint32_t* off;
+};
+
+// Camera-gather entries (one per layer): qa = gather point + normal.x,
+// qb = normal.y, normal.z, pixel (bits), index in the pixel's round (bits,
+// top bit = no gather); comp = colour factor k + weight w; E = irradiance;
+// ord = entries in (pixel, layer) order; cell / sorted = counting sort of
+// the gather entries by grid cell.
+struct EntryArrays {
+    float4* qa;
+    float4* qb;
+    float4* comp;
+    float4* E;
+    uint32_t* ord;
+    uint32_t* cell;
+    uint32_t* sorted;
+};
+
 struct TraceParams {
     uint32_t rrStart;
     float qNull, qReflect, qTransmit;
@@ -89,24 +119,42 @@ void launchCompactRays(hipStream_t s, const uint8_t* flags, const uint32_t* coun
 void launchCompactPhotons(hipStream_t s, const uint8_t* flags, const uint32_t* count, uint32_t maxItems,
                           uint32_t* tiles, uint32_t* total, const uint32_t* base, PhotonArrays src,
                           PhotonArrays dst, uint32_t dstCap);
+// Stable compaction of one bounce's events, appended to the event pool
+// (DevState::evPool items; this bounce's count goes to DevState::evTotal).
 void launchCompactEvents(hipStream_t s, const uint8_t* flags, const uint32_t* count, uint32_t maxItems,
-                         uint32_t* tiles, uint32_t* total, EventArrays src, EventArrays dst, uint32_t dstCap);
+                         uint32_t* tiles, DevState* st, EventArrays src, EventArrays pool, uint32_t poolCap);
 void launchFinishPhotonBounce(hipStream_t s, DevState* st, uint32_t cap);
 void launchSetQueue(hipStream_t s, DevState* st, const uint32_t* src);
 // Photon grid (counting sort by cell, stable in deposit order).
 void launchBuildGrid(hipStream_t s, const GridParams& g, DevState* st, uint32_t photonCap, PhotonArrays raw,
                      PhotonArrays sorted, uint32_t* cellId, uint32_t* sortedIdx, uint32_t* cellCount,
                      uint32_t* cellStart, uint32_t* tiles, uint32_t* scratchTotal);
-void launchGather(hipStream_t s, const DevScene& S, const DevGrid& G, const DevCamera& cam, DevState* st,
-                  uint32_t pass, int decorrelate, float invSpp, float4* fb, uint32_t* cursor, bool count);
+// Exclusive scan of n values into out[0..n] (out[n] = total).
+void launchExclusiveScan(hipStream_t s, const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* tiles,
+                         uint32_t* scratchTotal);
+// Camera gather of one pass (see pale_kernels.hip for the stages).
+void launchCamInit(hipStream_t s, const DevCamera& cam, const DevState* st, uint32_t pass, int decorrelate,
+                   PixelArrays px);
+void launchCamTrace(hipStream_t s, const DevScene& S, DevState* st, uint32_t npx, PixelArrays px, EntryArrays ent,
+                    uint32_t cap, uint32_t* cursor, bool count);
+void launchEntrySort(hipStream_t s, const GridParams& g, DevState* st, uint32_t cap, PixelArrays px,
+                     EntryArrays ent, uint32_t* cellCount, uint32_t* cellStart, uint32_t* tiles,
+                     uint32_t* scratchTotal);
+void launchEntryGather(hipStream_t s, const DevGrid& G, DevState* st, uint32_t cap, const uint32_t* nGather,
+                       EntryArrays ent, bool count);
+void launchCamComposite(hipStream_t s, uint32_t npx, PixelArrays px, EntryArrays ent, float invSpp, float4* fb);
 void launchPostprocess(hipStream_t s, const float4* fb, float4* ldr, uint32_t n);
 void launchAdjRaygen(hipStream_t s, const DevCamera& cam, const float* dloss, RayQueue q, PendingArrays pend,
                      DevState* st);
 void launchAdjTrace(hipStream_t s, const DevScene& S, DevState* st, uint32_t maxRays, uint32_t bounce, RayQueue in,
                     RayQueue staged, uint8_t* extFlags, EventArrays ev, uint8_t* evFlags, PendingArrays pend,
                     TraceParams tp, bool count);
-void launchAdjTransport(hipStream_t s, const DevScene& S, const DevGrid& G, DevState* st, uint32_t maxEvents,
-                        EventArrays ev, float invSpp, float* gradOpacity, bool count);
+// Transport of the pooled events (in cell order after launchEventSort).
+void launchEventSort(hipStream_t s, const GridParams& g, DevState* st, uint32_t poolCap, EventArrays ev,
+                     uint32_t* cell, uint32_t* sorted, uint32_t* cellCount, uint32_t* cellStart, uint32_t* tiles,
+                     uint32_t* scratchTotal);
+void launchAdjTransport(hipStream_t s, const DevScene& S, const DevGrid& G, DevState* st, uint32_t poolCap,
+                        EventArrays ev, const uint32_t* sorted, float invSpp, float* gradOpacity, bool count);
 void launchGatherPoints(hipStream_t s, const DevGrid& G, uint32_t n, const float* q, const float* nrm, float* out);
 void launchTraceRays(hipStream_t s, const DevScene& S, uint32_t n, const float* o, const float* d, int32_t* inst,
                      int32_t* prim, float* t, float* pos, float* alpha);

@@ -9,6 +9,7 @@
 // AdjointKernels.cpp:99-101).  Counts stay on the device; no kernel waits for
 // the host.
 #include <algorithm>
+#include <cstddef>
 
 #include "pale_kernels.h"
 
@@ -113,15 +114,24 @@ __global__ void k_seed_next(DevState* st, const uint32_t* table, uint32_t len, i
     st->live = 1;
 }
 
+// photonCount counts deposit attempts (the reference's counter T); only the
+// first `cap` are stored.
 __global__ void k_finish_photon_bounce(DevState* st, uint32_t cap) {
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    (void)cap;
     if (st->live) {
         st->qCount = st->extTotal;
-        uint32_t pc = st->photonCount + st->depTotal;
-        st->photonCount = pc < cap ? pc : cap;
+        st->photonCount += st->depTotal;
     } else {
         st->qCount = 0;
     }
+}
+
+// Events of one bounce were appended to the pool.
+__global__ void k_event_pool_add(DevState* st, uint32_t cap) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const uint32_t n = st->evPool + st->evTotal;
+    st->evPool = n < cap ? n : cap;
 }
 
 __global__ void k_set_queue(DevState* st, const uint32_t* src) {
@@ -337,9 +347,10 @@ __global__ __launch_bounds__(kScanThreads) void k_scatter_photons(const uint8_t*
 
 __global__ __launch_bounds__(kScanThreads) void k_scatter_events(const uint8_t* flags, const uint32_t* count,
                                                                  uint32_t maxItems, const uint32_t* tiles,
-                                                                 EventArrays src, EventArrays dst, uint32_t cap) {
+                                                                 const uint32_t* base, EventArrays src,
+                                                                 EventArrays dst, uint32_t cap) {
     const uint32_t n = loadCount(count, maxItems);
-    scatterTile(flags, n, tiles, 0u, cap, [&](uint32_t i, uint32_t j) {
+    scatterTile(flags, n, tiles, *base, cap, [&](uint32_t i, uint32_t j) {
         dst.e0[j] = src.e0[i];
         dst.e1[j] = src.e1[i];
         dst.e2[j] = src.e2[i];
@@ -436,26 +447,57 @@ __global__ __launch_bounds__(256) void k_cell_payload(const DevState* st, uint32
 }
 
 // ---------------------------------------------------- camera gather --
-// launchCameraGatherKernel (PrimalKernels.cpp:500-653): one pass, run by a
-// persistent grid.  A lane walks one pixel's surfel layers (one closest-hit
-// query + one irradiance gather per layer); when its pixel is finished it
-// takes the next pixel from the pass's cursor (one atomic per wave refill),
-// so no lane waits for the wave's deepest pixel.  Pixel arithmetic and the
-// framebuffer update are the reference's, so the image is unchanged.
+// launchCameraGatherKernel (PrimalKernels.cpp:500-653), split in stages so
+// that the irradiance gathers, which dominate the pass, run in grid-cell
+// order instead of pixel order:
+//
+//   k_cam_init      primary ray of every pixel (the reference's jitter);
+//   k_cam_trace     persistent lanes walk each pixel's layers (closest-hit
+//                   queries only) and append one entry per layer: the gather
+//                   point and normal, the layer's weight w and colour factor
+//                   k, in layer order within the pixel;
+//   k_entry_key     entry -> slot of its pixel's ordered list, and a
+//                   counting sort of the gather entries by photon-grid cell;
+//   k_entry_gather  one irradiance gather per entry, neighbouring lanes on
+//                   neighbouring points (shared photon rows, similar loops);
+//   k_cam_composite per pixel, C = C + w * (E * k) over the layers in order,
+//                   then the reference's framebuffer update.
+//
+// Every value is computed by the same expressions as in the reference's
+// single loop, and the per-pixel sums run in layer order, so the image is
+// bit-identical.  If the entry buffer fills, a pixel stops before the layer
+// that did not fit and resumes from there in the next round.
+enum : uint32_t { kPixActive = 0u, kPixFinished = 1u, kPixFlushed = 2u };
+constexpr uint32_t kEntryNoGather = 0x80000000u;
+
+__global__ __launch_bounds__(256) void k_cam_init(DevCamera cam, const DevState* st, uint32_t pass, int decorrelate,
+                                                  PixelArrays px) {
+    const uint32_t npx = cam.W * cam.H;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npx) return;
+    const uint32_t seed = decorrelate ? st->number + pass : st->number;
+    Rng rng(perItemSeed(seed, i));
+    const float jx = rng.nextFloat() - 0.5f;
+    const float jy = rng.nextFloat() - 0.5f;
+    F3 o, d;
+    primaryRay(cam, (float)(i % cam.W), (float)(i / cam.W), jx, jy, o, d);
+    px.ot[i] = f4(o, 1.0f);
+    px.dl[i] = f4(d, u2f(0u));
+    px.cs[i] = make_float4(0.0f, 0.0f, 0.0f, u2f(kPixActive));
+    px.cnt[i] = 0u;
+}
+
 template <bool kCount>
-__global__ __launch_bounds__(kTraceBlock) void k_gather(DevScene S, DevGrid G, DevCamera cam, DevState* st,
-                                                        uint32_t pass, int decorrelate, float invSpp, float4* fb,
-                                                        uint32_t* cursor) {
+__global__ __launch_bounds__(kTraceBlock) void k_cam_trace(DevScene S, DevState* st, uint32_t npx, PixelArrays px,
+                                                           EntryArrays ent, uint32_t cap, uint32_t* cursor) {
     extern __shared__ uint2 s_stack[];
     const LaneStack stk = laneStack(s_stack, S.stackDepth);
-    const uint32_t npx = cam.W * cam.H;
-    const uint32_t seed = decorrelate ? st->number + pass : st->number;
     const uint32_t lane = laneId();
     uint32_t layers = 0;
     Work w{};
     bool active = false, exhausted = false;
-    uint32_t pixel = 0;
-    F3 o = mk3(0, 0, 0), d = mk3(0, 0, 0), C = mk3(0, 0, 0);
+    uint32_t pixel = 0, r = 0, layer = 0;
+    F3 o = mk3(0, 0, 0), d = mk3(0, 0, 0);
     float tr = 1.0f;
     while (true) {
         if (!exhausted) {
@@ -465,62 +507,193 @@ __global__ __launch_bounds__(kTraceBlock) void k_gather(DevScene S, DevGrid G, D
                 uint32_t base = 0;
                 if (lane == leader) base = atomicAdd(cursor, (uint32_t)__popcll(need));
                 base = __shfl(base, (int)leader, 64);
-                if (base + (uint32_t)__popcll(need) >= npx) exhausted = true;
+                if (base + (uint32_t)__popcll(need) >= npx ||
+                    *reinterpret_cast<volatile const uint32_t*>(&st->overflow) != 0u)
+                    exhausted = true;
                 if (!active) {
                     const uint32_t p = base + (uint32_t)__popcll(need & ((1ull << lane) - 1ull));
-                    if (p < npx) {
+                    if (p < npx && f2u(px.cs[p].w) == kPixActive) {
                         pixel = p;
                         active = true;
-                        Rng rng(perItemSeed(seed, pixel));
-                        const float jx = rng.nextFloat() - 0.5f;
-                        const float jy = rng.nextFloat() - 0.5f;
-                        primaryRay(cam, (float)(pixel % cam.W), (float)(pixel / cam.W), jx, jy, o, d);
-                        C = mk3(0, 0, 0);
-                        tr = 1.0f;
+                        const float4 a = px.ot[p], b = px.dl[p];
+                        o = mk3(a.x, a.y, a.z);
+                        tr = a.w;
+                        d = mk3(b.x, b.y, b.z);
+                        layer = f2u(b.w);
+                        r = 0;
                     }
                 }
             }
         }
         if (!__any(active)) break;
+        // One layer: closest hit, then the entry it contributes.
+        bool want = false, terminal = true, gather = false;
+        F3 q = mk3(0, 0, 0), n = mk3(0, 0, 0), k = mk3(0, 0, 0), nextO = o;
+        float wgt = 0.0f, nextTr = tr;
+        Work wl{};
         if (active) {
             Hit h;
-            ++layers;
-            bool done = true;
-            if (intersectScene<kCount>(S, o, d, stk, h, w)) {
+            if (intersectScene<kCount>(S, o, d, stk, h, wl)) {
                 const DevInstance& I = S.inst[h.inst];
+                want = true;
                 if (I.type == kGeoCloud) {
                     const SurfelShade sh = S.shade[h.prim];
-                    F3 n = mk3(sh.nsx, sh.nsy, sh.nsz);
+                    n = mk3(sh.nsx, sh.nsy, sh.nsz);
                     if (dot(n, -d) < 0.0f) n = -n;
-                    const F3 E = gatherIrradiance<kCount>(G, h.pos, n, w);
-                    const F3 Lo = E * ((sh.alpha_r * mk3(sh.ar, sh.ag, sh.ab)) * kInvPi);
+                    q = h.pos;
+                    gather = true;
+                    k = (sh.alpha_r * mk3(sh.ar, sh.ag, sh.ab)) * kInvPi;
                     const float ae = sh.opacity * h.alpha;
-                    C = C + (tr * ae) * Lo;
-                    tr *= (1.0f - ae);
-                    o = h.pos + d * 1e-4f;
-                    done = false;
+                    wgt = tr * ae;
+                    nextTr = tr * (1.0f - ae);
+                    nextO = h.pos + d * 1e-4f;
+                    terminal = false;
                 } else {
                     const DevMaterial m = S.mats[I.material];
                     const F3 base = mk3(m.r, m.g, m.b);
+                    wgt = tr;
                     if (m.power > 0.0f) {
-                        C = C + tr * vmin(base * m.power, mk3(1.0f, 1.0f, 1.0f));
+                        k = vmin(base * m.power, mk3(1.0f, 1.0f, 1.0f));
                     } else {
-                        const F3 E = gatherIrradiance<kCount>(G, h.pos, hitNormal(S, h), w);
-                        C = C + tr * ((base * kInvPi) * E);
+                        q = h.pos;
+                        n = hitNormal(S, h);
+                        gather = true;
+                        k = base * kInvPi;
                     }
                 }
             }
-            if (done) {
-                const float4 f = fb[pixel];
-                fb[pixel] = make_float4(C.x * invSpp + f.x, C.y * invSpp + f.y, C.z * invSpp + f.z,
-                                        1.0f * invSpp + f.w);
+        }
+        // Wave-aggregated append of the entries of this iteration.
+        const unsigned long long m = __ballot(want);
+        uint32_t e = 0xFFFFFFFFu;
+        if (m) {
+            const uint32_t leader = (uint32_t)__ffsll((long long)m) - 1u;
+            uint32_t base = 0;
+            if (lane == leader) base = atomicAdd(&st->entries, (uint32_t)__popcll(m));
+            base = __shfl(base, (int)leader, 64);
+            if (want) e = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+        }
+        bool full = false;
+        if (active) {
+            bool finished = false;
+            if (want && e >= cap) {
+                // No room: stop before this layer, resume next round.
+                px.ot[pixel] = f4(o, tr);
+                px.dl[pixel] = f4(d, u2f(layer));
+                st->overflow = 1u;
+                px.cnt[pixel] = r;
+                active = false;
+                full = true;
+            } else {
+                ++layers;
+                if (kCount) {
+                    w.pops += wl.pops;
+                    w.aabb += wl.aabb;
+                    w.prims += wl.prims;
+                }
+                if (want) {
+                    ent.qa[e] = f4(q, n.x);
+                    ent.qb[e] = make_float4(n.y, n.z, u2f(pixel), u2f(r | (gather ? 0u : kEntryNoGather)));
+                    ent.comp[e] = f4(k, wgt);
+                    if (!gather) ent.E[e] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+                    ++r;
+                    ++layer;
+                }
+                if (terminal) {
+                    finished = true;
+                } else {
+                    o = nextO;
+                    tr = nextTr;
+                }
+            }
+            if (finished) {
+                px.cnt[pixel] = r;
+                px.cs[pixel].w = u2f(kPixFinished);
                 active = false;
             }
         }
+        if (__any(full)) exhausted = true;
     }
     const uint32_t ws = waveSum(layers);
     if (lane == 0 && ws) atomicAdd(&st->gatherQ, (unsigned long long)ws);
     if (kCount) flushWork(w, st, kWorkGather);
+}
+
+// Slot of every entry in its pixel's ordered list; grid cell and per-cell
+// count of the gather entries (the cell of worldToCellClamped,
+// KernelHelpers.h:920-934).
+__global__ __launch_bounds__(256) void k_entry_key(GridParams g, const DevState* st, uint32_t cap, PixelArrays px,
+                                                   EntryArrays ent, uint32_t* cellCount) {
+    const uint32_t n = min(st->entries, cap);
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 b = ent.qb[i];
+    const uint32_t tag = f2u(b.w);
+    ent.ord[px.off[f2u(b.z)] + (tag & ~kEntryNoGather)] = i;
+    uint32_t c = 0xFFFFFFFFu;
+    if (!(tag & kEntryNoGather)) {
+        const float4 a = ent.qa[i];
+        const F3 f = divv(mk3(a.x, a.y, a.z) - mk3(g.ox, g.oy, g.oz), mk3(g.cell, g.cell, g.cell));
+        const int x = min(max((int)floorf(f.x), 0), g.rx - 1);
+        const int y = min(max((int)floorf(f.y), 0), g.ry - 1);
+        const int z = min(max((int)floorf(f.z), 0), g.rz - 1);
+        c = (uint32_t)((z * g.ry + y) * g.rx + x);
+        atomicAdd(&cellCount[c], 1u);
+    }
+    ent.cell[i] = c;
+}
+
+// Counting-sort scatter of the gather entries; leaves cellCount zeroed.
+__global__ __launch_bounds__(256) void k_entry_scatter(const DevState* st, uint32_t cap, const uint32_t* cellStart,
+                                                       uint32_t* cellCount, EntryArrays ent) {
+    const uint32_t n = min(st->entries, cap);
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t c = ent.cell[i];
+    if (c == 0xFFFFFFFFu) return;
+    ent.sorted[cellStart[c] + atomicSub(&cellCount[c], 1u) - 1u] = i;
+}
+
+template <bool kCount>
+__global__ __launch_bounds__(256) void k_entry_gather(DevGrid G, DevState* st, const uint32_t* nGather,
+                                                      EntryArrays ent) {
+    const uint32_t n = *nGather;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    Work w{};
+    if (i < n) {
+        const uint32_t e = ent.sorted[i];
+        const float4 a = ent.qa[e], b = ent.qb[e];
+        const F3 E = gatherIrradiance<kCount>(G, mk3(a.x, a.y, a.z), mk3(a.w, b.x, b.y), w);
+        ent.E[e] = f4(E, 0.0f);
+    }
+    if (kCount) flushWork(w, st, kWorkGather);
+}
+
+__global__ __launch_bounds__(256) void k_cam_composite(uint32_t npx, PixelArrays px, EntryArrays ent, float invSpp,
+                                                       float4* fb) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npx) return;
+    const float4 cs = px.cs[p];
+    const uint32_t status = f2u(cs.w);
+    if (status == kPixFlushed) return;
+    F3 C = mk3(cs.x, cs.y, cs.z);
+    const uint32_t first = px.off[p], cnt = px.cnt[p];
+    for (uint32_t j = 0; j < cnt; ++j) {
+        const uint32_t e = ent.ord[first + j];
+        const float4 c = ent.comp[e], E = ent.E[e];
+        const F3 Lo = mk3(E.x, E.y, E.z) * mk3(c.x, c.y, c.z);
+        C = C + c.w * Lo;
+    }
+    // The next round's slot scan must see zero for every pixel it did not
+    // touch, finished ones included.
+    px.cnt[p] = 0u;
+    if (status == kPixFinished) {
+        const float4 f = fb[p];
+        fb[p] = make_float4(C.x * invSpp + f.x, C.y * invSpp + f.y, C.z * invSpp + f.z, 1.0f * invSpp + f.w);
+        px.cs[p] = make_float4(C.x, C.y, C.z, u2f(kPixFlushed));
+    } else {
+        px.cs[p] = make_float4(C.x, C.y, C.z, u2f(kPixActive));
+    }
 }
 
 // launchPostProcessKernel (PostprocessKernel.cpp:11-106), float LDR output.
@@ -704,15 +877,44 @@ __global__ __launch_bounds__(kTraceBlock) void k_adj_trace(DevScene S, DevState*
     if (kCount) flushWork(w, st, kWorkAdjoint);
 }
 
+// Counting sort of the pooled events by the photon-grid cell of their
+// gather point, so the transport's irradiance gathers run in cell order.
+__global__ __launch_bounds__(256) void k_event_key(GridParams g, const DevState* st, uint32_t cap, EventArrays ev,
+                                                   uint32_t* cell, uint32_t* cellCount) {
+    const uint32_t n = min(st->evPool, cap);
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 a = ev.e0[i];
+    const F3 f = divv(mk3(a.x, a.y, a.z) - mk3(g.ox, g.oy, g.oz), mk3(g.cell, g.cell, g.cell));
+    const int x = min(max((int)floorf(f.x), 0), g.rx - 1);
+    const int y = min(max((int)floorf(f.y), 0), g.ry - 1);
+    const int z = min(max((int)floorf(f.z), 0), g.rz - 1);
+    const uint32_t c = (uint32_t)((z * g.ry + y) * g.rx + x);
+    cell[i] = c;
+    atomicAdd(&cellCount[c], 1u);
+}
+
+__global__ __launch_bounds__(256) void k_event_scatter(const DevState* st, uint32_t cap, const uint32_t* cell,
+                                                       const uint32_t* cellStart, uint32_t* cellCount,
+                                                       uint32_t* sorted) {
+    const uint32_t n = min(st->evPool, cap);
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t c = cell[i];
+    sorted[cellStart[c] + atomicSub(&cellCount[c], 1u) - 1u] = i;
+}
+
 // launchAdjointTransportKernel (AdjointKernels.cpp:389-509).
 template <bool kCount>
-__global__ __launch_bounds__(256) void k_adj_transport(DevScene S, DevGrid G, DevState* st, uint32_t maxEvents,
-                                                       EventArrays ev, float invSpp, float* grad) {
-    const uint32_t n = min(st->evTotal, maxEvents);
-    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool active = j < n;
+__global__ __launch_bounds__(256) void k_adj_transport(DevScene S, DevGrid G, DevState* st, uint32_t poolCap,
+                                                       EventArrays ev, const uint32_t* sorted, float invSpp,
+                                                       float* grad) {
+    const uint32_t n = min(st->evPool, poolCap);
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool active = i < n;
     Work w{};
     if (active) {
+        const uint32_t j = sorted[i];
         const float4 e0 = ev.e0[j], e1 = ev.e1[j], e2 = ev.e2[j], e3 = ev.e3[j];
         const uint32_t kind = f2u(e0.w);
         const uint32_t target = f2u(e1.w);
@@ -835,11 +1037,14 @@ void launchCompactPhotons(hipStream_t s, const uint8_t* flags, const uint32_t* c
     k_scatter_photons<<<nt, kScanThreads, 0, s>>>(flags, count, maxItems, tiles, base, src, dst, dstCap);
 }
 void launchCompactEvents(hipStream_t s, const uint8_t* flags, const uint32_t* count, uint32_t maxItems,
-                         uint32_t* tiles, uint32_t* total, EventArrays src, EventArrays dst, uint32_t dstCap) {
+                         uint32_t* tiles, DevState* st, EventArrays src, EventArrays pool, uint32_t poolCap) {
     const uint32_t nt = blocks(maxItems, kTile);
+    uint32_t* total = reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(st) + offsetof(DevState, evTotal));
+    const uint32_t* base = reinterpret_cast<const uint32_t*>(reinterpret_cast<char*>(st) + offsetof(DevState, evPool));
     k_tile_count<<<nt, kScanThreads, 0, s>>>(flags, count, maxItems, tiles);
-    k_tile_scan<<<1, 1024, 0, s>>>(tiles, count, 0u, maxItems, total, dstCap);
-    k_scatter_events<<<nt, kScanThreads, 0, s>>>(flags, count, maxItems, tiles, src, dst, dstCap);
+    k_tile_scan<<<1, 1024, 0, s>>>(tiles, count, 0u, maxItems, total, 0xFFFFFFFFu);
+    k_scatter_events<<<nt, kScanThreads, 0, s>>>(flags, count, maxItems, tiles, base, src, pool, poolCap);
+    k_event_pool_add<<<1, 64, 0, s>>>(st, poolCap);
 }
 void launchFinishPhotonBounce(hipStream_t s, DevState* st, uint32_t cap) {
     k_finish_photon_bounce<<<1, 64, 0, s>>>(st, cap);
@@ -860,9 +1065,9 @@ void launchBuildGrid(hipStream_t s, const GridParams& g, DevState* st, uint32_t 
     k_cell_payload<<<pb, 256, 0, s>>>(st, cap, sortedIdx, raw, sorted);
 }
 
-// Persistent grid of the gather: as many workgroups as fit on the device.
+// Persistent grid of the camera trace: as many workgroups as fit on the device.
 template <bool kCount>
-static uint32_t gatherGrid(size_t lds) {
+static uint32_t camTraceGrid(size_t lds) {
     static uint32_t cached[2] = {0, 0};
     static size_t cachedLds[2] = {0, 0};
     const int k = kCount ? 1 : 0;
@@ -870,25 +1075,62 @@ static uint32_t gatherGrid(size_t lds) {
     int dev = 0, cus = 0, perCu = 0;
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, k_gather<kCount>, kTraceBlock, lds);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, k_cam_trace<kCount>, kTraceBlock, lds);
     cached[k] = (uint32_t)std::max(1, cus) * (uint32_t)std::max(1, perCu);
     cachedLds[k] = lds;
     return cached[k];
 }
 
-void launchGather(hipStream_t s, const DevScene& S, const DevGrid& G, const DevCamera& cam, DevState* st,
-                  uint32_t pass, int decorrelate, float invSpp, float4* fb, uint32_t* cursor, bool count) {
+void launchExclusiveScan(hipStream_t s, const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* tiles,
+                         uint32_t* scratchTotal) {
+    if (!n) return;
+    const uint32_t nt = blocks(n, kTile);
+    k_scan_local<<<nt, kScanThreads, 0, s>>>(in, out, n, tiles);
+    k_tile_scan<<<1, 1024, 0, s>>>(tiles, nullptr, n, n, scratchTotal, 0xFFFFFFFFu);
+    k_scan_add<<<blocks(n, 256), 256, 0, s>>>(out, n, tiles, scratchTotal);
+}
+
+void launchCamInit(hipStream_t s, const DevCamera& cam, const DevState* st, uint32_t pass, int decorrelate,
+                   PixelArrays px) {
     const uint32_t npx = cam.W * cam.H;
+    if (npx) k_cam_init<<<blocks(npx, 256), 256, 0, s>>>(cam, st, pass, decorrelate, px);
+}
+
+void launchCamTrace(hipStream_t s, const DevScene& S, DevState* st, uint32_t npx, PixelArrays px, EntryArrays ent,
+                    uint32_t cap, uint32_t* cursor, bool count) {
     if (!npx) return;
     const size_t lds = stackBytes(S);
     if (count) {
-        const uint32_t g = min(gatherGrid<true>(lds), blocks(npx, kTraceBlock));
-        k_gather<true><<<g, kTraceBlock, lds, s>>>(S, G, cam, st, pass, decorrelate, invSpp, fb, cursor);
+        const uint32_t g = min(camTraceGrid<true>(lds), blocks(npx, kTraceBlock));
+        k_cam_trace<true><<<g, kTraceBlock, lds, s>>>(S, st, npx, px, ent, cap, cursor);
     } else {
-        const uint32_t g = min(gatherGrid<false>(lds), blocks(npx, kTraceBlock));
-        k_gather<false><<<g, kTraceBlock, lds, s>>>(S, G, cam, st, pass, decorrelate, invSpp, fb, cursor);
+        const uint32_t g = min(camTraceGrid<false>(lds), blocks(npx, kTraceBlock));
+        k_cam_trace<false><<<g, kTraceBlock, lds, s>>>(S, st, npx, px, ent, cap, cursor);
     }
 }
+
+void launchEntrySort(hipStream_t s, const GridParams& g, DevState* st, uint32_t cap, PixelArrays px,
+                     EntryArrays ent, uint32_t* cellCount, uint32_t* cellStart, uint32_t* tiles,
+                     uint32_t* scratchTotal) {
+    if (!cap) return;
+    k_entry_key<<<blocks(cap, 256), 256, 0, s>>>(g, st, cap, px, ent, cellCount);
+    launchExclusiveScan(s, cellCount, cellStart, g.total, tiles, scratchTotal);
+    k_entry_scatter<<<blocks(cap, 256), 256, 0, s>>>(st, cap, cellStart, cellCount, ent);
+}
+
+void launchEntryGather(hipStream_t s, const DevGrid& G, DevState* st, uint32_t cap, const uint32_t* nGather,
+                       EntryArrays ent, bool count) {
+    if (!cap) return;
+    if (count)
+        k_entry_gather<true><<<blocks(cap, 256), 256, 0, s>>>(G, st, nGather, ent);
+    else
+        k_entry_gather<false><<<blocks(cap, 256), 256, 0, s>>>(G, st, nGather, ent);
+}
+
+void launchCamComposite(hipStream_t s, uint32_t npx, PixelArrays px, EntryArrays ent, float invSpp, float4* fb) {
+    if (npx) k_cam_composite<<<blocks(npx, 256), 256, 0, s>>>(npx, px, ent, invSpp, fb);
+}
+
 void launchPostprocess(hipStream_t s, const float4* fb, float4* ldr, uint32_t n) {
     if (n) k_post<<<blocks(n, 256), 256, 0, s>>>(fb, ldr, n);
 }
@@ -909,13 +1151,22 @@ void launchAdjTrace(hipStream_t s, const DevScene& S, DevState* st, uint32_t max
         k_adj_trace<false><<<g, kTraceBlock, stackBytes(S), s>>>(S, st, bounce, in, staged, extFlags, ev, evFlags,
                                                                  pend, maxRays, tp);
 }
-void launchAdjTransport(hipStream_t s, const DevScene& S, const DevGrid& G, DevState* st, uint32_t maxEvents,
-                        EventArrays ev, float invSpp, float* gradOpacity, bool count) {
-    if (!maxEvents) return;
+void launchEventSort(hipStream_t s, const GridParams& g, DevState* st, uint32_t poolCap, EventArrays ev,
+                     uint32_t* cell, uint32_t* sorted, uint32_t* cellCount, uint32_t* cellStart, uint32_t* tiles,
+                     uint32_t* scratchTotal) {
+    if (!poolCap) return;
+    k_event_key<<<blocks(poolCap, 256), 256, 0, s>>>(g, st, poolCap, ev, cell, cellCount);
+    launchExclusiveScan(s, cellCount, cellStart, g.total, tiles, scratchTotal);
+    k_event_scatter<<<blocks(poolCap, 256), 256, 0, s>>>(st, poolCap, cell, cellStart, cellCount, sorted);
+}
+void launchAdjTransport(hipStream_t s, const DevScene& S, const DevGrid& G, DevState* st, uint32_t poolCap,
+                        EventArrays ev, const uint32_t* sorted, float invSpp, float* gradOpacity, bool count) {
+    if (!poolCap) return;
     if (count)
-        k_adj_transport<true><<<blocks(maxEvents, 256), 256, 0, s>>>(S, G, st, maxEvents, ev, invSpp, gradOpacity);
+        k_adj_transport<true><<<blocks(poolCap, 256), 256, 0, s>>>(S, G, st, poolCap, ev, sorted, invSpp, gradOpacity);
     else
-        k_adj_transport<false><<<blocks(maxEvents, 256), 256, 0, s>>>(S, G, st, maxEvents, ev, invSpp, gradOpacity);
+        k_adj_transport<false><<<blocks(poolCap, 256), 256, 0, s>>>(S, G, st, poolCap, ev, sorted, invSpp,
+                                                                   gradOpacity);
 }
 void launchTraceRays(hipStream_t s, const DevScene& S, uint32_t n, const float* o, const float* d, int32_t* inst,
                      int32_t* prim, float* t, float* pos, float* alpha) {

@@ -37,16 +37,17 @@ struct StatusError : std::runtime_error {
 struct DevMem {
     void* p = nullptr;
     size_t bytes = 0;
+    bool pinned = false;  // host memory from hipHostMalloc
     DevMem() = default;
     DevMem(const DevMem&) = delete;
     DevMem& operator=(const DevMem&) = delete;
-    DevMem(DevMem&& o) noexcept : p(o.p), bytes(o.bytes) {
+    DevMem(DevMem&& o) noexcept : p(o.p), bytes(o.bytes), pinned(o.pinned) {
         o.p = nullptr;
         o.bytes = 0;
     }
     ~DevMem() { release(); }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p) (void)(pinned ? hipHostFree(p) : hipFree(p));
         p = nullptr;
         bytes = 0;
     }
@@ -105,18 +106,23 @@ struct pale_handle {
     uint32_t queueCap = 0;
     DevMem qA[3], qB[3], stg[3], extFlags, depStg[3], depFlags, tiles, counts;
     // Adjoint.
-    uint32_t adjCap = 0;
-    DevMem pend0, pend1, pend2, evStg[4], evCmp[4], evFlags, dloss, gradOpacity;
+    uint32_t adjCap = 0, poolCap = 0;
+    DevMem pend0, pend1, pend2, evStg[4], evCmp[4], evCell, evSorted, evFlags, dloss, gradOpacity;
+    // Camera gather: pixel state, entries of one round, their cell sort.
+    uint32_t pxCap = 0, entryCap = 0;
+    DevMem pxOt, pxDl, pxCs, pxCnt, pxOff, pxTiles, entQa, entQb, entComp, entE, entOrd, entCell, entSorted,
+        entStart, flagHost;
     // Device state and seed table.
     DevMem state, seedTable, cursors;
     // Per-camera framebuffers.
     std::vector<DevMem> fb, ldr;
     pale_stats stats{};
     bool countWork = false;
+    uint32_t camRounds = 0;
 
     // Per-kernel-class device timing: event pairs recorded around launches on
     // the handle's stream, read after the call's final synchronisation.
-    enum TimedClass { kTPhoton = 0, kTGather, kTAdjTrace, kTTransport, kTGrid, kTClasses };
+    enum TimedClass { kTPhoton = 0, kTGather, kTAdjTrace, kTTransport, kTGrid, kTCamTrace, kTCamSort, kTClasses };
     std::vector<hipEvent_t> evPool;
     std::vector<int> evClass;
     size_t evUsed = 0;
@@ -250,12 +256,13 @@ struct pale_handle {
     void setScene() {
         ensureQueues(nextPow2(std::max(S.photons, 1u)));
         if (!gridConfigured) {
-            const uint64_t requested = (uint64_t)S.photons * (uint64_t)(S.forward_passes * S.bounces);
+            // allocatePhotonMap (PathTracer.cpp:123-157): photons * passes *
+            // bounces, clamped to 10 GiB of 64-byte reference records.
+            const uint64_t requested = (uint64_t)S.photons * (uint64_t)S.forward_passes * (uint64_t)S.bounces;
             const uint64_t maxPhotons = (10ull * 1024ull * 1024ull * 1024ull) / 64ull;
-            if (requested > maxPhotons)
-                throw StatusError(PALE_ERR_INVALID_ARG,
-                                  "photons * forward_passes * bounces exceeds the 10 GiB photon-map limit");
-            photonCap = (uint32_t)requested;
+            photonCap = (uint32_t)std::min(requested, maxPhotons);
+            if ((uint64_t)S.photons * (uint64_t)S.forward_passes > 0xFFFFFFFFull)
+                throw StatusError(PALE_ERR_INVALID_ARG, "photons * forward_passes exceeds 2^32");
             // configurePhotonGrid (PathTracer.cpp:184-221)
             radius = S.gather_radius;
             const float cell = 0.5f * radius;
@@ -314,14 +321,20 @@ struct pale_handle {
 
     void ensureAdjoint(uint32_t npx) {
         ensureQueues(nextPow2(npx));
+        entStart.reserve(((size_t)grid.total + 1) * 4);
         if (npx <= adjCap) return;
         pend0.reserve((size_t)npx * sizeof(float4));
         pend1.reserve((size_t)npx * sizeof(float4));
         pend2.reserve((size_t)npx * sizeof(uint2));
+        // Event pool: the events of up to 24 bounces of npx paths wait for
+        // one cell-sorted transport launch.
+        poolCap = (uint32_t)std::min<uint64_t>((uint64_t)npx * 24u, 1u << 28);
         for (int k = 0; k < 4; ++k) {
             evStg[k].reserve((size_t)npx * sizeof(float4));
-            evCmp[k].reserve((size_t)npx * sizeof(float4));
+            evCmp[k].reserve((size_t)poolCap * sizeof(float4));
         }
+        evCell.reserve((size_t)poolCap * 4);
+        evSorted.reserve((size_t)poolCap * 4);
         evFlags.reserve(npx);
         adjCap = npx;
     }
@@ -343,6 +356,10 @@ struct pale_handle {
             stats.grid_ms = classMs[kTGrid];
             stats.photon_trace_launches = classLaunches[kTPhoton];
             stats.gather_launches = classLaunches[kTGather];
+            stats.camera_trace_ms = classMs[kTCamTrace];
+            stats.camera_sort_ms = classMs[kTCamSort];
+            stats.camera_trace_launches = classLaunches[kTCamTrace];
+            stats.camera_rounds = camRounds;
         } else {
             stats.adjoint_queries = h.adjointQ;
             stats.gradient_events = h.events;
@@ -354,6 +371,85 @@ struct pale_handle {
         for (int c = 0; c < 3; ++c)
             for (int u = 0; u < 7; ++u) stats.work[c][u] = h.work[c][u];
         stats.grid_cells = grid.total;
+    }
+
+    // Camera-gather buffers for images of up to npx pixels.  The entry
+    // buffer holds gather_entry_budget (default 24) layers per pixel on
+    // average: one round in practice; deeper images take more rounds.
+    void ensureCamera(uint32_t npx) {
+        if (npx > pxCap) {
+            pxOt.reserve((size_t)npx * 16);
+            pxDl.reserve((size_t)npx * 16);
+            pxCs.reserve((size_t)npx * 16);
+            pxCnt.reserve((size_t)npx * 4);
+            pxOff.reserve(((size_t)npx + 1) * 4);
+            pxTiles.reserve(((size_t)npx / kTile + 2) * 4);
+            pxCap = npx;
+        }
+        const uint64_t perPixel = S.gather_entry_budget ? S.gather_entry_budget : 24u;
+        const uint64_t floor = S.gather_entry_budget ? 64u : (1u << 16);
+        const uint32_t want = (uint32_t)std::min<uint64_t>(std::max<uint64_t>((uint64_t)npx * perPixel, floor), 1u << 28);
+        if (want != entryCap) {
+            entQa.reserve((size_t)want * 16);
+            entQb.reserve((size_t)want * 16);
+            entComp.reserve((size_t)want * 16);
+            entE.reserve((size_t)want * 16);
+            entOrd.reserve((size_t)want * 4);
+            entCell.reserve((size_t)want * 4);
+            entSorted.reserve((size_t)want * 4);
+            entryCap = want;
+        }
+        entStart.reserve(((size_t)grid.total + 1) * 4);
+        if (!flagHost.p) {
+            HIPCHECK(hipHostMalloc(&flagHost.p, 16, hipHostMallocDefault));
+            flagHost.bytes = 16;
+            flagHost.pinned = true;
+        }
+    }
+    PixelArrays pxa() const {
+        return PixelArrays{pxOt.as<float4>(), pxDl.as<float4>(), pxCs.as<float4>(), pxCnt.as<uint32_t>(),
+                           pxOff.as<uint32_t>()};
+    }
+    EntryArrays enta() const {
+        return EntryArrays{entQa.as<float4>(),    entQb.as<float4>(),    entComp.as<float4>(),
+                           entE.as<float4>(),     entOrd.as<uint32_t>(), entCell.as<uint32_t>(),
+                           entSorted.as<uint32_t>()};
+    }
+
+    // One gather pass of one camera (launchCameraGatherKernel,
+    // PrimalKernels.cpp:500-653), in rounds until every pixel is finished.
+    void cameraPass(const DevCamera& dc, uint32_t pass, float invSpp, float4* fbp) {
+        const uint32_t npx = dc.W * dc.H;
+        DevState* st = this->st();
+        const PixelArrays px = pxa();
+        const EntryArrays ent = enta();
+        const DevGrid G = devGrid();
+        uint32_t* entries = stField(offsetof(DevState, entries));
+        launchCamInit(stream, dc, st, pass, S.spp_decorrelate, px);
+        for (uint32_t round = 0;; ++round) {
+            launchFill(stream, entries, 2, 0u);  // entries, overflow
+            launchFill(stream, cursors.as<uint32_t>(), 1, 0u);
+            const size_t tt = tBegin(kTCamTrace);
+            launchCamTrace(stream, scene, st, npx, px, ent, entryCap, cursors.as<uint32_t>(), countWork);
+            tEnd(tt);
+            const size_t ts = tBegin(kTCamSort);
+            launchExclusiveScan(stream, px.cnt, px.off, npx, pxTiles.as<uint32_t>(), gridTotal.as<uint32_t>());
+            launchEntrySort(stream, grid, st, entryCap, px, ent, cellCount.as<uint32_t>(), entStart.as<uint32_t>(),
+                            gridTiles.as<uint32_t>(), gridTotal.as<uint32_t>());
+            tEnd(ts);
+            const size_t tk = tBegin(kTGather);
+            launchEntryGather(stream, G, st, entryCap, entStart.as<uint32_t>() + grid.total, ent, countWork);
+            tEnd(tk);
+            const size_t tc = tBegin(kTCamSort);
+            launchCamComposite(stream, npx, px, ent, invSpp, fbp);
+            tEnd(tc);
+            ++camRounds;
+            HIPCHECK(hipMemcpyAsync(flagHost.p, stField(offsetof(DevState, overflow)), 4, hipMemcpyDeviceToHost,
+                                    stream));
+            HIPCHECK(hipStreamSynchronize(stream));
+            if (*flagHost.as<uint32_t>() == 0u) break;
+            if (round > 4096) throw StatusError(PALE_ERR_INTERNAL, "camera gather: no progress");
+        }
     }
 
     // submitPhotonMappingKernel (SyclBridge.cpp:82-166) for the given cameras.
@@ -411,24 +507,18 @@ struct pale_handle {
                         gridTotal.as<uint32_t>());
         tEnd(tg);
         mapValid = true;
-        const DevGrid G = devGrid();
         const float invSpp = 1.0f / (float)S.gather_passes;
-        // One pixel cursor per (camera, pass) of the persistent gather.
-        const uint32_t nCursor = (uint32_t)std::max<size_t>(1, camIdx.size() * S.gather_passes);
-        cursors.reserve((size_t)nCursor * sizeof(uint32_t));
-        launchFill(stream, cursors.as<uint32_t>(), nCursor, 0u);
+        cursors.reserve(sizeof(uint32_t));
+        camRounds = 0;
         for (size_t k = 0; k < camIdx.size(); ++k) {
             const HostCamera& c = cams[camIdx[k]];
             const uint32_t npx = c.W * c.H;
+            ensureCamera(npx);
             launchSeedNext(stream, st, seedTable.as<uint32_t>(), tlen, -1);
             launchFillF(stream, fb[camIdx[k]].as<float>(), npx * 4, 0.0f);
             const DevCamera dc = devCam(c);
-            for (uint32_t pass = 0; pass < S.gather_passes; ++pass) {
-                const size_t tk = tBegin(kTGather);
-                launchGather(stream, scene, G, dc, st, pass, S.spp_decorrelate, invSpp, fb[camIdx[k]].as<float4>(),
-                             cursors.as<uint32_t>() + k * S.gather_passes + pass, countWork);
-                tEnd(tk);
-            }
+            for (uint32_t pass = 0; pass < S.gather_passes; ++pass)
+                cameraPass(dc, pass, invSpp, fb[camIdx[k]].as<float4>());
             launchPostprocess(stream, fb[camIdx[k]].as<float4>(), ldr[camIdx[k]].as<float4>(), npx);
         }
         HIPCHECK(hipEventRecord(ev1, stream));
@@ -438,6 +528,16 @@ struct pale_handle {
         HIPCHECK(hipEventElapsedTime(&ms, ev0, ev1));
         stats.forward_ms = ms;
         readStats(true);
+        if (stats.photons_deposited > photonCap) {
+            // The reference replaces stored photons by reservoir sampling
+            // once the map is full (KernelHelpers.h:1705-1718); this build
+            // stores the first `cap` and would give a different image.
+            mapValid = false;
+            throw StatusError(PALE_ERR_SIZE, "photon map full: " + std::to_string(stats.photons_deposited) +
+                                                 " deposits for capacity " + std::to_string(photonCap) +
+                                                 " (reservoir replacement is not supported; lower photons or "
+                                                 "forward_passes)");
+        }
     }
 
     // submitAdjointKernel (SyclBridge.cpp:169-294).
@@ -455,6 +555,7 @@ struct pale_handle {
         h.adjointQ = 0;
         h.events = 0;
         h.qCount = 0;
+        h.evPool = 0;
         HIPCHECK(hipMemcpyAsync(state.p, &h, sizeof(DevState), hipMemcpyHostToDevice, stream));
         HIPCHECK(hipStreamSynchronize(stream));
         // dL/dI of every camera goes to the device before the timed region.
@@ -471,8 +572,24 @@ struct pale_handle {
         DevState* st = this->st();
         uint32_t* qCount = stField(offsetof(DevState, qCount));
         uint32_t* extTotal = stField(offsetof(DevState, extTotal));
-        uint32_t* evTotal = stField(offsetof(DevState, evTotal));
         const float invSpp = 1.0f / (float)S.adjoint_passes;
+        // launchAdjointTransportKernel (AdjointKernels.cpp:389-509) over the
+        // pooled events of several bounces, in grid-cell order.  Gradients
+        // are sums of atomics, so the grouping does not change them beyond
+        // the order of the float additions.
+        uint64_t pooledBound = 0;
+        auto flush = [&]() {
+            if (!pooledBound) return;
+            const size_t tt = tBegin(kTTransport);
+            launchEventSort(stream, grid, st, poolCap, ea(evCmp), evCell.as<uint32_t>(), evSorted.as<uint32_t>(),
+                            cellCount.as<uint32_t>(), entStart.as<uint32_t>(), gridTiles.as<uint32_t>(),
+                            gridTotal.as<uint32_t>());
+            launchAdjTransport(stream, scene, G, st, poolCap, ea(evCmp), evSorted.as<uint32_t>(), invSpp,
+                               gradOpacity.as<float>(), countWork);
+            tEnd(tt);
+            launchFill(stream, stField(offsetof(DevState, evPool)), 1, 0u);
+            pooledBound = 0;
+        };
         for (size_t k = 0; k < camIdx.size(); ++k) {
             const HostCamera& c = cams[camIdx[k]];
             const uint32_t npx = c.W * c.H;
@@ -483,23 +600,22 @@ struct pale_handle {
                 DevMem* out = qB;
                 launchAdjRaygen(stream, dc, dloss.as<float>() + off[k], rq(in), pend, st);
                 for (uint32_t b = 0; b < S.adjoint_bounces; ++b) {
+                    if (pooledBound + npx > poolCap) flush();
                     const size_t ta = tBegin(kTAdjTrace);
                     launchAdjTrace(stream, scene, st, npx, b, rq(in), rq(stg), extFlags.as<uint8_t>(), ea(evStg),
                                    evFlags.as<uint8_t>(), pend, tp(), countWork);
                     tEnd(ta);
-                    launchCompactEvents(stream, evFlags.as<uint8_t>(), qCount, npx, tiles.as<uint32_t>(), evTotal,
-                                        ea(evStg), ea(evCmp), adjCap);
+                    launchCompactEvents(stream, evFlags.as<uint8_t>(), qCount, npx, tiles.as<uint32_t>(), st,
+                                        ea(evStg), ea(evCmp), poolCap);
+                    pooledBound += npx;
                     launchCompactRays(stream, extFlags.as<uint8_t>(), qCount, npx, tiles.as<uint32_t>(), extTotal,
                                       rq(stg), rq(out), queueCap);
-                    const size_t tt = tBegin(kTTransport);
-                    launchAdjTransport(stream, scene, G, st, npx, ea(evCmp), invSpp, gradOpacity.as<float>(),
-                                       countWork);
-                    tEnd(tt);
                     launchSetQueue(stream, st, extTotal);
                     std::swap(in, out);
                 }
             }
         }
+        flush();
         HIPCHECK(hipEventRecord(ev1, stream));
         HIPCHECK(hipStreamSynchronize(stream));
         HIPCHECK(hipGetLastError());
@@ -589,6 +705,7 @@ void pale_default_settings(pale_settings* out) {
     s.q_transmit = 0.0f;
     s.spp_decorrelate = 0;
     s.debug_images = 0;
+    s.gather_entry_budget = 0;
     *out = s;
 }
 
@@ -600,9 +717,9 @@ int pale_create(const pale_scene_desc* scene, const pale_settings* settings, pal
         int ndev = 0;
         if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
             throw StatusError(PALE_ERR_DEVICE, "no HIP device available");
+        // Zero passes are valid, as in the reference: nothing is traced for
+        // that stage (render_target.py asks for adjoint_passes = 0).
         h->S = *settings;
-        if (h->S.gather_passes == 0 || h->S.adjoint_passes == 0 || h->S.forward_passes == 0)
-            throw StatusError(PALE_ERR_INVALID_ARG, "passes must be >= 1");
         if (!(h->S.gather_radius > 0.0f)) throw StatusError(PALE_ERR_INVALID_ARG, "gather_radius must be > 0");
         for (uint32_t i = 0; i < scene->mesh_count; ++i) {
             const pale_mesh_desc& md = scene->meshes[i];

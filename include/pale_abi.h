@@ -119,6 +119,10 @@ typedef struct pale_settings {
                                   does (PrimalKernels.cpp:516-520); 1 = pass
                                   k uses seed + k (independent samples)     */
     int32_t debug_images;      /* "debug_images"                             */
+    uint32_t gather_entry_budget; /* "gather_entry_budget": camera-gather
+                                  entries per pixel held at once (0 = 24, at
+                                  least 65536 in all); deeper images take
+                                  more rounds, with the same result         */
 } pale_settings;
 
 /* Work counters of the last render calls (no reference counterpart; the
@@ -135,6 +139,12 @@ typedef struct pale_stats {
     /* Device time per kernel class (sum over launches) and launch counts. */
     double photon_trace_ms, gather_ms, adjoint_trace_ms, adjoint_transport_ms, grid_ms;
     uint32_t photon_trace_launches, gather_launches, adjoint_trace_launches, adjoint_transport_launches;
+    /* Camera gather stages: closest-hit layer walk, and the entry sort plus
+     * compositing; gather_ms above is its irradiance gathers.  camera_rounds
+     * = entry-buffer rounds of the last render_forward (one per pass when
+     * the buffer suffices). */
+    double camera_trace_ms, camera_sort_ms;
+    uint32_t camera_trace_launches, camera_rounds;
     /* Counting mode only (pale_set_count_work): work units of the
      * reference algorithm per kernel class, summed over the last
      * render_forward and any render_backward after it.  work[c][u] with

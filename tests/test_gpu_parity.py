@@ -209,3 +209,20 @@ def test_gather_points_bit_exact(c0_small):
     b = orc.gather(q, n)
     assert np.count_nonzero(b[:, 0]) > 1000
     assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def test_camera_gather_rounds_bit_exact():
+    # A one-entry-per-pixel budget forces several rounds per pass (about
+    # seven layers per pixel here): pixels stop at a layer that does not fit
+    # and resume from it; the image must not change.
+    spec = scene.config_scene(0, n=65536, width=64, height=64)
+    gpu, orc = _pair(spec, dict(SMALL, gather_entry_budget=1))
+    a = gpu.render_forward()["camera1_raw"]
+    rounds = gpu.stats()["camera_rounds"]
+    b = orc.render_forward()["camera1_raw"]
+    assert rounds > 2 * spec.settings["gather_passes"], rounds
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    one, _ = _pair(spec, SMALL)
+    c = one.render_forward()["camera1_raw"]
+    assert one.stats()["camera_rounds"] == spec.settings["gather_passes"]
+    assert np.array_equal(a.view(np.uint32), c.view(np.uint32))
