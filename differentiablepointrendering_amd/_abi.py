@@ -93,6 +93,7 @@ class Settings(C.Structure):
         ("spp_decorrelate", C.c_int32),
         ("debug_images", C.c_int32),
         ("gather_entry_budget", C.c_uint32),
+        ("photon_map_capacity", C.c_uint32),
     ]
 
 
@@ -151,6 +152,7 @@ DEFAULT_SETTINGS = dict(
     spp_decorrelate=0,
     debug_images=0,
     gather_entry_budget=0,
+    photon_map_capacity=0,
 )
 
 

@@ -25,7 +25,7 @@ struct DevState {
     uint32_t entries;      // camera-gather entries appended this round
     uint32_t overflow;     // the entry buffer filled this round
     uint32_t evPool;       // adjoint events waiting for the transport
-    uint32_t pad0;
+    uint32_t resv;         // this bounce may pass the photon capacity
     unsigned long long photonQ, gatherQ, adjointQ, deposits, events, pad;
     // Counting mode, per kernel class (photon, gather, adjoint): popped
     // nodes, AABB tests, primitive tests, gathers, gather cells, non-empty
@@ -98,6 +98,18 @@ struct TraceParams {
     float qNull, qReflect, qTransmit;
 };
 
+// Photon-map reservoir (depositPhotonSurface past capacity): per queue item
+// its deposit attempt index (valid when DevState::resv) and the slot it
+// replaces (kNoSlot if none); per stored slot the highest attempt that
+// claimed it.
+constexpr uint32_t kNoSlot = 0xFFFFFFFFu;
+struct Reservoir {
+    uint32_t* attempt;
+    uint32_t* slot;
+    uint32_t* owner;
+    uint32_t cap;
+};
+
 struct GridParams {
     float ox, oy, oz, cell;
     int rx, ry, rz;
@@ -112,7 +124,11 @@ void launchEmit(hipStream_t s, const DevScene& S, DevState* st, uint32_t photons
                 RayQueue staged, uint8_t* flags);
 void launchPhotonTrace(hipStream_t s, const DevScene& S, DevState* st, uint32_t maxRays, RayQueue in,
                        RayQueue staged, uint8_t* extFlags, PhotonArrays dep, uint8_t* depFlags, TraceParams tp,
-                       bool count);
+                       Reservoir rs, uint32_t* tiles, bool count);
+// After the bounce's deposits are compacted: deposits past capacity that the
+// reservoir accepted overwrite their slots, the highest attempt winning.
+void launchReservoirWrite(hipStream_t s, DevState* st, uint32_t maxRays, Reservoir rs, PhotonArrays dep,
+                          PhotonArrays raw);
 // Stable compaction of flagged items i < *count (items >= *count are ignored).
 void launchCompactRays(hipStream_t s, const uint8_t* flags, const uint32_t* count, uint32_t maxItems,
                        uint32_t* tiles, uint32_t* total, RayQueue src, RayQueue dst, uint32_t dstCap);

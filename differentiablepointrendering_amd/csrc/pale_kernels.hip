@@ -160,16 +160,32 @@ __global__ __launch_bounds__(256) void k_emit(DevScene S, const DevState* st, ui
 template <bool kCount>
 __global__ __launch_bounds__(kTraceBlock) void k_photon_trace(DevScene S, DevState* st, RayQueue in,
                                                               RayQueue out, uint8_t* extFlags, PhotonArrays dep,
-                                                              uint8_t* depFlags, TraceParams tp) {
+                                                              uint8_t* depFlags, TraceParams tp, Reservoir rs) {
     extern __shared__ uint2 s_stack[];
     const LaneStack stk = laneStack(s_stack, S.stackDepth);
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t n = st->live ? st->qCount : 0u;
     const bool active = i < n;
+    const bool resv = st->resv != 0u;
     Work w{};
     if (active) {
         uint8_t ef = 0, df = 0;
         Rng rng(perItemSeed(st->number, i));
+        // depositPhotonSurface past capacity (KernelHelpers.h:1705-1718): the
+        // attempt index comes from the probe pass; accept with probability
+        // cap / (attempt + 1), then pick the slot (two draws of this item).
+        auto reservoirDraws = [&]() {
+            if (!resv) return;
+            const uint32_t a = rs.attempt[i];
+            uint32_t sl = kNoSlot;
+            if (a >= rs.cap) {
+                const float u = rng.nextFloat();
+                const float accept = (float)rs.cap / (float)(a + 1u);
+                if (u < accept) sl = rng.nextUint() % rs.cap;
+            }
+            rs.slot[i] = sl;
+        };
+        if (resv) rs.slot[i] = kNoSlot;
         const float4 o4 = in.o[i], d4 = in.d[i], T4 = in.T[i];
         const F3 o = ld3(o4), d = ld3(d4), T = ld3(T4);
         const uint32_t bounce = f2u(d4.w);
@@ -184,6 +200,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_photon_trace(DevScene S, DevSta
                 dep.posPow[i] = f4(h.pos, T.x);
                 dep.dirPow[i] = f4(d, T.y);
                 dep.powZ[i] = T.z;
+                reservoirDraws();
                 const DevMaterial m = S.mats[I.material];
                 float pdf;
                 sampleCosine(rng, ng, ndir, pdf);
@@ -209,6 +226,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_photon_trace(DevScene S, DevSta
                     dep.posPow[i] = f4(h.pos, flux.x);
                     dep.dirPow[i] = f4(d, flux.y);
                     dep.powZ[i] = flux.z;
+                    reservoirDraws();
                     float pdf;
                     sampleUniformHemi(rng, on, ndir, pdf);
                     const float cosT = fmaxf(0.0f, dot(ndir, on));
@@ -246,6 +264,82 @@ __global__ __launch_bounds__(kTraceBlock) void k_photon_trace(DevScene S, DevSta
     }
     waveCount(active, &st->photonQ);
     if (kCount) flushWork(w, st, kWorkPhoton);
+}
+
+// Reservoir mode of one bounce: on when the deposits so far plus one per
+// queued photon could pass the capacity.
+__global__ void k_resv_check(DevState* st, uint32_t cap) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    st->resv = (st->live && (uint64_t)st->photonCount + (uint64_t)st->qCount > (uint64_t)cap) ? 1u : 0u;
+}
+
+// Probe pass of a reservoir bounce: which items deposit (the same hit and
+// the same first draw as k_photon_trace), so that every deposit learns its
+// attempt index in queue order before its random stream is used.
+__global__ __launch_bounds__(kTraceBlock) void k_photon_probe(DevScene S, DevState* st, RayQueue in,
+                                                              uint8_t* depFlags, TraceParams tp) {
+    if (!st->resv) return;
+    extern __shared__ uint2 s_stack[];
+    const LaneStack stk = laneStack(s_stack, S.stackDepth);
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n = st->live ? st->qCount : 0u;
+    if (i >= n) return;
+    Rng rng(perItemSeed(st->number, i));
+    Work w{};
+    Hit h;
+    uint8_t df = 0;
+    if (intersectScene<false>(S, ld3(in.o[i]), ld3(in.d[i]), stk, h, w)) {
+        if (S.inst[h.inst].type == kGeoMesh) {
+            df = 1;
+        } else {
+            const float u = rng.nextFloat();
+            df = (u >= tp.qNull && u < tp.qNull + tp.qReflect) ? 1 : 0;
+        }
+    }
+    depFlags[i] = df;
+}
+
+// attempt[i] = photonCount + rank of item i among the depositing items.
+__global__ __launch_bounds__(kScanThreads) void k_resv_rank(const uint8_t* flags, const DevState* st,
+                                                            const uint32_t* tiles, uint32_t* attempt) {
+    if (!st->resv) return;
+    const uint32_t n = st->live ? st->qCount : 0u;
+    const uint32_t first = blockIdx.x * kTile + threadIdx.x * kTileSpan;
+    uint8_t f[kTileSpan];
+    uint32_t s = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kTileSpan; ++k) {
+        f[k] = (first + k < n) ? flags[first + k] : 0;
+        s += f[k];
+    }
+    uint32_t total;
+    uint32_t off = blockExclusiveScan(s, total) + tiles[blockIdx.x] + st->photonCount;
+#pragma unroll
+    for (uint32_t k = 0; k < kTileSpan; ++k) {
+        if (f[k]) attempt[first + k] = off++;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_resv_claim(const DevState* st, Reservoir rs) {
+    if (!st->resv) return;
+    const uint32_t n = st->live ? st->qCount : 0u;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t sl = rs.slot[i];
+    if (sl != kNoSlot) atomicMax(&rs.owner[sl], rs.attempt[i]);
+}
+
+__global__ __launch_bounds__(256) void k_resv_write(const DevState* st, Reservoir rs, PhotonArrays dep,
+                                                    PhotonArrays raw) {
+    if (!st->resv) return;
+    const uint32_t n = st->live ? st->qCount : 0u;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t sl = rs.slot[i];
+    if (sl == kNoSlot || rs.owner[sl] != rs.attempt[i]) return;
+    raw.posPow[sl] = dep.posPow[i];
+    raw.dirPow[sl] = dep.dirPow[i];
+    raw.powZ[sl] = dep.powZ[i];
 }
 
 // -------------------------------------------------- stable compaction --
@@ -1013,13 +1107,31 @@ void launchEmit(hipStream_t s, const DevScene& S, DevState* st, uint32_t P, floa
 }
 void launchPhotonTrace(hipStream_t s, const DevScene& S, DevState* st, uint32_t maxRays, RayQueue in,
                        RayQueue staged, uint8_t* extFlags, PhotonArrays dep, uint8_t* depFlags, TraceParams tp,
-                       bool count) {
+                       Reservoir rs, uint32_t* tiles, bool count) {
     if (!maxRays) return;
     const dim3 g(blocks(maxRays, kTraceBlock));
+    if (rs.attempt) {
+        // The map can fill during this render: probe, rank, then trace.
+        k_resv_check<<<1, 64, 0, s>>>(st, rs.cap);
+        k_photon_probe<<<g, kTraceBlock, stackBytes(S), s>>>(S, st, in, depFlags, tp);
+        const uint32_t* qCount =
+            reinterpret_cast<const uint32_t*>(reinterpret_cast<const char*>(st) + offsetof(DevState, qCount));
+        const uint32_t nt = blocks(maxRays, kTile);
+        k_tile_count<<<nt, kScanThreads, 0, s>>>(depFlags, qCount, maxRays, tiles);
+        k_tile_scan<<<1, 1024, 0, s>>>(tiles, qCount, 0u, maxRays, tiles + nt, 0xFFFFFFFFu);
+        k_resv_rank<<<nt, kScanThreads, 0, s>>>(depFlags, st, tiles, rs.attempt);
+    }
     if (count)
-        k_photon_trace<true><<<g, kTraceBlock, stackBytes(S), s>>>(S, st, in, staged, extFlags, dep, depFlags, tp);
+        k_photon_trace<true><<<g, kTraceBlock, stackBytes(S), s>>>(S, st, in, staged, extFlags, dep, depFlags, tp, rs);
     else
-        k_photon_trace<false><<<g, kTraceBlock, stackBytes(S), s>>>(S, st, in, staged, extFlags, dep, depFlags, tp);
+        k_photon_trace<false><<<g, kTraceBlock, stackBytes(S), s>>>(S, st, in, staged, extFlags, dep, depFlags, tp,
+                                                                    rs);
+}
+void launchReservoirWrite(hipStream_t s, DevState* st, uint32_t maxRays, Reservoir rs, PhotonArrays dep,
+                          PhotonArrays raw) {
+    if (!maxRays || !rs.attempt) return;
+    k_resv_claim<<<blocks(maxRays, 256), 256, 0, s>>>(st, rs);
+    k_resv_write<<<blocks(maxRays, 256), 256, 0, s>>>(st, rs, dep, raw);
 }
 void launchCompactRays(hipStream_t s, const uint8_t* flags, const uint32_t* count, uint32_t maxItems,
                        uint32_t* tiles, uint32_t* total, RayQueue src, RayQueue dst, uint32_t dstCap) {

@@ -101,6 +101,10 @@ struct pale_handle {
     DevMem rawPosPow, rawDirPow, rawPowZ, srtPosPow, srtDirPow, srtPowZ, cellId, sortedIdx, cellCount, cellStart,
         gridTiles, gridTotal;
     bool mapValid = false;
+    // Reservoir replacement once the map is full (only when the capacity is
+    // below photons * passes * bounces, the most deposits a render makes).
+    bool reservoirOn = false;
+    DevMem resvAttempt, resvSlot, resvOwner;
 
     // Ray queues and staging.
     uint32_t queueCap = 0;
@@ -260,9 +264,11 @@ struct pale_handle {
             // bounces, clamped to 10 GiB of 64-byte reference records.
             const uint64_t requested = (uint64_t)S.photons * (uint64_t)S.forward_passes * (uint64_t)S.bounces;
             const uint64_t maxPhotons = (10ull * 1024ull * 1024ull * 1024ull) / 64ull;
-            photonCap = (uint32_t)std::min(requested, maxPhotons);
+            photonCap = S.photon_map_capacity ? S.photon_map_capacity : (uint32_t)std::min(requested, maxPhotons);
             if ((uint64_t)S.photons * (uint64_t)S.forward_passes > 0xFFFFFFFFull)
                 throw StatusError(PALE_ERR_INVALID_ARG, "photons * forward_passes exceeds 2^32");
+            reservoirOn = (uint64_t)photonCap < requested;
+            if (reservoirOn) resvOwner.reserve((size_t)photonCap * sizeof(uint32_t));
             // configurePhotonGrid (PathTracer.cpp:184-221)
             radius = S.gather_radius;
             const float cell = 0.5f * radius;
@@ -314,8 +320,10 @@ struct pale_handle {
         depStg[1].reserve((size_t)cap * sizeof(float4));
         depStg[2].reserve((size_t)cap * sizeof(float));
         depFlags.reserve(cap);
-        tiles.reserve(((size_t)cap / kTile + 2) * sizeof(uint32_t));
+        tiles.reserve(((size_t)cap / kTile + 4) * sizeof(uint32_t));
         counts.reserve(16 * sizeof(uint32_t));
+        resvAttempt.reserve((size_t)cap * sizeof(uint32_t));
+        resvSlot.reserve((size_t)cap * sizeof(uint32_t));
         queueCap = cap;
     }
 
@@ -476,6 +484,9 @@ struct pale_handle {
         uint32_t* depTotal = stField(offsetof(DevState, depTotal));
         uint32_t* photonCount = stField(offsetof(DevState, photonCount));
         const uint32_t tlen = (uint32_t)table.size();
+        const Reservoir rs{reservoirOn ? resvAttempt.as<uint32_t>() : nullptr, resvSlot.as<uint32_t>(),
+                           resvOwner.as<uint32_t>(), photonCap};
+        if (reservoirOn) launchFill(stream, resvOwner.as<uint32_t>(), photonCap, 0u);
         DevMem* in = qA;
         DevMem* out = qB;
         for (uint32_t pass = 0; pass < S.forward_passes; ++pass) {
@@ -488,13 +499,16 @@ struct pale_handle {
                 launchSeedNext(stream, st, seedTable.as<uint32_t>(), tlen, (int)b);
                 const size_t te = tBegin(kTPhoton);
                 launchPhotonTrace(stream, scene, st, P, rq(in), rq(stg), extFlags.as<uint8_t>(),
-                                  pa(depStg[0], depStg[1], depStg[2]), depFlags.as<uint8_t>(), tp(), countWork);
+                                  pa(depStg[0], depStg[1], depStg[2]), depFlags.as<uint8_t>(), tp(), rs,
+                                  tiles.as<uint32_t>(), countWork);
                 tEnd(te);
                 launchCompactRays(stream, extFlags.as<uint8_t>(), qCount, P, tiles.as<uint32_t>(), extTotal, rq(stg),
                                   rq(out), queueCap);
                 launchCompactPhotons(stream, depFlags.as<uint8_t>(), qCount, P, tiles.as<uint32_t>(), depTotal,
                                      photonCount, pa(depStg[0], depStg[1], depStg[2]),
                                      pa(rawPosPow, rawDirPow, rawPowZ), photonCap);
+                launchReservoirWrite(stream, st, P, rs, pa(depStg[0], depStg[1], depStg[2]),
+                                     pa(rawPosPow, rawDirPow, rawPowZ));
                 launchFinishPhotonBounce(stream, st, photonCap);
                 std::swap(in, out);
             }
@@ -528,16 +542,6 @@ struct pale_handle {
         HIPCHECK(hipEventElapsedTime(&ms, ev0, ev1));
         stats.forward_ms = ms;
         readStats(true);
-        if (stats.photons_deposited > photonCap) {
-            // The reference replaces stored photons by reservoir sampling
-            // once the map is full (KernelHelpers.h:1705-1718); this build
-            // stores the first `cap` and would give a different image.
-            mapValid = false;
-            throw StatusError(PALE_ERR_SIZE, "photon map full: " + std::to_string(stats.photons_deposited) +
-                                                 " deposits for capacity " + std::to_string(photonCap) +
-                                                 " (reservoir replacement is not supported; lower photons or "
-                                                 "forward_passes)");
-        }
     }
 
     // submitAdjointKernel (SyclBridge.cpp:169-294).

@@ -123,6 +123,12 @@ typedef struct pale_settings {
                                   entries per pixel held at once (0 = 24, at
                                   least 65536 in all); deeper images take
                                   more rounds, with the same result         */
+    uint32_t photon_map_capacity; /* "photon_map_capacity": stored photons at
+                                  most (0 = photons * forward_passes *
+                                  bounces, clamped to 10 GiB of 64-byte
+                                  records, as the reference allocates); past
+                                  it deposits replace stored photons by the
+                                  reference's reservoir sampling            */
 } pale_settings;
 
 /* Work counters of the last render calls (no reference counterpart; the

@@ -244,6 +244,7 @@ struct Settings {
     uint32_t rrStart;
     float qNull, qReflect, qTransmit;
     bool decorrelate;
+    uint32_t mapCapacity;
 };
 
 struct Grid {
@@ -688,7 +689,7 @@ static void setScene(Renderer& R) {
     if (!R.gridConfigured) {
         const uint64_t requested = (uint64_t)R.S.photons * (uint64_t)(R.S.forwardPasses * R.S.bounces);
         const uint64_t maxPhotons = (10ull * 1024ull * 1024ull * 1024ull) / 64ull;
-        R.photonCapacity = (uint32_t)std::min(requested, maxPhotons);
+        R.photonCapacity = R.S.mapCapacity ? R.S.mapCapacity : (uint32_t)std::min(requested, maxPhotons);
         configureGrid(R);
     }
 }
@@ -1651,6 +1652,7 @@ static Settings toSettings(const pale_settings& s) {
     S.qReflect = s.q_reflect;
     S.qTransmit = s.q_transmit;
     S.decorrelate = s.spp_decorrelate != 0;
+    S.mapCapacity = s.photon_map_capacity;
     return S;
 }
 

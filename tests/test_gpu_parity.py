@@ -226,3 +226,19 @@ def test_camera_gather_rounds_bit_exact():
     c = one.render_forward()["camera1_raw"]
     assert one.stats()["camera_rounds"] == spec.settings["gather_passes"]
     assert np.array_equal(a.view(np.uint32), c.view(np.uint32))
+
+
+def test_photon_reservoir_bit_exact():
+    # Capacity below the deposits: past it, deposits replace stored photons by
+    # the reference's reservoir sampling (two extra draws of the depositing
+    # photon); the photon map and the image must match the oracle's
+    # sequential order exactly.
+    spec = scene.config_scene(0, n=4096, width=64, height=64)
+    gpu, orc = _pair(spec, dict(SMALL, photon_map_capacity=1500))
+    ia, ib = gpu.render_forward(), orc.render_forward()
+    a, b = gpu.photons(), orc.photons()
+    assert a["attempted"] == b["attempted"] > 2 * 1500
+    assert a["count"] == b["count"] == 1500
+    assert np.array_equal(a["position"].view(np.uint32), b["position"].view(np.uint32))
+    assert np.array_equal(a["power"].view(np.uint32), b["power"].view(np.uint32))
+    assert np.array_equal(ia["camera1_raw"].view(np.uint32), ib["camera1_raw"].view(np.uint32))
