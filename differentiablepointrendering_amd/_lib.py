@@ -24,7 +24,8 @@ EXPORTED = (
     "pale_camera_info", "pale_render_forward", "pale_render_backward", "pale_point_count", "pale_get_points",
     "pale_apply_points", "pale_add_points", "pale_remove_points", "pale_rebuild_bvh", "pale_set_point_properties",
     "pale_set_point_opacity", "pale_set_point_beta", "pale_trace_rays", "pale_gather_irradiance", "pale_get_surfel_bvh", "pale_get_photons",
-    "pale_get_stats", "pale_set_count_work",
+    "pale_get_stats", "pale_set_count_work", "pale_render_forward_device", "pale_render_backward_device",
+    "pale_set_opacity_device",
 )
 
 
@@ -39,6 +40,16 @@ def load(path: str = LIB_PATH):
         return _lib
     if not os.path.exists(path):
         raise PaleError(f"HIP renderer library not built: {path} (run differentiablepointrendering_amd/build.py)")
+    # PyTorch ships its own HIP runtime under the same soname
+    # (libamdhip64.so.7).  Importing torch first makes the library bind to
+    # that runtime, so one process holds one HIP runtime and device tensors
+    # can be exchanged; loading ours first would leave torch a second
+    # runtime that finds no GPU.  Without torch the system ROCm runtime is
+    # used.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(path)
     H = C.c_void_p
     L.pale_last_error.restype = C.c_char_p
@@ -52,6 +63,9 @@ def load(path: str = LIB_PATH):
     L.pale_render_forward.argtypes = [H, C.c_char_p, C.POINTER(_fp), C.POINTER(_fp), _u32p]
     L.pale_render_backward.argtypes = [H, C.c_uint32, C.POINTER(C.c_char_p), C.POINTER(_fp), C.POINTER(_fp),
                                        C.POINTER(_fp)]
+    L.pale_render_forward_device.argtypes = [H, C.c_char_p, C.POINTER(_fp), C.POINTER(_fp), _u32p]
+    L.pale_render_backward_device.argtypes = [H, C.c_uint32, C.POINTER(C.c_char_p), C.POINTER(_fp), C.POINTER(_fp)]
+    L.pale_set_opacity_device.argtypes = [H, _fp, C.c_uint32]
     L.pale_point_count.argtypes = [H, _u32p]
     L.pale_get_points.argtypes = [H] + [_fp] * 8
     L.pale_apply_points.argtypes = [H, C.POINTER(_abi.PointsDesc)]

@@ -171,6 +171,7 @@ void launchEventSort(hipStream_t s, const GridParams& g, DevState* st, uint32_t 
                      uint32_t* scratchTotal);
 void launchAdjTransport(hipStream_t s, const DevScene& S, const DevGrid& G, DevState* st, uint32_t poolCap,
                         EventArrays ev, const uint32_t* sorted, float invSpp, float* gradOpacity, bool count);
+void launchSetOpacity(hipStream_t s, SurfelShade* shade, const float* opacity, uint32_t n);
 void launchGatherPoints(hipStream_t s, const DevGrid& G, uint32_t n, const float* q, const float* nrm, float* out);
 void launchTraceRays(hipStream_t s, const DevScene& S, uint32_t n, const float* o, const float* d, int32_t* inst,
                      int32_t* prim, float* t, float* pos, float* alpha);

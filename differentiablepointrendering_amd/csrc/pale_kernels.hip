@@ -1074,6 +1074,12 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_rays(DevScene S, uint32_t
     alpha[i] = hit ? h.alpha : 0.0f;
 }
 
+// Opacity of every surfel from device memory (BVH order).
+__global__ __launch_bounds__(256) void k_set_opacity(SurfelShade* shade, const float* opacity, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) shade[i].opacity = opacity[i];
+}
+
 // Irradiance estimate at given points (gatherDiffuseIrradianceAtPoint,
 // KernelHelpers.h:990-1037) for tests of the gather on its own.
 __global__ __launch_bounds__(256) void k_gather_points(DevGrid G, uint32_t n, const float* q, const float* nrm,
@@ -1289,6 +1295,10 @@ void launchTraceRays(hipStream_t s, const DevScene& S, uint32_t n, const float* 
 
 void launchGatherPoints(hipStream_t s, const DevGrid& G, uint32_t n, const float* q, const float* nrm, float* out) {
     if (n) k_gather_points<<<blocks(n, 256), 256, 0, s>>>(G, n, q, nrm, out);
+}
+
+void launchSetOpacity(hipStream_t s, SurfelShade* shade, const float* opacity, uint32_t n) {
+    if (n) k_set_opacity<<<blocks(n, 256), 256, 0, s>>>(shade, opacity, n);
 }
 
 }  // namespace pale

@@ -545,7 +545,8 @@ struct pale_handle {
     }
 
     // submitAdjointKernel (SyclBridge.cpp:169-294).
-    void renderBackward(const std::vector<uint32_t>& camIdx, const std::vector<const float*>& src) {
+    void renderBackward(const std::vector<uint32_t>& camIdx, const std::vector<const float*>& src,
+                        bool srcOnDevice = false) {
         const uint32_t N = (uint32_t)bp.points.size();
         uint32_t maxNpx = 1;
         for (uint32_t c : camIdx) maxNpx = std::max(maxNpx, cams[c].W * cams[c].H);
@@ -568,7 +569,7 @@ struct pale_handle {
         dloss.reserve(std::max<size_t>(off.back(), 1) * sizeof(float));
         for (size_t k = 0; k < camIdx.size(); ++k)
             HIPCHECK(hipMemcpyAsync(dloss.as<float>() + off[k], src[k], (off[k + 1] - off[k]) * sizeof(float),
-                                    hipMemcpyHostToDevice, stream));
+                                    srcOnDevice ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, stream));
         launchFillF(stream, gradOpacity.as<float>(), N, 0.0f);
         HIPCHECK(hipStreamSynchronize(stream));
         tReset();
@@ -803,6 +804,24 @@ int pale_camera_info(const pale_handle* h, uint32_t index, char name_out[PALE_NA
     return PALE_OK;
 }
 
+// Adjoint cameras in camera order that appear in the input
+// (pale_bindings.cpp:301-376): index, image pointer, input position.
+static void selectAdjoint(const pale_handle* h, uint32_t n, const char* const* names, const float* const* images,
+                          std::vector<uint32_t>& idx, std::vector<const float*>& src, std::vector<uint32_t>& which) {
+    for (uint32_t c = 0; c < h->cams.size(); ++c) {
+        if (!h->cams[c].adjoint) continue;
+        for (uint32_t k = 0; k < n; ++k) {
+            if (names[k] && h->cams[c].name == names[k]) {
+                if (!images[k]) throw StatusError(PALE_ERR_INVALID_ARG, "null image for " + h->cams[c].name);
+                idx.push_back(c);
+                src.push_back(images[k]);
+                which.push_back(k);
+                break;
+            }
+        }
+    }
+}
+
 int pale_render_forward(pale_handle* h, const char* camera_name, float* const* ldr_out, float* const* raw_out,
                         uint32_t* rendered) {
     if (!h) return fail(PALE_ERR_INVALID_ARG, "null handle");
@@ -826,21 +845,9 @@ int pale_render_backward(pale_handle* h, uint32_t n, const char* const* camera_n
     return guarded([&] {
         // Adjoint cameras in camera order that appear in the input
         // (pale_bindings.cpp:301-376).
-        std::vector<uint32_t> idx;
+        std::vector<uint32_t> idx, which;
         std::vector<const float*> src;
-        std::vector<uint32_t> which;
-        for (uint32_t c = 0; c < h->cams.size(); ++c) {
-            if (!h->cams[c].adjoint) continue;
-            for (uint32_t k = 0; k < n; ++k) {
-                if (camera_names[k] && h->cams[c].name == camera_names[k]) {
-                    if (!dloss_rgb[k]) throw StatusError(PALE_ERR_INVALID_ARG, "null image for " + h->cams[c].name);
-                    idx.push_back(c);
-                    src.push_back(dloss_rgb[k]);
-                    which.push_back(k);
-                    break;
-                }
-            }
-        }
+        selectAdjoint(h, n, camera_names, dloss_rgb, idx, src, which);
         h->renderBackward(idx, src);
         const size_t N = h->bp.points.size();
         const size_t widths[8] = {3, 3, 3, 2, 3, 1, 1, 1};
@@ -869,6 +876,70 @@ int pale_render_backward(pale_handle* h, uint32_t n, const char* const* camera_n
                     dst[4 * i + 3] = 1.0f;
                 }
             }
+        }
+    });
+}
+
+int pale_render_forward_device(pale_handle* h, const char* camera_name, float* const* ldr_dev, float* const* raw_dev,
+                               uint32_t* rendered) {
+    if (!h) return fail(PALE_ERR_INVALID_ARG, "null handle");
+    return guarded([&] {
+        const std::vector<uint32_t> idx = selectCameras(h, camera_name);
+        h->renderForward(idx);
+        for (size_t k = 0; k < idx.size(); ++k) {
+            const HostCamera& c = h->cams[idx[k]];
+            const size_t bytes = (size_t)c.W * c.H * 4 * sizeof(float);
+            if (raw_dev && raw_dev[k])
+                HIPCHECK(hipMemcpyAsync(raw_dev[k], h->fb[idx[k]].p, bytes, hipMemcpyDeviceToDevice, h->stream));
+            if (ldr_dev && ldr_dev[k])
+                HIPCHECK(hipMemcpyAsync(ldr_dev[k], h->ldr[idx[k]].p, bytes, hipMemcpyDeviceToDevice, h->stream));
+        }
+        HIPCHECK(hipStreamSynchronize(h->stream));
+        if (rendered) *rendered = (uint32_t)idx.size();
+    });
+}
+
+int pale_render_backward_device(pale_handle* h, uint32_t n, const char* const* camera_names,
+                                const float* const* dloss_rgb_dev, float* const* grads_dev) {
+    if (!h) return fail(PALE_ERR_INVALID_ARG, "null handle");
+    if (n && (!camera_names || !dloss_rgb_dev)) return fail(PALE_ERR_INVALID_ARG, "null camera names or images");
+    return guarded([&] {
+        std::vector<uint32_t> idx, which;
+        std::vector<const float*> src;
+        selectAdjoint(h, n, camera_names, dloss_rgb_dev, idx, src, which);
+        h->renderBackward(idx, src, true);
+        const uint32_t N = (uint32_t)h->bp.points.size();
+        const uint32_t widths[8] = {3, 3, 3, 2, 3, 1, 1, 1};
+        if (grads_dev) {
+            for (int g = 0; g < 8; ++g) {
+                if (!grads_dev[g] || !N) continue;
+                if (g == 5)
+                    HIPCHECK(hipMemcpyAsync(grads_dev[g], h->gradOpacity.p, (size_t)N * sizeof(float),
+                                            hipMemcpyDeviceToDevice, h->stream));
+                else
+                    launchFillF(h->stream, grads_dev[g], N * widths[g], 0.0f);
+            }
+        }
+        HIPCHECK(hipStreamSynchronize(h->stream));
+        HIPCHECK(hipGetLastError());
+    });
+}
+
+int pale_set_opacity_device(pale_handle* h, const float* opacity_dev, uint32_t n) {
+    if (!h || (n && !opacity_dev)) return fail(PALE_ERR_INVALID_ARG, "null argument");
+    return guarded([&] {
+        if (n != h->bp.points.size())
+            throw StatusError(PALE_ERR_SIZE, "opacity: expected " + std::to_string(h->bp.points.size()) + " values");
+        if (!n) return;
+        launchSetOpacity(h->stream, h->shade.as<SurfelShade>(), opacity_dev, n);
+        std::vector<float> v(n);
+        HIPCHECK(hipMemcpyAsync(v.data(), opacity_dev, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+        HIPCHECK(hipStreamSynchronize(h->stream));
+        HIPCHECK(hipGetLastError());
+        for (uint32_t i = 0; i < n; ++i) {
+            h->bp.points[i].opacity = v[i];
+            h->asset[i].opacity = v[i];
+            h->img.surfShade[i].opacity = v[i];
         }
     });
 }

@@ -155,6 +155,68 @@ class Renderer:
         images = {"adjoint_source": {nm: s for nm, s in zip(names, srcs)}}
         return grads, images
 
+    # ------------------------------------------------ device (torch) tensors --
+    @staticmethod
+    def _dptr(t):
+        return C.cast(C.c_void_p(t.data_ptr()), _fp)
+
+    def render_forward_device(self, camera_name: str = "") -> dict:
+        """render_forward into torch tensors on the renderer's GPU:
+        {name: (H,W,4) LDR, name + "_raw": (H,W,4) HDR}."""
+        import torch
+
+        cams = [c for c in self._cams if camera_name == "" or c[0] == camera_name]
+        ldr = [torch.empty((c[2], c[1], 4), dtype=torch.float32, device="cuda") for c in cams]
+        raw = [torch.empty((c[2], c[1], 4), dtype=torch.float32, device="cuda") for c in cams]
+        k = max(1, len(cams))
+        lp = (_fp * k)(*[self._dptr(a) for a in ldr])
+        rp = (_fp * k)(*[self._dptr(a) for a in raw])
+        n = C.c_uint32()
+        torch.cuda.current_stream().synchronize()
+        check(self._L.pale_render_forward_device(self._h, camera_name.encode(), lp, rp, C.byref(n)))
+        out = {}
+        for c, a, b in zip(cams, ldr, raw):
+            out[c[0]] = a
+            out[c[0] + "_raw"] = b
+        return out
+
+    def render_backward_device(self, dloss: dict) -> dict:
+        """render_backward from (H,W,3) dL/dI torch tensors on the GPU; returns
+        the gradient dict as torch tensors on the GPU."""
+        import torch
+
+        names = [c[0] for c in self._cams if c[3] and c[0] in dloss]
+        imgs = []
+        for nm in names:
+            cam = next(c for c in self._cams if c[0] == nm)
+            t = dloss[nm]
+            if t.dim() != 3 or t.shape[2] not in (3, 4):
+                raise RuntimeError("adjoint image for camera '" + nm + "' must be (H,W,3) or (H,W,4) float32")
+            if t.shape[0] != cam[2] or t.shape[1] != cam[1]:
+                raise RuntimeError(f"Adjoint image size mismatch for camera '{nm}': expected {cam[2]}x{cam[1]}, "
+                                   f"got {t.shape[0]}x{t.shape[1]}")
+            imgs.append(t[..., :3].to(device="cuda", dtype=torch.float32).contiguous())
+        n = self.point_count
+        grads = {k: torch.empty((n, w) if w else (n,), dtype=torch.float32, device="cuda")
+                 for k, w in _abi.POINT_SHAPES.items()}
+        gp = (_fp * 8)(*[self._dptr(grads[k]) for k in _abi.POINT_KEYS])
+        k = max(1, len(names))
+        cn = (C.c_char_p * k)(*[s.encode() for s in names])
+        ap = (_fp * k)(*[self._dptr(a) for a in imgs])
+        torch.cuda.current_stream().synchronize()
+        check(self._L.pale_render_backward_device(self._h, len(names), cn, ap, gp))
+        grads["color"] = grads["albedo"]
+        return grads
+
+    def set_opacity_device(self, opacity) -> None:
+        """Opacity of every surfel (BVH order) from a GPU tensor, without a
+        BVH rebuild."""
+        import torch
+
+        t = opacity.detach().to(device="cuda", dtype=torch.float32).contiguous().reshape(-1)
+        torch.cuda.current_stream().synchronize()
+        check(self._L.pale_set_opacity_device(self._h, self._dptr(t), t.numel()))
+
     # ------------------------------------------------------------- points --
     @property
     def point_count(self) -> int:

@@ -242,6 +242,31 @@ int pale_set_point_properties(pale_handle* h, const float translation[3],
 int pale_set_point_opacity(pale_handle* h, float opacity, int32_t index);
 int pale_set_point_beta(pale_handle* h, float beta, int32_t index);
 
+/* ---- device-memory variants: no reference counterpart (the reference's
+ * bindings copy through host numpy arrays, pale_bindings.cpp:180-277,
+ * 279-731).  For callers whose tensors live on the GPU (a
+ * torch.autograd.Function).  Pointers are device addresses on the handle's
+ * device; each call returns after its results are written, and reads its
+ * inputs after the caller's own work on them has finished (the caller
+ * synchronises its stream first). ---- */
+
+/* render_forward with device outputs: H*W*4 floats per rendered camera. */
+int pale_render_forward_device(pale_handle* h, const char* camera_name,
+                               float* const* ldr_dev, float* const* raw_dev,
+                               uint32_t* rendered);
+
+/* render_backward with device dL/dI images (H,W,3) and device gradient
+ * arrays (same order and sizes as pale_render_backward; any may be NULL). */
+int pale_render_backward_device(pale_handle* h, uint32_t n,
+                                const char* const* camera_names,
+                                const float* const* dloss_rgb_dev,
+                                float* const* grads_dev);
+
+/* Opacity of all n surfels (BVH order) from device memory, without a BVH
+ * rebuild (apply_point_optimization semantics for one parameter). */
+int pale_set_opacity_device(pale_handle* h, const float* opacity_dev,
+                            uint32_t n);
+
 /* ---- diagnostics: no reference counterpart; used by the parity tests ---- */
 
 /* Closest-hit query (the reference's intersectScene in FirstHit mode,
