@@ -48,8 +48,8 @@ def parse():
     ap.add_argument("--photons", type=int, default=None)
     ap.add_argument("--spp", type=int, default=None)
     ap.add_argument("--decorrelate", action="store_true", help="independent seed per gather pass")
-    ap.add_argument("--cpu-photon-stride", type=int, default=16)
-    ap.add_argument("--cpu-pixel-stride", type=int, default=256)
+    ap.add_argument("--cpu-photon-stride", type=int, default=8)
+    ap.add_argument("--cpu-pixel-stride", type=int, default=80)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-count", action="store_true", help="skip the work-counting step (roofline)")
     ap.add_argument("--copy-calibration", action="store_true",
