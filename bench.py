@@ -16,9 +16,10 @@ step's inputs are resident in device memory when the timed region starts;
 the wall-clock rate, which includes the host<->device image copies and the
 loss on the host, is reported beside it (mrays_per_s_wall).
 
-Also reported: the roofline of the dominant kernel (the camera gather) from
-the reference's algorithmic byte model per query, and the CPU oracle timed
-on a bounded sample of the same workload.
+Also reported: the roofline of the dominant kernel by device time, from the
+reference's algorithmic byte model per work unit (counted in a separate,
+untimed counting step), the same figures for the other camera-gather stage,
+and the CPU oracle timed on a bounded sample of the same workload.
 """
 from __future__ import annotations
 
@@ -53,8 +54,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-count", action="store_true", help="skip the work-counting step (roofline)")
     ap.add_argument("--copy-calibration", action="store_true",
-                    help="run only a device copy of 1 GiB (read) + 1 GiB (write): the reference kernel that scales "
-                         "rocprof FETCH_SIZE / WRITE_SIZE to bytes")
+                    help="run only an elementwise kernel reading 1 GiB and writing 1 GiB: the known-size kernel "
+                         "that scales rocprof FETCH_SIZE / WRITE_SIZE to bytes")
     return ap.parse_args()
 
 
