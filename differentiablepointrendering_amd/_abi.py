@@ -94,6 +94,7 @@ class Settings(C.Structure):
         ("debug_images", C.c_int32),
         ("gather_entry_budget", C.c_uint32),
         ("photon_map_capacity", C.c_uint32),
+        ("full_gradients", C.c_int32),
     ]
 
 
@@ -153,6 +154,7 @@ DEFAULT_SETTINGS = dict(
     debug_images=0,
     gather_entry_budget=0,
     photon_map_capacity=0,
+    full_gradients=0,
 )
 
 
@@ -168,7 +170,7 @@ def make_settings(d: Optional[dict] = None) -> Settings:
     for k, v in vals.items():
         if k in ("gather_radius", "q_null", "q_reflect", "q_transmit"):
             setattr(s, k, float(v))
-        elif k in ("spp_decorrelate", "debug_images"):
+        elif k in ("spp_decorrelate", "debug_images", "full_gradients"):
             setattr(s, k, int(bool(v)))
         else:
             setattr(s, k, int(v))

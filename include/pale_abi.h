@@ -129,6 +129,11 @@ typedef struct pale_settings {
                                   records, as the reference allocates); past
                                   it deposits replace stored photons by the
                                   reference's reservoir sampling            */
+    int32_t full_gradients;    /* "full_gradients": 0 = opacity only, as the
+                                  reference (the other gradient arrays are
+                                  zeros); 1 = also position, tangents,
+                                  scale, albedo and beta, through alpha_geom
+                                  and the BRDF of each adjoint event        */
 } pale_settings;
 
 /* Work counters of the last render calls (no reference counterpart; the

@@ -58,6 +58,8 @@ def lib():
         L.orc_apply_points.argtypes = [C.c_void_p, C.POINTER(_abi.PointsDesc)]
         L.orc_rebuild_bvh.argtypes = [C.c_void_p]
         L.orc_set_sample.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32]
+        L.orc_full_gradients.argtypes = [C.c_void_p, _fp]
+        L.orc_freeze_map.argtypes = [C.c_void_p, C.c_int32]
         _lib = L
     return _lib
 
@@ -158,6 +160,16 @@ class Oracle:
             raise RuntimeError(lib().orc_last_error().decode())
         return g
 
+    def full_gradients(self) -> Dict[str, np.ndarray]:
+        """Gradients of the last render_backward with full_gradients=1
+        (opacity comes from render_backward)."""
+        n = self.point_count
+        g = np.zeros((n, 16), np.float32)
+        if lib().orc_full_gradients(self.h, _p(g)) != 0:
+            raise RuntimeError("full gradients were not computed (full_gradients=0)")
+        return dict(position=g[:, 0:3].copy(), tangent_u=g[:, 3:6].copy(), tangent_v=g[:, 6:9].copy(),
+                    scale=g[:, 9:11].copy(), albedo=g[:, 11:14].copy(), beta=g[:, 15].copy())
+
     def photons(self):
         cnt = C.c_uint32(0)
         att = C.c_uint64(0)
@@ -196,6 +208,11 @@ class Oracle:
 
     def rebuild_bvh(self):
         lib().orc_rebuild_bvh(self.h)
+
+    def freeze_map(self, on: bool):
+        """Test mode: render_forward re-renders the cameras over the photon
+        map of the previous render (camera seeds unchanged)."""
+        lib().orc_freeze_map(self.h, 1 if on else 0)
 
     def set_sample(self, photon_stride: int, pixel_stride: int):
         """Bounded-sample mode (CPU baseline): photons/photon_stride photons,

@@ -245,6 +245,7 @@ struct Settings {
     float qNull, qReflect, qTransmit;
     bool decorrelate;
     uint32_t mapCapacity;
+    bool fullGradients;
 };
 
 struct Grid {
@@ -277,6 +278,10 @@ struct Renderer {
     std::vector<Material> materials;
     std::vector<Xf> xfs;
     std::vector<Instance> inst;
+    uint32_t cloudInstance = kInvalid;
+    // Test mode: render_forward keeps the previous photon map.
+    bool freezeMap = false;
+    uint64_t photonDraws = 0;
     std::vector<Point> points;
     std::vector<uint32_t> perm;  // perm[slot] = asset index before the build
     std::vector<Node> blas;
@@ -295,6 +300,10 @@ struct Renderer {
     bool gridConfigured = false;
     // Gradients.
     std::vector<float> gradOpacity;
+    // Full gradients (this build's extension; the reference returns zeros):
+    // per surfel position 3, tangent_u 3, tangent_v 3, scale 2, albedo 3,
+    // beta 1.
+    std::vector<float> gradFull;
     pale_stats stats{};
     // Bounded-sample mode of the CPU baseline: trace photons/photonStride
     // photons and only pixels with index % pixelStride == 0.
@@ -495,6 +504,7 @@ static void buildScene(Renderer& R) {
         for (int k = 0; k < 16; ++k) I.m[k] = (k % 5 == 0) ? 1.0f : 0.0f;
         R.xfs.push_back(Xf{I, I});
         R.inst.push_back(Instance{kCloud, 0, kInvalid, (uint32_t)R.xfs.size() - 1, 0});
+        R.cloudInstance = (uint32_t)R.inst.size() - 1;
     }
     // collectLights (SceneBuild.cpp:263-343): built before the BLAS build
     // reorders triangles; the emissive-triangle indices refer to that order.
@@ -1348,6 +1358,23 @@ static void postprocess(const std::vector<float>& hdr, float* ldr, size_t npx) {
 static void renderForward(Renderer& R, const std::vector<uint32_t>& camIdx, float* const* ldr, float* const* raw) {
     const Settings& S = R.S;
     std::mt19937_64 seedGen(S.seed);
+    if (R.freezeMap && R.mapValid) {
+        // Test mode: keep the photon map of the previous render and only
+        // re-render the cameras (same camera seeds).
+        for (uint64_t k = 0; k < R.photonDraws; ++k) (void)seedGen();
+        workClass(1);
+        std::vector<float> fb;
+        for (size_t k = 0; k < camIdx.size(); ++k) {
+            const uint32_t number = (uint32_t)seedGen();
+            const Camera& c = R.cams[camIdx[k]];
+            cameraGather(R, c, number, fb);
+            const size_t npx = (size_t)c.W * c.H;
+            if (raw && raw[k]) std::memcpy(raw[k], fb.data(), npx * 4 * sizeof(float));
+            if (ldr && ldr[k]) postprocess(fb, ldr[k], npx);
+        }
+        return;
+    }
+    uint64_t draws = 0;
     R.photonCounter = 0;
     R.photons.assign(R.photonCapacity, Photon{});
     const uint32_t P = std::max(1u, S.photons / R.photonStride);
@@ -1356,6 +1383,7 @@ static void renderForward(Renderer& R, const std::vector<uint32_t>& camIdx, floa
     workClass(0);
     for (uint32_t pass = 0; pass < S.forwardPasses; ++pass) {
         uint32_t number = (uint32_t)seedGen();
+        ++draws;
         std::vector<RayState> q, nq;
         q.reserve(P);
         // launchRayGenEmitterKernel (PrimalKernels.cpp:13-62)
@@ -1373,12 +1401,14 @@ static void renderForward(Renderer& R, const std::vector<uint32_t>& camIdx, floa
         }
         for (uint32_t b = 0; b < S.bounces && !q.empty(); ++b) {
             number = (uint32_t)seedGen();
+            ++draws;
             nq.clear();
             photonBounce(R, q, nq, number);
             q.swap(nq);
         }
     }
     R.photonCount = (uint32_t)std::min<uint64_t>(R.photonCounter, R.photonCapacity);
+    R.photonDraws = draws;
     buildGrid(R);
     R.mapValid = true;
     R.stats.grid_cells = R.grid.total;
@@ -1401,12 +1431,14 @@ struct Pending {
     uint32_t prim = kInvalid, inst = kInvalid;
     float alpha = 0.0f;
     F3 pos{0, 0, 0}, T{0, 0, 0};
+    F3 ro{0, 0, 0}, rd{0, 0, 0};  // the ray that hit `prim`
 };
 struct Completed {
     int kind = kNone;
     uint32_t prim = 0, inst = 0;
     float alpha = 0.0f;
     F3 pos{0, 0, 0}, normal{0, 0, 0}, T{0, 0, 0};
+    F3 ro{0, 0, 0}, rd{0, 0, 0};  // the ray that hit the surfel whose alpha the event weighs
     uint32_t endInst = 0, endPrim = 0;
     int endType = -1;
     float endAlpha = 0.0f, endPdf = 0.0f, endCos = 0.0f;
@@ -1418,6 +1450,111 @@ struct ARay {
     uint32_t bounce, pixel, pathId;
 };
 
+// ---------------------------------------------------- full gradients --
+// Forward-mode derivatives of alpha_geom (intersectSurfel, KernelHelpers.h:
+// 637-669) with respect to the 14 geometric parameters of one surfel:
+// center (0-2), tangent_u (3-5), tangent_v (6-8), s_u (9), s_v (10),
+// beta (11).  Dual numbers carry value and gradient through the same
+// expressions the test evaluates; the build's GPU code uses a hand-written
+// reverse pass, so the two agree only if both are right.
+struct Dual {
+    double v;
+    double g[12];
+};
+static Dual dconst(double v) {
+    Dual r{};
+    r.v = v;
+    return r;
+}
+static Dual dvar(double v, int i) {
+    Dual r = dconst(v);
+    r.g[i] = 1.0;
+    return r;
+}
+static Dual operator+(const Dual& a, const Dual& b) {
+    Dual r{};
+    r.v = a.v + b.v;
+    for (int i = 0; i < 12; ++i) r.g[i] = a.g[i] + b.g[i];
+    return r;
+}
+static Dual operator-(const Dual& a, const Dual& b) {
+    Dual r{};
+    r.v = a.v - b.v;
+    for (int i = 0; i < 12; ++i) r.g[i] = a.g[i] - b.g[i];
+    return r;
+}
+static Dual operator*(const Dual& a, const Dual& b) {
+    Dual r{};
+    r.v = a.v * b.v;
+    for (int i = 0; i < 12; ++i) r.g[i] = a.g[i] * b.v + a.v * b.g[i];
+    return r;
+}
+static Dual operator/(const Dual& a, const Dual& b) {
+    Dual r{};
+    r.v = a.v / b.v;
+    for (int i = 0; i < 12; ++i) r.g[i] = (a.g[i] * b.v - a.v * b.g[i]) / (b.v * b.v);
+    return r;
+}
+static Dual dsqrt(const Dual& a) {
+    Dual r{};
+    r.v = std::sqrt(a.v);
+    for (int i = 0; i < 12; ++i) r.g[i] = a.g[i] / (2.0 * r.v);
+    return r;
+}
+struct D3 {
+    Dual x, y, z;
+};
+static D3 operator+(const D3& a, const D3& b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+static D3 operator-(const D3& a, const D3& b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+static D3 operator*(const D3& a, const Dual& s) { return {a.x * s, a.y * s, a.z * s}; }
+static Dual ddot(const D3& a, const D3& b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static D3 dcross(const D3& a, const D3& b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+static D3 dnormalize(const D3& a) { return a * (dconst(1.0) / dsqrt(ddot(a, a))); }
+static D3 dconst3(F3 v) { return {dconst(v.x), dconst(v.y), dconst(v.z)}; }
+static D3 dvar3(F3 v, int i) { return {dvar(v.x, i), dvar(v.y, i + 1), dvar(v.z, i + 2)}; }
+
+// alpha_geom of surfel s for the object-space ray (o, d) and its gradient
+// (12 values, order above).  False if the ray does not hit the support.
+static bool alphaGradient(const Point& s, F3 o, F3 d, float grad[12]) {
+    const D3 p = dvar3(s.p, 0), tu = dvar3(s.tu, 3), tv = dvar3(s.tv, 6);
+    const Dual su = dvar(s.su, 9), sv = dvar(s.sv, 10);
+    const D3 uh = dnormalize(tu);
+    const D3 vh = dnormalize(tv - uh * ddot(uh, tv));
+    const D3 n = dnormalize(dcross(uh, vh));
+    const D3 O = dconst3(o), Dd = dconst3(d);
+    const Dual nd = ddot(n, Dd);
+    const Dual t = ddot(n, p - O) / nd;
+    const D3 r = (O + Dd * t) - p;
+    const Dual u = ddot(tu, r) / su, v = ddot(tv, r) / sv;
+    const Dual R2 = u * u + v * v;
+    if (!(R2.v < 1.0)) return false;
+    const double k = 4.0 * std::exp((double)s.beta);
+    const double a = std::pow(1.0 - R2.v, k);
+    const double dadR2 = -k * a / (1.0 - R2.v);
+    for (int i = 0; i < 11; ++i) grad[i] = (float)(dadR2 * R2.g[i]);
+    grad[11] = (float)(a * k * std::log(1.0 - R2.v));
+    return true;
+}
+
+// Adds the full gradient of one event: `w` is the event's scalar weight
+// sum_rgb(T * Lo) * invSpp (the factor of alpha_geom * eta in its
+// contribution), `sgn` its sign.
+static void addGeometryGradient(Renderer& R, uint32_t prim, F3 ro, F3 rd, float w) {
+    const Point& s = R.points[prim];
+    Ray wr;
+    wr.o = ro;
+    wr.d = rd;
+    const Ray obj = toObjectSpace(wr, R.xfs[R.inst[R.cloudInstance].xf]);
+    float g[12];
+    if (!alphaGradient(s, obj.o, obj.d, g)) return;
+    float* out = &R.gradFull[(size_t)prim * 16];
+    const float f = s.opacity * w;
+    for (int i = 0; i < 11; ++i) out[i] += g[i] * f;
+    out[15] += g[11] * f;
+}
+
 // launchAdjointTransportKernel (AdjointKernels.cpp:389-509), one event.
 static void transport(Renderer& R, const Completed& c, float invSpp) {
     if (c.kind == kProjection) {
@@ -1427,6 +1564,15 @@ static void transport(Renderer& R, const Completed& c, float invSpp) {
         const F3 Lr = fr * E;
         const F3 g = (c.alpha * c.T) * Lr;
         R.gradOpacity[c.prim] += fsum(g) * invSpp;
+        if (R.S.fullGradients) {
+            addGeometryGradient(R, c.prim, c.ro, c.rd, fsum(c.T * Lr) * invSpp);
+            // d/d(albedo_c) of alpha * eta * (alpha_r * albedo_c / pi) * E_c
+            const F3 ga = ((c.alpha * s.opacity * s.alpha_r * kInvPi) * (c.T * E)) * invSpp;
+            float* out = &R.gradFull[(size_t)c.prim * 16];
+            out[11] += ga.x;
+            out[12] += ga.y;
+            out[13] += ga.z;
+        }
     }
     if (c.kind == kNullT) {
         F3 base;
@@ -1449,6 +1595,7 @@ static void transport(Renderer& R, const Completed& c, float invSpp) {
         const F3 Lo = Le + Lr;
         const F3 g = ((-c.alpha) * c.T) * Lo;
         R.gradOpacity[c.prim] += fsum(g) * invSpp;
+        if (R.S.fullGradients) addGeometryGradient(R, c.prim, c.ro, c.rd, -(fsum(c.T * Lo) * invSpp));
     }
     if (c.kind == kReflectS) {
         const Material& m = R.materials[R.inst[c.inst].material];
@@ -1456,6 +1603,7 @@ static void transport(Renderer& R, const Completed& c, float invSpp) {
         const F3 p = divs(c.T, c.endPdf);
         const F3 g = (p * Lo) * c.endAlpha;
         R.gradOpacity[c.endPrim] += fsum(g) * invSpp;
+        if (R.S.fullGradients) addGeometryGradient(R, c.endPrim, c.ro, c.rd, fsum(p * Lo) * invSpp);
     }
 }
 
@@ -1468,6 +1616,7 @@ static void renderBackward(Renderer& R, const std::vector<uint32_t>& camIdx,
     const uint64_t number = 42;
     workClass(2);
     R.gradOpacity.assign(R.points.size(), 0.0f);
+    R.gradFull.assign(S.fullGradients ? R.points.size() * 16 : 0, 0.0f);
     const float invSpp = 1.0f / (float)S.adjointPasses;
     for (size_t k = 0; k < camIdx.size(); ++k) {
         const Camera& c = R.cams[camIdx[k]];
@@ -1535,6 +1684,10 @@ static void renderBackward(Renderer& R, const std::vector<uint32_t>& camIdx,
                             e.endPdf = pdf;
                             e.endType = ein.type;
                             e.endCos = std::fmax(0.0f, dot(dir, no));
+                            // Null: the ray that crossed `prim`; ReflectScatter:
+                            // this ray, which hit the endpoint surfel.
+                            e.ro = nullDone ? p.ro : rs.ray.o;
+                            e.rd = nullDone ? p.rd : rs.ray.d;
                             done.push_back(e);
                             pend[rs.pathId] = Pending{};
                         }
@@ -1579,6 +1732,8 @@ static void renderBackward(Renderer& R, const std::vector<uint32_t>& camIdx,
                                 p.alpha = h.alpha;
                                 p.pos = h.pos;
                                 p.T = divs(rs.T, S.qNull);
+                                p.ro = rs.ray.o;
+                                p.rd = rs.ray.d;
                                 pend[rs.pathId] = p;
                             }
                             nq.push_back(nx);
@@ -1603,6 +1758,8 @@ static void renderBackward(Renderer& R, const std::vector<uint32_t>& camIdx,
                                 e.pos = h.pos;
                                 e.normal = no;
                                 e.T = divs(rs.T, S.qReflect);
+                                e.ro = rs.ray.o;
+                                e.rd = rs.ray.d;
                                 done.push_back(e);
                             }
                             nq.push_back(nx);
@@ -1653,6 +1810,7 @@ static Settings toSettings(const pale_settings& s) {
     S.qTransmit = s.q_transmit;
     S.decorrelate = s.spp_decorrelate != 0;
     S.mapCapacity = s.photon_map_capacity;
+    S.fullGradients = s.full_gradients != 0;
     return S;
 }
 
@@ -1859,6 +2017,20 @@ int orc_render_backward(void* h, uint32_t n, const char* const* names, const flo
     }
 }
 
+// Full gradients of the last backward pass, 16 floats per surfel: position
+// (3), tangent_u (3), tangent_v (3), scale (2), albedo (3), unused, beta.
+int orc_full_gradients(void* h, float* out) {
+    try {
+        Renderer& R = *static_cast<Renderer*>(h);
+        if (R.gradFull.empty()) return PALE_ERR_INVALID_ARG;
+        std::memcpy(out, R.gradFull.data(), R.gradFull.size() * sizeof(float));
+        return 0;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return PALE_ERR_INTERNAL;
+    }
+}
+
 int orc_get_photons(void* h, uint32_t* count, uint64_t* attempted, float* pos, float* power, float* dir) {
     Renderer& R = *static_cast<Renderer*>(h);
     *count = R.photonCount;
@@ -1926,6 +2098,11 @@ int orc_apply_points(void* h, const pale_points_desc* p) {
         a.beta = q.beta;
     }
     setScene(R);
+    return 0;
+}
+
+int orc_freeze_map(void* h, int32_t on) {
+    static_cast<Renderer*>(h)->freezeMap = on != 0;
     return 0;
 }
 

@@ -71,3 +71,70 @@ def test_fd_opacity_empty_scene():
         assert fd < 0 and an < 0, (v, an, fd)
         assert err <= 1e-4 or rel <= 0.05, (v, an, fd, rel)  # the reference's bar
         assert rel <= 0.02, (v, an, fd, rel)  # this estimator's observed accuracy, with margin
+
+
+def test_fd_full_gradients_frozen_photon_map():
+    # Full gradients (position, tangents, scale, albedo, beta; this build's
+    # extension: the reference's adjoint fills only opacity) differentiate
+    # alpha_geom and the BRDF of each adjoint event, not the irradiance
+    # estimate's dependence on where the hit lands.  With the photon map held
+    # fixed and parameters that keep the hit point in place (in-plane
+    # position, scales, in-plane rotation, beta, albedo, opacity), central
+    # differences of the loss must match them.  The surfel is anisotropic and
+    # smooth at its rim (beta = 1), so the footprint has no jump.
+    base = dict(photons=100_000, bounces=5, forward_passes=5, gather_passes=4, seed=42, spp_decorrelate=1)
+    from oracle.oracle import Oracle
+
+    meshes, cams = assets.load_xml(os.path.join(ROOT, "GradientTests", "empty", "empty.xml"), ROOT)
+    ini = assets.load_points(os.path.join(ROOT, "GradientTests", "empty", "initial.ply"))
+    ini["opacity"][:] = 0.6
+    ini["beta"][:] = 1.0
+    ini["scale"][0] = [0.9, 0.6]
+    tgt = {k: v.copy() for k, v in ini.items()}
+    tgt["position"][0] += [0.05, 0.0, -0.04]
+    tgt["scale"][0] *= 0.9
+    tgt["beta"][0] += 0.5
+    tgt["albedo"][0] = [0.5, 0.6, 0.7]
+    target = Oracle(meshes, cams, tgt, dict(base, adjoint_bounces=0, adjoint_passes=0)).render_forward()[
+        "camera1_raw"][..., :3].copy()
+    o = Oracle(meshes, cams, ini, dict(base, adjoint_bounces=1, adjoint_passes=64, full_gradients=1))
+    p0 = o.get_points()
+    o.render_forward()
+    o.freeze_map(True)
+
+    def loss(p):
+        o.apply_points(p)
+        o.rebuild_bvh()
+        img = o.render_forward()["camera1_raw"][..., :3]
+        return 0.5 * float(np.mean((img - target) ** 2)), img
+
+    _, img = loss(p0)
+    diff = img - target
+    g_opacity = o.render_backward({"camera1": (diff / diff.size).astype(np.float32)})
+    g = o.full_gradients()
+    g["opacity"] = g_opacity
+    tu, tv = p0["tangent_u"][0], p0["tangent_v"][0]
+
+    def moved(key, comp, dv):
+        q = {k: v.copy() for k, v in p0.items()}
+        if key == "rotation":  # in-plane rotation of the tangent frame
+            c, s = np.cos(dv), np.sin(dv)
+            q["tangent_u"][0] = c * tu + s * tv
+            q["tangent_v"][0] = -s * tu + c * tv
+        elif comp is None:
+            q[key][0] += dv
+        else:
+            q[key][0, comp] += dv
+        return q
+
+    cases = [("opacity", None, 5e-3, 0.05), ("beta", None, 1e-2, 0.05), ("position", 0, 2e-3, 0.05),
+             ("position", 2, 2e-3, 0.05), ("scale", 0, 2e-3, 0.05), ("scale", 1, 2e-3, 0.05),
+             ("albedo", 1, 1e-2, 0.05), ("rotation", None, 2e-3, 0.3)]
+    for key, comp, h, tol in cases:
+        fd = (loss(moved(key, comp, h))[0] - loss(moved(key, comp, -h))[0]) / (2 * h)
+        if key == "rotation":
+            an = float(g["tangent_u"][0] @ tv - g["tangent_v"][0] @ tu)
+        else:
+            an = float(g[key][0] if comp is None else g[key][0, comp])
+        assert np.sign(an) == np.sign(fd), (key, comp, an, fd)
+        assert abs(an - fd) <= tol * abs(fd), (key, comp, an, fd)
