@@ -31,6 +31,7 @@ struct DevScene {
     uint32_t nTlas;
     uint32_t nLights;
     uint32_t stackDepth;
+    uint32_t cloudInst;  // instance of the point cloud (kInvalid if none)
 };
 
 struct DevGrid {
@@ -351,6 +352,57 @@ __device__ bool intersectScene(const DevScene& S, F3 o, F3 d, const LaneStack& s
         }
     }
     return found;
+}
+
+// Gradient of alpha_geom (intersectSurfel, KernelHelpers.h:637-669) with
+// respect to the surfel's center, raw tangents, scales and beta, for the
+// object-space ray (o, d): a reverse pass through the test's expressions
+// (orthonormal frame from normalize(t_u) and Gram-Schmidt t_v, plane hit,
+// (u, v) from the raw tangents, alpha = (1 - r^2)^(4 e^beta)).  g[0..2]
+// center, g[3..5] t_u, g[6..8] t_v, g[9] s_u, g[10] s_v, g[11] beta.  False
+// if the ray misses the support.
+__device__ bool alphaGradient(const SurfelGeo& s, F3 o, F3 d, float g[12]) {
+    const F3 p = mk3(s.px, s.py, s.pz), tu = mk3(s.tux, s.tuy, s.tuz), tv = mk3(s.tvx, s.tvy, s.tvz);
+    const float lu = sqrtf(dot(tu, tu));
+    const F3 uh = tu * (1.0f / lu);
+    const float sdot = dot(uh, tv);
+    const F3 wv = tv - uh * sdot;
+    const float lw = sqrtf(dot(wv, wv));
+    const F3 vh = wv * (1.0f / lw);
+    const F3 m = cross(uh, vh);
+    const float lm = sqrtf(dot(m, m));
+    const F3 n = m * (1.0f / lm);
+    const float nd = dot(n, d);
+    if (fabsf(nd) < 1e-6f) return false;
+    const F3 po = p - o;
+    const float t = dot(n, po) / nd;
+    const F3 r = (o + d * t) - p;
+    const float U = dot(tu, r) / s.su, V = dot(tv, r) / s.sv;
+    const float R = U * U + V * V;
+    if (!(R < 1.0f)) return false;
+    const float k = s.e4b;
+    const float a = pow_r(1.0f - R, k);
+    const float dadR = -k * a / (1.0f - R);
+    const float gU = dadR * 2.0f * U, gV = dadR * 2.0f * V;
+    const F3 gr = tu * (gU / s.su) + tv * (gV / s.sv);
+    const float gt = dot(gr, d);
+    const float gnp = gt / nd, gnd = -gt * t / nd;
+    const F3 gp = n * gnp - gr;
+    const F3 gn = po * gnp + d * gnd;
+    const F3 gm = (gn - n * dot(n, gn)) * (1.0f / lm);
+    F3 guh = cross(vh, gm);
+    const F3 gvh = cross(gm, uh);
+    const F3 gw = (gvh - vh * dot(vh, gvh)) * (1.0f / lw);
+    F3 gtv = r * (gV / s.sv) + (gw - uh * dot(uh, gw));
+    guh = guh - gw * sdot - tv * dot(uh, gw);
+    const F3 gtu = r * (gU / s.su) + (guh - uh * dot(uh, guh)) * (1.0f / lu);
+    g[0] = gp.x; g[1] = gp.y; g[2] = gp.z;
+    g[3] = gtu.x; g[4] = gtu.y; g[5] = gtu.z;
+    g[6] = gtv.x; g[7] = gtv.y; g[8] = gtv.z;
+    g[9] = -gU * U / s.su;
+    g[10] = -gV * V / s.sv;
+    g[11] = a * (float)log((double)(1.0f - R)) * k;
+    return true;
 }
 
 // buildIntersectionNormal (KernelHelpers.h:480-503)

@@ -51,20 +51,27 @@ struct PhotonArrays {
 };
 
 // Adjoint gradient event: e0 = gather position + kind, e1 = gather normal +
-// target surfel, e2 = throughput + alpha, e3 = kind-specific words.
+// target surfel, e2 = throughput + alpha, e3 = kind-specific words; with
+// full gradients also e4 / e5 = origin / direction of the ray that hit the
+// surfel whose alpha the event weighs (null pointers otherwise).
 struct EventArrays {
     float4* e0;
     float4* e1;
     float4* e2;
     float4* e3;
+    float4* e4;
+    float4* e5;
 };
 
 // Pending adjoint state per path: p0 = position + kind, p1 = throughput +
-// alpha, p2 = (primitive, instance) bits.
+// alpha, p2 = (primitive, instance) bits; p3 / p4 = the ray of a pending
+// NullTransmittance (full gradients only).
 struct PendingArrays {
     float4* p0;
     float4* p1;
     uint2* p2;
+    float4* p3;
+    float4* p4;
 };
 
 // Camera gather, per pixel: ot = ray origin + throughput tr, dl = direction +
@@ -96,6 +103,7 @@ struct EntryArrays {
 struct TraceParams {
     uint32_t rrStart;
     float qNull, qReflect, qTransmit;
+    uint32_t fullGradients;  // record hit rays for the full gradients
 };
 
 // Photon-map reservoir (depositPhotonSurface past capacity): per queue item
@@ -169,8 +177,14 @@ void launchAdjTrace(hipStream_t s, const DevScene& S, DevState* st, uint32_t max
 void launchEventSort(hipStream_t s, const GridParams& g, DevState* st, uint32_t poolCap, EventArrays ev,
                      uint32_t* cell, uint32_t* sorted, uint32_t* cellCount, uint32_t* cellStart, uint32_t* tiles,
                      uint32_t* scratchTotal);
+// gradFull (16 floats per surfel: position 3, tangent_u 3, tangent_v 3,
+// scale 2, albedo 3, unused, beta) is null unless full gradients are on.
 void launchAdjTransport(hipStream_t s, const DevScene& S, const DevGrid& G, DevState* st, uint32_t poolCap,
-                        EventArrays ev, const uint32_t* sorted, float invSpp, float* gradOpacity, bool count);
+                        EventArrays ev, const uint32_t* sorted, float invSpp, float* gradOpacity, float* gradFull,
+                        bool count);
+// Split gradFull into the ABI's arrays (any may be null).
+void launchSplitGradients(hipStream_t s, const float* gradFull, uint32_t n, float* position, float* tangentU,
+                          float* tangentV, float* scale, float* albedo, float* beta);
 void launchSetOpacity(hipStream_t s, SurfelShade* shade, const float* opacity, uint32_t n);
 void launchGatherPoints(hipStream_t s, const DevGrid& G, uint32_t n, const float* q, const float* nrm, float* out);
 void launchTraceRays(hipStream_t s, const DevScene& S, uint32_t n, const float* o, const float* d, int32_t* inst,

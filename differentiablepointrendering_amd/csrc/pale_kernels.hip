@@ -444,11 +444,16 @@ __global__ __launch_bounds__(kScanThreads) void k_scatter_events(const uint8_t* 
                                                                  const uint32_t* base, EventArrays src,
                                                                  EventArrays dst, uint32_t cap) {
     const uint32_t n = loadCount(count, maxItems);
+    const bool rays = src.e4 != nullptr;
     scatterTile(flags, n, tiles, *base, cap, [&](uint32_t i, uint32_t j) {
         dst.e0[j] = src.e0[i];
         dst.e1[j] = src.e1[i];
         dst.e2[j] = src.e2[i];
         dst.e3[j] = src.e3[i];
+        if (rays) {
+            dst.e4[j] = src.e4[i];
+            dst.e5[j] = src.e5[i];
+        }
     });
 }
 
@@ -882,11 +887,19 @@ __global__ __launch_bounds__(kTraceBlock) void k_adj_trace(DevScene S, DevState*
                         ev.e1[i] = f4(on, u2f(h.prim));
                         ev.e2[i] = f4(Tp, h.alpha);
                         ev.e3[i] = make_float4(u2f(S.inst[p2.y].material), pdf, endCos, 0.0f);
+                        if (tp.fullGradients) {  // this ray hit the endpoint surfel
+                            ev.e4[i] = f4(o, 0.0f);
+                            ev.e5[i] = f4(d, 0.0f);
+                        }
                     } else {
                         ev.e1[i] = f4(on, u2f(p2.x));
                         ev.e2[i] = f4(Tp, p1.w);
                         ev.e3[i] = make_float4(u2f(cloud ? kGeoCloud : kGeoMesh), u2f(cloud ? h.prim : I.material),
                                                endCos, 0.0f);
+                        if (tp.fullGradients) {  // the ray that crossed the pending surfel
+                            ev.e4[i] = pend.p3[path];
+                            ev.e5[i] = pend.p4[path];
+                        }
                     }
                     vf = 1;
                     pend.p0[path] = make_float4(0.0f, 0.0f, 0.0f, u2f(kNone));
@@ -950,12 +963,20 @@ __global__ __launch_bounds__(kTraceBlock) void k_adj_trace(DevScene S, DevState*
                     pend.p0[path] = f4(h.pos, u2f(newKind));
                     pend.p1[path] = f4(pT, pAlpha);
                     pend.p2[path] = pIds;
+                    if (tp.fullGradients && newKind == kNullT) {
+                        pend.p3[path] = f4(o, 0.0f);
+                        pend.p4[path] = f4(d, 0.0f);
+                    }
                 }
                 if (projection && path < pendCap) {
                     ev.e0[i] = f4(h.pos, u2f(kProjection));
                     ev.e1[i] = f4(on, u2f(h.prim));
                     ev.e2[i] = f4(divs(T, tp.qReflect), h.alpha);
                     ev.e3[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                    if (tp.fullGradients) {
+                        ev.e4[i] = f4(o, 0.0f);
+                        ev.e5[i] = f4(d, 0.0f);
+                    }
                     vf = 1;
                 }
                 ef = 1;
@@ -998,11 +1019,30 @@ __global__ __launch_bounds__(256) void k_event_scatter(const DevState* st, uint3
     sorted[cellStart[c] + atomicSub(&cellCount[c], 1u) - 1u] = i;
 }
 
+// Full-gradient contribution of one event: d(alpha_geom)/d(geometry) of
+// surfel `prim` for the recorded ray, times eta * w (w = the event's weight
+// sum_rgb(T * Lo) * invSpp with its sign).
+__device__ void addGeometryGradient(const DevScene& S, uint32_t prim, float4 ro, float4 rd, float w, float* full) {
+    const DevInstance& I = S.inst[S.cloudInst];
+    float hv[4];
+    mulv4(I.w2o, ro.x, ro.y, ro.z, 1.0f, hv);
+    const F3 oo = divs(mk3(hv[0], hv[1], hv[2]), hv[3]);
+    mulv4(I.w2o, rd.x, rd.y, rd.z, 0.0f, hv);
+    const F3 od = normalize(mk3(hv[0], hv[1], hv[2]));
+    float g[12];
+    if (!alphaGradient(S.geo[prim], oo, od, g)) return;
+    const float f = S.shade[prim].opacity * w;
+    float* out = full + (size_t)prim * 16;
+#pragma unroll
+    for (int k = 0; k < 11; ++k) atomicAdd(out + k, g[k] * f);
+    atomicAdd(out + 15, g[11] * f);
+}
+
 // launchAdjointTransportKernel (AdjointKernels.cpp:389-509).
 template <bool kCount>
 __global__ __launch_bounds__(256) void k_adj_transport(DevScene S, DevGrid G, DevState* st, uint32_t poolCap,
                                                        EventArrays ev, const uint32_t* sorted, float invSpp,
-                                                       float* grad) {
+                                                       float* grad, float* full) {
     const uint32_t n = min(st->evPool, poolCap);
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const bool active = i < n;
@@ -1022,6 +1062,14 @@ __global__ __launch_bounds__(256) void k_adj_transport(DevScene S, DevGrid G, De
             const F3 Lr = fr * E;
             const F3 g = (a * T) * Lr;
             val = fsum(g) * invSpp;
+            if (full) {
+                addGeometryGradient(S, target, ev.e4[j], ev.e5[j], fsum(T * Lr) * invSpp, full);
+                const F3 ga = ((a * sh.opacity * sh.alpha_r * kInvPi) * (T * E)) * invSpp;
+                float* out = full + (size_t)target * 16;
+                atomicAdd(out + 11, ga.x);
+                atomicAdd(out + 12, ga.y);
+                atomicAdd(out + 13, ga.z);
+            }
         } else if (kind == kNullT) {
             F3 base;
             float power;
@@ -1040,12 +1088,14 @@ __global__ __launch_bounds__(256) void k_adj_transport(DevScene S, DevGrid G, De
             const F3 Lo = Le + Lr;
             const F3 g = ((-a) * T) * Lo;
             val = fsum(g) * invSpp;
+            if (full) addGeometryGradient(S, target, ev.e4[j], ev.e5[j], -(fsum(T * Lo) * invSpp), full);
         } else if (kind == kReflectS) {
             const DevMaterial m = S.mats[f2u(e3.x)];
             const F3 Lo = (E * mk3(m.r, m.g, m.b)) * kInvPi;
             const F3 p = divs(T, e3.y);
             const F3 g = (p * Lo) * a;
             val = fsum(g) * invSpp;
+            if (full) addGeometryGradient(S, target, ev.e4[j], ev.e5[j], fsum(p * Lo) * invSpp, full);
         }
         atomicAdd(&grad[target], val);
     }
@@ -1072,6 +1122,27 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_rays(DevScene S, uint32_t
     pos[3 * i + 1] = hit ? h.pos.y : 0.0f;
     pos[3 * i + 2] = hit ? h.pos.z : 0.0f;
     alpha[i] = hit ? h.alpha : 0.0f;
+}
+
+// Full-gradient array (16 floats per surfel) into the ABI's arrays.
+__global__ __launch_bounds__(256) void k_split_gradients(const float* full, uint32_t n, float* position,
+                                                         float* tangentU, float* tangentV, float* scale,
+                                                         float* albedo, float* beta) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float* g = full + (size_t)i * 16;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        if (position) position[3 * (size_t)i + k] = g[k];
+        if (tangentU) tangentU[3 * (size_t)i + k] = g[3 + k];
+        if (tangentV) tangentV[3 * (size_t)i + k] = g[6 + k];
+        if (albedo) albedo[3 * (size_t)i + k] = g[11 + k];
+    }
+    if (scale) {
+        scale[2 * (size_t)i] = g[9];
+        scale[2 * (size_t)i + 1] = g[10];
+    }
+    if (beta) beta[i] = g[15];
 }
 
 // Opacity of every surfel from device memory (BVH order).
@@ -1278,13 +1349,20 @@ void launchEventSort(hipStream_t s, const GridParams& g, DevState* st, uint32_t 
     k_event_scatter<<<blocks(poolCap, 256), 256, 0, s>>>(st, poolCap, cell, cellStart, cellCount, sorted);
 }
 void launchAdjTransport(hipStream_t s, const DevScene& S, const DevGrid& G, DevState* st, uint32_t poolCap,
-                        EventArrays ev, const uint32_t* sorted, float invSpp, float* gradOpacity, bool count) {
+                        EventArrays ev, const uint32_t* sorted, float invSpp, float* gradOpacity, float* gradFull,
+                        bool count) {
     if (!poolCap) return;
     if (count)
-        k_adj_transport<true><<<blocks(poolCap, 256), 256, 0, s>>>(S, G, st, poolCap, ev, sorted, invSpp, gradOpacity);
+        k_adj_transport<true><<<blocks(poolCap, 256), 256, 0, s>>>(S, G, st, poolCap, ev, sorted, invSpp, gradOpacity,
+                                                                  gradFull);
     else
         k_adj_transport<false><<<blocks(poolCap, 256), 256, 0, s>>>(S, G, st, poolCap, ev, sorted, invSpp,
-                                                                   gradOpacity);
+                                                                   gradOpacity, gradFull);
+}
+void launchSplitGradients(hipStream_t s, const float* gradFull, uint32_t n, float* position, float* tangentU,
+                          float* tangentV, float* scale, float* albedo, float* beta) {
+    if (n) k_split_gradients<<<blocks(n, 256), 256, 0, s>>>(gradFull, n, position, tangentU, tangentV, scale, albedo,
+                                                            beta);
 }
 void launchTraceRays(hipStream_t s, const DevScene& S, uint32_t n, const float* o, const float* d, int32_t* inst,
                      int32_t* prim, float* t, float* pos, float* alpha) {

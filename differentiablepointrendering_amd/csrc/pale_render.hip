@@ -111,7 +111,9 @@ struct pale_handle {
     DevMem qA[3], qB[3], stg[3], extFlags, depStg[3], depFlags, tiles, counts;
     // Adjoint.
     uint32_t adjCap = 0, poolCap = 0;
-    DevMem pend0, pend1, pend2, evStg[4], evCmp[4], evCell, evSorted, evFlags, dloss, gradOpacity;
+    DevMem pend0, pend1, pend2, pend3, pend4, evStg[6], evCmp[6], evCell, evSorted, evFlags, dloss, gradOpacity,
+        gradFull;
+    bool adjFull = false;  // event rays allocated (full gradients)
     // Camera gather: pixel state, entries of one round, their cell sort.
     uint32_t pxCap = 0, entryCap = 0;
     DevMem pxOt, pxDl, pxCs, pxCnt, pxOff, pxTiles, entQa, entQb, entComp, entE, entOrd, entCell, entSorted,
@@ -181,11 +183,15 @@ struct pale_handle {
     static PhotonArrays pa(DevMem& a, DevMem& b, DevMem& c) {
         return PhotonArrays{a.as<float4>(), b.as<float4>(), c.as<float>()};
     }
-    static EventArrays ea(DevMem* m) {
-        return EventArrays{m[0].as<float4>(), m[1].as<float4>(), m[2].as<float4>(), m[3].as<float4>()};
+    EventArrays ea(DevMem* m) const {
+        const bool full = S.full_gradients != 0;
+        return EventArrays{m[0].as<float4>(), m[1].as<float4>(), m[2].as<float4>(), m[3].as<float4>(),
+                           full ? m[4].as<float4>() : nullptr, full ? m[5].as<float4>() : nullptr};
     }
 
-    TraceParams tp() const { return TraceParams{S.rr_start, S.q_null, S.q_reflect, S.q_transmit}; }
+    TraceParams tp() const {
+        return TraceParams{S.rr_start, S.q_null, S.q_reflect, S.q_transmit, S.full_gradients ? 1u : 0u};
+    }
 
     DevGrid devGrid() const {
         DevGrid g{};
@@ -239,6 +245,9 @@ struct pale_handle {
         scene.lights = lights.as<DevLight>();
         scene.emTris = emTris.as<DevEmTri>();
         scene.nInst = (uint32_t)img.instances.size();
+        scene.cloudInst = kInvalid;
+        for (uint32_t i = 0; i < img.instances.size(); ++i)
+            if (img.instances[i].type == kGeoCloud) scene.cloudInst = i;
         scene.nTlas = (uint32_t)img.tlas.size();
         scene.nLights = (uint32_t)img.lights.size();
         scene.stackDepth = img.stackDepth;
@@ -330,6 +339,7 @@ struct pale_handle {
     void ensureAdjoint(uint32_t npx) {
         ensureQueues(nextPow2(npx));
         entStart.reserve(((size_t)grid.total + 1) * 4);
+        if (S.full_gradients && !adjFull) adjCap = 0;  // also allocate the event rays
         if (npx <= adjCap) return;
         pend0.reserve((size_t)npx * sizeof(float4));
         pend1.reserve((size_t)npx * sizeof(float4));
@@ -337,9 +347,15 @@ struct pale_handle {
         // Event pool: the events of up to 24 bounces of npx paths wait for
         // one cell-sorted transport launch.
         poolCap = (uint32_t)std::min<uint64_t>((uint64_t)npx * 24u, 1u << 28);
-        for (int k = 0; k < 4; ++k) {
+        const int arrays = S.full_gradients ? 6 : 4;
+        for (int k = 0; k < arrays; ++k) {
             evStg[k].reserve((size_t)npx * sizeof(float4));
             evCmp[k].reserve((size_t)poolCap * sizeof(float4));
+        }
+        if (S.full_gradients) {
+            pend3.reserve((size_t)npx * sizeof(float4));
+            pend4.reserve((size_t)npx * sizeof(float4));
+            adjFull = true;
         }
         evCell.reserve((size_t)poolCap * 4);
         evSorted.reserve((size_t)poolCap * 4);
@@ -571,6 +587,10 @@ struct pale_handle {
             HIPCHECK(hipMemcpyAsync(dloss.as<float>() + off[k], src[k], (off[k + 1] - off[k]) * sizeof(float),
                                     srcOnDevice ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, stream));
         launchFillF(stream, gradOpacity.as<float>(), N, 0.0f);
+        if (S.full_gradients) {
+            gradFull.reserve(std::max<size_t>(1, (size_t)N * 16) * sizeof(float));
+            launchFillF(stream, gradFull.as<float>(), N * 16, 0.0f);
+        }
         HIPCHECK(hipStreamSynchronize(stream));
         tReset();
         HIPCHECK(hipEventRecord(ev0, stream));
@@ -590,7 +610,8 @@ struct pale_handle {
                             cellCount.as<uint32_t>(), entStart.as<uint32_t>(), gridTiles.as<uint32_t>(),
                             gridTotal.as<uint32_t>());
             launchAdjTransport(stream, scene, G, st, poolCap, ea(evCmp), evSorted.as<uint32_t>(), invSpp,
-                               gradOpacity.as<float>(), countWork);
+                               gradOpacity.as<float>(), S.full_gradients ? gradFull.as<float>() : nullptr,
+                               countWork);
             tEnd(tt);
             launchFill(stream, stField(offsetof(DevState, evPool)), 1, 0u);
             pooledBound = 0;
@@ -599,7 +620,8 @@ struct pale_handle {
             const HostCamera& c = cams[camIdx[k]];
             const uint32_t npx = c.W * c.H;
             const DevCamera dc = devCam(c);
-            const PendingArrays pend{pend0.as<float4>(), pend1.as<float4>(), pend2.as<uint2>()};
+            const PendingArrays pend{pend0.as<float4>(), pend1.as<float4>(), pend2.as<uint2>(), pend3.as<float4>(),
+                                     pend4.as<float4>()};
             for (uint32_t spp = 0; spp < S.adjoint_passes; ++spp) {
                 DevMem* in = qA;
                 DevMem* out = qB;
@@ -852,10 +874,20 @@ int pale_render_backward(pale_handle* h, uint32_t n, const char* const* camera_n
         const size_t N = h->bp.points.size();
         const size_t widths[8] = {3, 3, 3, 2, 3, 1, 1, 1};
         if (grads) {
+            std::vector<float> full;
+            if (h->S.full_gradients && N) {
+                full.resize(N * 16);
+                HIPCHECK(hipMemcpy(full.data(), h->gradFull.p, N * 16 * sizeof(float), hipMemcpyDeviceToHost));
+            }
+            // position, tangent_u, tangent_v, scale, albedo, opacity, beta, shape
+            const int first[8] = {0, 3, 6, 9, 11, -1, 15, -1};
             for (int g = 0; g < 8; ++g) {
                 if (!grads[g]) continue;
                 if (g == 5) {
                     if (N) HIPCHECK(hipMemcpy(grads[g], h->gradOpacity.p, N * sizeof(float), hipMemcpyDeviceToHost));
+                } else if (!full.empty() && first[g] >= 0) {
+                    for (size_t i = 0; i < N; ++i)
+                        for (size_t k = 0; k < widths[g]; ++k) grads[g][i * widths[g] + k] = full[i * 16 + first[g] + k];
                 } else {
                     std::memset(grads[g], 0, N * widths[g] * sizeof(float));
                 }
@@ -910,15 +942,19 @@ int pale_render_backward_device(pale_handle* h, uint32_t n, const char* const* c
         h->renderBackward(idx, src, true);
         const uint32_t N = (uint32_t)h->bp.points.size();
         const uint32_t widths[8] = {3, 3, 3, 2, 3, 1, 1, 1};
-        if (grads_dev) {
+        if (grads_dev && N) {
+            const bool full = h->S.full_gradients != 0;
             for (int g = 0; g < 8; ++g) {
-                if (!grads_dev[g] || !N) continue;
+                if (!grads_dev[g]) continue;
                 if (g == 5)
                     HIPCHECK(hipMemcpyAsync(grads_dev[g], h->gradOpacity.p, (size_t)N * sizeof(float),
                                             hipMemcpyDeviceToDevice, h->stream));
-                else
+                else if (!full || g == 7)
                     launchFillF(h->stream, grads_dev[g], N * widths[g], 0.0f);
             }
+            if (full)
+                launchSplitGradients(h->stream, h->gradFull.as<float>(), N, grads_dev[0], grads_dev[1], grads_dev[2],
+                                     grads_dev[3], grads_dev[4], grads_dev[6]);
         }
         HIPCHECK(hipStreamSynchronize(h->stream));
         HIPCHECK(hipGetLastError());

@@ -261,3 +261,27 @@ def test_duplicates_and_points_moved_without_rebuild():
     orc.apply_points(p)
     a, b = gpu.render_forward()["camera1_raw"], orc.render_forward()["camera1_raw"]
     assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def test_full_gradients_match_oracle():
+    # full_gradients=1: position, tangents, scale, albedo and beta gradients
+    # (reverse pass on the GPU, forward-mode duals in the oracle; FP32 atomics
+    # add in another order, so a tolerance relative to each array's largest
+    # entry).  Opacity keeps the reference's gradient.
+    spec = scene.config_scene(0, n=4096, width=64, height=64)
+    gpu, orc = _pair(spec, dict(SMALL, full_gradients=1))
+    ib = orc.render_forward()
+    gpu.render_forward()
+    raw = ib["camera1_raw"][..., :3]
+    dl = ((raw - 0.5 * raw) / raw.size).astype(np.float32)
+    g, _ = gpu.render_backward({"camera1": dl})
+    go = orc.render_backward({"camera1": dl})
+    full = orc.full_gradients()
+    scale = np.abs(go).max()
+    np.testing.assert_allclose(g["opacity"], go, rtol=1e-4, atol=1e-4 * scale)
+    for k in ("position", "tangent_u", "tangent_v", "scale", "albedo", "beta"):
+        ref = full[k]
+        m = np.abs(ref).max()
+        assert m > 0, k
+        np.testing.assert_allclose(g[k], ref, rtol=1e-3, atol=1e-3 * m, err_msg=k)
+    assert not np.any(g["shape"])

@@ -71,3 +71,18 @@ def test_autograd_opacity_gradient():
     scale = np.abs(g["opacity"]).max()
     assert scale > 0
     np.testing.assert_allclose(op.grad.cpu().numpy(), g["opacity"], rtol=1e-4, atol=1e-4 * scale)
+
+
+def test_device_full_gradients_match_host():
+    import torch
+
+    spec = scene.config_scene(0, n=4096, width=64, height=64)
+    r = _renderer(spec, dict(photons=20_000, full_gradients=1))
+    raw = r.render_forward()["camera1_raw"][..., :3]
+    dl = (raw / raw.size).astype(np.float32)
+    gh, _ = r.render_backward({"camera1": dl})
+    gd = r.render_backward_device({"camera1": torch.from_numpy(dl).cuda()})
+    for k in ("position", "tangent_u", "scale", "albedo", "beta", "opacity"):
+        m = np.abs(gh[k]).max()
+        assert m > 0, k
+        np.testing.assert_allclose(gd[k].cpu().numpy(), gh[k], rtol=1e-3, atol=1e-3 * m, err_msg=k)
