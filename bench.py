@@ -4,7 +4,9 @@
 One step = one forward pass (photon tracing, photon grid, camera gather,
 post-processing) plus one adjoint pass of the L2 loss against a target image
 of the same scene with jittered centers (the reference's training-step pair,
-python/training.py:621-662), on the plain single-view workload:
+python/training.py:621-662), returning the image and the gradients of every
+surfel parameter (--gradients opacity: only the opacity gradient, as the
+reference's adjoint), on the plain single-view workload:
 
   1,048,576 clustered surfels (64 Gaussian clusters, sigma 0.1), 800x800,
   16 gather passes and 16 adjoint samples per pixel, depth 6, 2M photons,
@@ -49,6 +51,8 @@ def parse():
     ap.add_argument("--photons", type=int, default=None)
     ap.add_argument("--spp", type=int, default=None)
     ap.add_argument("--decorrelate", action="store_true", help="independent seed per gather pass")
+    ap.add_argument("--gradients", choices=("full", "opacity"), default="full",
+                    help="full: all parameter gradients (the benchmark's workload); opacity: the reference's adjoint")
     ap.add_argument("--cpu-photon-stride", type=int, default=8)
     ap.add_argument("--cpu-pixel-stride", type=int, default=80)
     ap.add_argument("--no-cpu", action="store_true")
@@ -113,6 +117,7 @@ def main():
     if a.spp:
         settings["gather_passes"] = settings["adjoint_passes"] = a.spp
     settings["spp_decorrelate"] = 1 if a.decorrelate else 0
+    settings["full_gradients"] = 1 if a.gradients == "full" else 0
     cam = spec.cameras[0]
     W, H = cam.width, cam.height
     n = len(spec.points["position"])
@@ -233,10 +238,11 @@ def main():
         "config": {
             "workload": f"plain single view: {n} clustered surfels, {W}x{H}, {settings['gather_passes']} spp, "
                         f"depth {settings['bounces']}, {settings['photons']} photons, radius "
-                        f"{settings['gather_radius']}, forward + opacity adjoint",
+                        f"{settings['gather_radius']}, forward + adjoint ({a.gradients} gradients)",
             "surfels": n, "width": W, "height": H, "spp": settings["gather_passes"],
             "photons": settings["photons"], "depth": settings["bounces"],
             "gather_passes_share_seed": not a.decorrelate,
+            "gradients": a.gradients,
         },
         "queries_per_step": int(q_step),
         "ms_per_step_wall": round(float(np.mean(wall_ms)), 3),
