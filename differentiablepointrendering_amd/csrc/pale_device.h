@@ -195,6 +195,93 @@ __device__ bool traverseCloud(const DevScene& S, const DevInstance& I, F3 o, F3 
     return true;
 }
 
+// The same traversal without the counters, and without a stack round trip
+// for the nearer child: the reference pushes the farther child, then the
+// nearer, and pops the nearer at once with an unchanged best, so descending
+// into the nearer child directly visits the same nodes in the same order.
+__device__ bool traverseCloudFast(const DevScene& S, const DevInstance& I, F3 o, F3 d, const LaneStack& st,
+                                  uint32_t base, LocalHit& out) {
+    const F3 inv = safeInvDir(d);
+    float best = __builtin_huge_valf();
+    float te;
+    if (!slab(o, inv, I.rmnx, I.rmny, I.rmnz, I.rmxx, I.rmxy, I.rmxz, best, 1e-6f, te)) return false;
+    uint32_t sp = base;
+    uint32_t ref = I.rootRef;
+    uint32_t bestIdx = 0;
+    float bestR2 = 0.0f;
+    F3 bestX = mk3(0, 0, 0);
+    bool any = false;
+    while (true) {
+        if (ref & kLeafBit) {
+            const uint32_t first = leafFirst(ref), cnt = leafCount(ref);
+            for (uint32_t k = 0; k < cnt; ++k) {
+                const uint32_t si = first + k;
+                const float4* g = reinterpret_cast<const float4*>(S.geo + si);
+                const float4 g0 = g[0], g1 = g[1];
+                const F3 n = mk3(g1.x, g1.y, g1.z);
+                const float nd = dot(n, d);
+                if (fabsf(nd) < 1e-6f) continue;
+                const F3 p = mk3(g0.x, g0.y, g0.z);
+                const float t = dot(n, p - o) / nd;
+                if (t <= 1e-5f || t >= best) continue;
+                const float4 g2 = g[2], g3 = g[3];
+                const F3 x = o + d * t;
+                const F3 r = x - p;
+                const float uu = dot(mk3(g2.x, g2.y, g2.z), r) / g0.w;
+                const float vv = dot(mk3(g3.x, g3.y, g3.z), r) / g1.w;
+                const float r2 = uu * uu + vv * vv;
+                if (r2 >= 1.0f) continue;
+                any = true;
+                best = t;
+                bestIdx = si;
+                bestX = x;
+                bestR2 = r2;
+            }
+        } else {
+            const float4* nd4 = reinterpret_cast<const float4*>(S.nodes + ref);
+            const float4 a = nd4[0], b = nd4[1], c = nd4[2];
+            const uint4 refs = reinterpret_cast<const uint4*>(S.nodes + ref)[3];
+            float tl = __builtin_huge_valf(), tr = __builtin_huge_valf();
+            const bool hl = slab(o, inv, a.x, a.y, a.z, a.w, b.x, b.y, best, 1e-6f, tl);
+            const bool hr = slab(o, inv, b.z, b.w, c.x, c.y, c.z, c.w, best, 1e-6f, tr);
+            if (hl && hr) {
+                if (tl <= tr) {
+                    st.push(sp, refs.y, tr);
+                    ref = refs.x;
+                } else {
+                    st.push(sp, refs.x, tl);
+                    ref = refs.y;
+                }
+                continue;
+            }
+            if (hl) {
+                ref = refs.x;
+                continue;
+            }
+            if (hr) {
+                ref = refs.y;
+                continue;
+            }
+        }
+        // pop, skipping entries that start beyond the current best
+        bool found = false;
+        while (sp > base) {
+            const uint2 e = st.get(--sp);
+            if (__uint_as_float(e.y) > best) continue;
+            ref = e.x;
+            found = true;
+            break;
+        }
+        if (!found) break;
+    }
+    if (!any) return false;
+    out.t = best;
+    out.prim = bestIdx;
+    out.pos = bestX;
+    out.alpha = pow_r(1.0f - bestR2, S.geo[bestIdx].e4b);
+    return true;
+}
+
 // intersectTriangle (KernelHelpers.h:248-277)
 __device__ __forceinline__ bool intersectTri(F3 o, F3 d, F3 v0, F3 v1, F3 v2, float tMin, float& tOut) {
     const F3 e1 = v1 - v0, e2 = v2 - v0;
@@ -336,8 +423,13 @@ __device__ bool intersectScene(const DevScene& S, F3 o, F3 d, const LaneStack& s
         mulv4(I.w2o, d.x, d.y, d.z, 0.0f, hv);
         const F3 od = normalize(mk3(hv[0], hv[1], hv[2]));
         LocalHit lh;
-        const bool acc = (I.type == kGeoMesh) ? traverseMesh<kCount>(S, I, oo, od, st, sp, lh, w)
-                                              : traverseCloud<kCount>(S, I, oo, od, st, sp, lh, w);
+        bool acc;
+        if (I.type == kGeoMesh)
+            acc = traverseMesh<kCount>(S, I, oo, od, st, sp, lh, w);
+        else if (kCount)
+            acc = traverseCloud<kCount>(S, I, oo, od, st, sp, lh, w);
+        else
+            acc = traverseCloudFast(S, I, oo, od, st, sp, lh);
         if (acc) {
             const float tw = dot(lh.pos - o, d);
             if (tw > 0.0f && tw < bestW) {

@@ -587,7 +587,7 @@ __global__ __launch_bounds__(256) void k_cam_init(DevCamera cam, const DevState*
 }
 
 template <bool kCount>
-__global__ __launch_bounds__(kTraceBlock) void k_cam_trace(DevScene S, DevState* st, uint32_t npx, PixelArrays px,
+__global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_cam_trace(DevScene S, DevState* st, uint32_t npx, PixelArrays px,
                                                            EntryArrays ent, uint32_t cap, uint32_t* cursor) {
     extern __shared__ uint2 s_stack[];
     const LaneStack stk = laneStack(s_stack, S.stackDepth);
