@@ -159,8 +159,8 @@ void launchExclusiveScan(hipStream_t s, const uint32_t* in, uint32_t* out, uint3
 // Camera gather of one pass (see pale_kernels.hip for the stages).
 void launchCamInit(hipStream_t s, const DevCamera& cam, const DevState* st, uint32_t pass, int decorrelate,
                    PixelArrays px);
-void launchCamTrace(hipStream_t s, const DevScene& S, DevState* st, uint32_t npx, PixelArrays px, EntryArrays ent,
-                    uint32_t cap, uint32_t* cursor, bool count);
+void launchCamTrace(hipStream_t s, const DevScene& S, DevState* st, uint32_t npx, uint32_t width, PixelArrays px,
+                    EntryArrays ent, uint32_t cap, uint32_t* cursor, bool count);
 void launchEntrySort(hipStream_t s, const GridParams& g, DevState* st, uint32_t cap, PixelArrays px,
                      EntryArrays ent, uint32_t* cellCount, uint32_t* cellStart, uint32_t* tiles,
                      uint32_t* scratchTotal);

@@ -587,11 +587,17 @@ __global__ __launch_bounds__(256) void k_cam_init(DevCamera cam, const DevState*
 }
 
 template <bool kCount>
-__global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_cam_trace(DevScene S, DevState* st, uint32_t npx, PixelArrays px,
-                                                           EntryArrays ent, uint32_t cap, uint32_t* cursor) {
+__global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_cam_trace(
+    DevScene S, DevState* st, uint32_t npx, uint32_t width, PixelArrays px, EntryArrays ent, uint32_t cap,
+    uint32_t* cursor) {
     extern __shared__ uint2 s_stack[];
     const LaneStack stk = laneStack(s_stack, S.stackDepth);
     const uint32_t lane = laneId();
+    // Pixels are handed out in 8x8 tiles (a wave's first rays are neighbours);
+    // the order does not change any pixel's result.
+    const uint32_t height = npx / width;
+    const uint32_t tilesX = (width + 7u) / 8u;
+    const uint32_t slots = tilesX * ((height + 7u) / 8u) * 64u;
     uint32_t layers = 0;
     Work w{};
     bool active = false, exhausted = false;
@@ -606,12 +612,16 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(4))
                 uint32_t base = 0;
                 if (lane == leader) base = atomicAdd(cursor, (uint32_t)__popcll(need));
                 base = __shfl(base, (int)leader, 64);
-                if (base + (uint32_t)__popcll(need) >= npx ||
+                if (base + (uint32_t)__popcll(need) >= slots ||
                     *reinterpret_cast<volatile const uint32_t*>(&st->overflow) != 0u)
                     exhausted = true;
                 if (!active) {
-                    const uint32_t p = base + (uint32_t)__popcll(need & ((1ull << lane) - 1ull));
-                    if (p < npx && f2u(px.cs[p].w) == kPixActive) {
+                    const uint32_t slot = base + (uint32_t)__popcll(need & ((1ull << lane) - 1ull));
+                    const uint32_t tile = slot >> 6, within = slot & 63u;
+                    const uint32_t pxx = (tile % tilesX) * 8u + (within & 7u);
+                    const uint32_t pxy = (tile / tilesX) * 8u + (within >> 3);
+                    const uint32_t p = pxy * width + pxx;
+                    if (slot < slots && pxx < width && pxy < height && f2u(px.cs[p].w) == kPixActive) {
                         pixel = p;
                         active = true;
                         const float4 a = px.ot[p], b = px.dl[p];
@@ -1285,16 +1295,16 @@ void launchCamInit(hipStream_t s, const DevCamera& cam, const DevState* st, uint
     if (npx) k_cam_init<<<blocks(npx, 256), 256, 0, s>>>(cam, st, pass, decorrelate, px);
 }
 
-void launchCamTrace(hipStream_t s, const DevScene& S, DevState* st, uint32_t npx, PixelArrays px, EntryArrays ent,
-                    uint32_t cap, uint32_t* cursor, bool count) {
+void launchCamTrace(hipStream_t s, const DevScene& S, DevState* st, uint32_t npx, uint32_t width, PixelArrays px,
+                    EntryArrays ent, uint32_t cap, uint32_t* cursor, bool count) {
     if (!npx) return;
     const size_t lds = stackBytes(S);
     if (count) {
         const uint32_t g = min(camTraceGrid<true>(lds), blocks(npx, kTraceBlock));
-        k_cam_trace<true><<<g, kTraceBlock, lds, s>>>(S, st, npx, px, ent, cap, cursor);
+        k_cam_trace<true><<<g, kTraceBlock, lds, s>>>(S, st, npx, width, px, ent, cap, cursor);
     } else {
         const uint32_t g = min(camTraceGrid<false>(lds), blocks(npx, kTraceBlock));
-        k_cam_trace<false><<<g, kTraceBlock, lds, s>>>(S, st, npx, px, ent, cap, cursor);
+        k_cam_trace<false><<<g, kTraceBlock, lds, s>>>(S, st, npx, width, px, ent, cap, cursor);
     }
 }
 

@@ -454,7 +454,7 @@ struct pale_handle {
             launchFill(stream, entries, 2, 0u);  // entries, overflow
             launchFill(stream, cursors.as<uint32_t>(), 1, 0u);
             const size_t tt = tBegin(kTCamTrace);
-            launchCamTrace(stream, scene, st, npx, px, ent, entryCap, cursors.as<uint32_t>(), countWork);
+            launchCamTrace(stream, scene, st, npx, dc.W, px, ent, entryCap, cursors.as<uint32_t>(), countWork);
             tEnd(tt);
             const size_t ts = tBegin(kTCamSort);
             launchExclusiveScan(stream, px.cnt, px.off, npx, pxTiles.as<uint32_t>(), gridTotal.as<uint32_t>());
