@@ -22,11 +22,24 @@ Also reported: the roofline of the dominant kernel by device time, from the
 reference's algorithmic byte model per work unit (counted in a separate,
 untimed counting step), the same figures for the other camera-gather stage,
 and the CPU oracle timed on a bounded sample of the same workload.
+
+Served form ("served"): four resident clustered scenes (100k, 500k, 1M and
+3M surfels with photon maps of 1, 2, 3 and 4M photons, depth 6, radius
+0.02, two replicas each) take a Poisson stream of view requests (seed 1):
+scene uniform, forward only with p = 0.7 else forward + adjoint against a
+target image of the scene with jittered centers, resolution log-uniform in
+area between 256x256 and 1920x1080, spp uniform in {1, 4, 16, 64}.  A bounded
+prefix of the stream is run (--served requests; the full stream is 10,000),
+first one request at a time (the plain single-view path on the same
+requests) and then as they arrive; reported are p50/p99 latency from
+arrival to completion, throughput of both runs and whether every result is
+the same alone and in the mix.
 """
 from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -57,6 +70,10 @@ def parse():
     ap.add_argument("--cpu-pixel-stride", type=int, default=80)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-count", action="store_true", help="skip the work-counting step (roofline)")
+    ap.add_argument("--served", type=int, default=40, help="requests of the served stream (0: skip)")
+    ap.add_argument("--served-rate", type=float, default=1.0, help="Poisson arrival rate, requests per second")
+    ap.add_argument("--served-replicas", type=int, default=2)
+    ap.add_argument("--served-only", action="store_true", help="run only the served form")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only an elementwise kernel reading 1 GiB and writing 1 GiB: the known-size kernel "
                          "that scales rocprof FETCH_SIZE / WRITE_SIZE to bytes")
@@ -97,6 +114,149 @@ def load_traffic(path):
     return None
 
 
+SERVED_SCENES = ((100_000, 1_000_000), (500_000, 2_000_000), (1_000_000, 3_000_000), (3_000_000, 4_000_000))
+
+
+def served_requests(count: int, rate: float, seed: int = 1):
+    """The request stream of the served form: (arrival s, scene, camera, spp,
+    adjoint)."""
+    from differentiablepointrendering_amd import scene
+
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    lo, hi = math.log(256 * 256), math.log(1920 * 1080)
+    for i in range(count):
+        t += rng.exponential(1.0 / rate)
+        sc = int(rng.integers(0, len(SERVED_SCENES)))
+        adjoint = bool(rng.random() >= 0.7)
+        area = math.exp(rng.uniform(lo, hi))
+        aspect = rng.uniform(1.0, 16.0 / 9.0)
+        w = max(16, int(round(math.sqrt(area * aspect))))
+        h = max(16, int(round(area / w)))
+        spp = int(rng.choice([1, 4, 16, 64]))
+        d = rng.normal(size=3)
+        eye = 4.0 * d / np.linalg.norm(d)
+        up = (0.0, 1.0, 0.0) if abs(eye[1]) < 3.9 else (1.0, 0.0, 0.0)
+        cam = scene.look_at_camera(f"r{i}", eye, (0.0, 0.0, 0.0), up, w, h, 45.0)
+        out.append((t, sc, cam, spp, adjoint))
+    return out
+
+
+def served_leg(a):
+    import threading
+
+    import torch
+
+    from differentiablepointrendering_amd import scene
+    from differentiablepointrendering_amd.pale import Renderer
+    from differentiablepointrendering_amd.serve import SceneServer, ViewRequest, latency_summary
+
+    free, total = torch.cuda.mem_get_info()
+    budget = int(0.9 * total)
+    stream = served_requests(a.served, a.served_rate)
+    t0 = time.time()
+    srv = SceneServer(hbm_budget_bytes=budget)
+    targets = SceneServer(hbm_budget_bytes=budget)
+    names = []
+    resident = {}
+    for k, (n, photons) in enumerate(SERVED_SCENES):
+        name = f"clustered_{n // 1000}k"
+        names.append(name)
+        pts, tgt = scene.clustered_surfels(n, seed=k)
+        settings = dict(photons=photons, bounces=6, forward_passes=1, gather_passes=16, adjoint_bounces=6,
+                        adjoint_passes=16, seed=0, gather_radius=0.02)
+        meshes = [scene.light_quad()]
+        resident[name] = srv.add_scene(name, meshes, pts, settings, replicas=a.served_replicas)
+        if any(r[1] == k and r[4] for r in stream):
+            targets.add_scene(name, meshes, tgt, settings, replicas=1)
+        print(f"[bench] served scene {name}: {resident[name] / 2**30:.2f} GiB resident "
+              f"({a.served_replicas} replicas)", file=sys.stderr, flush=True)
+    # Target images of the forward + adjoint requests (untimed).
+    reqs = []
+    for i, (t, sc, cam, spp, adjoint) in enumerate(stream):
+        target = None
+        if adjoint:
+            target = targets.render(ViewRequest(names[sc], cam, spp, 10_000 + i)).raw[..., :3].copy()
+        reqs.append((t, ViewRequest(names[sc], cam, spp, 1 + i, target=target)))
+    targets.close()
+    setup_s = time.time() - t0
+    print(f"[bench] served setup {setup_s:.1f}s; {len(reqs)} requests, "
+          f"{sum(r.adjoint for _, r in reqs)} with adjoint", file=sys.stderr, flush=True)
+
+    # One request at a time: the plain single-view path on the same requests.
+    tp = time.perf_counter()
+    alone = []
+    for i, (_, r) in enumerate(reqs):
+        alone.append(srv.render(r))
+        if (i + 1) % 10 == 0:
+            print(f"[bench] served alone {i + 1}/{len(reqs)}", file=sys.stderr, flush=True)
+    plain_s = time.perf_counter() - tp
+    plain = latency_summary(alone, plain_s)
+
+    # As they arrive (Poisson), continuously scheduled.
+    futs = []
+    ts = time.perf_counter()
+    for t, r in reqs:
+        dt = ts + t - time.perf_counter()
+        if dt > 0:
+            time.sleep(dt)
+        futs.append(srv.submit(r))
+    mixed = [f.result() for f in futs]
+    makespan = max(m.finished for m in mixed) - ts
+    served = latency_summary(mixed, makespan)
+    same_images = same_grads = True
+    mismatches = []
+    for i, (x, y) in enumerate(zip(alone, mixed)):
+        img_ok = np.array_equal(x.raw.view(np.uint32), y.raw.view(np.uint32))
+        grad_ok = True
+        gerr = None
+        if x.grads is not None:
+            gx, gy = x.grads["opacity"], y.grads["opacity"]
+            scale = max(float(np.abs(gx).max()), 1e-30)
+            grad_ok = bool(np.allclose(gy, gx, rtol=1e-4, atol=1e-4 * scale))
+            gerr = float(np.abs(gy - gx).max() / scale)
+        same_images &= img_ok
+        same_grads &= grad_ok
+        if not (img_ok and grad_ok):
+            mismatches.append({"request": i, "scene": x.request.scene, "replicas": [x.replica, y.replica],
+                               "size": [x.request.camera.width, x.request.camera.height], "spp": x.request.spp,
+                               "pixels_differing": int(np.sum(np.any(x.raw != y.raw, axis=-1))),
+                               "max_abs_pixel_diff": float(np.abs(x.raw - y.raw).max()),
+                               "grad_max_err_over_max": gerr})
+    srv.close()
+    dev_bytes_after = Renderer.device_bytes()
+    out = {
+        "workload": f"{len(reqs)} requests (prefix of the seed-1 stream) over {len(names)} resident scenes "
+                    f"{'/'.join(names)}, Poisson {a.served_rate}/s, 30% forward+adjoint, 256x256..1920x1080, "
+                    f"spp {{1,4,16,64}}, {a.served_replicas} replicas per scene",
+        "arrival_rate_per_s": a.served_rate,
+        "p50_ms": round(served["p50_ms"], 2),
+        "p99_ms": round(served["p99_ms"], 2),
+        "mean_wait_ms": round(served["mean_wait_ms"], 2),
+        "mrays_per_s": round(served["mrays_per_s"], 2),
+        "requests_per_s": round(served["requests_per_s"], 3),
+        "makespan_s": round(makespan, 3),
+        "plain_one_at_a_time": {"mrays_per_s": round(plain["mrays_per_s"], 2),
+                                "requests_per_s": round(plain["requests_per_s"], 3),
+                                "p50_ms": round(plain["p50_ms"], 2), "p99_ms": round(plain["p99_ms"], 2),
+                                "total_s": round(plain_s, 3)},
+        "queries": served["queries"],
+        "same_results_alone_and_mixed": bool(same_images and same_grads),
+        "same_images": bool(same_images),
+        "same_gradients": bool(same_grads),
+        "mismatches": mismatches[:8],
+        "hbm_budget_bytes": budget,
+        "resident_bytes": resident,
+        "setup_s": round(setup_s, 1),
+        "device_bytes_after_close": dev_bytes_after,
+    }
+    print(f"[bench] served p50 {out['p50_ms']} ms p99 {out['p99_ms']} ms, {out['mrays_per_s']} Mrays/s "
+          f"(one at a time {out['plain_one_at_a_time']['mrays_per_s']}), same results: "
+          f"{out['same_results_alone_and_mixed']}", file=sys.stderr, flush=True)
+    return out
+
+
 def main():
     a = parse()
     if a.copy_calibration:
@@ -105,6 +265,9 @@ def main():
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {a.gpus}).",
               file=sys.stderr)
         return 2
+    if a.served_only:
+        print(json.dumps({"served": served_leg(a)}))
+        return 0
 
     from differentiablepointrendering_amd import scene
     from differentiablepointrendering_amd.pale import Renderer
@@ -259,6 +422,10 @@ def main():
     if work is not None:
         out["work_per_step"] = work
         out["kernels"] = kernels
+    r.close()
+    del r
+    if a.served > 0:
+        out["served"] = served_leg(a)
     print(json.dumps(out))
     return 0
 

@@ -241,6 +241,19 @@ class PointsHolder:
             setattr(self.desc, k, fptr(self.arrays[k]))
 
 
+def fill_camera_desc(d: CameraDesc, c: Camera) -> CameraDesc:
+    d.name = c.name.encode()[: PALE_NAME_LEN - 1]
+    d.width = int(c.width)
+    d.height = int(c.height)
+    d.fovy_deg = float(np.float32(c.fovy_deg))
+    d.view[:] = [float(x) for x in f32(c.view).reshape(16)]
+    d.inv_view[:] = [float(x) for x in f32(c.inv_view).reshape(16)]
+    d.position[:] = [float(x) for x in f32(c.position).reshape(3)]
+    d.forward[:] = [float(x) for x in f32(c.forward).reshape(3)]
+    d.use_for_adjoint = 1 if c.use_for_adjoint else 0
+    return d
+
+
 class SceneHolder:
     """Builds a SceneDesc and keeps every buffer it references alive."""
 
@@ -265,16 +278,7 @@ class SceneHolder:
             d.name = m.name.encode()[: PALE_NAME_LEN - 1]
         cam_arr = (CameraDesc * max(1, len(cameras)))()
         for i, c in enumerate(cameras):
-            d = cam_arr[i]
-            d.name = c.name.encode()[: PALE_NAME_LEN - 1]
-            d.width = int(c.width)
-            d.height = int(c.height)
-            d.fovy_deg = float(np.float32(c.fovy_deg))
-            d.view[:] = [float(x) for x in f32(c.view).reshape(16)]
-            d.inv_view[:] = [float(x) for x in f32(c.inv_view).reshape(16)]
-            d.position[:] = [float(x) for x in f32(c.position).reshape(3)]
-            d.forward[:] = [float(x) for x in f32(c.forward).reshape(3)]
-            d.use_for_adjoint = 1 if c.use_for_adjoint else 0
+            fill_camera_desc(cam_arr[i], c)
         self.points = PointsHolder(points)
         self._mesh_arr = mesh_arr
         self._cam_arr = cam_arr

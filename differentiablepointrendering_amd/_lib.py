@@ -25,7 +25,9 @@ EXPORTED = (
     "pale_apply_points", "pale_add_points", "pale_remove_points", "pale_rebuild_bvh", "pale_set_point_properties",
     "pale_set_point_opacity", "pale_set_point_beta", "pale_trace_rays", "pale_gather_irradiance", "pale_get_surfel_bvh", "pale_get_photons",
     "pale_get_stats", "pale_set_count_work", "pale_render_forward_device", "pale_render_backward_device",
-    "pale_set_opacity_device",
+    "pale_set_opacity_device", "pale_build_photon_map", "pale_render_view", "pale_render_view_device",
+    "pale_render_view_backward", "pale_device_bytes", "pale_view_bytes", "pale_view_buffer_bytes",
+    "pale_release_view_buffers",
 )
 
 
@@ -66,6 +68,15 @@ def load(path: str = LIB_PATH):
     L.pale_render_forward_device.argtypes = [H, C.c_char_p, C.POINTER(_fp), C.POINTER(_fp), _u32p]
     L.pale_render_backward_device.argtypes = [H, C.c_uint32, C.POINTER(C.c_char_p), C.POINTER(_fp), C.POINTER(_fp)]
     L.pale_set_opacity_device.argtypes = [H, _fp, C.c_uint32]
+    L.pale_build_photon_map.argtypes = [H]
+    L.pale_device_bytes.argtypes = [C.POINTER(C.c_uint64)]
+    L.pale_view_bytes.argtypes = [H, C.c_uint32, C.c_uint32, C.c_int32, C.POINTER(C.c_uint64)]
+    L.pale_release_view_buffers.argtypes = [H]
+    L.pale_view_buffer_bytes.argtypes = [H, C.POINTER(C.c_uint64)]
+    _camp = C.POINTER(_abi.CameraDesc)
+    L.pale_render_view.argtypes = [H, _camp, C.c_uint32, C.c_uint32, _fp, _fp]
+    L.pale_render_view_device.argtypes = [H, _camp, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p]
+    L.pale_render_view_backward.argtypes = [H, _camp, C.c_uint32, _fp, C.POINTER(_fp)]
     L.pale_point_count.argtypes = [H, _u32p]
     L.pale_get_points.argtypes = [H] + [_fp] * 8
     L.pale_apply_points.argtypes = [H, C.POINTER(_abi.PointsDesc)]

@@ -128,6 +128,7 @@ struct GridParams {
 void launchFill(hipStream_t s, uint32_t* p, uint32_t n, uint32_t v);
 void launchFillF(hipStream_t s, float* p, uint32_t n, float v);
 void launchSeedNext(hipStream_t s, DevState* st, const uint32_t* table, uint32_t tableLen, int bounce);
+void launchBeginView(hipStream_t s, DevState* st, uint32_t number);
 void launchEmit(hipStream_t s, const DevScene& S, DevState* st, uint32_t photons, float invEmitted,
                 RayQueue staged, uint8_t* flags);
 void launchPhotonTrace(hipStream_t s, const DevScene& S, DevState* st, uint32_t maxRays, RayQueue in,

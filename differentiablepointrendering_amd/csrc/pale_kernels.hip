@@ -8,6 +8,7 @@
 // work item from its queue slot, PrimalKernels.cpp:80-82,
 // AdjointKernels.cpp:99-101).  Counts stay on the device; no kernel waits for
 // the host.
+#include <mutex>
 #include <algorithm>
 #include <cstddef>
 
@@ -112,6 +113,20 @@ __global__ void k_seed_next(DevState* st, const uint32_t* table, uint32_t len, i
     st->number = c < len ? table[c] : 0u;
     st->cursor = c + 1;
     st->live = 1;
+}
+
+// Start of a served view: its passes draw from `number` (+ pass); the
+// camera-gather counters restart, the photon map's stay.
+__global__ void k_begin_view(DevState* st, uint32_t number) {
+    if (blockIdx.x != 0) return;
+    const uint32_t t = threadIdx.x;
+    if (t < 8) st->work[kWorkGather][t] = 0ull;
+    if (t == 0) {
+        st->number = number;
+        st->gatherQ = 0ull;
+        st->entries = 0u;
+        st->overflow = 0u;
+    }
 }
 
 // photonCount counts deposit attempts (the reference's counter T); only the
@@ -634,7 +649,13 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(4))
                 }
             }
         }
-        if (!__any(active)) break;
+        if (!__any(active)) {
+            // Every lane drew a finished pixel (or none): draw again until the
+            // slots run out or the entry buffer has filled.  Leaving here
+            // earlier would strand the pixels no other wave draws.
+            if (exhausted) break;
+            continue;
+        }
         // One layer: closest hit, then the entry it contributes.
         bool want = false, terminal = true, gather = false;
         F3 q = mk3(0, 0, 0), n = mk3(0, 0, 0), k = mk3(0, 0, 0), nextO = o;
@@ -1188,6 +1209,7 @@ void launchFillF(hipStream_t s, float* p, uint32_t n, float v) {
 void launchSeedNext(hipStream_t s, DevState* st, const uint32_t* table, uint32_t len, int bounce) {
     k_seed_next<<<1, 64, 0, s>>>(st, table, len, bounce);
 }
+void launchBeginView(hipStream_t s, DevState* st, uint32_t number) { k_begin_view<<<1, 64, 0, s>>>(st, number); }
 void launchEmit(hipStream_t s, const DevScene& S, DevState* st, uint32_t P, float invEmitted, RayQueue staged,
                 uint8_t* flags) {
     if (P) k_emit<<<blocks(P, 256), 256, 0, s>>>(S, st, P, invEmitted, staged, flags);
@@ -1265,11 +1287,14 @@ void launchBuildGrid(hipStream_t s, const GridParams& g, DevState* st, uint32_t 
 }
 
 // Persistent grid of the camera trace: as many workgroups as fit on the device.
+// Renderers on several host threads share the cache.
 template <bool kCount>
 static uint32_t camTraceGrid(size_t lds) {
+    static std::mutex lock;
     static uint32_t cached[2] = {0, 0};
     static size_t cachedLds[2] = {0, 0};
     const int k = kCount ? 1 : 0;
+    std::lock_guard<std::mutex> guard(lock);
     if (cached[k] && cachedLds[k] == lds) return cached[k];
     int dev = 0, cus = 0, perCu = 0;
     (void)hipGetDevice(&dev);

@@ -7,6 +7,7 @@
 // queue counts, seeds and deposit counts stay in device memory (DevState), so
 // the host synchronises once per call, when results are copied out.
 #include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <random>
 #include <stdexcept>
@@ -34,6 +35,9 @@ struct StatusError : std::runtime_error {
         if (e_ != hipSuccess) throw StatusError(PALE_ERR_DEVICE, std::string(#x) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
+// Device bytes held by every renderer of the process (pale_device_bytes).
+std::atomic<uint64_t> g_deviceBytes{0};
+
 struct DevMem {
     void* p = nullptr;
     size_t bytes = 0;
@@ -47,7 +51,14 @@ struct DevMem {
     }
     ~DevMem() { release(); }
     void release() {
-        if (p) (void)(pinned ? hipHostFree(p) : hipFree(p));
+        if (p) {
+            if (pinned) {
+                (void)hipHostFree(p);
+            } else {
+                (void)hipFree(p);
+                g_deviceBytes -= bytes;
+            }
+        }
         p = nullptr;
         bytes = 0;
     }
@@ -56,6 +67,7 @@ struct DevMem {
         release();
         HIPCHECK(hipMalloc(&p, std::max<size_t>(b, 16)));
         bytes = std::max<size_t>(b, 16);
+        g_deviceBytes += bytes;
     }
     template <typename T>
     T* as() const {
@@ -116,6 +128,7 @@ struct pale_handle {
     bool adjFull = false;  // event rays allocated (full gradients)
     // Camera gather: pixel state, entries of one round, their cell sort.
     uint32_t pxCap = 0, entryCap = 0;
+    DevMem viewFb, viewLdr;  // images of pale_render_view
     DevMem pxOt, pxDl, pxCs, pxCnt, pxOff, pxTiles, entQa, entQb, entComp, entE, entOrd, entCell, entSorted,
         entStart, flagHost;
     // Device state and seed table.
@@ -317,50 +330,119 @@ struct pale_handle {
         }
     }
 
-    void ensureQueues(uint32_t cap) {
+    // Buffer sizing runs either for real (Allocate) or as a dry run that
+    // adds up the growth a request would cause (Probe: HBM admission).
+    struct Allocate {
+        static constexpr bool kDry = false;
+        void operator()(DevMem& m, size_t b) const { m.reserve(b); }
+    };
+    struct Probe {
+        static constexpr bool kDry = true;
+        uint64_t add = 0;
+        void operator()(const DevMem& m, size_t b) {
+            if (b <= m.bytes && m.p) return;
+            add += std::max<size_t>(b, 16) - m.bytes;
+        }
+    };
+
+    template <typename A>
+    void sizeQueues(uint32_t cap, A& a) {
         if (cap <= queueCap) return;
         for (int k = 0; k < 3; ++k) {
-            qA[k].reserve((size_t)cap * sizeof(float4));
-            qB[k].reserve((size_t)cap * sizeof(float4));
-            stg[k].reserve((size_t)cap * sizeof(float4));
+            a(qA[k], (size_t)cap * sizeof(float4));
+            a(qB[k], (size_t)cap * sizeof(float4));
+            a(stg[k], (size_t)cap * sizeof(float4));
         }
-        extFlags.reserve(cap);
-        depStg[0].reserve((size_t)cap * sizeof(float4));
-        depStg[1].reserve((size_t)cap * sizeof(float4));
-        depStg[2].reserve((size_t)cap * sizeof(float));
-        depFlags.reserve(cap);
-        tiles.reserve(((size_t)cap / kTile + 4) * sizeof(uint32_t));
-        counts.reserve(16 * sizeof(uint32_t));
-        resvAttempt.reserve((size_t)cap * sizeof(uint32_t));
-        resvSlot.reserve((size_t)cap * sizeof(uint32_t));
-        queueCap = cap;
+        a(extFlags, cap);
+        a(depStg[0], (size_t)cap * sizeof(float4));
+        a(depStg[1], (size_t)cap * sizeof(float4));
+        a(depStg[2], (size_t)cap * sizeof(float));
+        a(depFlags, cap);
+        a(tiles, ((size_t)cap / kTile + 4) * sizeof(uint32_t));
+        a(counts, 16 * sizeof(uint32_t));
+        a(resvAttempt, (size_t)cap * sizeof(uint32_t));
+        a(resvSlot, (size_t)cap * sizeof(uint32_t));
+        if (!A::kDry) queueCap = cap;
+    }
+    void ensureQueues(uint32_t cap) {
+        Allocate a;
+        sizeQueues(cap, a);
     }
 
-    void ensureAdjoint(uint32_t npx) {
-        ensureQueues(nextPow2(npx));
-        entStart.reserve(((size_t)grid.total + 1) * 4);
-        if (S.full_gradients && !adjFull) adjCap = 0;  // also allocate the event rays
-        if (npx <= adjCap) return;
-        pend0.reserve((size_t)npx * sizeof(float4));
-        pend1.reserve((size_t)npx * sizeof(float4));
-        pend2.reserve((size_t)npx * sizeof(uint2));
+    template <typename A>
+    void sizeAdjoint(uint32_t npx, A& a) {
+        sizeQueues(nextPow2(npx), a);
+        a(entStart, ((size_t)grid.total + 1) * 4);
+        uint32_t have = adjCap;
+        if (S.full_gradients && !adjFull) have = 0;  // also allocate the event rays
+        if (npx <= have) return;
+        a(pend0, (size_t)npx * sizeof(float4));
+        a(pend1, (size_t)npx * sizeof(float4));
+        a(pend2, (size_t)npx * sizeof(uint2));
         // Event pool: the events of up to 24 bounces of npx paths wait for
         // one cell-sorted transport launch.
-        poolCap = (uint32_t)std::min<uint64_t>((uint64_t)npx * 24u, 1u << 28);
+        const uint32_t pool = (uint32_t)std::min<uint64_t>((uint64_t)npx * 24u, 1u << 28);
         const int arrays = S.full_gradients ? 6 : 4;
         for (int k = 0; k < arrays; ++k) {
-            evStg[k].reserve((size_t)npx * sizeof(float4));
-            evCmp[k].reserve((size_t)poolCap * sizeof(float4));
+            a(evStg[k], (size_t)npx * sizeof(float4));
+            a(evCmp[k], (size_t)pool * sizeof(float4));
         }
         if (S.full_gradients) {
-            pend3.reserve((size_t)npx * sizeof(float4));
-            pend4.reserve((size_t)npx * sizeof(float4));
-            adjFull = true;
+            a(pend3, (size_t)npx * sizeof(float4));
+            a(pend4, (size_t)npx * sizeof(float4));
         }
-        evCell.reserve((size_t)poolCap * 4);
-        evSorted.reserve((size_t)poolCap * 4);
-        evFlags.reserve(npx);
-        adjCap = npx;
+        a(evCell, (size_t)pool * 4);
+        a(evSorted, (size_t)pool * 4);
+        a(evFlags, npx);
+        if (!A::kDry) {
+            poolCap = pool;
+            if (S.full_gradients) adjFull = true;
+            adjCap = npx;
+        }
+    }
+    void ensureAdjoint(uint32_t npx) {
+        Allocate a;
+        sizeAdjoint(npx, a);
+    }
+
+    // Device bytes a view of npx pixels (with its adjoint) would add.
+    uint64_t viewGrowth(uint32_t npx, bool adjoint) {
+        Probe p;
+        sizeCamera(npx, p);
+        p(cursors, sizeof(uint32_t));
+        p(viewFb, (size_t)npx * sizeof(float4));
+        p(viewLdr, (size_t)npx * sizeof(float4));
+        if (adjoint) {
+            sizeAdjoint(npx, p);
+            p(dloss, (size_t)npx * 3 * sizeof(float));
+            if (S.full_gradients) p(gradFull, std::max<size_t>(1, bp.points.size() * 16) * sizeof(float));
+        }
+        return p.add;
+    }
+
+    // Frees the buffers that grow with the image size (camera gather, view
+    // images, adjoint); the scene, BVH, photon map and photon-stage queues
+    // stay.  They are allocated again by the next view.
+    std::vector<DevMem*> viewBuffers() {
+        std::vector<DevMem*> v = {&pxOt,    &pxDl,    &pxCs,    &pxCnt,  &pxOff, &pxTiles, &entQa,  &entQb,
+                                  &entComp, &entE,    &entOrd,  &entCell, &entSorted, &viewFb, &viewLdr, &pend0,
+                                  &pend1,   &pend2,   &pend3,   &pend4,  &evCell, &evSorted, &evFlags, &dloss};
+        for (int k = 0; k < 6; ++k) {
+            v.push_back(&evStg[k]);
+            v.push_back(&evCmp[k]);
+        }
+        return v;
+    }
+    uint64_t viewBufferBytes() {
+        uint64_t b = 0;
+        for (DevMem* m : viewBuffers()) b += m->p ? m->bytes : 0;
+        return b;
+    }
+    void releaseViewBuffers() {
+        for (DevMem* m : viewBuffers()) m->release();
+        pxCap = entryCap = 0;
+        adjCap = poolCap = 0;
+        adjFull = false;
     }
 
     void resetState() {
@@ -400,30 +482,36 @@ struct pale_handle {
     // Camera-gather buffers for images of up to npx pixels.  The entry
     // buffer holds gather_entry_budget (default 24) layers per pixel on
     // average: one round in practice; deeper images take more rounds.
-    void ensureCamera(uint32_t npx) {
+    template <typename A>
+    void sizeCamera(uint32_t npx, A& a) {
         if (npx > pxCap) {
-            pxOt.reserve((size_t)npx * 16);
-            pxDl.reserve((size_t)npx * 16);
-            pxCs.reserve((size_t)npx * 16);
-            pxCnt.reserve((size_t)npx * 4);
-            pxOff.reserve(((size_t)npx + 1) * 4);
-            pxTiles.reserve(((size_t)npx / kTile + 2) * 4);
-            pxCap = npx;
+            a(pxOt, (size_t)npx * 16);
+            a(pxDl, (size_t)npx * 16);
+            a(pxCs, (size_t)npx * 16);
+            a(pxCnt, (size_t)npx * 4);
+            a(pxOff, ((size_t)npx + 1) * 4);
+            a(pxTiles, ((size_t)npx / kTile + 2) * 4);
+            if (!A::kDry) pxCap = npx;
         }
         const uint64_t perPixel = S.gather_entry_budget ? S.gather_entry_budget : 24u;
         const uint64_t floor = S.gather_entry_budget ? 64u : (1u << 16);
         const uint32_t want = (uint32_t)std::min<uint64_t>(std::max<uint64_t>((uint64_t)npx * perPixel, floor), 1u << 28);
         if (want != entryCap) {
-            entQa.reserve((size_t)want * 16);
-            entQb.reserve((size_t)want * 16);
-            entComp.reserve((size_t)want * 16);
-            entE.reserve((size_t)want * 16);
-            entOrd.reserve((size_t)want * 4);
-            entCell.reserve((size_t)want * 4);
-            entSorted.reserve((size_t)want * 4);
-            entryCap = want;
+            a(entQa, (size_t)want * 16);
+            a(entQb, (size_t)want * 16);
+            a(entComp, (size_t)want * 16);
+            a(entE, (size_t)want * 16);
+            a(entOrd, (size_t)want * 4);
+            a(entCell, (size_t)want * 4);
+            a(entSorted, (size_t)want * 4);
+            if (!A::kDry) entryCap = want;
         }
-        entStart.reserve(((size_t)grid.total + 1) * 4);
+        a(entStart, ((size_t)grid.total + 1) * 4);
+    }
+
+    void ensureCamera(uint32_t npx) {
+        Allocate a;
+        sizeCamera(npx, a);
         if (!flagHost.p) {
             HIPCHECK(hipHostMalloc(&flagHost.p, 16, hipHostMallocDefault));
             flagHost.bytes = 16;
@@ -442,14 +530,14 @@ struct pale_handle {
 
     // One gather pass of one camera (launchCameraGatherKernel,
     // PrimalKernels.cpp:500-653), in rounds until every pixel is finished.
-    void cameraPass(const DevCamera& dc, uint32_t pass, float invSpp, float4* fbp) {
+    void cameraPass(const DevCamera& dc, uint32_t pass, float invSpp, float4* fbp, int decorrelate) {
         const uint32_t npx = dc.W * dc.H;
         DevState* st = this->st();
         const PixelArrays px = pxa();
         const EntryArrays ent = enta();
         const DevGrid G = devGrid();
         uint32_t* entries = stField(offsetof(DevState, entries));
-        launchCamInit(stream, dc, st, pass, S.spp_decorrelate, px);
+        launchCamInit(stream, dc, st, pass, decorrelate, px);
         for (uint32_t round = 0;; ++round) {
             launchFill(stream, entries, 2, 0u);  // entries, overflow
             launchFill(stream, cursors.as<uint32_t>(), 1, 0u);
@@ -478,18 +566,55 @@ struct pale_handle {
 
     // submitPhotonMappingKernel (SyclBridge.cpp:82-166) for the given cameras.
     void renderForward(const std::vector<uint32_t>& camIdx) {
-        const uint32_t P = S.photons;
-        if (P > queueCap) ensureQueues(nextPow2(P));
         // Seed schedule: mt19937_64(seed), truncated to 32 bits
         // (SyclBridge.cpp:83-143, Random::number is uint32_t).
-        std::mt19937_64 gen(S.seed);
-        std::vector<uint32_t> table((size_t)S.forward_passes * (1 + S.bounces) + camIdx.size() + 1);
-        for (uint32_t& v : table) v = (uint32_t)gen();
-        upload(stream, seedTable, table);
-        HIPCHECK(hipStreamSynchronize(stream));
+        const uint32_t tlen = uploadSeedTable((uint32_t)camIdx.size());
         resetState();
         tReset();
         HIPCHECK(hipEventRecord(ev0, stream));
+        tracePhotons(tlen);
+        DevState* st = this->st();
+        const float invSpp = 1.0f / (float)S.gather_passes;
+        cursors.reserve(sizeof(uint32_t));
+        camRounds = 0;
+        for (size_t k = 0; k < camIdx.size(); ++k) {
+            const HostCamera& c = cams[camIdx[k]];
+            const uint32_t npx = c.W * c.H;
+            ensureCamera(npx);
+            launchSeedNext(stream, st, seedTable.as<uint32_t>(), tlen, -1);
+            launchFillF(stream, fb[camIdx[k]].as<float>(), npx * 4, 0.0f);
+            const DevCamera dc = devCam(c);
+            for (uint32_t pass = 0; pass < S.gather_passes; ++pass)
+                cameraPass(dc, pass, invSpp, fb[camIdx[k]].as<float4>(), S.spp_decorrelate);
+            launchPostprocess(stream, fb[camIdx[k]].as<float4>(), ldr[camIdx[k]].as<float4>(), npx);
+        }
+        finishForward();
+    }
+
+    uint32_t uploadSeedTable(uint32_t cameras) {
+        std::mt19937_64 gen(S.seed);
+        std::vector<uint32_t> table((size_t)S.forward_passes * (1 + S.bounces) + cameras + 1);
+        for (uint32_t& v : table) v = (uint32_t)gen();
+        upload(stream, seedTable, table);
+        HIPCHECK(hipStreamSynchronize(stream));
+        return (uint32_t)table.size();
+    }
+
+    void finishForward() {
+        HIPCHECK(hipEventRecord(ev1, stream));
+        HIPCHECK(hipStreamSynchronize(stream));
+        HIPCHECK(hipGetLastError());
+        float ms = 0.0f;
+        HIPCHECK(hipEventElapsedTime(&ms, ev0, ev1));
+        stats.forward_ms = ms;
+        readStats(true);
+    }
+
+    // The photon stage of submitPhotonMappingKernel (SyclBridge.cpp:82-141):
+    // emission, bounces, grid.
+    void tracePhotons(uint32_t tlen) {
+        const uint32_t P = S.photons;
+        if (P > queueCap) ensureQueues(nextPow2(P));
         DevState* st = this->st();
         const uint32_t emitted = S.photons * S.forward_passes;
         const float invEmitted = 1.0f / (float)emitted;
@@ -499,7 +624,6 @@ struct pale_handle {
         uint32_t* extTotal = stField(offsetof(DevState, extTotal));
         uint32_t* depTotal = stField(offsetof(DevState, depTotal));
         uint32_t* photonCount = stField(offsetof(DevState, photonCount));
-        const uint32_t tlen = (uint32_t)table.size();
         const Reservoir rs{reservoirOn ? resvAttempt.as<uint32_t>() : nullptr, resvSlot.as<uint32_t>(),
                            resvOwner.as<uint32_t>(), photonCap};
         if (reservoirOn) launchFill(stream, resvOwner.as<uint32_t>(), photonCap, 0u);
@@ -537,35 +661,54 @@ struct pale_handle {
                         gridTotal.as<uint32_t>());
         tEnd(tg);
         mapValid = true;
-        const float invSpp = 1.0f / (float)S.gather_passes;
-        cursors.reserve(sizeof(uint32_t));
+    }
+
+    // Photon map only (serving: views reuse it).
+    void buildPhotonMap() {
+        const uint32_t tlen = uploadSeedTable(0);
+        resetState();
+        tReset();
         camRounds = 0;
-        for (size_t k = 0; k < camIdx.size(); ++k) {
-            const HostCamera& c = cams[camIdx[k]];
-            const uint32_t npx = c.W * c.H;
-            ensureCamera(npx);
-            launchSeedNext(stream, st, seedTable.as<uint32_t>(), tlen, -1);
-            launchFillF(stream, fb[camIdx[k]].as<float>(), npx * 4, 0.0f);
-            const DevCamera dc = devCam(c);
-            for (uint32_t pass = 0; pass < S.gather_passes; ++pass)
-                cameraPass(dc, pass, invSpp, fb[camIdx[k]].as<float4>());
-            launchPostprocess(stream, fb[camIdx[k]].as<float4>(), ldr[camIdx[k]].as<float4>(), npx);
-        }
-        HIPCHECK(hipEventRecord(ev1, stream));
-        HIPCHECK(hipStreamSynchronize(stream));
-        HIPCHECK(hipGetLastError());
-        float ms = 0.0f;
-        HIPCHECK(hipEventElapsedTime(&ms, ev0, ev1));
-        stats.forward_ms = ms;
-        readStats(true);
+        HIPCHECK(hipEventRecord(ev0, stream));
+        tracePhotons(tlen);
+        finishForward();
+    }
+
+    // One view over the current photon map: spp passes seeded by seed + k.
+    void renderView(const HostCamera& c, uint32_t spp, uint32_t seed) {
+        if (!mapValid) throw StatusError(PALE_ERR_INVALID_ARG, "render_view: no photon map; build it first");
+        if (spp == 0) throw StatusError(PALE_ERR_INVALID_ARG, "render_view: spp must be >= 1");
+        const uint32_t npx = c.W * c.H;
+        ensureCamera(npx);
+        viewFb.reserve((size_t)npx * sizeof(float4));
+        viewLdr.reserve((size_t)npx * sizeof(float4));
+        tReset();
+        camRounds = 0;
+        HIPCHECK(hipEventRecord(ev0, stream));
+        DevState* st = this->st();
+        launchBeginView(stream, st, seed);
+        launchFillF(stream, viewFb.as<float>(), npx * 4, 0.0f);
+        const DevCamera dc = devCam(c);
+        const float invSpp = 1.0f / (float)spp;
+        cursors.reserve(sizeof(uint32_t));
+        for (uint32_t pass = 0; pass < spp; ++pass) cameraPass(dc, pass, invSpp, viewFb.as<float4>(), 1);
+        launchPostprocess(stream, viewFb.as<float4>(), viewLdr.as<float4>(), npx);
+        finishForward();
     }
 
     // submitAdjointKernel (SyclBridge.cpp:169-294).
     void renderBackward(const std::vector<uint32_t>& camIdx, const std::vector<const float*>& src,
                         bool srcOnDevice = false) {
+        std::vector<const HostCamera*> cp;
+        for (uint32_t c : camIdx) cp.push_back(&cams[c]);
+        renderBackwardCams(cp, src, srcOnDevice, S.adjoint_passes);
+    }
+
+    void renderBackwardCams(const std::vector<const HostCamera*>& camList, const std::vector<const float*>& src,
+                            bool srcOnDevice, uint32_t samples) {
         const uint32_t N = (uint32_t)bp.points.size();
         uint32_t maxNpx = 1;
-        for (uint32_t c : camIdx) maxNpx = std::max(maxNpx, cams[c].W * cams[c].H);
+        for (const HostCamera* c : camList) maxNpx = std::max(maxNpx, c->W * c->H);
         ensureAdjoint(maxNpx);
         const DevGrid G = devGrid();
         // Keep the forward's deposit count (the gather normalisation T) and
@@ -580,10 +723,10 @@ struct pale_handle {
         HIPCHECK(hipMemcpyAsync(state.p, &h, sizeof(DevState), hipMemcpyHostToDevice, stream));
         HIPCHECK(hipStreamSynchronize(stream));
         // dL/dI of every camera goes to the device before the timed region.
-        std::vector<size_t> off(camIdx.size() + 1, 0);
-        for (size_t k = 0; k < camIdx.size(); ++k) off[k + 1] = off[k] + (size_t)cams[camIdx[k]].W * cams[camIdx[k]].H * 3;
+        std::vector<size_t> off(camList.size() + 1, 0);
+        for (size_t k = 0; k < camList.size(); ++k) off[k + 1] = off[k] + (size_t)camList[k]->W * camList[k]->H * 3;
         dloss.reserve(std::max<size_t>(off.back(), 1) * sizeof(float));
-        for (size_t k = 0; k < camIdx.size(); ++k)
+        for (size_t k = 0; k < camList.size(); ++k)
             HIPCHECK(hipMemcpyAsync(dloss.as<float>() + off[k], src[k], (off[k + 1] - off[k]) * sizeof(float),
                                     srcOnDevice ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, stream));
         launchFillF(stream, gradOpacity.as<float>(), N, 0.0f);
@@ -597,7 +740,7 @@ struct pale_handle {
         DevState* st = this->st();
         uint32_t* qCount = stField(offsetof(DevState, qCount));
         uint32_t* extTotal = stField(offsetof(DevState, extTotal));
-        const float invSpp = 1.0f / (float)S.adjoint_passes;
+        const float invSpp = 1.0f / (float)samples;
         // launchAdjointTransportKernel (AdjointKernels.cpp:389-509) over the
         // pooled events of several bounces, in grid-cell order.  Gradients
         // are sums of atomics, so the grouping does not change them beyond
@@ -616,13 +759,13 @@ struct pale_handle {
             launchFill(stream, stField(offsetof(DevState, evPool)), 1, 0u);
             pooledBound = 0;
         };
-        for (size_t k = 0; k < camIdx.size(); ++k) {
-            const HostCamera& c = cams[camIdx[k]];
+        for (size_t k = 0; k < camList.size(); ++k) {
+            const HostCamera& c = *camList[k];
             const uint32_t npx = c.W * c.H;
             const DevCamera dc = devCam(c);
             const PendingArrays pend{pend0.as<float4>(), pend1.as<float4>(), pend2.as<uint2>(), pend3.as<float4>(),
                                      pend4.as<float4>()};
-            for (uint32_t spp = 0; spp < S.adjoint_passes; ++spp) {
+            for (uint32_t spp = 0; spp < samples; ++spp) {
                 DevMem* in = qA;
                 DevMem* out = qB;
                 launchAdjRaygen(stream, dc, dloss.as<float>() + off[k], rq(in), pend, st);
@@ -736,6 +879,22 @@ void pale_default_settings(pale_settings* out) {
     *out = s;
 }
 
+static HostCamera hostCamera(const pale_camera_desc& cd) {
+    HostCamera c;
+    c.name = std::string(cd.name, strnlen(cd.name, PALE_NAME_LEN));
+    c.W = cd.width;
+    c.H = cd.height;
+    if (!c.W || !c.H) throw StatusError(PALE_ERR_INVALID_ARG, "camera " + c.name + ": empty image");
+    c.fovy = cd.fovy_deg;
+    std::copy(cd.view, cd.view + 16, c.view);
+    std::copy(cd.inv_view, cd.inv_view + 16, c.invView);
+    c.pos = ld3h(cd.position);
+    c.fwd = ld3h(cd.forward);
+    c.adjoint = cd.use_for_adjoint != 0;
+    cameraFocal(c);
+    return c;
+}
+
 int pale_create(const pale_scene_desc* scene, const pale_settings* settings, pale_handle** out) {
     if (!scene || !settings || !out) return fail(PALE_ERR_INVALID_ARG, "pale_create: null argument");
     *out = nullptr;
@@ -762,21 +921,7 @@ int pale_create(const pale_scene_desc* scene, const pale_settings* settings, pal
             m.name = std::string(md.name, strnlen(md.name, PALE_NAME_LEN));
             h->meshes.push_back(std::move(m));
         }
-        for (uint32_t i = 0; i < scene->camera_count; ++i) {
-            const pale_camera_desc& cd = scene->cameras[i];
-            HostCamera c;
-            c.name = std::string(cd.name, strnlen(cd.name, PALE_NAME_LEN));
-            c.W = cd.width;
-            c.H = cd.height;
-            c.fovy = cd.fovy_deg;
-            std::copy(cd.view, cd.view + 16, c.view);
-            std::copy(cd.inv_view, cd.inv_view + 16, c.invView);
-            c.pos = ld3h(cd.position);
-            c.fwd = ld3h(cd.forward);
-            c.adjoint = cd.use_for_adjoint != 0;
-            cameraFocal(c);
-            h->cams.push_back(c);
-        }
+        for (uint32_t i = 0; i < scene->camera_count; ++i) h->cams.push_back(hostCamera(scene->cameras[i]));
         h->asset = toAsset(scene->points);
         HIPCHECK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
         HIPCHECK(hipEventCreate(&h->ev0));
@@ -844,6 +989,52 @@ static void selectAdjoint(const pale_handle* h, uint32_t n, const char* const* n
     }
 }
 
+// Gradient arrays in ABI order: position, tangent_u, tangent_v, scale,
+// albedo, opacity, beta, shape (pale_bindings.cpp:410-450).
+static const size_t kGradWidth[8] = {3, 3, 3, 2, 3, 1, 1, 1};
+
+static void copyGradsHost(pale_handle* h, float* const* grads) {
+    const size_t N = h->bp.points.size();
+    if (!grads) return;
+    std::vector<float> full;
+    if (h->S.full_gradients && N) {
+        full.resize(N * 16);
+        HIPCHECK(hipMemcpy(full.data(), h->gradFull.p, N * 16 * sizeof(float), hipMemcpyDeviceToHost));
+    }
+    const int first[8] = {0, 3, 6, 9, 11, -1, 15, -1};  // columns of the N x 16 accumulator
+    for (int g = 0; g < 8; ++g) {
+        if (!grads[g]) continue;
+        if (g == 5) {
+            if (N) HIPCHECK(hipMemcpy(grads[g], h->gradOpacity.p, N * sizeof(float), hipMemcpyDeviceToHost));
+        } else if (!full.empty() && first[g] >= 0) {
+            for (size_t i = 0; i < N; ++i)
+                for (size_t k = 0; k < kGradWidth[g]; ++k) grads[g][i * kGradWidth[g] + k] = full[i * 16 + first[g] + k];
+        } else {
+            std::memset(grads[g], 0, N * kGradWidth[g] * sizeof(float));
+        }
+    }
+}
+
+static void copyGradsDevice(pale_handle* h, float* const* grads_dev) {
+    const uint32_t N = (uint32_t)h->bp.points.size();
+    if (grads_dev && N) {
+        const bool full = h->S.full_gradients != 0;
+        for (int g = 0; g < 8; ++g) {
+            if (!grads_dev[g]) continue;
+            if (g == 5)
+                HIPCHECK(hipMemcpyAsync(grads_dev[g], h->gradOpacity.p, (size_t)N * sizeof(float),
+                                        hipMemcpyDeviceToDevice, h->stream));
+            else if (!full || g == 7)
+                launchFillF(h->stream, grads_dev[g], N * (uint32_t)kGradWidth[g], 0.0f);
+        }
+        if (full)
+            launchSplitGradients(h->stream, h->gradFull.as<float>(), N, grads_dev[0], grads_dev[1], grads_dev[2],
+                                 grads_dev[3], grads_dev[4], grads_dev[6]);
+    }
+    HIPCHECK(hipStreamSynchronize(h->stream));
+    HIPCHECK(hipGetLastError());
+}
+
 int pale_render_forward(pale_handle* h, const char* camera_name, float* const* ldr_out, float* const* raw_out,
                         uint32_t* rendered) {
     if (!h) return fail(PALE_ERR_INVALID_ARG, "null handle");
@@ -871,28 +1062,7 @@ int pale_render_backward(pale_handle* h, uint32_t n, const char* const* camera_n
         std::vector<const float*> src;
         selectAdjoint(h, n, camera_names, dloss_rgb, idx, src, which);
         h->renderBackward(idx, src);
-        const size_t N = h->bp.points.size();
-        const size_t widths[8] = {3, 3, 3, 2, 3, 1, 1, 1};
-        if (grads) {
-            std::vector<float> full;
-            if (h->S.full_gradients && N) {
-                full.resize(N * 16);
-                HIPCHECK(hipMemcpy(full.data(), h->gradFull.p, N * 16 * sizeof(float), hipMemcpyDeviceToHost));
-            }
-            // position, tangent_u, tangent_v, scale, albedo, opacity, beta, shape
-            const int first[8] = {0, 3, 6, 9, 11, -1, 15, -1};
-            for (int g = 0; g < 8; ++g) {
-                if (!grads[g]) continue;
-                if (g == 5) {
-                    if (N) HIPCHECK(hipMemcpy(grads[g], h->gradOpacity.p, N * sizeof(float), hipMemcpyDeviceToHost));
-                } else if (!full.empty() && first[g] >= 0) {
-                    for (size_t i = 0; i < N; ++i)
-                        for (size_t k = 0; k < widths[g]; ++k) grads[g][i * widths[g] + k] = full[i * 16 + first[g] + k];
-                } else {
-                    std::memset(grads[g], 0, N * widths[g] * sizeof(float));
-                }
-            }
-        }
+        copyGradsHost(h, grads);
         if (adjoint_source_out) {
             // The adjoint-source image is the uploaded dL/dI with alpha 1
             // (pale_bindings.cpp:352-392, :470-482), in input order.
@@ -940,24 +1110,7 @@ int pale_render_backward_device(pale_handle* h, uint32_t n, const char* const* c
         std::vector<const float*> src;
         selectAdjoint(h, n, camera_names, dloss_rgb_dev, idx, src, which);
         h->renderBackward(idx, src, true);
-        const uint32_t N = (uint32_t)h->bp.points.size();
-        const uint32_t widths[8] = {3, 3, 3, 2, 3, 1, 1, 1};
-        if (grads_dev && N) {
-            const bool full = h->S.full_gradients != 0;
-            for (int g = 0; g < 8; ++g) {
-                if (!grads_dev[g]) continue;
-                if (g == 5)
-                    HIPCHECK(hipMemcpyAsync(grads_dev[g], h->gradOpacity.p, (size_t)N * sizeof(float),
-                                            hipMemcpyDeviceToDevice, h->stream));
-                else if (!full || g == 7)
-                    launchFillF(h->stream, grads_dev[g], N * widths[g], 0.0f);
-            }
-            if (full)
-                launchSplitGradients(h->stream, h->gradFull.as<float>(), N, grads_dev[0], grads_dev[1], grads_dev[2],
-                                     grads_dev[3], grads_dev[4], grads_dev[6]);
-        }
-        HIPCHECK(hipStreamSynchronize(h->stream));
-        HIPCHECK(hipGetLastError());
+        copyGradsDevice(h, grads_dev);
     });
 }
 
@@ -977,6 +1130,71 @@ int pale_set_opacity_device(pale_handle* h, const float* opacity_dev, uint32_t n
             h->asset[i].opacity = v[i];
             h->img.surfShade[i].opacity = v[i];
         }
+    });
+}
+
+int pale_device_bytes(uint64_t* out) {
+    if (!out) return fail(PALE_ERR_INVALID_ARG, "null argument");
+    *out = g_deviceBytes.load();
+    return PALE_OK;
+}
+
+int pale_view_bytes(pale_handle* h, uint32_t width, uint32_t height, int32_t adjoint, uint64_t* out) {
+    if (!h || !out) return fail(PALE_ERR_INVALID_ARG, "null argument");
+    return guarded([&] { *out = h->viewGrowth(width * height, adjoint != 0); });
+}
+
+int pale_view_buffer_bytes(pale_handle* h, uint64_t* out) {
+    if (!h || !out) return fail(PALE_ERR_INVALID_ARG, "null argument");
+    return guarded([&] { *out = h->viewBufferBytes(); });
+}
+
+int pale_release_view_buffers(pale_handle* h) {
+    if (!h) return fail(PALE_ERR_INVALID_ARG, "null handle");
+    return guarded([&] {
+        HIPCHECK(hipStreamSynchronize(h->stream));
+        h->releaseViewBuffers();
+    });
+}
+
+int pale_build_photon_map(pale_handle* h) {
+    if (!h) return fail(PALE_ERR_INVALID_ARG, "null handle");
+    return guarded([&] { h->buildPhotonMap(); });
+}
+
+int pale_render_view(pale_handle* h, const pale_camera_desc* camera, uint32_t spp, uint32_t seed, float* ldr_out,
+                     float* raw_out) {
+    if (!h || !camera) return fail(PALE_ERR_INVALID_ARG, "null argument");
+    return guarded([&] {
+        const HostCamera c = hostCamera(*camera);
+        h->renderView(c, spp, seed);
+        const size_t bytes = (size_t)c.W * c.H * 4 * sizeof(float);
+        if (raw_out) HIPCHECK(hipMemcpy(raw_out, h->viewFb.p, bytes, hipMemcpyDeviceToHost));
+        if (ldr_out) HIPCHECK(hipMemcpy(ldr_out, h->viewLdr.p, bytes, hipMemcpyDeviceToHost));
+    });
+}
+
+int pale_render_view_device(pale_handle* h, const pale_camera_desc* camera, uint32_t spp, uint32_t seed,
+                            float* ldr_dev, float* raw_dev) {
+    if (!h || !camera) return fail(PALE_ERR_INVALID_ARG, "null argument");
+    return guarded([&] {
+        const HostCamera c = hostCamera(*camera);
+        h->renderView(c, spp, seed);
+        const size_t bytes = (size_t)c.W * c.H * 4 * sizeof(float);
+        if (raw_dev) HIPCHECK(hipMemcpyAsync(raw_dev, h->viewFb.p, bytes, hipMemcpyDeviceToDevice, h->stream));
+        if (ldr_dev) HIPCHECK(hipMemcpyAsync(ldr_dev, h->viewLdr.p, bytes, hipMemcpyDeviceToDevice, h->stream));
+        HIPCHECK(hipStreamSynchronize(h->stream));
+    });
+}
+
+int pale_render_view_backward(pale_handle* h, const pale_camera_desc* camera, uint32_t spp, const float* dloss_rgb,
+                              float* const* grads) {
+    if (!h || !camera || !dloss_rgb) return fail(PALE_ERR_INVALID_ARG, "null argument");
+    return guarded([&] {
+        if (spp == 0) throw StatusError(PALE_ERR_INVALID_ARG, "render_view_backward: spp must be >= 1");
+        const HostCamera c = hostCamera(*camera);
+        h->renderBackwardCams({&c}, {dloss_rgb}, false, spp);
+        copyGradsHost(h, grads);
     });
 }
 

@@ -217,6 +217,72 @@ class Renderer:
         torch.cuda.current_stream().synchronize()
         check(self._L.pale_set_opacity_device(self._h, self._dptr(t), t.numel()))
 
+    # ------------------------------------------------------- served views --
+    def build_photon_map(self) -> None:
+        """Photon stage of render_forward only; views then gather from it."""
+        check(self._L.pale_build_photon_map(self._h))
+
+    @staticmethod
+    def device_bytes() -> int:
+        """Device bytes held by every renderer of the process."""
+        v = C.c_uint64()
+        check(load().pale_device_bytes(C.byref(v)))
+        return v.value
+
+    def view_bytes(self, width: int, height: int, adjoint: bool) -> int:
+        """Device bytes a width x height view (with its adjoint) would add."""
+        v = C.c_uint64()
+        check(self._L.pale_view_bytes(self._h, int(width), int(height), 1 if adjoint else 0, C.byref(v)))
+        return v.value
+
+    def view_buffer_bytes(self) -> int:
+        """Device bytes of the image-sized buffers (what release_view_buffers frees)."""
+        v = C.c_uint64()
+        check(self._L.pale_view_buffer_bytes(self._h, C.byref(v)))
+        return v.value
+
+    def release_view_buffers(self) -> None:
+        """Frees the image-sized buffers; scene and photon map stay."""
+        check(self._L.pale_release_view_buffers(self._h))
+
+    def render_view(self, camera: _abi.Camera, spp: int, seed: int) -> dict:
+        """One view of the resident scene over the current photon map:
+        {"ldr": (H,W,4), "raw": (H,W,4)}.  Pass k draws the seed seed + k, so
+        the image depends only on the scene, the settings and (camera, spp,
+        seed)."""
+        d = _abi.fill_camera_desc(_abi.CameraDesc(), camera)
+        ldr = np.empty((camera.height, camera.width, 4), np.float32)
+        raw = np.empty((camera.height, camera.width, 4), np.float32)
+        check(self._L.pale_render_view(self._h, C.byref(d), int(spp), int(seed) & 0xFFFFFFFF, _p(ldr), _p(raw)))
+        return {"ldr": ldr, "raw": raw}
+
+    def render_view_device(self, camera: _abi.Camera, spp: int, seed: int) -> dict:
+        """render_view into torch tensors on the renderer's GPU."""
+        import torch
+
+        d = _abi.fill_camera_desc(_abi.CameraDesc(), camera)
+        ldr = torch.empty((camera.height, camera.width, 4), dtype=torch.float32, device="cuda")
+        raw = torch.empty((camera.height, camera.width, 4), dtype=torch.float32, device="cuda")
+        torch.cuda.current_stream().synchronize()
+        check(self._L.pale_render_view_device(self._h, C.byref(d), int(spp), int(seed) & 0xFFFFFFFF,
+                                              C.c_void_p(ldr.data_ptr()), C.c_void_p(raw.data_ptr())))
+        return {"ldr": ldr, "raw": raw}
+
+    def render_view_backward(self, camera: _abi.Camera, spp: int, dloss_rgb: np.ndarray) -> dict:
+        """Adjoint of one view: spp adjoint samples of dL/dI (H,W,3) through
+        the current photon map; the gradient dict of render_backward."""
+        a = np.ascontiguousarray(np.asarray(dloss_rgb)[..., :3], dtype=np.float32)
+        if a.shape != (camera.height, camera.width, 3):
+            raise RuntimeError(f"render_view_backward: dL/dI must be ({camera.height},{camera.width},3), "
+                               f"got {a.shape}")
+        d = _abi.fill_camera_desc(_abi.CameraDesc(), camera)
+        n = self.point_count
+        grads = {k: np.empty((n, w) if w else (n,), np.float32) for k, w in _abi.POINT_SHAPES.items()}
+        gp = (_fp * 8)(*[_p(grads[k]) for k in _abi.POINT_KEYS])
+        check(self._L.pale_render_view_backward(self._h, C.byref(d), int(spp), _p(a), gp))
+        grads["color"] = grads["albedo"]
+        return grads
+
     # ------------------------------------------------------------- points --
     @property
     def point_count(self) -> int:

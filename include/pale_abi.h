@@ -272,6 +272,53 @@ int pale_render_backward_device(pale_handle* h, uint32_t n,
 int pale_set_opacity_device(pale_handle* h, const float* opacity_dev,
                             uint32_t n);
 
+/* ---- served views: no reference counterpart (the reference renders only
+ * the cameras of its scene, pale_bindings.cpp:180-277).  For a server that
+ * keeps a scene resident and renders views of it on request: the photon map
+ * is built once and every view gathers from it.  A view is any camera
+ * description (its name and use_for_adjoint are ignored); its spp gather
+ * passes draw the seeds seed + 0 .. seed + spp - 1, so the image depends only
+ * on (scene, settings, camera, spp, seed) and never on other requests.
+ * Calls on one handle are serialised by the caller; handles on their own
+ * host threads run concurrently. ---- */
+
+/* Device bytes held by all renderers of the process (buffers grow on demand
+ * and are kept until pale_destroy): the measure of HBM admission. */
+int pale_device_bytes(uint64_t* out);
+
+/* Device bytes that serving a width x height view (adjoint != 0: with its
+ * adjoint) would add to this renderer's buffers; 0 when they already fit. */
+int pale_view_bytes(pale_handle* h, uint32_t width, uint32_t height,
+                    int32_t adjoint, uint64_t* out);
+
+/* Device bytes of this renderer's image-sized buffers (camera gather, view
+ * images, adjoint): what pale_release_view_buffers frees. */
+int pale_view_buffer_bytes(pale_handle* h, uint64_t* out);
+
+/* Frees the buffers that grow with the image size (camera gather, view
+ * images, adjoint); scene, BVH and photon map stay resident. */
+int pale_release_view_buffers(pale_handle* h);
+
+/* Photon stage of render_forward only (emission, bounces, grid). */
+int pale_build_photon_map(pale_handle* h);
+
+/* One view over the current photon map: H*W*4 floats into each non-NULL
+ * output (tone-mapped, linear).  Fails if no photon map was built. */
+int pale_render_view(pale_handle* h, const pale_camera_desc* camera,
+                     uint32_t spp, uint32_t seed, float* ldr_out,
+                     float* raw_out);
+
+/* pale_render_view into device memory. */
+int pale_render_view_device(pale_handle* h, const pale_camera_desc* camera,
+                            uint32_t spp, uint32_t seed, float* ldr_dev,
+                            float* raw_dev);
+
+/* Adjoint of one view: spp adjoint samples of dL/dI (H,W,3) through the
+ * current photon map; gradients as pale_render_backward. */
+int pale_render_view_backward(pale_handle* h, const pale_camera_desc* camera,
+                              uint32_t spp, const float* dloss_rgb,
+                              float* const* grads);
+
 /* ---- diagnostics: no reference counterpart; used by the parity tests ---- */
 
 /* Closest-hit query (the reference's intersectScene in FirstHit mode,

@@ -60,6 +60,9 @@ def lib():
         L.orc_set_sample.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32]
         L.orc_full_gradients.argtypes = [C.c_void_p, _fp]
         L.orc_freeze_map.argtypes = [C.c_void_p, C.c_int32]
+        L.orc_build_photon_map.argtypes = [C.c_void_p]
+        L.orc_render_view.argtypes = [C.c_void_p, C.POINTER(_abi.CameraDesc), C.c_uint32, C.c_uint32, _fp, _fp]
+        L.orc_render_view_backward.argtypes = [C.c_void_p, C.POINTER(_abi.CameraDesc), C.c_uint32, _fp, _fp]
         _lib = L
     return _lib
 
@@ -158,6 +161,26 @@ class Oracle:
         rc = lib().orc_render_backward(self.h, len(names), cn, ap, _p(g))
         if rc != 0:
             raise RuntimeError(lib().orc_last_error().decode())
+        return g
+
+    def build_photon_map(self):
+        if lib().orc_build_photon_map(self.h) != 0:
+            raise RuntimeError(lib().orc_last_error().decode())
+
+    def render_view(self, camera, spp: int, seed: int) -> Dict[str, np.ndarray]:
+        d = _abi.fill_camera_desc(_abi.CameraDesc(), camera)
+        ldr = np.zeros((camera.height, camera.width, 4), np.float32)
+        raw = np.zeros((camera.height, camera.width, 4), np.float32)
+        if lib().orc_render_view(self.h, C.byref(d), int(spp), int(seed) & 0xFFFFFFFF, _p(ldr), _p(raw)) != 0:
+            raise RuntimeError("render_view failed (no photon map, or spp == 0)")
+        return {"ldr": ldr, "raw": raw}
+
+    def render_view_backward(self, camera, spp: int, dloss: np.ndarray) -> np.ndarray:
+        d = _abi.fill_camera_desc(_abi.CameraDesc(), camera)
+        a = _abi.f32(np.asarray(dloss)[..., :3])
+        g = np.zeros(self.point_count, np.float32)
+        if lib().orc_render_view_backward(self.h, C.byref(d), int(spp), _p(a), _p(g)) != 0:
+            raise RuntimeError("render_view_backward failed (no photon map, or spp == 0)")
         return g
 
     def full_gradients(self) -> Dict[str, np.ndarray]:

@@ -1280,14 +1280,16 @@ static Ray primaryRay(const Camera& c, const CamConst& k, float px, float py, fl
 }
 
 // launchCameraGatherKernel (PrimalKernels.cpp:500-653) for every pass.
-static void cameraGather(Renderer& R, const Camera& c, uint32_t baseSeed, std::vector<float>& fb) {
+// A served view passes its own pass count and always decorrelates.
+static void cameraGather(Renderer& R, const Camera& c, uint32_t baseSeed, std::vector<float>& fb, uint32_t passes,
+                         bool decorrelate) {
     const uint32_t W = c.W, H = c.H, npx = W * H;
     fb.assign((size_t)npx * 4, 0.0f);
     const CamConst k = camConst(c);
-    const float spp = (float)R.S.gatherPasses;
+    const float spp = (float)passes;
     const float invSpp = 1.0f / spp;
-    for (uint32_t pass = 0; pass < R.S.gatherPasses; ++pass) {
-        const uint64_t seed = R.S.decorrelate ? (uint64_t)(uint32_t)(baseSeed + pass) : (uint64_t)baseSeed;
+    for (uint32_t pass = 0; pass < passes; ++pass) {
+        const uint64_t seed = decorrelate ? (uint64_t)(uint32_t)(baseSeed + pass) : (uint64_t)baseSeed;
         for (uint32_t pi = 0; pi < npx; ++pi) {
             if (pi % R.pixelStride) continue;
             const uint32_t px = pi % W, py = pi / W;
@@ -1367,7 +1369,7 @@ static void renderForward(Renderer& R, const std::vector<uint32_t>& camIdx, floa
         for (size_t k = 0; k < camIdx.size(); ++k) {
             const uint32_t number = (uint32_t)seedGen();
             const Camera& c = R.cams[camIdx[k]];
-            cameraGather(R, c, number, fb);
+            cameraGather(R, c, number, fb, R.S.gatherPasses, R.S.decorrelate);
             const size_t npx = (size_t)c.W * c.H;
             if (raw && raw[k]) std::memcpy(raw[k], fb.data(), npx * 4 * sizeof(float));
             if (ldr && ldr[k]) postprocess(fb, ldr[k], npx);
@@ -1417,7 +1419,7 @@ static void renderForward(Renderer& R, const std::vector<uint32_t>& camIdx, floa
     for (size_t k = 0; k < camIdx.size(); ++k) {
         const uint32_t number = (uint32_t)seedGen();
         const Camera& c = R.cams[camIdx[k]];
-        cameraGather(R, c, number, fb);
+        cameraGather(R, c, number, fb, R.S.gatherPasses, R.S.decorrelate);
         const size_t npx = (size_t)c.W * c.H;
         if (raw && raw[k]) std::memcpy(raw[k], fb.data(), npx * 4 * sizeof(float));
         if (ldr && ldr[k]) postprocess(fb, ldr[k], npx);
@@ -1610,19 +1612,19 @@ static void transport(Renderer& R, const Completed& c, float invSpp) {
 // submitAdjointKernel (SyclBridge.cpp:169-294) for the given cameras.  The
 // adjoint kernels seed from PathTracerSettings::random.number, which keeps
 // its default 42 (AdjointKernels.cpp:27,94; GPUDataStructures.h:395-398).
-static void renderBackward(Renderer& R, const std::vector<uint32_t>& camIdx,
-                           const std::vector<const float*>& dloss) {
+static void renderBackward(Renderer& R, const std::vector<const Camera*>& camList,
+                           const std::vector<const float*>& dloss, uint32_t passes) {
     const Settings& S = R.S;
     const uint64_t number = 42;
     workClass(2);
     R.gradOpacity.assign(R.points.size(), 0.0f);
     R.gradFull.assign(S.fullGradients ? R.points.size() * 16 : 0, 0.0f);
-    const float invSpp = 1.0f / (float)S.adjointPasses;
-    for (size_t k = 0; k < camIdx.size(); ++k) {
-        const Camera& c = R.cams[camIdx[k]];
+    const float invSpp = 1.0f / (float)passes;
+    for (size_t k = 0; k < camList.size(); ++k) {
+        const Camera& c = *camList[k];
         const CamConst kc = camConst(c);
         const uint32_t W = c.W, H = c.H, npx = W * H;
-        for (uint32_t spp = 0; spp < S.adjointPasses; ++spp) {
+        for (uint32_t spp = 0; spp < passes; ++spp) {
             // launchRayGenAdjointKernel (AdjointKernels.cpp:15-84)
             std::vector<ARay> q, nq;
             q.reserve(npx / R.pixelStride + 1);
@@ -1848,6 +1850,20 @@ static void rebuild(Renderer& R) {
 
 using namespace orc;
 
+static Camera cameraFrom(const pale_camera_desc& cd) {
+    Camera c;
+    c.name = std::string(cd.name, strnlen(cd.name, PALE_NAME_LEN));
+    c.W = cd.width;
+    c.H = cd.height;
+    c.fovy = cd.fovy_deg;
+    c.view = toM4(cd.view);
+    c.invView = toM4(cd.inv_view);
+    c.pos = mk(cd.position[0], cd.position[1], cd.position[2]);
+    c.fwd = mk(cd.forward[0], cd.forward[1], cd.forward[2]);
+    c.adjoint = cd.use_for_adjoint != 0;
+    return c;
+}
+
 extern "C" {
 
 const char* orc_last_error(void) { return g_err.c_str(); }
@@ -1868,20 +1884,7 @@ void* orc_create(const pale_scene_desc* scene, const pale_settings* settings) {
             m.mat = Material{mk(md.base_color[0], md.base_color[1], md.base_color[2]), md.power};
             R->meshes.push_back(m);
         }
-        for (uint32_t i = 0; i < scene->camera_count; ++i) {
-            const pale_camera_desc& cd = scene->cameras[i];
-            Camera c;
-            c.name = std::string(cd.name, strnlen(cd.name, PALE_NAME_LEN));
-            c.W = cd.width;
-            c.H = cd.height;
-            c.fovy = cd.fovy_deg;
-            c.view = toM4(cd.view);
-            c.invView = toM4(cd.inv_view);
-            c.pos = mk(cd.position[0], cd.position[1], cd.position[2]);
-            c.fwd = mk(cd.forward[0], cd.forward[1], cd.forward[2]);
-            c.adjoint = cd.use_for_adjoint != 0;
-            R->cams.push_back(c);
-        }
+        for (uint32_t i = 0; i < scene->camera_count; ++i) R->cams.push_back(cameraFrom(scene->cameras[i]));
         R->asset = toAsset(scene->points);
         rebuild(*R);
         return R;
@@ -2006,9 +2009,67 @@ int orc_render_backward(void* h, uint32_t n, const char* const* names, const flo
         R.stats.adjoint_queries = 0;
         R.stats.gradient_events = 0;
         const auto t0 = std::chrono::steady_clock::now();
-        renderBackward(R, idx, src);
+        std::vector<const Camera*> camList;
+        for (uint32_t c : idx) camList.push_back(&R.cams[c]);
+        renderBackward(R, camList, src, R.S.adjointPasses);
         R.stats.backward_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         copyWork(R);
+        if (opacityGrad) std::memcpy(opacityGrad, R.gradOpacity.data(), R.gradOpacity.size() * sizeof(float));
+        return 0;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return PALE_ERR_INTERNAL;
+    }
+}
+
+// Served views (no reference counterpart): the photon stage alone, then any
+// camera over that map with spp decorrelated passes seeded seed + pass, and
+// its adjoint with spp samples.
+int orc_build_photon_map(void* h) {
+    try {
+        Renderer& R = *static_cast<Renderer*>(h);
+        R.stats = pale_stats{};
+        for (Work& w : g_workClass) w = Work{};
+        const bool frozen = R.freezeMap;
+        R.freezeMap = false;
+        renderForward(R, {}, nullptr, nullptr);
+        R.freezeMap = frozen;
+        copyWork(R);
+        return 0;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return PALE_ERR_INTERNAL;
+    }
+}
+
+int orc_render_view(void* h, const pale_camera_desc* cd, uint32_t spp, uint32_t seed, float* ldr, float* raw) {
+    try {
+        Renderer& R = *static_cast<Renderer*>(h);
+        if (!R.mapValid || spp == 0) return PALE_ERR_INVALID_ARG;
+        const Camera c = cameraFrom(*cd);
+        std::vector<float> fb;
+        workClass(1);
+        cameraGather(R, c, seed, fb, spp, true);
+        const size_t npx = (size_t)c.W * c.H;
+        if (raw) std::memcpy(raw, fb.data(), npx * 4 * sizeof(float));
+        if (ldr) postprocess(fb, ldr, npx);
+        return 0;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return PALE_ERR_INTERNAL;
+    }
+}
+
+int orc_render_view_backward(void* h, const pale_camera_desc* cd, uint32_t spp, const float* dloss,
+                             float* opacityGrad) {
+    try {
+        Renderer& R = *static_cast<Renderer*>(h);
+        if (!R.mapValid || spp == 0) return PALE_ERR_INVALID_ARG;
+        const Camera c = cameraFrom(*cd);
+        uint32_t cap = 1;
+        while (cap < c.W * c.H) cap <<= 1u;
+        R.rayCapacity = std::max(R.rayCapacity, cap);
+        renderBackward(R, {&c}, {dloss}, spp);
         if (opacityGrad) std::memcpy(opacityGrad, R.gradOpacity.data(), R.gradOpacity.size() * sizeof(float));
         return 0;
     } catch (const std::exception& e) {

@@ -228,6 +228,24 @@ def test_camera_gather_rounds_bit_exact():
     assert np.array_equal(a.view(np.uint32), c.view(np.uint32))
 
 
+def test_camera_gather_rounds_resume_every_pixel():
+    # Many more 8x8 tiles than waves, an image border of background pixels
+    # that finish in the first round and a width that is not a multiple of
+    # 8: in later rounds most tiles a wave draws are already finished, and
+    # every pixel still unfinished must be drawn and completed.
+    spec = scene.config_scene(0, n=65536, width=333, height=250)
+    settings = dict(SMALL, gather_passes=1, adjoint_passes=1)
+    gpu, orc = _pair(spec, dict(settings, gather_entry_budget=2))
+    a = gpu.render_forward()["camera1_raw"]
+    assert gpu.stats()["camera_rounds"] >= 3
+    b = orc.render_forward()["camera1_raw"]
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    one, _ = _pair(spec, settings)
+    c = one.render_forward()["camera1_raw"]
+    assert one.stats()["camera_rounds"] == 1
+    assert np.array_equal(a.view(np.uint32), c.view(np.uint32))
+
+
 def test_photon_reservoir_bit_exact():
     # Capacity below the deposits: past it, deposits replace stored photons by
     # the reference's reservoir sampling (two extra draws of the depositing

@@ -354,3 +354,24 @@ def test_photon_reservoir_is_unbiased():
     ef, ep = full.gather(q, n), part.gather(q, n)
     rel = abs(ep.sum() - ef.sum()) / ef.sum()
     assert rel < 0.05, rel
+
+
+def test_view_passes_are_seeded_consecutively():
+    """A view of spp passes seeded s accumulates the passes seeded s, s+1, ...
+    with weight 1/spp each, in pass order (f = C * (1/spp) + f)."""
+    spec = scene.config_scene(0, n=512, width=24, height=24)
+    o = Oracle.from_spec(spec, dict(photons=5000))
+    o.build_photon_map()
+    cam = scene.look_at_camera("v", (1.0, 1.0, 3.7), (0, 0, 0), (0, 1, 0), 20, 16, 45.0)
+    a = o.render_view(cam, 1, 41)["raw"]
+    b = o.render_view(cam, 1, 42)["raw"]
+    ab = o.render_view(cam, 2, 41)["raw"]
+    half = np.float32(0.5)
+    want = b * half + a * half
+    assert np.array_equal(ab.view(np.uint32), want.view(np.uint32))
+    assert not np.array_equal(a, b)
+    # The photon map of build_photon_map is the one render_forward builds.
+    p0 = o.photons()
+    o.render_forward()
+    p1 = o.photons()
+    assert np.array_equal(p0["position"], p1["position"]) and p0["count"] == p1["count"] > 0
