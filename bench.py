@@ -31,9 +31,10 @@ target image of the scene with jittered centers, resolution log-uniform in
 area between 256x256 and 1920x1080, spp uniform in {1, 4, 16, 64}.  A bounded
 prefix of the stream is run (--served requests; the full stream is 10,000),
 first one request at a time (the plain single-view path on the same
-requests) and then as they arrive; reported are p50/p99 latency from
-arrival to completion, throughput of both runs and whether every result is
-the same alone and in the mix.
+requests), then all at once (the served form's capacity: views overlap on
+the device) and then as they arrive; reported are p50/p99 latency from
+arrival to completion at the arrival rate, throughput of the three runs and
+whether every result is the same alone and in both mixes.
 """
 from __future__ import annotations
 
@@ -194,7 +195,15 @@ def served_leg(a):
     plain_s = time.perf_counter() - tp
     plain = latency_summary(alone, plain_s)
 
-    # As they arrive (Poisson), continuously scheduled.
+    # All at once: the capacity of the served form (views overlap on the
+    # device), on the same requests.
+    tb = time.perf_counter()
+    burst = [f.result() for f in [srv.submit(r) for _, r in reqs]]
+    burst_s = max(m.finished for m in burst) - tb
+    cap = latency_summary(burst, burst_s)
+    print(f"[bench] served burst: {cap['mrays_per_s']:.1f} Mrays/s in {burst_s:.1f}s", file=sys.stderr, flush=True)
+
+    # As they arrive (Poisson), continuously scheduled: the latency.
     futs = []
     ts = time.perf_counter()
     for t, r in reqs:
@@ -207,7 +216,7 @@ def served_leg(a):
     served = latency_summary(mixed, makespan)
     same_images = same_grads = True
     mismatches = []
-    for i, (x, y) in enumerate(zip(alone, mixed)):
+    for i, (x, y) in [(i, p) for i, p in enumerate(zip(alone, mixed))] + [(i, p) for i, p in enumerate(zip(alone, burst))]:
         img_ok = np.array_equal(x.raw.view(np.uint32), y.raw.view(np.uint32))
         grad_ok = True
         gerr = None
@@ -237,6 +246,10 @@ def served_leg(a):
         "mrays_per_s": round(served["mrays_per_s"], 2),
         "requests_per_s": round(served["requests_per_s"], 3),
         "makespan_s": round(makespan, 3),
+        "capacity_all_at_once": {"mrays_per_s": round(cap["mrays_per_s"], 2),
+                                 "requests_per_s": round(cap["requests_per_s"], 3),
+                                 "p50_ms": round(cap["p50_ms"], 2), "p99_ms": round(cap["p99_ms"], 2),
+                                 "total_s": round(burst_s, 3)},
         "plain_one_at_a_time": {"mrays_per_s": round(plain["mrays_per_s"], 2),
                                 "requests_per_s": round(plain["requests_per_s"], 3),
                                 "p50_ms": round(plain["p50_ms"], 2), "p99_ms": round(plain["p99_ms"], 2),
@@ -251,8 +264,9 @@ def served_leg(a):
         "setup_s": round(setup_s, 1),
         "device_bytes_after_close": dev_bytes_after,
     }
-    print(f"[bench] served p50 {out['p50_ms']} ms p99 {out['p99_ms']} ms, {out['mrays_per_s']} Mrays/s "
-          f"(one at a time {out['plain_one_at_a_time']['mrays_per_s']}), same results: "
+    print(f"[bench] served p50 {out['p50_ms']} ms p99 {out['p99_ms']} ms at {a.served_rate}/s; capacity "
+          f"{out['capacity_all_at_once']['mrays_per_s']} Mrays/s all at once vs "
+          f"{out['plain_one_at_a_time']['mrays_per_s']} one at a time; same results: "
           f"{out['same_results_alone_and_mixed']}", file=sys.stderr, flush=True)
     return out
 
@@ -352,7 +366,7 @@ def main():
                              "achieved_gbs": per_launch / avg_s / 1e9, "total_ms_per_step": ms / a.steps}
         dom = max(kernels, key=lambda k: kernels[k]["total_ms_per_step"])
         kd = kernels[dom]
-        traffic = load_traffic(os.path.join(ROOT, "profiles", f"{dom}_c1_traffic.json"))
+        traffic = load_traffic(os.path.join(ROOT, "profiles", f"{dom}_1m_surfels_800px_traffic.json"))
         roofline = {
             "kernel": dom,
             "bound": "hbm",
