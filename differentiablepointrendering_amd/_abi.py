@@ -52,6 +52,18 @@ class CameraDesc(C.Structure):
     ]
 
 
+class BatchTuning(C.Structure):
+    _fields_ = [
+        ("slots_per_batch", C.c_uint32),
+        ("entries_per_slot", C.c_uint32),
+        ("graph_rounds", C.c_uint32),
+        ("grid_fraction", C.c_float),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class PointsDesc(C.Structure):
     _fields_ = [
         ("count", C.c_uint32),

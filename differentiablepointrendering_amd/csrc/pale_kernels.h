@@ -85,6 +85,31 @@ struct PixelArrays {
     uint32_t* off;
 };
 
+// A batch of camera-gather images (views x passes) walked by one persistent
+// launch: the slot space is a list of 8x8 pixel tiles, each naming its
+// image (seg) and its tile within it.  Segment = one pass of one view; its
+// pixels occupy slots slotBase .. slotBase + W*H - 1 of the pixel arrays.
+struct BatchSeg {
+    uint32_t view, pass, W, H, tilesX, slotBase, seed, pad;
+};
+// The passes of one view in a batch (consecutive segments, pass order): the
+// framebuffer update of its pixels, fb = C * invSpp + fb per pass in order.
+struct BatchRun {
+    uint32_t view, firstSeg, numSegs, npx, fbOffset, resolveBase;
+    float invSpp;
+    uint32_t pad;
+};
+struct BatchHeader {
+    uint32_t nTiles, nSlots, nSegs, nRuns, nResolve, pad[3];
+};
+struct BatchDesc {
+    const uint2* tiles;  // (segment, tile within the segment's image)
+    const BatchSeg* segs;
+    const BatchRun* runs;
+    const BatchHeader* hdr;
+    const DevCamera* cams;  // per view
+};
+
 // Camera-gather entries (one per layer): qa = gather point + normal.x,
 // qb = normal.y, normal.z, pixel (bits), index in the pixel's round (bits,
 // top bit = no gather); comp = colour factor k + weight w; E = irradiance;
@@ -168,6 +193,16 @@ void launchEntrySort(hipStream_t s, const GridParams& g, DevState* st, uint32_t 
 void launchEntryGather(hipStream_t s, const DevGrid& G, DevState* st, uint32_t cap, const uint32_t* nGather,
                        EntryArrays ent, bool count);
 void launchCamComposite(hipStream_t s, uint32_t npx, PixelArrays px, EntryArrays ent, float invSpp, float4* fb);
+// Batched camera gather: primary rays of every slot of the batch; the layer
+// walk over the batch's tiles (persistent grid scaled by gridFraction);
+// composite of up to maxSlots slots (count read from the header) without the
+// framebuffer update; the pass-ordered framebuffer update per view.
+void launchBatchInit(hipStream_t s, BatchDesc b, uint32_t maxTiles, PixelArrays px);
+void launchCamTraceBatch(hipStream_t s, const DevScene& S, DevState* st, BatchDesc b, uint32_t maxSlots,
+                         PixelArrays px, EntryArrays ent, uint32_t cap, uint32_t* cursor, float gridFraction);
+void launchCamCompositeBatch(hipStream_t s, uint32_t maxSlots, const BatchHeader* hdr, PixelArrays px,
+                             EntryArrays ent);
+void launchBatchResolve(hipStream_t s, BatchDesc b, uint32_t maxResolve, PixelArrays px, float4* fb);
 void launchPostprocess(hipStream_t s, const float4* fb, float4* ldr, uint32_t n);
 void launchAdjRaygen(hipStream_t s, const DevCamera& cam, const float* dloss, RayQueue q, PendingArrays pend,
                      DevState* st);

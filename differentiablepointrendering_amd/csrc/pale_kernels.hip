@@ -601,18 +601,45 @@ __global__ __launch_bounds__(256) void k_cam_init(DevCamera cam, const DevState*
     px.cnt[i] = 0u;
 }
 
-template <bool kCount>
+// Slot -> pixel maps of the layer walk.  Pixels are handed out in 8x8 tiles
+// (a wave's first rays are neighbours); the order does not change any
+// pixel's result.  ImageMap: the tiles of one image.  BatchMap: a list of
+// tiles of several images (views x passes).
+struct ImageMap {
+    uint32_t width, height, tilesX;
+    __device__ __forceinline__ uint32_t slots() const { return tilesX * ((height + 7u) / 8u) * 64u; }
+    __device__ __forceinline__ bool locate(uint32_t slot, uint32_t& p) const {
+        const uint32_t tile = slot >> 6, within = slot & 63u;
+        const uint32_t pxx = (tile % tilesX) * 8u + (within & 7u);
+        const uint32_t pxy = (tile / tilesX) * 8u + (within >> 3);
+        p = pxy * width + pxx;
+        return pxx < width && pxy < height;
+    }
+};
+
+struct BatchMap {
+    const uint2* tiles;
+    const BatchSeg* segs;
+    const BatchHeader* hdr;
+    __device__ __forceinline__ uint32_t slots() const { return hdr->nTiles * 64u; }
+    __device__ __forceinline__ bool locate(uint32_t slot, uint32_t& p) const {
+        const uint2 t = tiles[slot >> 6];
+        const BatchSeg& s = segs[t.x];
+        const uint32_t within = slot & 63u;
+        const uint32_t pxx = (t.y % s.tilesX) * 8u + (within & 7u);
+        const uint32_t pxy = (t.y / s.tilesX) * 8u + (within >> 3);
+        p = s.slotBase + pxy * s.W + pxx;
+        return pxx < s.W && pxy < s.H;
+    }
+};
+
+template <bool kCount, typename Map>
 __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_cam_trace(
-    DevScene S, DevState* st, uint32_t npx, uint32_t width, PixelArrays px, EntryArrays ent, uint32_t cap,
-    uint32_t* cursor) {
+    DevScene S, DevState* st, Map map, PixelArrays px, EntryArrays ent, uint32_t cap, uint32_t* cursor) {
     extern __shared__ uint2 s_stack[];
     const LaneStack stk = laneStack(s_stack, S.stackDepth);
     const uint32_t lane = laneId();
-    // Pixels are handed out in 8x8 tiles (a wave's first rays are neighbours);
-    // the order does not change any pixel's result.
-    const uint32_t height = npx / width;
-    const uint32_t tilesX = (width + 7u) / 8u;
-    const uint32_t slots = tilesX * ((height + 7u) / 8u) * 64u;
+    const uint32_t slots = map.slots();
     uint32_t layers = 0;
     Work w{};
     bool active = false, exhausted = false;
@@ -632,11 +659,8 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(4))
                     exhausted = true;
                 if (!active) {
                     const uint32_t slot = base + (uint32_t)__popcll(need & ((1ull << lane) - 1ull));
-                    const uint32_t tile = slot >> 6, within = slot & 63u;
-                    const uint32_t pxx = (tile % tilesX) * 8u + (within & 7u);
-                    const uint32_t pxy = (tile / tilesX) * 8u + (within >> 3);
-                    const uint32_t p = pxy * width + pxx;
-                    if (slot < slots && pxx < width && pxy < height && f2u(px.cs[p].w) == kPixActive) {
+                    uint32_t p = 0;
+                    if (slot < slots && map.locate(slot, p) && f2u(px.cs[p].w) == kPixActive) {
                         pixel = p;
                         active = true;
                         const float4 a = px.ot[p], b = px.dl[p];
@@ -799,10 +823,12 @@ __global__ __launch_bounds__(256) void k_entry_gather(DevGrid G, DevState* st, c
     if (kCount) flushWork(w, st, kWorkGather);
 }
 
-__global__ __launch_bounds__(256) void k_cam_composite(uint32_t npx, PixelArrays px, EntryArrays ent, float invSpp,
-                                                       float4* fb) {
+// fb == nullptr (batches): a finished pixel keeps its colour in cs and the
+// framebuffer update is left to k_batch_resolve.  nDev: device-side count.
+__global__ __launch_bounds__(256) void k_cam_composite(uint32_t npx, const uint32_t* nDev, PixelArrays px,
+                                                       EntryArrays ent, float invSpp, float4* fb) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= npx) return;
+    if (p >= (nDev ? min(*nDev, npx) : npx)) return;
     const float4 cs = px.cs[p];
     const uint32_t status = f2u(cs.w);
     if (status == kPixFlushed) return;
@@ -818,12 +844,60 @@ __global__ __launch_bounds__(256) void k_cam_composite(uint32_t npx, PixelArrays
     // touch, finished ones included.
     px.cnt[p] = 0u;
     if (status == kPixFinished) {
-        const float4 f = fb[p];
-        fb[p] = make_float4(C.x * invSpp + f.x, C.y * invSpp + f.y, C.z * invSpp + f.z, 1.0f * invSpp + f.w);
+        if (fb) {
+            const float4 f = fb[p];
+            fb[p] = make_float4(C.x * invSpp + f.x, C.y * invSpp + f.y, C.z * invSpp + f.z, 1.0f * invSpp + f.w);
+        }
         px.cs[p] = make_float4(C.x, C.y, C.z, u2f(kPixFlushed));
     } else {
         px.cs[p] = make_float4(C.x, C.y, C.z, u2f(kPixActive));
     }
+}
+
+// Primary rays of a batch: one thread per tile pixel; the same jitter and
+// ray as k_cam_init for (view, pass) with seed = view seed + pass.
+__global__ __launch_bounds__(256) void k_batch_init(BatchDesc b, PixelArrays px) {
+    const uint32_t t = blockIdx.x * 4u + (threadIdx.x >> 6);
+    if (t >= b.hdr->nTiles) return;
+    const uint2 td = b.tiles[t];
+    const BatchSeg s = b.segs[td.x];
+    const uint32_t within = threadIdx.x & 63u;
+    const uint32_t pxx = (td.y % s.tilesX) * 8u + (within & 7u);
+    const uint32_t pxy = (td.y / s.tilesX) * 8u + (within >> 3);
+    if (pxx >= s.W || pxy >= s.H) return;
+    const uint32_t pixel = pxy * s.W + pxx, slot = s.slotBase + pixel;
+    Rng rng(perItemSeed(s.seed, pixel));
+    const float jx = rng.nextFloat() - 0.5f;
+    const float jy = rng.nextFloat() - 0.5f;
+    F3 o, d;
+    primaryRay(b.cams[s.view], (float)pxx, (float)pxy, jx, jy, o, d);
+    px.ot[slot] = f4(o, 1.0f);
+    px.dl[slot] = f4(d, u2f(0u));
+    px.cs[slot] = make_float4(0.0f, 0.0f, 0.0f, u2f(kPixActive));
+    px.cnt[slot] = 0u;
+}
+
+// Framebuffer update of a batch, per view pixel over the view's passes in
+// pass order: the sequence of k_cam_composite's updates.
+__global__ __launch_bounds__(256) void k_batch_resolve(BatchDesc b, PixelArrays px, float4* fb) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= b.hdr->nResolve) return;
+    uint32_t lo = 0, hi = b.hdr->nRuns - 1u;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1u) >> 1;
+        if (b.runs[mid].resolveBase <= i)
+            lo = mid;
+        else
+            hi = mid - 1u;
+    }
+    const BatchRun r = b.runs[lo];
+    const uint32_t pixel = i - r.resolveBase;
+    float4 f = fb[r.fbOffset + pixel];
+    for (uint32_t k = 0; k < r.numSegs; ++k) {
+        const float4 c = px.cs[b.segs[r.firstSeg + k].slotBase + pixel];
+        f = make_float4(c.x * r.invSpp + f.x, c.y * r.invSpp + f.y, c.z * r.invSpp + f.z, 1.0f * r.invSpp + f.w);
+    }
+    fb[r.fbOffset + pixel] = f;
 }
 
 // launchPostProcessKernel (PostprocessKernel.cpp:11-106), float LDR output.
@@ -1288,7 +1362,7 @@ void launchBuildGrid(hipStream_t s, const GridParams& g, DevState* st, uint32_t 
 
 // Persistent grid of the camera trace: as many workgroups as fit on the device.
 // Renderers on several host threads share the cache.
-template <bool kCount>
+template <bool kCount, typename Map>
 static uint32_t camTraceGrid(size_t lds) {
     static std::mutex lock;
     static uint32_t cached[2] = {0, 0};
@@ -1299,7 +1373,7 @@ static uint32_t camTraceGrid(size_t lds) {
     int dev = 0, cus = 0, perCu = 0;
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, k_cam_trace<kCount>, kTraceBlock, lds);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, k_cam_trace<kCount, Map>, kTraceBlock, lds);
     cached[k] = (uint32_t)std::max(1, cus) * (uint32_t)std::max(1, perCu);
     cachedLds[k] = lds;
     return cached[k];
@@ -1324,13 +1398,38 @@ void launchCamTrace(hipStream_t s, const DevScene& S, DevState* st, uint32_t npx
                     EntryArrays ent, uint32_t cap, uint32_t* cursor, bool count) {
     if (!npx) return;
     const size_t lds = stackBytes(S);
+    const ImageMap map{width, npx / width, (width + 7u) / 8u};
     if (count) {
-        const uint32_t g = min(camTraceGrid<true>(lds), blocks(npx, kTraceBlock));
-        k_cam_trace<true><<<g, kTraceBlock, lds, s>>>(S, st, npx, width, px, ent, cap, cursor);
+        const uint32_t g = min(camTraceGrid<true, ImageMap>(lds), blocks(npx, kTraceBlock));
+        k_cam_trace<true><<<g, kTraceBlock, lds, s>>>(S, st, map, px, ent, cap, cursor);
     } else {
-        const uint32_t g = min(camTraceGrid<false>(lds), blocks(npx, kTraceBlock));
-        k_cam_trace<false><<<g, kTraceBlock, lds, s>>>(S, st, npx, width, px, ent, cap, cursor);
+        const uint32_t g = min(camTraceGrid<false, ImageMap>(lds), blocks(npx, kTraceBlock));
+        k_cam_trace<false><<<g, kTraceBlock, lds, s>>>(S, st, map, px, ent, cap, cursor);
     }
+}
+
+void launchBatchInit(hipStream_t s, BatchDesc b, uint32_t maxTiles, PixelArrays px) {
+    if (maxTiles) k_batch_init<<<blocks(maxTiles, 4), 256, 0, s>>>(b, px);
+}
+
+void launchCamTraceBatch(hipStream_t s, const DevScene& S, DevState* st, BatchDesc b, uint32_t maxSlots,
+                         PixelArrays px, EntryArrays ent, uint32_t cap, uint32_t* cursor, float gridFraction) {
+    if (!maxSlots) return;
+    const size_t lds = stackBytes(S);
+    const BatchMap map{b.tiles, b.segs, b.hdr};
+    const uint32_t full = camTraceGrid<false, BatchMap>(lds);
+    const uint32_t g = min(max(1u, (uint32_t)((float)full * gridFraction)), blocks(maxSlots, kTraceBlock));
+    k_cam_trace<false><<<g, kTraceBlock, lds, s>>>(S, st, map, px, ent, cap, cursor);
+}
+
+void launchCamCompositeBatch(hipStream_t s, uint32_t maxSlots, const BatchHeader* hdr, PixelArrays px,
+                             EntryArrays ent) {
+    if (maxSlots)
+        k_cam_composite<<<blocks(maxSlots, 256), 256, 0, s>>>(maxSlots, &hdr->nSlots, px, ent, 0.0f, nullptr);
+}
+
+void launchBatchResolve(hipStream_t s, BatchDesc b, uint32_t maxResolve, PixelArrays px, float4* fb) {
+    if (maxResolve) k_batch_resolve<<<blocks(maxResolve, 256), 256, 0, s>>>(b, px, fb);
 }
 
 void launchEntrySort(hipStream_t s, const GridParams& g, DevState* st, uint32_t cap, PixelArrays px,
@@ -1352,7 +1451,7 @@ void launchEntryGather(hipStream_t s, const DevGrid& G, DevState* st, uint32_t c
 }
 
 void launchCamComposite(hipStream_t s, uint32_t npx, PixelArrays px, EntryArrays ent, float invSpp, float4* fb) {
-    if (npx) k_cam_composite<<<blocks(npx, 256), 256, 0, s>>>(npx, px, ent, invSpp, fb);
+    if (npx) k_cam_composite<<<blocks(npx, 256), 256, 0, s>>>(npx, nullptr, px, ent, invSpp, fb);
 }
 
 void launchPostprocess(hipStream_t s, const float4* fb, float4* ldr, uint32_t n) {

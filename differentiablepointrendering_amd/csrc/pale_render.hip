@@ -129,6 +129,12 @@ struct pale_handle {
     // Camera gather: pixel state, entries of one round, their cell sort.
     uint32_t pxCap = 0, entryCap = 0;
     DevMem viewFb, viewLdr;  // images of pale_render_view
+    // Batched views (pale_render_views): descriptors of the current batch,
+    // the framebuffers of all views, the captured graph of a batch's rounds.
+    DevMem bTiles, bSegs, bRuns, bHdr, bCams, viewsFb, viewsLdr;
+    pale_batch_tuning tune{};
+    hipGraphExec_t bGraph = nullptr;
+    std::vector<uint8_t> bGraphKey;
     DevMem pxOt, pxDl, pxCs, pxCnt, pxOff, pxTiles, entQa, entQb, entComp, entE, entOrd, entCell, entSorted,
         entStart, flagHost;
     // Device state and seed table.
@@ -183,6 +189,7 @@ struct pale_handle {
 
     ~pale_handle() {
         if (stream) (void)hipStreamSynchronize(stream);
+        if (bGraph) (void)hipGraphExecDestroy(bGraph);
         for (hipEvent_t e : evPool) (void)hipEventDestroy(e);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
@@ -483,7 +490,7 @@ struct pale_handle {
     // buffer holds gather_entry_budget (default 24) layers per pixel on
     // average: one round in practice; deeper images take more rounds.
     template <typename A>
-    void sizeCamera(uint32_t npx, A& a) {
+    void sizeCamera(uint32_t npx, A& a, uint32_t perPixelOverride = 0) {
         if (npx > pxCap) {
             a(pxOt, (size_t)npx * 16);
             a(pxDl, (size_t)npx * 16);
@@ -493,8 +500,9 @@ struct pale_handle {
             a(pxTiles, ((size_t)npx / kTile + 2) * 4);
             if (!A::kDry) pxCap = npx;
         }
-        const uint64_t perPixel = S.gather_entry_budget ? S.gather_entry_budget : 24u;
-        const uint64_t floor = S.gather_entry_budget ? 64u : (1u << 16);
+        const uint32_t budget = perPixelOverride ? perPixelOverride : S.gather_entry_budget;
+        const uint64_t perPixel = budget ? budget : 24u;
+        const uint64_t floor = budget ? 64u : (1u << 16);
         const uint32_t want = (uint32_t)std::min<uint64_t>(std::max<uint64_t>((uint64_t)npx * perPixel, floor), 1u << 28);
         if (want != entryCap) {
             a(entQa, (size_t)want * 16);
@@ -509,9 +517,9 @@ struct pale_handle {
         a(entStart, ((size_t)grid.total + 1) * 4);
     }
 
-    void ensureCamera(uint32_t npx) {
+    void ensureCamera(uint32_t npx, uint32_t perPixelOverride = 0) {
         Allocate a;
-        sizeCamera(npx, a);
+        sizeCamera(npx, a, perPixelOverride);
         if (!flagHost.p) {
             HIPCHECK(hipHostMalloc(&flagHost.p, 16, hipHostMallocDefault));
             flagHost.bytes = 16;
@@ -673,6 +681,217 @@ struct pale_handle {
         tracePhotons(tlen);
         finishForward();
     }
+
+    // ---------------------------------------------------------- batches --
+    // Views x passes walked together: segments in pass-major order (every
+    // view's pass 0, then pass 1, ...) are cut into batches of at most
+    // slots_per_batch pixel slots; a batch is one persistent layer walk per
+    // round over all its tiles, and a view's framebuffer is updated pass by
+    // pass in order (k_batch_resolve), so each image equals renderView's.
+    struct HostBatch {
+        std::vector<uint2> tiles;
+        std::vector<BatchSeg> segs;
+        std::vector<BatchRun> runs;
+        BatchHeader hdr{};
+    };
+
+    BatchDesc batchDesc() const {
+        return BatchDesc{bTiles.as<uint2>(), bSegs.as<BatchSeg>(), bRuns.as<BatchRun>(), bHdr.as<BatchHeader>(),
+                         bCams.as<DevCamera>()};
+    }
+
+    void batchRound(uint32_t maxSlots, float fraction) {
+        const PixelArrays px = pxa();
+        const EntryArrays ent = enta();
+        launchFill(stream, stField(offsetof(DevState, entries)), 2, 0u);  // entries, overflow
+        launchFill(stream, cursors.as<uint32_t>(), 1, 0u);
+        launchCamTraceBatch(stream, scene, st(), batchDesc(), maxSlots, px, ent, entryCap, cursors.as<uint32_t>(),
+                            fraction);
+        launchExclusiveScan(stream, px.cnt, px.off, maxSlots, pxTiles.as<uint32_t>(), gridTotal.as<uint32_t>());
+        launchEntrySort(stream, grid, st(), entryCap, px, ent, cellCount.as<uint32_t>(), entStart.as<uint32_t>(),
+                        gridTiles.as<uint32_t>(), gridTotal.as<uint32_t>());
+        launchEntryGather(stream, devGrid(), st(), entryCap, entStart.as<uint32_t>() + grid.total, ent, false);
+        launchCamCompositeBatch(stream, maxSlots, bHdr.as<BatchHeader>(), px, ent);
+    }
+
+    template <typename T>
+    static void keyAdd(std::vector<uint8_t>& k, const T& v) {
+        const uint8_t* p = reinterpret_cast<const uint8_t*>(&v);
+        k.insert(k.end(), p, p + sizeof(T));
+    }
+
+    // The graph of a batch's first R rounds (init included); kernels read the
+    // batch's sizes from the header, so one graph serves every batch of the
+    // same capacities.  Recaptured when any argument it holds changes.
+    void ensureBatchGraph(uint32_t maxTiles, uint32_t maxSlots, uint32_t rounds, float fraction) {
+        std::vector<uint8_t> key;
+        const PixelArrays px = pxa();
+        const EntryArrays ent = enta();
+        const BatchDesc bd = batchDesc();
+        const DevGrid G = devGrid();
+        keyAdd(key, px);
+        keyAdd(key, ent);
+        keyAdd(key, bd);
+        keyAdd(key, scene);
+        keyAdd(key, G);
+        keyAdd(key, grid);
+        keyAdd(key, state.p);
+        keyAdd(key, cursors.p);
+        keyAdd(key, entStart.p);
+        keyAdd(key, pxTiles.p);
+        keyAdd(key, gridTiles.p);
+        keyAdd(key, gridTotal.p);
+        keyAdd(key, cellCount.p);
+        keyAdd(key, maxTiles);
+        keyAdd(key, maxSlots);
+        keyAdd(key, entryCap);
+        keyAdd(key, rounds);
+        keyAdd(key, fraction);
+        if (bGraph && key == bGraphKey) return;
+        if (bGraph) {
+            HIPCHECK(hipGraphExecDestroy(bGraph));
+            bGraph = nullptr;
+        }
+        hipGraph_t g = nullptr;
+        HIPCHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed));
+        launchBatchInit(stream, bd, maxTiles, px);
+        for (uint32_t r = 0; r < rounds; ++r) batchRound(maxSlots, fraction);
+        HIPCHECK(hipStreamEndCapture(stream, &g));
+        const hipError_t e = hipGraphInstantiate(&bGraph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        HIPCHECK(e);
+        bGraphKey = key;
+    }
+
+    void renderViews(const std::vector<HostCamera>& vc, const std::vector<uint32_t>& spp,
+                     const std::vector<uint32_t>& seeds) {
+        if (!mapValid) throw StatusError(PALE_ERR_INVALID_ARG, "render_views: no photon map; build it first");
+        const uint32_t n = (uint32_t)vc.size();
+        std::vector<size_t> fbOff(n + 1, 0);
+        uint32_t maxSpp = 0;
+        for (uint32_t v = 0; v < n; ++v) {
+            if (spp[v] == 0) throw StatusError(PALE_ERR_INVALID_ARG, "render_views: spp must be >= 1");
+            fbOff[v + 1] = fbOff[v] + (size_t)vc[v].W * vc[v].H;
+            maxSpp = std::max(maxSpp, spp[v]);
+        }
+        if (fbOff[n] > 0xFFFFFFFFull) throw StatusError(PALE_ERR_INVALID_ARG, "render_views: too many pixels");
+        const uint64_t budget = tune.slots_per_batch ? tune.slots_per_batch : (4u << 20);
+        const float fraction = tune.grid_fraction > 0.0f ? std::min(tune.grid_fraction, 1.0f) : 1.0f;
+        // Segments in pass-major order, cut into batches.
+        std::vector<HostBatch> batches;
+        uint64_t cur = budget + 1;
+        for (uint32_t p = 0; p < maxSpp; ++p) {
+            for (uint32_t v = 0; v < n; ++v) {
+                if (spp[v] <= p) continue;
+                const HostCamera& c = vc[v];
+                const uint32_t npx = c.W * c.H;
+                if (cur > 0 && cur + npx > budget) {
+                    batches.emplace_back();
+                    cur = 0;
+                }
+                HostBatch& b = batches.back();
+                const uint32_t tx = (c.W + 7u) / 8u, ty = (c.H + 7u) / 8u;
+                const uint32_t si = (uint32_t)b.segs.size();
+                b.segs.push_back(BatchSeg{v, p, c.W, c.H, tx, (uint32_t)cur, seeds[v] + p, 0u});
+                for (uint32_t t = 0; t < tx * ty; ++t) b.tiles.push_back(make_uint2(si, t));
+                cur += npx;
+            }
+        }
+        uint32_t maxTiles = 0, maxSlots = 0, maxResolve = 0;
+        for (HostBatch& b : batches) {
+            // One run per view present, its segments in pass order (in
+            // pass-major order a view's segments of a batch are not adjacent).
+            std::vector<std::vector<uint32_t>> byView(n);
+            for (uint32_t s = 0; s < b.segs.size(); ++s) byView[b.segs[s].view].push_back(s);
+            std::vector<BatchSeg> ordered;
+            std::vector<uint32_t> remap(b.segs.size());
+            uint32_t resolve = 0;
+            for (uint32_t v = 0; v < n; ++v) {
+                if (byView[v].empty()) continue;
+                BatchRun r{};
+                r.view = v;
+                r.firstSeg = (uint32_t)ordered.size();
+                r.numSegs = (uint32_t)byView[v].size();
+                r.npx = vc[v].W * vc[v].H;
+                r.fbOffset = (uint32_t)fbOff[v];
+                r.resolveBase = resolve;
+                r.invSpp = 1.0f / (float)spp[v];
+                for (uint32_t s : byView[v]) {
+                    remap[s] = (uint32_t)ordered.size();
+                    ordered.push_back(b.segs[s]);
+                }
+                resolve += r.npx;
+                b.runs.push_back(r);
+            }
+            for (uint2& t : b.tiles) t.x = remap[t.x];
+            b.segs.swap(ordered);
+            uint32_t slots = 0;
+            for (const BatchSeg& s : b.segs) slots += s.W * s.H;
+            b.hdr = BatchHeader{(uint32_t)b.tiles.size(), slots, (uint32_t)b.segs.size(), (uint32_t)b.runs.size(),
+                                resolve, {0, 0, 0}};
+            maxTiles = std::max(maxTiles, b.hdr.nTiles);
+            maxSlots = std::max(maxSlots, slots);
+            maxResolve = std::max(maxResolve, resolve);
+        }
+        // Buffers (all before any capture).
+        const uint32_t perSlot = tune.entries_per_slot ? tune.entries_per_slot : 24u;
+        ensureCamera(maxSlots, perSlot);
+        cursors.reserve(sizeof(uint32_t));
+        bTiles.reserve((size_t)maxTiles * sizeof(uint2));
+        size_t maxSegs = 1, maxRuns = 1;
+        for (const HostBatch& b : batches) {
+            maxSegs = std::max(maxSegs, b.segs.size());
+            maxRuns = std::max(maxRuns, b.runs.size());
+        }
+        bSegs.reserve(maxSegs * sizeof(BatchSeg));
+        bRuns.reserve(maxRuns * sizeof(BatchRun));
+        bHdr.reserve(sizeof(BatchHeader));
+        bCams.reserve(std::max<size_t>(n, 1) * sizeof(DevCamera));
+        viewsFb.reserve(std::max<size_t>(fbOff[n], 1) * sizeof(float4));
+        viewsLdr.reserve(std::max<size_t>(fbOff[n], 1) * sizeof(float4));
+        std::vector<DevCamera> dcs(n);
+        for (uint32_t v = 0; v < n; ++v) dcs[v] = devCam(vc[v]);
+        upload(stream, bCams, dcs);
+        const uint32_t rounds = tune.graph_rounds;
+        if (rounds) ensureBatchGraph(maxTiles, maxSlots, rounds, fraction);
+        tReset();
+        camRounds = 0;
+        HIPCHECK(hipEventRecord(ev0, stream));
+        launchBeginView(stream, st(), 0u);
+        launchFill(stream, pxCnt.as<uint32_t>(), maxSlots, 0u);
+        launchFillF(stream, viewsFb.as<float>(), (uint32_t)fbOff[n] * 4u, 0.0f);
+        const BatchDesc bd = batchDesc();
+        for (HostBatch& b : batches) {
+            upload(stream, bTiles, b.tiles);
+            upload(stream, bSegs, b.segs);
+            upload(stream, bRuns, b.runs);
+            HIPCHECK(hipMemcpyAsync(bHdr.p, &b.hdr, sizeof(BatchHeader), hipMemcpyHostToDevice, stream));
+            if (rounds) {
+                HIPCHECK(hipGraphLaunch(bGraph, stream));
+                camRounds += rounds;
+            } else {
+                launchBatchInit(stream, bd, b.hdr.nTiles, pxa());
+                batchRound(maxSlots, fraction);
+                ++camRounds;
+            }
+            for (uint32_t extra = 0;; ++extra) {
+                HIPCHECK(hipMemcpyAsync(flagHost.p, stField(offsetof(DevState, overflow)), 4,
+                                        hipMemcpyDeviceToHost, stream));
+                HIPCHECK(hipStreamSynchronize(stream));
+                if (*flagHost.as<uint32_t>() == 0u) break;
+                if (extra > 4096) throw StatusError(PALE_ERR_INTERNAL, "batched camera gather: no progress");
+                batchRound(maxSlots, fraction);
+                ++camRounds;
+            }
+            launchBatchResolve(stream, bd, b.hdr.nResolve, pxa(), viewsFb.as<float4>());
+        }
+        launchPostprocess(stream, viewsFb.as<float4>(), viewsLdr.as<float4>(), (uint32_t)fbOff[n]);
+        viewOffsets = fbOff;
+        batchCount = (uint32_t)batches.size();
+        finishForward();
+    }
+    std::vector<size_t> viewOffsets;
+    uint32_t batchCount = 0;
 
     // One view over the current photon map: spp passes seeded by seed + k.
     void renderView(const HostCamera& c, uint32_t spp, uint32_t seed) {
@@ -1130,6 +1349,39 @@ int pale_set_opacity_device(pale_handle* h, const float* opacity_dev, uint32_t n
             h->asset[i].opacity = v[i];
             h->img.surfShade[i].opacity = v[i];
         }
+    });
+}
+
+int pale_set_batch_tuning(pale_handle* h, const pale_batch_tuning* t) {
+    if (!h || !t) return fail(PALE_ERR_INVALID_ARG, "null argument");
+    if (!(t->grid_fraction >= 0.0f && t->grid_fraction <= 1.0f))
+        return fail(PALE_ERR_INVALID_ARG, "grid_fraction must be in [0, 1]");
+    if (t->graph_rounds > 64) return fail(PALE_ERR_INVALID_ARG, "graph_rounds must be <= 64");
+    h->tune = *t;
+    return PALE_OK;
+}
+
+int pale_get_batch_tuning(const pale_handle* h, pale_batch_tuning* t) {
+    if (!h || !t) return fail(PALE_ERR_INVALID_ARG, "null argument");
+    *t = h->tune;
+    return PALE_OK;
+}
+
+int pale_render_views(pale_handle* h, uint32_t n, const pale_camera_desc* cameras, const uint32_t* spp,
+                      const uint32_t* seeds, float* const* ldr_out, float* const* raw_out, uint32_t* batches_out) {
+    if (!h || (n && (!cameras || !spp || !seeds))) return fail(PALE_ERR_INVALID_ARG, "null argument");
+    return guarded([&] {
+        std::vector<HostCamera> vc;
+        for (uint32_t i = 0; i < n; ++i) vc.push_back(hostCamera(cameras[i]));
+        h->renderViews(vc, std::vector<uint32_t>(spp, spp + n), std::vector<uint32_t>(seeds, seeds + n));
+        for (uint32_t i = 0; i < n; ++i) {
+            const size_t off = h->viewOffsets[i], bytes = (h->viewOffsets[i + 1] - off) * sizeof(float4);
+            if (raw_out && raw_out[i])
+                HIPCHECK(hipMemcpy(raw_out[i], h->viewsFb.as<float4>() + off, bytes, hipMemcpyDeviceToHost));
+            if (ldr_out && ldr_out[i])
+                HIPCHECK(hipMemcpy(ldr_out[i], h->viewsLdr.as<float4>() + off, bytes, hipMemcpyDeviceToHost));
+        }
+        if (batches_out) *batches_out = h->batchCount;
     });
 }
 

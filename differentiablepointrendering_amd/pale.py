@@ -256,6 +256,40 @@ class Renderer:
         check(self._L.pale_render_view(self._h, C.byref(d), int(spp), int(seed) & 0xFFFFFFFF, _p(ldr), _p(raw)))
         return {"ldr": ldr, "raw": raw}
 
+    def set_batch_tuning(self, slots_per_batch: int = 0, entries_per_slot: int = 0, graph_rounds: int = 0,
+                         grid_fraction: float = 0.0) -> None:
+        """Launch shape of render_views (0 = default); never changes results."""
+        t = _abi.BatchTuning(int(slots_per_batch), int(entries_per_slot), int(graph_rounds), float(grid_fraction))
+        check(self._L.pale_set_batch_tuning(self._h, C.byref(t)))
+
+    def batch_tuning(self) -> dict:
+        t = _abi.BatchTuning()
+        check(self._L.pale_get_batch_tuning(self._h, C.byref(t)))
+        return t.as_dict()
+
+    def render_views(self, cameras, spp, seeds) -> List[dict]:
+        """Several views over the current photon map, batched: a list of
+        {"ldr", "raw"} equal to render_view(cameras[i], spp[i], seeds[i])."""
+        n = len(cameras)
+        if len(spp) != n or len(seeds) != n:
+            raise RuntimeError("render_views: cameras, spp and seeds must have the same length")
+        descs = (_abi.CameraDesc * max(1, n))()
+        for i, c in enumerate(cameras):
+            _abi.fill_camera_desc(descs[i], c)
+        sp = np.ascontiguousarray(spp, dtype=np.uint32)
+        sd = np.ascontiguousarray(np.asarray(seeds, dtype=np.int64) & 0xFFFFFFFF, dtype=np.uint32)
+        ldr = [np.empty((c.height, c.width, 4), np.float32) for c in cameras]
+        raw = [np.empty((c.height, c.width, 4), np.float32) for c in cameras]
+        k = max(1, n)
+        lp = (_fp * k)(*[_p(a) for a in ldr])
+        rp = (_fp * k)(*[_p(a) for a in raw])
+        nb = C.c_uint32()
+        u32 = C.POINTER(C.c_uint32)
+        check(self._L.pale_render_views(self._h, n, descs, sp.ctypes.data_as(u32), sd.ctypes.data_as(u32), lp, rp,
+                                        C.byref(nb)))
+        self.last_batches = nb.value
+        return [{"ldr": a, "raw": b} for a, b in zip(ldr, raw)]
+
     def render_view_device(self, camera: _abi.Camera, spp: int, seed: int) -> dict:
         """render_view into torch tensors on the renderer's GPU."""
         import torch

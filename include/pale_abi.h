@@ -319,6 +319,32 @@ int pale_render_view_backward(pale_handle* h, const pale_camera_desc* camera,
                               uint32_t spp, const float* dloss_rgb,
                               float* const* grads);
 
+/* ---- batched views: no reference counterpart (the reference renders one
+ * camera after another, SyclBridge.cpp:144-157).  Many views of uneven
+ * resolution and spp over the current photon map in one persistent
+ * layer-walk launch per batch and round: pixel slots of all views and passes
+ * are cut into batches of slots_per_batch; the rounds of a batch run as a
+ * captured HIP graph when graph_rounds > 0.  Every image equals
+ * pale_render_view(cameras[i], spp[i], seeds[i]) bit for bit, whatever the
+ * tuning. ---- */
+
+typedef struct pale_batch_tuning {
+    uint32_t slots_per_batch;  /* pixel slots (pixels x passes) per batch; 0 = 4Mi       */
+    uint32_t entries_per_slot; /* camera-gather entries per slot and round; 0 = 24        */
+    uint32_t graph_rounds;     /* rounds in the captured graph; 0 = eager launches        */
+    float grid_fraction;       /* layer-walk grid / full occupancy grid, (0,1]; 0 = 1     */
+} pale_batch_tuning;
+
+int pale_set_batch_tuning(pale_handle* h, const pale_batch_tuning* tuning);
+int pale_get_batch_tuning(const pale_handle* h, pale_batch_tuning* tuning);
+
+/* n views; each output (H_i*W_i*4 floats) may be NULL.  batches_out (may be
+ * NULL) receives the number of batches the views took. */
+int pale_render_views(pale_handle* h, uint32_t n, const pale_camera_desc* cameras,
+                      const uint32_t* spp, const uint32_t* seeds,
+                      float* const* ldr_out, float* const* raw_out,
+                      uint32_t* batches_out);
+
 /* ---- diagnostics: no reference counterpart; used by the parity tests ---- */
 
 /* Closest-hit query (the reference's intersectScene in FirstHit mode,

@@ -31,15 +31,16 @@ def test_ctypes_layout_matches_c(tmp_path):
     prog = tmp_path / "sizes.c"
     prog.write_text(
         '#include <stdio.h>\n#include <stddef.h>\n#include "pale_abi.h"\n'
-        "int main(void){printf(\"%zu %zu %zu %zu %zu %zu %zu\\n\", sizeof(pale_mesh_desc), sizeof(pale_camera_desc),"
-        " sizeof(pale_points_desc), sizeof(pale_scene_desc), sizeof(pale_settings), sizeof(pale_stats),"
-        " offsetof(pale_settings, gather_radius));return 0;}\n")
+        "int main(void){printf(\"%zu %zu %zu %zu %zu %zu %zu %zu %zu\\n\", sizeof(pale_mesh_desc),"
+        " sizeof(pale_camera_desc), sizeof(pale_points_desc), sizeof(pale_scene_desc), sizeof(pale_settings),"
+        " sizeof(pale_stats), offsetof(pale_settings, gather_radius), sizeof(pale_batch_tuning),"
+        " offsetof(pale_batch_tuning, grid_fraction));return 0;}\n")
     exe = tmp_path / "sizes"
     subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(prog), "-o", str(exe)], check=True)
     got = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
     want = [C.sizeof(_abi.MeshDesc), C.sizeof(_abi.CameraDesc), C.sizeof(_abi.PointsDesc),
             C.sizeof(_abi.SceneDesc), C.sizeof(_abi.Settings), C.sizeof(_abi.Stats),
-            _abi.Settings.gather_radius.offset]
+            _abi.Settings.gather_radius.offset, C.sizeof(_abi.BatchTuning), _abi.BatchTuning.grid_fraction.offset]
     assert got == want
 
 

@@ -1,0 +1,92 @@
+"""Batched views on the GPU: every image of pale_render_views equals the
+single-view path (pale_render_view, itself checked against the oracle) bit
+for bit, whatever the launch tuning (batch size, entry budget and so the
+number of rounds, graph capture, grid size), across graph replays with
+eager work in between; and the first views match the oracle directly.
+"""
+import numpy as np
+import pytest
+
+from differentiablepointrendering_amd import scene
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [(37, 29), (64, 48), (8, 8), (100, 60), (13, 71), (50, 50), (1, 1), (71, 9)]
+SPP = [1, 3, 2, 1, 2, 4, 1, 2]
+
+
+def _views():
+    rng = np.random.default_rng(11)
+    cams = []
+    for i, (w, h) in enumerate(SIZES):
+        d = rng.normal(size=3)
+        eye = 4.0 * d / np.linalg.norm(d)
+        up = (0.0, 1.0, 0.0) if abs(eye[1]) < 3.9 else (1.0, 0.0, 0.0)
+        cams.append(scene.look_at_camera(f"v{i}", eye, (0.0, 0.0, 0.0), up, w, h, 45.0))
+    seeds = [1000 + 17 * i for i in range(len(cams))]
+    return cams, SPP, seeds
+
+
+@pytest.fixture(scope="module")
+def renderer():
+    from differentiablepointrendering_amd.pale import Renderer
+
+    spec = scene.config_scene(0, n=4096, width=16, height=16)
+    print("case: batched views, config-0 scene n=4096", flush=True)
+    r = Renderer.from_spec(spec, dict(photons=20_000))
+    r.build_photon_map()
+    cams, spp, seeds = _views()
+    single = [r.render_view(c, s, sd)["raw"] for c, s, sd in zip(cams, spp, seeds)]
+    return spec, r, single
+
+
+TUNINGS = [
+    dict(),
+    dict(slots_per_batch=3000),                      # many batches, views split across passes
+    dict(slots_per_batch=3000, entries_per_slot=2),  # several rounds per batch
+    dict(graph_rounds=1),
+    dict(slots_per_batch=5000, entries_per_slot=2, graph_rounds=2, grid_fraction=0.5),
+    dict(slots_per_batch=1, graph_rounds=1),         # one segment per batch
+]
+
+
+@pytest.mark.parametrize("tuning", TUNINGS, ids=[str(t) for t in TUNINGS])
+def test_batched_views_equal_single_views(renderer, tuning):
+    _, r, single = renderer
+    cams, spp, seeds = _views()
+    print(f"case: tuning {tuning}", flush=True)
+    r.set_batch_tuning(**tuning)
+    for replay in range(3):
+        out = r.render_views(cams, spp, seeds)
+        for i, (o, s) in enumerate(zip(out, single)):
+            assert np.array_equal(o["raw"].view(np.uint32), s.view(np.uint32)), (tuning, replay, i)
+        # eager work on the same stream between replays
+        r.render_view(cams[1], 2, 5)
+    if tuning.get("slots_per_batch") == 3000:
+        assert r.last_batches > 3
+    r.set_batch_tuning()
+
+
+def test_batched_views_match_oracle(renderer):
+    from oracle.oracle import Oracle
+
+    spec, r, single = renderer
+    orc = Oracle.from_spec(spec, dict(photons=20_000))
+    orc.build_photon_map()
+    cams, spp, seeds = _views()
+    r.set_batch_tuning(slots_per_batch=3000, entries_per_slot=2, graph_rounds=1)
+    out = r.render_views(cams, spp, seeds)
+    r.set_batch_tuning()
+    for i in range(4):
+        want = orc.render_view(cams[i], spp[i], seeds[i])
+        np.testing.assert_allclose(out[i]["raw"], want["raw"], rtol=1e-4, atol=1e-7)
+        np.testing.assert_allclose(out[i]["ldr"], want["ldr"], rtol=1e-4, atol=1e-6)
+
+
+def test_batch_tuning_validation(renderer):
+    _, r, _ = renderer
+    with pytest.raises(RuntimeError):
+        r.set_batch_tuning(grid_fraction=1.5)
+    with pytest.raises(RuntimeError):
+        r.render_views([scene.look_at_camera("v", (0, 0, 4), (0, 0, 0), (0, 1, 0), 8, 8, 45.0)], [0], [1])
+    assert r.render_views([], [], []) == []
