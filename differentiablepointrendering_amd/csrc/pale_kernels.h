@@ -90,7 +90,9 @@ struct PixelArrays {
 // image (seg) and its tile within it.  Segment = one pass of one view; its
 // pixels occupy slots slotBase .. slotBase + W*H - 1 of the pixel arrays.
 struct BatchSeg {
-    uint32_t view, pass, W, H, tilesX, slotBase, seed, pad;
+    uint32_t view, pass, W, H, tilesX, slotBase, seed, pixOffset;  // pixOffset: the view's first pixel
+    float invSpp;                                                   // 1 / the view's spp
+    uint32_t pad[3];
 };
 // The passes of one view in a batch (consecutive segments, pass order): the
 // framebuffer update of its pixels, fb = C * invSpp + fb per pass in order.
@@ -125,10 +127,18 @@ struct EntryArrays {
     uint32_t* sorted;
 };
 
+// Adjoint trace parameters.  Outside batches every ray weighs invSpp and
+// seeds its draws with its queue index.  In a batch (segs != null) a ray
+// carries its segment in T.w, weighs that segment's invSpp, and seeds with
+// its index among its segment's rays (queue index - segFirst[segment]), the
+// index it has when its view is traced alone.
 struct TraceParams {
     uint32_t rrStart;
     float qNull, qReflect, qTransmit;
     uint32_t fullGradients;  // record hit rays for the full gradients
+    float invSpp;
+    const BatchSeg* segs;
+    const uint32_t* segFirst;
 };
 
 // Photon-map reservoir (depositPhotonSurface past capacity): per queue item
@@ -204,6 +214,11 @@ void launchCamCompositeBatch(hipStream_t s, uint32_t maxSlots, const BatchHeader
                              EntryArrays ent);
 void launchBatchResolve(hipStream_t s, BatchDesc b, uint32_t maxResolve, PixelArrays px, float4* fb);
 void launchPostprocess(hipStream_t s, const float4* fb, float4* ldr, uint32_t n);
+// Batched adjoint: primary rays of every slot of the batch (queue index =
+// slot, T.w = segment); first queue index of every segment present.
+void launchAdjRaygenBatch(hipStream_t s, BatchDesc b, uint32_t maxTiles, const float* dloss, RayQueue q,
+                          PendingArrays pend, DevState* st);
+void launchSegFirst(hipStream_t s, const DevState* st, uint32_t maxRays, RayQueue q, uint32_t* segFirst);
 void launchAdjRaygen(hipStream_t s, const DevCamera& cam, const float* dloss, RayQueue q, PendingArrays pend,
                      DevState* st);
 void launchAdjTrace(hipStream_t s, const DevScene& S, DevState* st, uint32_t maxRays, uint32_t bounce, RayQueue in,
@@ -216,8 +231,7 @@ void launchEventSort(hipStream_t s, const GridParams& g, DevState* st, uint32_t 
 // gradFull (16 floats per surfel: position 3, tangent_u 3, tangent_v 3,
 // scale 2, albedo 3, unused, beta) is null unless full gradients are on.
 void launchAdjTransport(hipStream_t s, const DevScene& S, const DevGrid& G, DevState* st, uint32_t poolCap,
-                        EventArrays ev, const uint32_t* sorted, float invSpp, float* gradOpacity, float* gradFull,
-                        bool count);
+                        EventArrays ev, const uint32_t* sorted, float* gradOpacity, float* gradFull, bool count);
 // Split gradFull into the ABI's arrays (any may be null).
 void launchSplitGradients(hipStream_t s, const float* gradFull, uint32_t n, float* position, float* tangentU,
                           float* tangentV, float* scale, float* albedo, float* beta);

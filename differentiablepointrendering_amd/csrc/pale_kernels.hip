@@ -938,6 +938,41 @@ __global__ __launch_bounds__(256) void k_adj_raygen(DevCamera cam, const float* 
     pend.p0[i] = make_float4(0.0f, 0.0f, 0.0f, u2f(kNone));
 }
 
+// Adjoint primary rays of a batch: one thread per tile pixel, the jitter and
+// ray of k_adj_raygen for the pixel of its view; queue index = slot.
+__global__ __launch_bounds__(256) void k_adj_raygen_batch(BatchDesc b, const float* dloss, RayQueue q,
+                                                          PendingArrays pend, DevState* st) {
+    const uint32_t t = blockIdx.x * 4u + (threadIdx.x >> 6);
+    if (blockIdx.x == 0 && threadIdx.x == 0) st->qCount = b.hdr->nSlots;
+    if (t >= b.hdr->nTiles) return;
+    const uint2 td = b.tiles[t];
+    const BatchSeg s = b.segs[td.x];
+    const uint32_t within = threadIdx.x & 63u;
+    const uint32_t pxx = (td.y % s.tilesX) * 8u + (within & 7u);
+    const uint32_t pxy = (td.y / s.tilesX) * 8u + (within >> 3);
+    if (pxx >= s.W || pxy >= s.H) return;
+    const uint32_t pixel = pxy * s.W + pxx, slot = s.slotBase + pixel;
+    Rng rng(perItemSeed(42ull, pixel));
+    const float jx = rng.nextFloat() - 0.5f;
+    const float jy = rng.nextFloat() - 0.5f;
+    F3 o, d;
+    primaryRay(b.cams[s.view], (float)pxx, (float)pxy, jx, jy, o, d);
+    const size_t src = 3 * ((size_t)s.pixOffset + pixel);
+    q.o[slot] = f4(o, u2f(slot));
+    q.d[slot] = f4(d, u2f(0u));
+    q.T[slot] = make_float4(dloss[src], dloss[src + 1], dloss[src + 2], u2f(td.x));
+    pend.p0[slot] = make_float4(0.0f, 0.0f, 0.0f, u2f(kNone));
+}
+
+// First queue index of every segment present in a batch's queue (the queue
+// stays in slot order: raygen writes slot order, compaction is stable).
+__global__ __launch_bounds__(256) void k_seg_first(const DevState* st, RayQueue q, uint32_t* segFirst) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= st->qCount) return;
+    const uint32_t seg = f2u(q.T[i].w);
+    if (i == 0 || f2u(q.T[i - 1].w) != seg) segFirst[seg] = i;
+}
+
 // launchAdjointIntersectKernel (AdjointKernels.cpp:87-386) for one queue.
 template <bool kCount>
 __global__ __launch_bounds__(kTraceBlock) void k_adj_trace(DevScene S, DevState* st, uint32_t bounceIndex,
@@ -952,8 +987,11 @@ __global__ __launch_bounds__(kTraceBlock) void k_adj_trace(DevScene S, DevState*
     Work w{};
     if (active) {
         uint8_t ef = 0, vf = 0;
-        Rng rng(perItemSeed(42ull, i));
         const float4 o4 = in.o[i], d4 = in.d[i], T4 = in.T[i];
+        const uint32_t seg = f2u(T4.w);
+        const uint32_t rank = tp.segFirst ? i - tp.segFirst[seg] : i;
+        const float invS = tp.segs ? tp.segs[seg].invSpp : tp.invSpp;
+        Rng rng(perItemSeed(42ull, rank));
         const F3 o = ld3(o4), d = ld3(d4), T = ld3(T4);
         const uint32_t path = f2u(o4.w);
         const uint32_t bounce = f2u(d4.w);
@@ -991,7 +1029,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_adj_trace(DevScene S, DevState*
                     if (reflectDone) {
                         ev.e1[i] = f4(on, u2f(h.prim));
                         ev.e2[i] = f4(Tp, h.alpha);
-                        ev.e3[i] = make_float4(u2f(S.inst[p2.y].material), pdf, endCos, 0.0f);
+                        ev.e3[i] = make_float4(u2f(S.inst[p2.y].material), pdf, endCos, invS);
                         if (tp.fullGradients) {  // this ray hit the endpoint surfel
                             ev.e4[i] = f4(o, 0.0f);
                             ev.e5[i] = f4(d, 0.0f);
@@ -1000,7 +1038,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_adj_trace(DevScene S, DevState*
                         ev.e1[i] = f4(on, u2f(p2.x));
                         ev.e2[i] = f4(Tp, p1.w);
                         ev.e3[i] = make_float4(u2f(cloud ? kGeoCloud : kGeoMesh), u2f(cloud ? h.prim : I.material),
-                                               endCos, 0.0f);
+                                               endCos, invS);
                         if (tp.fullGradients) {  // the ray that crossed the pending surfel
                             ev.e4[i] = pend.p3[path];
                             ev.e5[i] = pend.p4[path];
@@ -1077,7 +1115,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_adj_trace(DevScene S, DevState*
                     ev.e0[i] = f4(h.pos, u2f(kProjection));
                     ev.e1[i] = f4(on, u2f(h.prim));
                     ev.e2[i] = f4(divs(T, tp.qReflect), h.alpha);
-                    ev.e3[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                    ev.e3[i] = make_float4(0.0f, 0.0f, 0.0f, invS);
                     if (tp.fullGradients) {
                         ev.e4[i] = f4(o, 0.0f);
                         ev.e5[i] = f4(d, 0.0f);
@@ -1087,7 +1125,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_adj_trace(DevScene S, DevState*
                 ef = 1;
                 out.o[i] = f4(no, o4.w);
                 out.d[i] = f4(ndir, u2f(bounce + 1));
-                out.T[i] = f4(nT, 0.0f);
+                out.T[i] = f4(nT, T4.w);
             }
         }
         extFlags[i] = ef;
@@ -1145,9 +1183,10 @@ __device__ void addGeometryGradient(const DevScene& S, uint32_t prim, float4 ro,
 
 // launchAdjointTransportKernel (AdjointKernels.cpp:389-509).
 template <bool kCount>
+// Each event carries the 1/spp of the view whose adjoint ray made it (e3.w).
 __global__ __launch_bounds__(256) void k_adj_transport(DevScene S, DevGrid G, DevState* st, uint32_t poolCap,
-                                                       EventArrays ev, const uint32_t* sorted, float invSpp,
-                                                       float* grad, float* full) {
+                                                       EventArrays ev, const uint32_t* sorted, float* grad,
+                                                       float* full) {
     const uint32_t n = min(st->evPool, poolCap);
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const bool active = i < n;
@@ -1157,6 +1196,7 @@ __global__ __launch_bounds__(256) void k_adj_transport(DevScene S, DevGrid G, De
         const float4 e0 = ev.e0[j], e1 = ev.e1[j], e2 = ev.e2[j], e3 = ev.e3[j];
         const uint32_t kind = f2u(e0.w);
         const uint32_t target = f2u(e1.w);
+        const float invSpp = e3.w;
         const F3 T = ld3(e2);
         const float a = e2.w;
         const F3 E = gatherIrradiance<kCount>(G, ld3(e0), ld3(e1), w);
@@ -1457,6 +1497,13 @@ void launchCamComposite(hipStream_t s, uint32_t npx, PixelArrays px, EntryArrays
 void launchPostprocess(hipStream_t s, const float4* fb, float4* ldr, uint32_t n) {
     if (n) k_post<<<blocks(n, 256), 256, 0, s>>>(fb, ldr, n);
 }
+void launchAdjRaygenBatch(hipStream_t s, BatchDesc b, uint32_t maxTiles, const float* dloss, RayQueue q,
+                          PendingArrays pend, DevState* st) {
+    k_adj_raygen_batch<<<blocks(max(maxTiles, 1u), 4), 256, 0, s>>>(b, dloss, q, pend, st);
+}
+void launchSegFirst(hipStream_t s, const DevState* st, uint32_t maxRays, RayQueue q, uint32_t* segFirst) {
+    if (maxRays) k_seg_first<<<blocks(maxRays, 256), 256, 0, s>>>(st, q, segFirst);
+}
 void launchAdjRaygen(hipStream_t s, const DevCamera& cam, const float* dloss, RayQueue q, PendingArrays pend,
                      DevState* st) {
     const uint32_t npx = cam.W * cam.H;
@@ -1483,14 +1530,13 @@ void launchEventSort(hipStream_t s, const GridParams& g, DevState* st, uint32_t 
     k_event_scatter<<<blocks(poolCap, 256), 256, 0, s>>>(st, poolCap, cell, cellStart, cellCount, sorted);
 }
 void launchAdjTransport(hipStream_t s, const DevScene& S, const DevGrid& G, DevState* st, uint32_t poolCap,
-                        EventArrays ev, const uint32_t* sorted, float invSpp, float* gradOpacity, float* gradFull,
-                        bool count) {
+                        EventArrays ev, const uint32_t* sorted, float* gradOpacity, float* gradFull, bool count) {
     if (!poolCap) return;
     if (count)
-        k_adj_transport<true><<<blocks(poolCap, 256), 256, 0, s>>>(S, G, st, poolCap, ev, sorted, invSpp, gradOpacity,
+        k_adj_transport<true><<<blocks(poolCap, 256), 256, 0, s>>>(S, G, st, poolCap, ev, sorted, gradOpacity,
                                                                   gradFull);
     else
-        k_adj_transport<false><<<blocks(poolCap, 256), 256, 0, s>>>(S, G, st, poolCap, ev, sorted, invSpp,
+        k_adj_transport<false><<<blocks(poolCap, 256), 256, 0, s>>>(S, G, st, poolCap, ev, sorted,
                                                                    gradOpacity, gradFull);
 }
 void launchSplitGradients(hipStream_t s, const float* gradFull, uint32_t n, float* position, float* tangentU,

@@ -131,7 +131,7 @@ struct pale_handle {
     DevMem viewFb, viewLdr;  // images of pale_render_view
     // Batched views (pale_render_views): descriptors of the current batch,
     // the framebuffers of all views, the captured graph of a batch's rounds.
-    DevMem bTiles, bSegs, bRuns, bHdr, bCams, viewsFb, viewsLdr;
+    DevMem bTiles, bSegs, bRuns, bHdr, bCams, bSegFirst, viewsFb, viewsLdr;
     pale_batch_tuning tune{};
     hipGraphExec_t bGraph = nullptr;
     std::vector<uint8_t> bGraphKey;
@@ -209,8 +209,9 @@ struct pale_handle {
                            full ? m[4].as<float4>() : nullptr, full ? m[5].as<float4>() : nullptr};
     }
 
-    TraceParams tp() const {
-        return TraceParams{S.rr_start, S.q_null, S.q_reflect, S.q_transmit, S.full_gradients ? 1u : 0u};
+    TraceParams tp(float invSpp = 1.0f, const BatchSeg* segs = nullptr, const uint32_t* segFirst = nullptr) const {
+        return TraceParams{S.rr_start, S.q_null, S.q_reflect, S.q_transmit, S.full_gradients ? 1u : 0u,
+                           invSpp, segs, segFirst};
     }
 
     DevGrid devGrid() const {
@@ -916,22 +917,11 @@ struct pale_handle {
     }
 
     // submitAdjointKernel (SyclBridge.cpp:169-294).
-    void renderBackward(const std::vector<uint32_t>& camIdx, const std::vector<const float*>& src,
-                        bool srcOnDevice = false) {
-        std::vector<const HostCamera*> cp;
-        for (uint32_t c : camIdx) cp.push_back(&cams[c]);
-        renderBackwardCams(cp, src, srcOnDevice, S.adjoint_passes);
-    }
-
-    void renderBackwardCams(const std::vector<const HostCamera*>& camList, const std::vector<const float*>& src,
-                            bool srcOnDevice, uint32_t samples) {
+    // Adjoint prologue: keep the forward's deposit count (the gather
+    // normalisation T) and reset only the adjoint counters; dL/dI of every
+    // view to the device (off = float offsets); zero gradients; start timing.
+    void beginBackward(const std::vector<size_t>& off, const std::vector<const float*>& src, bool srcOnDevice) {
         const uint32_t N = (uint32_t)bp.points.size();
-        uint32_t maxNpx = 1;
-        for (const HostCamera* c : camList) maxNpx = std::max(maxNpx, c->W * c->H);
-        ensureAdjoint(maxNpx);
-        const DevGrid G = devGrid();
-        // Keep the forward's deposit count (the gather normalisation T) and
-        // reset only the adjoint counters.
         DevState h{};
         HIPCHECK(hipMemcpyAsync(&h, state.p, sizeof(DevState), hipMemcpyDeviceToHost, stream));
         HIPCHECK(hipStreamSynchronize(stream));
@@ -941,11 +931,8 @@ struct pale_handle {
         h.evPool = 0;
         HIPCHECK(hipMemcpyAsync(state.p, &h, sizeof(DevState), hipMemcpyHostToDevice, stream));
         HIPCHECK(hipStreamSynchronize(stream));
-        // dL/dI of every camera goes to the device before the timed region.
-        std::vector<size_t> off(camList.size() + 1, 0);
-        for (size_t k = 0; k < camList.size(); ++k) off[k + 1] = off[k] + (size_t)camList[k]->W * camList[k]->H * 3;
         dloss.reserve(std::max<size_t>(off.back(), 1) * sizeof(float));
-        for (size_t k = 0; k < camList.size(); ++k)
+        for (size_t k = 0; k + 1 < off.size(); ++k)
             HIPCHECK(hipMemcpyAsync(dloss.as<float>() + off[k], src[k], (off[k + 1] - off[k]) * sizeof(float),
                                     srcOnDevice ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, stream));
         launchFillF(stream, gradOpacity.as<float>(), N, 0.0f);
@@ -956,6 +943,148 @@ struct pale_handle {
         HIPCHECK(hipStreamSynchronize(stream));
         tReset();
         HIPCHECK(hipEventRecord(ev0, stream));
+    }
+
+    // Transport of the pooled events in grid-cell order (one launch).
+    void flushEvents() {
+        const size_t tt = tBegin(kTTransport);
+        DevState* st = this->st();
+        launchEventSort(stream, grid, st, poolCap, ea(evCmp), evCell.as<uint32_t>(), evSorted.as<uint32_t>(),
+                        cellCount.as<uint32_t>(), entStart.as<uint32_t>(), gridTiles.as<uint32_t>(),
+                        gridTotal.as<uint32_t>());
+        launchAdjTransport(stream, scene, devGrid(), st, poolCap, ea(evCmp), evSorted.as<uint32_t>(),
+                           gradOpacity.as<float>(), S.full_gradients ? gradFull.as<float>() : nullptr, countWork);
+        tEnd(tt);
+        launchFill(stream, stField(offsetof(DevState, evPool)), 1, 0u);
+    }
+
+    void finishBackward() {
+        HIPCHECK(hipEventRecord(ev1, stream));
+        HIPCHECK(hipStreamSynchronize(stream));
+        HIPCHECK(hipGetLastError());
+        float ms = 0.0f;
+        HIPCHECK(hipEventElapsedTime(&ms, ev0, ev1));
+        stats.backward_ms = ms;
+        readStats(false);
+    }
+
+    // Adjoint of several views (gradients summed): (view, sample) segments in
+    // sample-major order cut into batches like the forward's; each batch is
+    // one adjoint wavefront over all its rays.  A ray seeds its draws with
+    // its index among its segment's rays and its events weigh its view's
+    // 1/spp, so each view contributes what renderBackwardCams gives it.
+    void renderViewsBackward(const std::vector<HostCamera>& vc, const std::vector<uint32_t>& spp,
+                             const std::vector<const float*>& src, bool srcOnDevice) {
+        if (!mapValid) throw StatusError(PALE_ERR_INVALID_ARG, "render_views_backward: no photon map");
+        const uint32_t n = (uint32_t)vc.size();
+        std::vector<size_t> pixOff(n + 1, 0), off(n + 1, 0);
+        uint32_t maxSpp = 0;
+        for (uint32_t v = 0; v < n; ++v) {
+            if (spp[v] == 0) throw StatusError(PALE_ERR_INVALID_ARG, "render_views_backward: spp must be >= 1");
+            pixOff[v + 1] = pixOff[v] + (size_t)vc[v].W * vc[v].H;
+            off[v + 1] = pixOff[v + 1] * 3;
+            maxSpp = std::max(maxSpp, spp[v]);
+        }
+        if (pixOff[n] > 0x7FFFFFFFull) throw StatusError(PALE_ERR_INVALID_ARG, "render_views_backward: too many pixels");
+        const uint64_t budget = tune.slots_per_batch ? tune.slots_per_batch : (4u << 20);
+        std::vector<HostBatch> batches;
+        uint64_t cur = budget + 1;
+        for (uint32_t p = 0; p < maxSpp; ++p) {
+            for (uint32_t v = 0; v < n; ++v) {
+                if (spp[v] <= p) continue;
+                const HostCamera& c = vc[v];
+                const uint32_t npx = c.W * c.H;
+                if (cur > 0 && cur + npx > budget) {
+                    batches.emplace_back();
+                    cur = 0;
+                }
+                HostBatch& b = batches.back();
+                const uint32_t tx = (c.W + 7u) / 8u, ty = (c.H + 7u) / 8u;
+                const uint32_t si = (uint32_t)b.segs.size();
+                BatchSeg sg{v, p, c.W, c.H, tx, (uint32_t)cur, 42u, (uint32_t)pixOff[v], 1.0f / (float)spp[v], {0, 0, 0}};
+                b.segs.push_back(sg);
+                for (uint32_t t = 0; t < tx * ty; ++t) b.tiles.push_back(make_uint2(si, t));
+                cur += npx;
+            }
+        }
+        uint32_t maxTiles = 0, maxSlots = 1;
+        size_t maxSegs = 1;
+        for (HostBatch& b : batches) {
+            uint32_t slots = 0;
+            for (const BatchSeg& s : b.segs) slots += s.W * s.H;
+            b.hdr = BatchHeader{(uint32_t)b.tiles.size(), slots, (uint32_t)b.segs.size(), 0u, 0u, {0, 0, 0}};
+            maxTiles = std::max(maxTiles, b.hdr.nTiles);
+            maxSlots = std::max(maxSlots, slots);
+            maxSegs = std::max(maxSegs, b.segs.size());
+        }
+        ensureAdjoint(maxSlots);
+        bTiles.reserve((size_t)maxTiles * sizeof(uint2));
+        bSegs.reserve(maxSegs * sizeof(BatchSeg));
+        bRuns.reserve(sizeof(BatchRun));
+        bHdr.reserve(sizeof(BatchHeader));
+        bSegFirst.reserve(maxSegs * sizeof(uint32_t));
+        bCams.reserve(std::max<size_t>(n, 1) * sizeof(DevCamera));
+        std::vector<DevCamera> dcs(n);
+        for (uint32_t v = 0; v < n; ++v) dcs[v] = devCam(vc[v]);
+        upload(stream, bCams, dcs);
+        beginBackward(off, src, srcOnDevice);
+        DevState* st = this->st();
+        uint32_t* qCount = stField(offsetof(DevState, qCount));
+        uint32_t* extTotal = stField(offsetof(DevState, extTotal));
+        const BatchDesc bd = batchDesc();
+        const TraceParams tpb = tp(1.0f, bSegs.as<BatchSeg>(), bSegFirst.as<uint32_t>());
+        const PendingArrays pend{pend0.as<float4>(), pend1.as<float4>(), pend2.as<uint2>(), pend3.as<float4>(),
+                                 pend4.as<float4>()};
+        uint64_t pooledBound = 0;
+        for (HostBatch& b : batches) {
+            // Events carry their weight (1/spp) and surfel, not their
+            // segment: the pool spans batches.
+            upload(stream, bTiles, b.tiles);
+            upload(stream, bSegs, b.segs);
+            HIPCHECK(hipMemcpyAsync(bHdr.p, &b.hdr, sizeof(BatchHeader), hipMemcpyHostToDevice, stream));
+            const uint32_t nr = b.hdr.nSlots;
+            DevMem* in = qA;
+            DevMem* out = qB;
+            launchAdjRaygenBatch(stream, bd, b.hdr.nTiles, dloss.as<float>(), rq(in), pend, st);
+            for (uint32_t k = 0; k < S.adjoint_bounces; ++k) {
+                if (pooledBound + nr > poolCap) {
+                    flushEvents();
+                    pooledBound = 0;
+                }
+                launchSegFirst(stream, st, nr, rq(in), bSegFirst.as<uint32_t>());
+                const size_t ta = tBegin(kTAdjTrace);
+                launchAdjTrace(stream, scene, st, nr, k, rq(in), rq(stg), extFlags.as<uint8_t>(), ea(evStg),
+                               evFlags.as<uint8_t>(), pend, tpb, countWork);
+                tEnd(ta);
+                launchCompactEvents(stream, evFlags.as<uint8_t>(), qCount, nr, tiles.as<uint32_t>(), st, ea(evStg),
+                                    ea(evCmp), poolCap);
+                pooledBound += nr;
+                launchCompactRays(stream, extFlags.as<uint8_t>(), qCount, nr, tiles.as<uint32_t>(), extTotal,
+                                  rq(stg), rq(out), queueCap);
+                launchSetQueue(stream, st, extTotal);
+                std::swap(in, out);
+            }
+        }
+        if (pooledBound) flushEvents();
+        batchCount = (uint32_t)batches.size();
+        finishBackward();
+    }
+
+    void renderBackward(const std::vector<uint32_t>& camIdx, const std::vector<const float*>& src,
+                        bool srcOnDevice = false) {
+        std::vector<const HostCamera*> cp;
+        for (uint32_t c : camIdx) cp.push_back(&cams[c]);
+        renderBackwardCams(cp, src, srcOnDevice, S.adjoint_passes);
+    }
+
+    void renderBackwardCams(const std::vector<const HostCamera*>& camList, const std::vector<const float*>& src,
+                            bool srcOnDevice, uint32_t samples) {
+        uint32_t maxNpx = 1;
+        for (const HostCamera* c : camList) maxNpx = std::max(maxNpx, c->W * c->H);
+        ensureAdjoint(maxNpx);
+        std::vector<size_t> off(camList.size() + 1, 0);
+        for (size_t k = 0; k < camList.size(); ++k) off[k + 1] = off[k] + (size_t)camList[k]->W * camList[k]->H * 3;
+        beginBackward(off, src, srcOnDevice);
         DevState* st = this->st();
         uint32_t* qCount = stField(offsetof(DevState, qCount));
         uint32_t* extTotal = stField(offsetof(DevState, extTotal));
@@ -967,15 +1096,7 @@ struct pale_handle {
         uint64_t pooledBound = 0;
         auto flush = [&]() {
             if (!pooledBound) return;
-            const size_t tt = tBegin(kTTransport);
-            launchEventSort(stream, grid, st, poolCap, ea(evCmp), evCell.as<uint32_t>(), evSorted.as<uint32_t>(),
-                            cellCount.as<uint32_t>(), entStart.as<uint32_t>(), gridTiles.as<uint32_t>(),
-                            gridTotal.as<uint32_t>());
-            launchAdjTransport(stream, scene, G, st, poolCap, ea(evCmp), evSorted.as<uint32_t>(), invSpp,
-                               gradOpacity.as<float>(), S.full_gradients ? gradFull.as<float>() : nullptr,
-                               countWork);
-            tEnd(tt);
-            launchFill(stream, stField(offsetof(DevState, evPool)), 1, 0u);
+            flushEvents();
             pooledBound = 0;
         };
         for (size_t k = 0; k < camList.size(); ++k) {
@@ -992,7 +1113,7 @@ struct pale_handle {
                     if (pooledBound + npx > poolCap) flush();
                     const size_t ta = tBegin(kTAdjTrace);
                     launchAdjTrace(stream, scene, st, npx, b, rq(in), rq(stg), extFlags.as<uint8_t>(), ea(evStg),
-                                   evFlags.as<uint8_t>(), pend, tp(), countWork);
+                                   evFlags.as<uint8_t>(), pend, tp(invSpp), countWork);
                     tEnd(ta);
                     launchCompactEvents(stream, evFlags.as<uint8_t>(), qCount, npx, tiles.as<uint32_t>(), st,
                                         ea(evStg), ea(evCmp), poolCap);
@@ -1005,13 +1126,7 @@ struct pale_handle {
             }
         }
         flush();
-        HIPCHECK(hipEventRecord(ev1, stream));
-        HIPCHECK(hipStreamSynchronize(stream));
-        HIPCHECK(hipGetLastError());
-        float ms = 0.0f;
-        HIPCHECK(hipEventElapsedTime(&ms, ev0, ev1));
-        stats.backward_ms = ms;
-        readStats(false);
+        finishBackward();
     }
 };
 
@@ -1381,6 +1496,23 @@ int pale_render_views(pale_handle* h, uint32_t n, const pale_camera_desc* camera
             if (ldr_out && ldr_out[i])
                 HIPCHECK(hipMemcpy(ldr_out[i], h->viewsLdr.as<float4>() + off, bytes, hipMemcpyDeviceToHost));
         }
+        if (batches_out) *batches_out = h->batchCount;
+    });
+}
+
+int pale_render_views_backward(pale_handle* h, uint32_t n, const pale_camera_desc* cameras, const uint32_t* spp,
+                               const float* const* dloss_rgb, float* const* grads, uint32_t* batches_out) {
+    if (!h || (n && (!cameras || !spp || !dloss_rgb))) return fail(PALE_ERR_INVALID_ARG, "null argument");
+    return guarded([&] {
+        std::vector<HostCamera> vc;
+        std::vector<const float*> src;
+        for (uint32_t i = 0; i < n; ++i) {
+            vc.push_back(hostCamera(cameras[i]));
+            if (!dloss_rgb[i]) throw StatusError(PALE_ERR_INVALID_ARG, "null dL/dI image");
+            src.push_back(dloss_rgb[i]);
+        }
+        h->renderViewsBackward(vc, std::vector<uint32_t>(spp, spp + n), src, false);
+        copyGradsHost(h, grads);
         if (batches_out) *batches_out = h->batchCount;
     });
 }

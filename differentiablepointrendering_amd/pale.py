@@ -290,6 +290,32 @@ class Renderer:
         self.last_batches = nb.value
         return [{"ldr": a, "raw": b} for a, b in zip(ldr, raw)]
 
+    def render_views_backward(self, cameras, spp, dloss) -> dict:
+        """Adjoint of several views (dL/dI (H,W,3) each) over the current
+        photon map; the gradient dict of render_backward, summed over views."""
+        n = len(cameras)
+        if len(spp) != n or len(dloss) != n:
+            raise RuntimeError("render_views_backward: cameras, spp and dloss must have the same length")
+        descs = (_abi.CameraDesc * max(1, n))()
+        arrs = []
+        for i, c in enumerate(cameras):
+            _abi.fill_camera_desc(descs[i], c)
+            a = np.ascontiguousarray(np.asarray(dloss[i])[..., :3], dtype=np.float32)
+            if a.shape != (c.height, c.width, 3):
+                raise RuntimeError(f"render_views_backward: dL/dI {i} must be ({c.height},{c.width},3), got {a.shape}")
+            arrs.append(a)
+        sp = np.ascontiguousarray(spp, dtype=np.uint32)
+        pts = self.point_count
+        grads = {k: np.empty((pts, w) if w else (pts,), np.float32) for k, w in _abi.POINT_SHAPES.items()}
+        gp = (_fp * 8)(*[_p(grads[k]) for k in _abi.POINT_KEYS])
+        ap = (_fp * max(1, n))(*[_p(a) for a in arrs])
+        nb = C.c_uint32()
+        check(self._L.pale_render_views_backward(self._h, n, descs, sp.ctypes.data_as(C.POINTER(C.c_uint32)), ap,
+                                                 gp, C.byref(nb)))
+        self.last_batches = nb.value
+        grads["color"] = grads["albedo"]
+        return grads
+
     def render_view_device(self, camera: _abi.Camera, spp: int, seed: int) -> dict:
         """render_view into torch tensors on the renderer's GPU."""
         import torch

@@ -345,6 +345,16 @@ int pale_render_views(pale_handle* h, uint32_t n, const pale_camera_desc* camera
                       float* const* ldr_out, float* const* raw_out,
                       uint32_t* batches_out);
 
+/* Adjoint of n views over the current photon map, gradients summed over the
+ * views (arrays as pale_render_backward).  Each view contributes what
+ * pale_render_view_backward(cameras[i], spp[i], dloss_rgb[i]) gives (the
+ * same rays and events; float atomics add in another order). */
+int pale_render_views_backward(pale_handle* h, uint32_t n,
+                               const pale_camera_desc* cameras,
+                               const uint32_t* spp,
+                               const float* const* dloss_rgb,
+                               float* const* grads, uint32_t* batches_out);
+
 /* ---- diagnostics: no reference counterpart; used by the parity tests ---- */
 
 /* Closest-hit query (the reference's intersectScene in FirstHit mode,
