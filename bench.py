@@ -75,6 +75,9 @@ def parse():
     ap.add_argument("--served-rate", type=float, default=1.0, help="Poisson arrival rate, requests per second")
     ap.add_argument("--served-replicas", type=int, default=2)
     ap.add_argument("--served-only", action="store_true", help="run only the served form")
+    ap.add_argument("--batched", type=int, default=32, help="views of the batched form (0: skip)")
+    ap.add_argument("--batched-only", action="store_true", help="run only the batched form")
+    ap.add_argument("--no-tune", action="store_true", help="batched form with the default tuning")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only an elementwise kernel reading 1 GiB and writing 1 GiB: the known-size kernel "
                          "that scales rocprof FETCH_SIZE / WRITE_SIZE to bytes")
@@ -142,6 +145,122 @@ def served_requests(count: int, rate: float, seed: int = 1):
         cam = scene.look_at_camera(f"r{i}", eye, (0.0, 0.0, 0.0), up, w, h, 45.0)
         out.append((t, sc, cam, spp, adjoint))
     return out
+
+
+def batched_cameras(count: int, seed: int = 2):
+    """Views of the batched form: eyes uniform on the sphere of radius 4
+    looking at the origin, area log-uniform between 128x96 and 1024x768
+    (aspect 4:3), spp uniform in {4, 8, 16}."""
+    from differentiablepointrendering_amd import scene
+
+    rng = np.random.default_rng(seed)
+    lo, hi = math.log(128 * 96), math.log(1024 * 768)
+    cams, spp = [], []
+    for i in range(count):
+        area = math.exp(rng.uniform(lo, hi))
+        w = max(8, int(round(math.sqrt(area * 4.0 / 3.0))))
+        h = max(6, int(round(w * 0.75)))
+        d = rng.normal(size=3)
+        eye = 4.0 * d / np.linalg.norm(d)
+        up = (0.0, 1.0, 0.0) if abs(eye[1]) < 3.9 else (1.0, 0.0, 0.0)
+        cams.append(scene.look_at_camera(f"v{i}", eye, (0.0, 0.0, 0.0), up, w, h, 45.0))
+        spp.append(int(rng.choice([4, 8, 16])))
+    return cams, spp
+
+
+def batched_leg(a):
+    from differentiablepointrendering_amd import autotune, scene
+    from differentiablepointrendering_amd.pale import Renderer
+
+    t0 = time.time()
+    n = 500_000
+    pts, tgt = scene.clustered_surfels(n, seed=0)
+    settings = dict(photons=2_000_000, bounces=6, forward_passes=1, gather_passes=16, adjoint_bounces=6,
+                    adjoint_passes=16, seed=0, gather_radius=0.02,
+                    full_gradients=1 if a.gradients == "full" else 0)
+    cams, spp = batched_cameras(a.batched)
+    seeds = [100 + i for i in range(len(cams))]
+    slots = sum(c.width * c.height * s for c, s in zip(cams, spp))
+    meshes = [scene.light_quad()]
+    t = Renderer.from_scene(meshes, [], tgt, settings)
+    t.build_photon_map()
+    targets = [x["raw"][..., :3].copy() for x in t.render_views(cams, spp, seeds)]
+    t.close()
+    r = Renderer.from_scene(meshes, [], pts, settings)
+    r.build_photon_map()
+    print(f"[bench] batched: {len(cams)} views, {slots / 1e6:.1f} M pixel samples, setup {time.time() - t0:.1f}s",
+          file=sys.stderr, flush=True)
+
+    def dloss(raw, tg):
+        d = raw[..., :3] - tg
+        return (d / d.size).astype(np.float32)
+
+    keys = ("opacity", "position", "scale", "albedo", "beta") if a.gradients == "full" else ("opacity",)
+    # Plain: one view after another through the single-view path.
+    tw = time.time()
+    p_ms, p_q, p_img, p_grad = 0.0, 0, [], {k: 0.0 for k in keys}
+    for c, s, sd, tg in zip(cams, spp, seeds, targets):
+        img = r.render_view(c, s, sd)["raw"]
+        st = r.stats()
+        p_ms += st["forward_ms"]
+        p_q += st["gather_queries"]
+        g = r.render_view_backward(c, s, dloss(img, tg))
+        st = r.stats()
+        p_ms += st["backward_ms"]
+        p_q += st["adjoint_queries"]
+        p_img.append(img)
+        for k in keys:
+            p_grad[k] = p_grad[k] + g[k]
+    plain_wall = time.time() - tw
+    # Autotuner on a sample: the first 8 views at up to 4 spp.
+    tune = None
+    if not a.no_tune:
+        ns = min(8, len(cams))
+        sc, ss = cams[:ns], [min(4, s) for s in spp[:ns]]
+        sd = [dloss(np.zeros((c.height, c.width, 4), np.float32), tg) for c, tg in zip(sc, targets[:ns])]
+        tune = autotune.tune(r, sc, ss, seeds[:ns], sd, total_slots=slots)
+        print(f"[bench] batched tuning {tune['tuning']} ({tune['seconds']}s)", file=sys.stderr, flush=True)
+    else:
+        r.set_batch_tuning(**autotune.DEFAULT)
+    # Batched: all views in one call each way (one warm-up first).
+    r.render_views(cams[:2], spp[:2], seeds[:2])
+    tw = time.time()
+    out = r.render_views(cams, spp, seeds)
+    st = r.stats()
+    b_ms, b_q, batches = st["forward_ms"], st["gather_queries"], r.last_batches
+    g = r.render_views_backward(cams, spp, [dloss(o["raw"], tg) for o, tg in zip(out, targets)])
+    st = r.stats()
+    b_ms += st["backward_ms"]
+    b_q += st["adjoint_queries"]
+    batched_wall = time.time() - tw
+    same_images = all(np.array_equal(o["raw"].view(np.uint32), p.view(np.uint32)) for o, p in zip(out, p_img))
+    grad_err = {}
+    for k in keys:
+        scale = max(float(np.abs(p_grad[k]).max()), 1e-30)
+        grad_err[k] = float(np.abs(g[k] - p_grad[k]).max() / scale)
+    r.close()
+    res = {
+        "workload": f"{len(cams)} views (eyes on the radius-4 sphere, 128x96..1024x768 log-uniform area, spp "
+                    f"{{4,8,16}}) of {n} clustered surfels, 2M photons, depth 6, forward + adjoint ({a.gradients} "
+                    f"gradients, summed over views)",
+        "pixel_samples": slots,
+        "mrays_per_s": round(b_q / (b_ms * 1e-3) / 1e6, 2),
+        "ms": round(b_ms, 2),
+        "batches": batches,
+        "plain_one_view_at_a_time": {"mrays_per_s": round(p_q / (p_ms * 1e-3) / 1e6, 2), "ms": round(p_ms, 2)},
+        "speedup_vs_plain": round((p_ms / b_ms) * (b_q / max(p_q, 1)), 3),
+        "wall_s": {"batched": round(batched_wall, 2), "plain": round(plain_wall, 2)},
+        "same_queries": b_q == p_q,
+        "same_images_as_plain": bool(same_images),
+        "gradient_max_err_over_max": grad_err,
+        "tuning": tune["tuning"] if tune else autotune.DEFAULT,
+        "autotune": {"sample": "first 8 views at up to 4 spp", "seconds": tune["seconds"], "table": tune["table"]}
+        if tune else None,
+    }
+    print(f"[bench] batched {res['mrays_per_s']} Mrays/s vs one view at a time "
+          f"{res['plain_one_view_at_a_time']['mrays_per_s']}; same images {same_images}; grad err {grad_err}",
+          file=sys.stderr, flush=True)
+    return res
 
 
 def served_leg(a):
@@ -281,6 +400,9 @@ def main():
         return 2
     if a.served_only:
         print(json.dumps({"served": served_leg(a)}))
+        return 0
+    if a.batched_only:
+        print(json.dumps({"batched": batched_leg(a)}))
         return 0
 
     from differentiablepointrendering_amd import scene
@@ -438,6 +560,8 @@ def main():
         out["kernels"] = kernels
     r.close()
     del r
+    if a.batched > 0:
+        out["batched"] = batched_leg(a)
     if a.served > 0:
         out["served"] = served_leg(a)
     print(json.dumps(out))
