@@ -78,6 +78,11 @@ def parse():
     ap.add_argument("--batched", type=int, default=32, help="views of the batched form (0: skip)")
     ap.add_argument("--batched-only", action="store_true", help="run only the batched form")
     ap.add_argument("--no-tune", action="store_true", help="batched form with the default tuning")
+    ap.add_argument("--batch-passes", type=int, choices=(0, 1), default=1,
+                    help="plain form: 1 = all gather passes / adjoint samples of the view as batches")
+    ap.add_argument("--tuning", type=str, default="12582912,24,0,1",
+                    help="plain form batch tuning as slots,entries,graph_rounds,grid_fraction (the default holds "
+                         "all 16 passes of 800x800 in one batch, eager launches so each stage is timed)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only an elementwise kernel reading 1 GiB and writing 1 GiB: the known-size kernel "
                          "that scales rocprof FETCH_SIZE / WRITE_SIZE to bytes")
@@ -417,6 +422,7 @@ def main():
         settings["gather_passes"] = settings["adjoint_passes"] = a.spp
     settings["spp_decorrelate"] = 1 if a.decorrelate else 0
     settings["full_gradients"] = 1 if a.gradients == "full" else 0
+    settings["batch_passes"] = a.batch_passes
     cam = spec.cameras[0]
     W, H = cam.width, cam.height
     n = len(spec.points["position"])
@@ -430,7 +436,11 @@ def main():
     del tgt
     t_build = time.time()
     r = Renderer.from_spec(spec, settings)
+    if a.tuning:
+        sl, en, gr, fr = a.tuning.split(",")
+        r.set_batch_tuning(int(sl), int(en), int(gr), float(fr))
     build_s = time.time() - t_build
+    r_tuning = r.batch_tuning()
     print(f"[bench] renderer ready (host BVH build {build_s:.2f}s)", file=sys.stderr, flush=True)
 
     def step():
@@ -541,6 +551,8 @@ def main():
             "surfels": n, "width": W, "height": H, "spp": settings["gather_passes"],
             "photons": settings["photons"], "depth": settings["bounces"],
             "gather_passes_share_seed": not a.decorrelate,
+            "batch_passes": a.batch_passes,
+            "batch_tuning": r_tuning,
             "gradients": a.gradients,
         },
         "queries_per_step": int(q_step),

@@ -107,6 +107,7 @@ class Settings(C.Structure):
         ("gather_entry_budget", C.c_uint32),
         ("photon_map_capacity", C.c_uint32),
         ("full_gradients", C.c_int32),
+        ("batch_passes", C.c_int32),
     ]
 
 
@@ -167,6 +168,7 @@ DEFAULT_SETTINGS = dict(
     gather_entry_budget=0,
     photon_map_capacity=0,
     full_gradients=0,
+    batch_passes=1,
 )
 
 
@@ -182,7 +184,7 @@ def make_settings(d: Optional[dict] = None) -> Settings:
     for k, v in vals.items():
         if k in ("gather_radius", "q_null", "q_reflect", "q_transmit"):
             setattr(s, k, float(v))
-        elif k in ("spp_decorrelate", "debug_images", "full_gradients"):
+        elif k in ("spp_decorrelate", "debug_images", "full_gradients", "batch_passes"):
             setattr(s, k, int(bool(v)))
         else:
             setattr(s, k, int(v))

@@ -209,7 +209,8 @@ void launchCamComposite(hipStream_t s, uint32_t npx, PixelArrays px, EntryArrays
 // framebuffer update; the pass-ordered framebuffer update per view.
 void launchBatchInit(hipStream_t s, BatchDesc b, uint32_t maxTiles, PixelArrays px);
 void launchCamTraceBatch(hipStream_t s, const DevScene& S, DevState* st, BatchDesc b, uint32_t maxSlots,
-                         PixelArrays px, EntryArrays ent, uint32_t cap, uint32_t* cursor, float gridFraction);
+                         PixelArrays px, EntryArrays ent, uint32_t cap, uint32_t* cursor, float gridFraction,
+                         bool count);
 void launchCamCompositeBatch(hipStream_t s, uint32_t maxSlots, const BatchHeader* hdr, PixelArrays px,
                              EntryArrays ent);
 void launchBatchResolve(hipStream_t s, BatchDesc b, uint32_t maxResolve, PixelArrays px, float4* fb);

@@ -1453,13 +1453,20 @@ void launchBatchInit(hipStream_t s, BatchDesc b, uint32_t maxTiles, PixelArrays 
 }
 
 void launchCamTraceBatch(hipStream_t s, const DevScene& S, DevState* st, BatchDesc b, uint32_t maxSlots,
-                         PixelArrays px, EntryArrays ent, uint32_t cap, uint32_t* cursor, float gridFraction) {
+                         PixelArrays px, EntryArrays ent, uint32_t cap, uint32_t* cursor, float gridFraction,
+                         bool count) {
     if (!maxSlots) return;
     const size_t lds = stackBytes(S);
     const BatchMap map{b.tiles, b.segs, b.hdr};
-    const uint32_t full = camTraceGrid<false, BatchMap>(lds);
-    const uint32_t g = min(max(1u, (uint32_t)((float)full * gridFraction)), blocks(maxSlots, kTraceBlock));
-    k_cam_trace<false><<<g, kTraceBlock, lds, s>>>(S, st, map, px, ent, cap, cursor);
+    if (count) {
+        const uint32_t full = camTraceGrid<true, BatchMap>(lds);
+        const uint32_t g = min(max(1u, (uint32_t)((float)full * gridFraction)), blocks(maxSlots, kTraceBlock));
+        k_cam_trace<true><<<g, kTraceBlock, lds, s>>>(S, st, map, px, ent, cap, cursor);
+    } else {
+        const uint32_t full = camTraceGrid<false, BatchMap>(lds);
+        const uint32_t g = min(max(1u, (uint32_t)((float)full * gridFraction)), blocks(maxSlots, kTraceBlock));
+        k_cam_trace<false><<<g, kTraceBlock, lds, s>>>(S, st, map, px, ent, cap, cursor);
+    }
 }
 
 void launchCamCompositeBatch(hipStream_t s, uint32_t maxSlots, const BatchHeader* hdr, PixelArrays px,

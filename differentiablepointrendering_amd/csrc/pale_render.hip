@@ -582,10 +582,40 @@ struct pale_handle {
         tReset();
         HIPCHECK(hipEventRecord(ev0, stream));
         tracePhotons(tlen);
+        camRounds = 0;
+        if (S.batch_passes && S.gather_passes > 0 && !camIdx.empty()) {
+            // Every camera draws one seed from the table where the photon
+            // stage left the cursor (k_seed_next); its passes use it
+            // (+ pass when decorrelated).  All passes of all cameras run as
+            // batches; each image equals the pass-by-pass one.
+            DevState hs{};
+            HIPCHECK(hipMemcpyAsync(&hs, state.p, sizeof(DevState), hipMemcpyDeviceToHost, stream));
+            HIPCHECK(hipStreamSynchronize(stream));
+            std::vector<HostCamera> vc;
+            std::vector<uint32_t> base;
+            for (size_t k = 0; k < camIdx.size(); ++k) {
+                vc.push_back(cams[camIdx[k]]);
+                const size_t c = (size_t)hs.cursor + k;
+                base.push_back(c < tlen ? seedHost[c] : 0u);
+            }
+            const std::vector<uint32_t> spp(vc.size(), S.gather_passes);
+            const bool dec = S.spp_decorrelate != 0;
+            ForwardPlan plan =
+                planForward(vc, spp, [&](uint32_t v, uint32_t p) { return dec ? base[v] + p : base[v]; });
+            runForward(plan);
+            for (size_t k = 0; k < camIdx.size(); ++k) {
+                const size_t off = plan.fbOff[k], bytes = (plan.fbOff[k + 1] - off) * sizeof(float4);
+                HIPCHECK(hipMemcpyAsync(fb[camIdx[k]].p, viewsFb.as<float4>() + off, bytes, hipMemcpyDeviceToDevice,
+                                        stream));
+                HIPCHECK(hipMemcpyAsync(ldr[camIdx[k]].p, viewsLdr.as<float4>() + off, bytes,
+                                        hipMemcpyDeviceToDevice, stream));
+            }
+            finishForward();
+            return;
+        }
         DevState* st = this->st();
         const float invSpp = 1.0f / (float)S.gather_passes;
         cursors.reserve(sizeof(uint32_t));
-        camRounds = 0;
         for (size_t k = 0; k < camIdx.size(); ++k) {
             const HostCamera& c = cams[camIdx[k]];
             const uint32_t npx = c.W * c.H;
@@ -602,12 +632,13 @@ struct pale_handle {
 
     uint32_t uploadSeedTable(uint32_t cameras) {
         std::mt19937_64 gen(S.seed);
-        std::vector<uint32_t> table((size_t)S.forward_passes * (1 + S.bounces) + cameras + 1);
-        for (uint32_t& v : table) v = (uint32_t)gen();
-        upload(stream, seedTable, table);
+        seedHost.assign((size_t)S.forward_passes * (1 + S.bounces) + cameras + 1, 0u);
+        for (uint32_t& v : seedHost) v = (uint32_t)gen();
+        upload(stream, seedTable, seedHost);
         HIPCHECK(hipStreamSynchronize(stream));
-        return (uint32_t)table.size();
+        return (uint32_t)seedHost.size();
     }
+    std::vector<uint32_t> seedHost;  // the seed table on the host
 
     void finishForward() {
         HIPCHECK(hipEventRecord(ev1, stream));
@@ -701,18 +732,29 @@ struct pale_handle {
                          bCams.as<DevCamera>()};
     }
 
-    void batchRound(uint32_t maxSlots, float fraction) {
+    // One round of a batch.  timed: per-stage HIP events and work counting
+    // (eager launches only; a captured round records neither).
+    void batchRound(uint32_t maxSlots, float fraction, bool timed) {
         const PixelArrays px = pxa();
         const EntryArrays ent = enta();
+        const bool count = timed && countWork;
         launchFill(stream, stField(offsetof(DevState, entries)), 2, 0u);  // entries, overflow
         launchFill(stream, cursors.as<uint32_t>(), 1, 0u);
+        const size_t tt = timed ? tBegin(kTCamTrace) : 0;
         launchCamTraceBatch(stream, scene, st(), batchDesc(), maxSlots, px, ent, entryCap, cursors.as<uint32_t>(),
-                            fraction);
+                            fraction, count);
+        if (timed) tEnd(tt);
+        const size_t ts = timed ? tBegin(kTCamSort) : 0;
         launchExclusiveScan(stream, px.cnt, px.off, maxSlots, pxTiles.as<uint32_t>(), gridTotal.as<uint32_t>());
         launchEntrySort(stream, grid, st(), entryCap, px, ent, cellCount.as<uint32_t>(), entStart.as<uint32_t>(),
                         gridTiles.as<uint32_t>(), gridTotal.as<uint32_t>());
-        launchEntryGather(stream, devGrid(), st(), entryCap, entStart.as<uint32_t>() + grid.total, ent, false);
+        if (timed) tEnd(ts);
+        const size_t tk = timed ? tBegin(kTGather) : 0;
+        launchEntryGather(stream, devGrid(), st(), entryCap, entStart.as<uint32_t>() + grid.total, ent, count);
+        if (timed) tEnd(tk);
+        const size_t tc = timed ? tBegin(kTCamSort) : 0;
         launchCamCompositeBatch(stream, maxSlots, bHdr.as<BatchHeader>(), px, ent);
+        if (timed) tEnd(tc);
     }
 
     template <typename T>
@@ -756,7 +798,7 @@ struct pale_handle {
         hipGraph_t g = nullptr;
         HIPCHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed));
         launchBatchInit(stream, bd, maxTiles, px);
-        for (uint32_t r = 0; r < rounds; ++r) batchRound(maxSlots, fraction);
+        for (uint32_t r = 0; r < rounds; ++r) batchRound(maxSlots, fraction, false);
         HIPCHECK(hipStreamEndCapture(stream, &g));
         const hipError_t e = hipGraphInstantiate(&bGraph, g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
@@ -764,22 +806,32 @@ struct pale_handle {
         bGraphKey = key;
     }
 
-    void renderViews(const std::vector<HostCamera>& vc, const std::vector<uint32_t>& spp,
-                     const std::vector<uint32_t>& seeds) {
-        if (!mapValid) throw StatusError(PALE_ERR_INVALID_ARG, "render_views: no photon map; build it first");
+    // Segments (view, pass) in pass-major order cut into batches, their
+    // descriptors, buffers and (graph_rounds > 0) the batch graph.  seedOf
+    // gives the seed of a pass.
+    struct ForwardPlan {
+        std::vector<HostBatch> batches;
+        std::vector<size_t> fbOff;
+        uint32_t maxTiles = 0, maxSlots = 0, rounds = 0;
+        float fraction = 1.0f;
+    };
+
+    template <typename SeedFn>
+    ForwardPlan planForward(const std::vector<HostCamera>& vc, const std::vector<uint32_t>& spp, SeedFn seedOf) {
+        ForwardPlan plan;
         const uint32_t n = (uint32_t)vc.size();
-        std::vector<size_t> fbOff(n + 1, 0);
+        plan.fbOff.assign(n + 1, 0);
         uint32_t maxSpp = 0;
         for (uint32_t v = 0; v < n; ++v) {
             if (spp[v] == 0) throw StatusError(PALE_ERR_INVALID_ARG, "render_views: spp must be >= 1");
-            fbOff[v + 1] = fbOff[v] + (size_t)vc[v].W * vc[v].H;
+            plan.fbOff[v + 1] = plan.fbOff[v] + (size_t)vc[v].W * vc[v].H;
             maxSpp = std::max(maxSpp, spp[v]);
         }
-        if (fbOff[n] > 0xFFFFFFFFull) throw StatusError(PALE_ERR_INVALID_ARG, "render_views: too many pixels");
+        if (plan.fbOff[n] > 0xFFFFFFFFull) throw StatusError(PALE_ERR_INVALID_ARG, "render_views: too many pixels");
         const uint64_t budget = tune.slots_per_batch ? tune.slots_per_batch : (4u << 20);
-        const float fraction = tune.grid_fraction > 0.0f ? std::min(tune.grid_fraction, 1.0f) : 1.0f;
-        // Segments in pass-major order, cut into batches.
-        std::vector<HostBatch> batches;
+        plan.fraction = tune.grid_fraction > 0.0f ? std::min(tune.grid_fraction, 1.0f) : 1.0f;
+        plan.rounds = countWork ? 0u : tune.graph_rounds;
+        std::vector<HostBatch>& batches = plan.batches;
         uint64_t cur = budget + 1;
         for (uint32_t p = 0; p < maxSpp; ++p) {
             for (uint32_t v = 0; v < n; ++v) {
@@ -793,17 +845,19 @@ struct pale_handle {
                 HostBatch& b = batches.back();
                 const uint32_t tx = (c.W + 7u) / 8u, ty = (c.H + 7u) / 8u;
                 const uint32_t si = (uint32_t)b.segs.size();
-                b.segs.push_back(BatchSeg{v, p, c.W, c.H, tx, (uint32_t)cur, seeds[v] + p, 0u});
+                BatchSeg sg{v, p, c.W, c.H, tx, (uint32_t)cur, seedOf(v, p), (uint32_t)plan.fbOff[v],
+                            1.0f / (float)spp[v], {0, 0, 0}};
+                b.segs.push_back(sg);
                 for (uint32_t t = 0; t < tx * ty; ++t) b.tiles.push_back(make_uint2(si, t));
                 cur += npx;
             }
         }
-        uint32_t maxTiles = 0, maxSlots = 0, maxResolve = 0;
+        size_t maxSegs = 1, maxRuns = 1;
         for (HostBatch& b : batches) {
             // One run per view present, its segments in pass order (in
             // pass-major order a view's segments of a batch are not adjacent).
             std::vector<std::vector<uint32_t>> byView(n);
-            for (uint32_t s = 0; s < b.segs.size(); ++s) byView[b.segs[s].view].push_back(s);
+            for (uint32_t s2 = 0; s2 < b.segs.size(); ++s2) byView[b.segs[s2].view].push_back(s2);
             std::vector<BatchSeg> ordered;
             std::vector<uint32_t> remap(b.segs.size());
             uint32_t resolve = 0;
@@ -814,12 +868,12 @@ struct pale_handle {
                 r.firstSeg = (uint32_t)ordered.size();
                 r.numSegs = (uint32_t)byView[v].size();
                 r.npx = vc[v].W * vc[v].H;
-                r.fbOffset = (uint32_t)fbOff[v];
+                r.fbOffset = (uint32_t)plan.fbOff[v];
                 r.resolveBase = resolve;
                 r.invSpp = 1.0f / (float)spp[v];
-                for (uint32_t s : byView[v]) {
-                    remap[s] = (uint32_t)ordered.size();
-                    ordered.push_back(b.segs[s]);
+                for (uint32_t s2 : byView[v]) {
+                    remap[s2] = (uint32_t)ordered.size();
+                    ordered.push_back(b.segs[s2]);
                 }
                 resolve += r.npx;
                 b.runs.push_back(r);
@@ -827,52 +881,50 @@ struct pale_handle {
             for (uint2& t : b.tiles) t.x = remap[t.x];
             b.segs.swap(ordered);
             uint32_t slots = 0;
-            for (const BatchSeg& s : b.segs) slots += s.W * s.H;
+            for (const BatchSeg& sg : b.segs) slots += sg.W * sg.H;
             b.hdr = BatchHeader{(uint32_t)b.tiles.size(), slots, (uint32_t)b.segs.size(), (uint32_t)b.runs.size(),
                                 resolve, {0, 0, 0}};
-            maxTiles = std::max(maxTiles, b.hdr.nTiles);
-            maxSlots = std::max(maxSlots, slots);
-            maxResolve = std::max(maxResolve, resolve);
-        }
-        // Buffers (all before any capture).
-        const uint32_t perSlot = tune.entries_per_slot ? tune.entries_per_slot : 24u;
-        ensureCamera(maxSlots, perSlot);
-        cursors.reserve(sizeof(uint32_t));
-        bTiles.reserve((size_t)maxTiles * sizeof(uint2));
-        size_t maxSegs = 1, maxRuns = 1;
-        for (const HostBatch& b : batches) {
+            plan.maxTiles = std::max(plan.maxTiles, b.hdr.nTiles);
+            plan.maxSlots = std::max(plan.maxSlots, slots);
             maxSegs = std::max(maxSegs, b.segs.size());
             maxRuns = std::max(maxRuns, b.runs.size());
         }
+        // Buffers (all before any capture).
+        // Entries per slot: the tuning's, else the settings' per-pixel budget.
+        ensureCamera(std::max(plan.maxSlots, 1u), tune.entries_per_slot);
+        cursors.reserve(sizeof(uint32_t));
+        bTiles.reserve((size_t)std::max(plan.maxTiles, 1u) * sizeof(uint2));
         bSegs.reserve(maxSegs * sizeof(BatchSeg));
         bRuns.reserve(maxRuns * sizeof(BatchRun));
         bHdr.reserve(sizeof(BatchHeader));
         bCams.reserve(std::max<size_t>(n, 1) * sizeof(DevCamera));
-        viewsFb.reserve(std::max<size_t>(fbOff[n], 1) * sizeof(float4));
-        viewsLdr.reserve(std::max<size_t>(fbOff[n], 1) * sizeof(float4));
+        viewsFb.reserve(std::max<size_t>(plan.fbOff[n], 1) * sizeof(float4));
+        viewsLdr.reserve(std::max<size_t>(plan.fbOff[n], 1) * sizeof(float4));
         std::vector<DevCamera> dcs(n);
         for (uint32_t v = 0; v < n; ++v) dcs[v] = devCam(vc[v]);
         upload(stream, bCams, dcs);
-        const uint32_t rounds = tune.graph_rounds;
-        if (rounds) ensureBatchGraph(maxTiles, maxSlots, rounds, fraction);
-        tReset();
-        camRounds = 0;
-        HIPCHECK(hipEventRecord(ev0, stream));
-        launchBeginView(stream, st(), 0u);
-        launchFill(stream, pxCnt.as<uint32_t>(), maxSlots, 0u);
-        launchFillF(stream, viewsFb.as<float>(), (uint32_t)fbOff[n] * 4u, 0.0f);
+        HIPCHECK(hipStreamSynchronize(stream));
+        if (plan.rounds) ensureBatchGraph(plan.maxTiles, plan.maxSlots, plan.rounds, plan.fraction);
+        return plan;
+    }
+
+    // The batches of a plan into viewsFb / viewsLdr (all views, concatenated).
+    void runForward(ForwardPlan& plan) {
+        const uint32_t total = (uint32_t)plan.fbOff.back();
+        launchFill(stream, pxCnt.as<uint32_t>(), std::max(plan.maxSlots, 1u), 0u);
+        launchFillF(stream, viewsFb.as<float>(), total * 4u, 0.0f);
         const BatchDesc bd = batchDesc();
-        for (HostBatch& b : batches) {
+        for (HostBatch& b : plan.batches) {
             upload(stream, bTiles, b.tiles);
             upload(stream, bSegs, b.segs);
             upload(stream, bRuns, b.runs);
             HIPCHECK(hipMemcpyAsync(bHdr.p, &b.hdr, sizeof(BatchHeader), hipMemcpyHostToDevice, stream));
-            if (rounds) {
+            if (plan.rounds) {
                 HIPCHECK(hipGraphLaunch(bGraph, stream));
-                camRounds += rounds;
+                camRounds += plan.rounds;
             } else {
                 launchBatchInit(stream, bd, b.hdr.nTiles, pxa());
-                batchRound(maxSlots, fraction);
+                batchRound(plan.maxSlots, plan.fraction, true);
                 ++camRounds;
             }
             for (uint32_t extra = 0;; ++extra) {
@@ -881,14 +933,25 @@ struct pale_handle {
                 HIPCHECK(hipStreamSynchronize(stream));
                 if (*flagHost.as<uint32_t>() == 0u) break;
                 if (extra > 4096) throw StatusError(PALE_ERR_INTERNAL, "batched camera gather: no progress");
-                batchRound(maxSlots, fraction);
+                batchRound(plan.maxSlots, plan.fraction, true);
                 ++camRounds;
             }
             launchBatchResolve(stream, bd, b.hdr.nResolve, pxa(), viewsFb.as<float4>());
         }
-        launchPostprocess(stream, viewsFb.as<float4>(), viewsLdr.as<float4>(), (uint32_t)fbOff[n]);
-        viewOffsets = fbOff;
-        batchCount = (uint32_t)batches.size();
+        launchPostprocess(stream, viewsFb.as<float4>(), viewsLdr.as<float4>(), total);
+        viewOffsets = plan.fbOff;
+        batchCount = (uint32_t)plan.batches.size();
+    }
+
+    void renderViews(const std::vector<HostCamera>& vc, const std::vector<uint32_t>& spp,
+                     const std::vector<uint32_t>& seeds) {
+        if (!mapValid) throw StatusError(PALE_ERR_INVALID_ARG, "render_views: no photon map; build it first");
+        ForwardPlan plan = planForward(vc, spp, [&](uint32_t v, uint32_t p) { return seeds[v] + p; });
+        tReset();
+        camRounds = 0;
+        HIPCHECK(hipEventRecord(ev0, stream));
+        launchBeginView(stream, st(), 0u);
+        runForward(plan);
         finishForward();
     }
     std::vector<size_t> viewOffsets;
@@ -1072,6 +1135,12 @@ struct pale_handle {
 
     void renderBackward(const std::vector<uint32_t>& camIdx, const std::vector<const float*>& src,
                         bool srcOnDevice = false) {
+        if (S.batch_passes && S.adjoint_passes > 0) {
+            std::vector<HostCamera> vc;
+            for (uint32_t c : camIdx) vc.push_back(cams[c]);
+            renderViewsBackward(vc, std::vector<uint32_t>(vc.size(), S.adjoint_passes), src, srcOnDevice);
+            return;
+        }
         std::vector<const HostCamera*> cp;
         for (uint32_t c : camIdx) cp.push_back(&cams[c]);
         renderBackwardCams(cp, src, srcOnDevice, S.adjoint_passes);
@@ -1210,6 +1279,9 @@ void pale_default_settings(pale_settings* out) {
     s.spp_decorrelate = 0;
     s.debug_images = 0;
     s.gather_entry_budget = 0;
+    s.photon_map_capacity = 0;
+    s.full_gradients = 0;
+    s.batch_passes = 1;
     *out = s;
 }
 

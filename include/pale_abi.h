@@ -134,6 +134,14 @@ typedef struct pale_settings {
                                   zeros); 1 = also position, tangents,
                                   scale, albedo and beta, through alpha_geom
                                   and the BRDF of each adjoint event        */
+    int32_t batch_passes;      /* "batch_passes" (default 1): 1 = render_forward walks
+                                  the gather passes of all its cameras, and
+                                  render_backward traces the adjoint samples
+                                  of all its cameras, as batches (the
+                                  launch shape of pale_set_batch_tuning);
+                                  0 = one pass / sample after another.
+                                  Same images either way; gradients equal
+                                  up to the order of float atomics          */
 } pale_settings;
 
 /* Work counters of the last render calls (no reference counterpart; the

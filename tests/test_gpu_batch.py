@@ -155,3 +155,31 @@ def test_batched_adjoint_matches_oracle(renderer):
     want = sum(orc.render_view_backward(c, s, d) for c, s, d in zip(cams, spp, dl))
     got = r.render_views_backward(cams, spp, dl)["opacity"]
     _close(got, want)
+
+
+@pytest.mark.parametrize("decorrelate", [0, 1])
+def test_batch_passes_render_forward_and_backward(decorrelate):
+    # render_forward / render_backward with batch_passes = 1 walk all gather
+    # passes (and trace all adjoint samples) of all cameras as batches; the
+    # images equal the pass-by-pass ones bit for bit, the gradients up to the
+    # order of float atomics.
+    from differentiablepointrendering_amd.pale import Renderer
+
+    spec = scene.config_scene(0, n=4096, width=48, height=40)
+    spec.cameras.append(scene.look_at_camera("camera2", (1.0, 2.0, 3.2), (0, 0, 0), (0, 1, 0), 33, 21, 45.0))
+    base = dict(photons=20_000, gather_passes=3, adjoint_passes=2, spp_decorrelate=decorrelate, full_gradients=1)
+    print(f"case: batch_passes, decorrelate={decorrelate}", flush=True)
+    a = Renderer.from_spec(spec, base)
+    b = Renderer.from_spec(spec, dict(base, batch_passes=1))
+    b.set_batch_tuning(slots_per_batch=3000, entries_per_slot=3)
+    ia, ib = a.render_forward(), b.render_forward()
+    for k in ia:
+        assert np.array_equal(ia[k].view(np.uint32), ib[k].view(np.uint32)), k
+    assert b.last_batches if hasattr(b, "last_batches") else True
+    dl = {c.name: (ia[c.name + "_raw"][..., :3] * 1e-3).astype(np.float32) for c in spec.cameras}
+    ga, _ = a.render_backward(dl)
+    gb, _ = b.render_backward(dl)
+    for k in ("opacity", "position", "scale", "albedo", "beta"):
+        scale = float(np.abs(ga[k]).max())
+        assert scale > 0, k
+        np.testing.assert_allclose(gb[k], ga[k], rtol=1e-3, atol=1e-3 * scale, err_msg=k)

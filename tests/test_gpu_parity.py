@@ -215,17 +215,22 @@ def test_camera_gather_rounds_bit_exact():
     # A one-entry-per-pixel budget forces several rounds per pass (about
     # seven layers per pixel here): pixels stop at a layer that does not fit
     # and resume from it; the image must not change.
+    # Pass by pass (batch_passes = 0), then all passes as one batch.
     spec = scene.config_scene(0, n=65536, width=64, height=64)
-    gpu, orc = _pair(spec, dict(SMALL, gather_entry_budget=1))
+    gpu, orc = _pair(spec, dict(SMALL, gather_entry_budget=1, batch_passes=0))
     a = gpu.render_forward()["camera1_raw"]
     rounds = gpu.stats()["camera_rounds"]
     b = orc.render_forward()["camera1_raw"]
     assert rounds > 2 * spec.settings["gather_passes"], rounds
     assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
-    one, _ = _pair(spec, SMALL)
+    one, _ = _pair(spec, dict(SMALL, batch_passes=0))
     c = one.render_forward()["camera1_raw"]
     assert one.stats()["camera_rounds"] == spec.settings["gather_passes"]
     assert np.array_equal(a.view(np.uint32), c.view(np.uint32))
+    batched, _ = _pair(spec, dict(SMALL, gather_entry_budget=1, batch_passes=1))
+    d = batched.render_forward()["camera1_raw"]
+    assert batched.stats()["camera_rounds"] > 2, batched.stats()["camera_rounds"]
+    assert np.array_equal(a.view(np.uint32), d.view(np.uint32))
 
 
 def test_camera_gather_rounds_resume_every_pixel():
@@ -234,7 +239,7 @@ def test_camera_gather_rounds_resume_every_pixel():
     # 8: in later rounds most tiles a wave draws are already finished, and
     # every pixel still unfinished must be drawn and completed.
     spec = scene.config_scene(0, n=65536, width=333, height=250)
-    settings = dict(SMALL, gather_passes=1, adjoint_passes=1)
+    settings = dict(SMALL, gather_passes=1, adjoint_passes=1, batch_passes=0)
     gpu, orc = _pair(spec, dict(settings, gather_entry_budget=2))
     a = gpu.render_forward()["camera1_raw"]
     assert gpu.stats()["camera_rounds"] >= 3
