@@ -776,11 +776,12 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(4))
 // Slot of every entry in its pixel's ordered list; grid cell and per-cell
 // count of the gather entries (the cell of worldToCellClamped,
 // KernelHelpers.h:920-934).
+// The entry kernels stride over the entries with a bounded grid: the count
+// is known only on the device and the capacity is far above it.
 __global__ __launch_bounds__(256) void k_entry_key(GridParams g, const DevState* st, uint32_t cap, PixelArrays px,
                                                    EntryArrays ent, uint32_t* cellCount) {
     const uint32_t n = min(st->entries, cap);
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const float4 b = ent.qb[i];
     const uint32_t tag = f2u(b.w);
     ent.ord[px.off[f2u(b.z)] + (tag & ~kEntryNoGather)] = i;
@@ -795,22 +796,25 @@ __global__ __launch_bounds__(256) void k_entry_key(GridParams g, const DevState*
         atomicAdd(&cellCount[c], 1u);
     }
     ent.cell[i] = c;
+    }
 }
 
 // Counting-sort scatter of the gather entries; leaves cellCount zeroed.
 __global__ __launch_bounds__(256) void k_entry_scatter(const DevState* st, uint32_t cap, const uint32_t* cellStart,
                                                        uint32_t* cellCount, EntryArrays ent) {
     const uint32_t n = min(st->entries, cap);
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t c = ent.cell[i];
-    if (c == 0xFFFFFFFFu) return;
-    ent.sorted[cellStart[c] + atomicSub(&cellCount[c], 1u) - 1u] = i;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t c = ent.cell[i];
+        if (c != 0xFFFFFFFFu) ent.sorted[cellStart[c] + atomicSub(&cellCount[c], 1u) - 1u] = i;
+    }
 }
 
 template <bool kCount>
 __global__ __launch_bounds__(256) void k_entry_gather(DevGrid G, DevState* st, const uint32_t* nGather,
                                                       EntryArrays ent) {
+    // One lane per entry (a one-shot grid over the capacity): gathers vary
+    // in cost, and fresh waves take the place of finished ones, which a
+    // strided loop would not.
     const uint32_t n = *nGather;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     Work w{};
@@ -1140,26 +1144,26 @@ __global__ __launch_bounds__(kTraceBlock) void k_adj_trace(DevScene S, DevState*
 __global__ __launch_bounds__(256) void k_event_key(GridParams g, const DevState* st, uint32_t cap, EventArrays ev,
                                                    uint32_t* cell, uint32_t* cellCount) {
     const uint32_t n = min(st->evPool, cap);
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const float4 a = ev.e0[i];
-    const F3 f = divv(mk3(a.x, a.y, a.z) - mk3(g.ox, g.oy, g.oz), mk3(g.cell, g.cell, g.cell));
-    const int x = min(max((int)floorf(f.x), 0), g.rx - 1);
-    const int y = min(max((int)floorf(f.y), 0), g.ry - 1);
-    const int z = min(max((int)floorf(f.z), 0), g.rz - 1);
-    const uint32_t c = (uint32_t)((z * g.ry + y) * g.rx + x);
-    cell[i] = c;
-    atomicAdd(&cellCount[c], 1u);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const float4 a = ev.e0[i];
+        const F3 f = divv(mk3(a.x, a.y, a.z) - mk3(g.ox, g.oy, g.oz), mk3(g.cell, g.cell, g.cell));
+        const int x = min(max((int)floorf(f.x), 0), g.rx - 1);
+        const int y = min(max((int)floorf(f.y), 0), g.ry - 1);
+        const int z = min(max((int)floorf(f.z), 0), g.rz - 1);
+        const uint32_t c = (uint32_t)((z * g.ry + y) * g.rx + x);
+        cell[i] = c;
+        atomicAdd(&cellCount[c], 1u);
+    }
 }
 
 __global__ __launch_bounds__(256) void k_event_scatter(const DevState* st, uint32_t cap, const uint32_t* cell,
                                                        const uint32_t* cellStart, uint32_t* cellCount,
                                                        uint32_t* sorted) {
     const uint32_t n = min(st->evPool, cap);
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t c = cell[i];
-    sorted[cellStart[c] + atomicSub(&cellCount[c], 1u) - 1u] = i;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t c = cell[i];
+        sorted[cellStart[c] + atomicSub(&cellCount[c], 1u) - 1u] = i;
+    }
 }
 
 // Full-gradient contribution of one event: d(alpha_geom)/d(geometry) of
@@ -1181,9 +1185,10 @@ __device__ void addGeometryGradient(const DevScene& S, uint32_t prim, float4 ro,
     atomicAdd(out + 15, g[11] * f);
 }
 
-// launchAdjointTransportKernel (AdjointKernels.cpp:389-509).
+// launchAdjointTransportKernel (AdjointKernels.cpp:389-509), one lane per
+// pooled event.  Each event carries the 1/spp of the view whose adjoint ray
+// made it (e3.w).
 template <bool kCount>
-// Each event carries the 1/spp of the view whose adjoint ray made it (e3.w).
 __global__ __launch_bounds__(256) void k_adj_transport(DevScene S, DevGrid G, DevState* st, uint32_t poolCap,
                                                        EventArrays ev, const uint32_t* sorted, float* grad,
                                                        float* full) {
@@ -1312,6 +1317,10 @@ __global__ __launch_bounds__(256) void k_gather_points(DevGrid G, uint32_t n, co
 
 // ------------------------------------------------------------ launchers --
 static inline uint32_t blocks(uint32_t n, uint32_t b) { return (n + b - 1) / b; }
+// Grid of the light per-item kernels (keys and scatters of the entry and
+// event sorts) that stride over buffers whose count is known only on the
+// device: 256 CUs x 32 workgroups of 256 lanes.
+constexpr uint32_t kEntryBlocks = 8192;
 static inline size_t stackBytes(const DevScene& S) { return (size_t)S.stackDepth * sizeof(uint2) * kTraceBlock; }
 
 void launchFill(hipStream_t s, uint32_t* p, uint32_t n, uint32_t v) {
@@ -1483,9 +1492,10 @@ void launchEntrySort(hipStream_t s, const GridParams& g, DevState* st, uint32_t 
                      EntryArrays ent, uint32_t* cellCount, uint32_t* cellStart, uint32_t* tiles,
                      uint32_t* scratchTotal) {
     if (!cap) return;
-    k_entry_key<<<blocks(cap, 256), 256, 0, s>>>(g, st, cap, px, ent, cellCount);
+    const uint32_t gb = min(blocks(cap, 256), kEntryBlocks);
+    k_entry_key<<<gb, 256, 0, s>>>(g, st, cap, px, ent, cellCount);
     launchExclusiveScan(s, cellCount, cellStart, g.total, tiles, scratchTotal);
-    k_entry_scatter<<<blocks(cap, 256), 256, 0, s>>>(st, cap, cellStart, cellCount, ent);
+    k_entry_scatter<<<gb, 256, 0, s>>>(st, cap, cellStart, cellCount, ent);
 }
 
 void launchEntryGather(hipStream_t s, const DevGrid& G, DevState* st, uint32_t cap, const uint32_t* nGather,
@@ -1532,9 +1542,9 @@ void launchEventSort(hipStream_t s, const GridParams& g, DevState* st, uint32_t 
                      uint32_t* cell, uint32_t* sorted, uint32_t* cellCount, uint32_t* cellStart, uint32_t* tiles,
                      uint32_t* scratchTotal) {
     if (!poolCap) return;
-    k_event_key<<<blocks(poolCap, 256), 256, 0, s>>>(g, st, poolCap, ev, cell, cellCount);
+    k_event_key<<<min(blocks(poolCap, 256), kEntryBlocks), 256, 0, s>>>(g, st, poolCap, ev, cell, cellCount);
     launchExclusiveScan(s, cellCount, cellStart, g.total, tiles, scratchTotal);
-    k_event_scatter<<<blocks(poolCap, 256), 256, 0, s>>>(st, poolCap, cell, cellStart, cellCount, sorted);
+    k_event_scatter<<<min(blocks(poolCap, 256), kEntryBlocks), 256, 0, s>>>(st, poolCap, cell, cellStart, cellCount, sorted);
 }
 void launchAdjTransport(hipStream_t s, const DevScene& S, const DevGrid& G, DevState* st, uint32_t poolCap,
                         EventArrays ev, const uint32_t* sorted, float* gradOpacity, float* gradFull, bool count) {
