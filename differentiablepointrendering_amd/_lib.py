@@ -71,7 +71,7 @@ def load(path: str = LIB_PATH):
     L.pale_set_opacity_device.argtypes = [H, _fp, C.c_uint32]
     L.pale_build_photon_map.argtypes = [H]
     L.pale_device_bytes.argtypes = [C.POINTER(C.c_uint64)]
-    L.pale_view_bytes.argtypes = [H, C.c_uint32, C.c_uint32, C.c_int32, C.POINTER(C.c_uint64)]
+    L.pale_view_bytes.argtypes = [H, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int32, C.POINTER(C.c_uint64)]
     L.pale_release_view_buffers.argtypes = [H]
     L.pale_view_buffer_bytes.argtypes = [H, C.POINTER(C.c_uint64)]
     L.pale_set_batch_tuning.argtypes = [H, C.POINTER(_abi.BatchTuning)]

@@ -128,7 +128,9 @@ struct pale_handle {
     bool adjFull = false;  // event rays allocated (full gradients)
     // Camera gather: pixel state, entries of one round, their cell sort.
     uint32_t pxCap = 0, entryCap = 0;
-    DevMem viewFb, viewLdr;  // images of pale_render_view
+    DevMem viewFb, viewLdr;  // images of pale_render_view (pass by pass)
+    float4* outFb = nullptr;  // where the last pale_render_view left its images
+    float4* outLdr = nullptr;
     // Batched views (pale_render_views): descriptors of the current batch,
     // the framebuffers of all views, the captured graph of a batch's rounds.
     DevMem bTiles, bSegs, bRuns, bHdr, bCams, bSegFirst, viewsFb, viewsLdr;
@@ -347,9 +349,24 @@ struct pale_handle {
     struct Probe {
         static constexpr bool kDry = true;
         uint64_t add = 0;
+        std::vector<std::pair<const DevMem*, size_t>> planned;  // sizes this dry run has grown
         void operator()(const DevMem& m, size_t b) {
-            if (b <= m.bytes && m.p) return;
-            add += std::max<size_t>(b, 16) - m.bytes;
+            // DevMem::reserve: no change if allocated and large enough, else
+            // a new allocation of max(b, 16) bytes in place of the old one.
+            size_t have = m.p ? m.bytes : 0;
+            bool allocated = m.p != nullptr;
+            auto it = std::find_if(planned.begin(), planned.end(), [&](const auto& x) { return x.first == &m; });
+            if (it != planned.end()) {
+                have = it->second;
+                allocated = true;
+            }
+            if (allocated && b <= have) return;
+            const size_t now = std::max<size_t>(b, 16);
+            add += now - have;
+            if (it != planned.end())
+                it->second = now;
+            else
+                planned.emplace_back(&m, now);
         }
     };
 
@@ -413,16 +430,49 @@ struct pale_handle {
         sizeAdjoint(npx, a);
     }
 
-    // Device bytes a view of npx pixels (with its adjoint) would add.
-    uint64_t viewGrowth(uint32_t npx, bool adjoint) {
+    // Passes of one view per batch (whole passes; a pass larger than the
+    // batch size is a batch of its own), as planForward / renderViewsBackward
+    // cut them.
+    uint32_t passesPerBatch(uint32_t npx, uint32_t spp) const {
+        const uint64_t budget = tune.slots_per_batch ? tune.slots_per_batch : (4u << 20);
+        const uint64_t k = npx >= budget ? 1u : budget / npx;
+        return (uint32_t)std::min<uint64_t>(std::max<uint64_t>(k, 1u), spp);
+    }
+
+    // Device bytes a W x H view of spp passes (with its adjoint) would add:
+    // the buffer sizing of renderView / renderViewBackward, run dry.
+    uint64_t viewGrowth(uint32_t W, uint32_t H, uint32_t spp, bool adjoint) {
         Probe p;
-        sizeCamera(npx, p);
+        const uint32_t npx = W * H;
+        if (!S.batch_passes) {
+            sizeCamera(npx, p);
+            p(cursors, sizeof(uint32_t));
+            p(viewFb, (size_t)npx * sizeof(float4));
+            p(viewLdr, (size_t)npx * sizeof(float4));
+            if (adjoint) {
+                sizeAdjoint(npx, p);
+                p(dloss, (size_t)npx * 3 * sizeof(float));
+                if (S.full_gradients) p(gradFull, std::max<size_t>(1, bp.points.size() * 16) * sizeof(float));
+            }
+            return p.add;
+        }
+        const uint32_t k = passesPerBatch(npx, std::max(spp, 1u));
+        const uint32_t slots = k * npx, tiles = k * ((W + 7u) / 8u) * ((H + 7u) / 8u);
+        sizeCamera(std::max(slots, 1u), p, tune.entries_per_slot);
         p(cursors, sizeof(uint32_t));
-        p(viewFb, (size_t)npx * sizeof(float4));
-        p(viewLdr, (size_t)npx * sizeof(float4));
+        p(bTiles, (size_t)std::max(tiles, 1u) * sizeof(uint2));
+        p(bSegs, (size_t)k * sizeof(BatchSeg));
+        p(bRuns, sizeof(BatchRun));
+        p(bHdr, sizeof(BatchHeader));
+        p(bCams, sizeof(DevCamera));
+        p(viewsFb, (size_t)std::max(npx, 1u) * sizeof(float4));
+        p(viewsLdr, (size_t)std::max(npx, 1u) * sizeof(float4));
         if (adjoint) {
-            sizeAdjoint(npx, p);
-            p(dloss, (size_t)npx * 3 * sizeof(float));
+            sizeAdjoint(std::max(slots, 1u), p);
+            p(bTiles, (size_t)tiles * sizeof(uint2));
+            p(bSegs, (size_t)k * sizeof(BatchSeg));
+            p(bSegFirst, (size_t)k * sizeof(uint32_t));
+            p(dloss, (size_t)std::max(npx, 1u) * 3 * sizeof(float));
             if (S.full_gradients) p(gradFull, std::max<size_t>(1, bp.points.size() * 16) * sizeof(float));
         }
         return p.add;
@@ -434,7 +484,8 @@ struct pale_handle {
     std::vector<DevMem*> viewBuffers() {
         std::vector<DevMem*> v = {&pxOt,    &pxDl,    &pxCs,    &pxCnt,  &pxOff, &pxTiles, &entQa,  &entQb,
                                   &entComp, &entE,    &entOrd,  &entCell, &entSorted, &viewFb, &viewLdr, &pend0,
-                                  &pend1,   &pend2,   &pend3,   &pend4,  &evCell, &evSorted, &evFlags, &dloss};
+                                  &pend1,   &pend2,   &pend3,   &pend4,  &evCell, &evSorted, &evFlags, &dloss,
+                                  &bTiles,  &bSegs,   &bRuns,   &bHdr,   &bCams, &bSegFirst, &viewsFb, &viewsLdr};
         for (int k = 0; k < 6; ++k) {
             v.push_back(&evStg[k]);
             v.push_back(&evCmp[k]);
@@ -961,6 +1012,14 @@ struct pale_handle {
     void renderView(const HostCamera& c, uint32_t spp, uint32_t seed) {
         if (!mapValid) throw StatusError(PALE_ERR_INVALID_ARG, "render_view: no photon map; build it first");
         if (spp == 0) throw StatusError(PALE_ERR_INVALID_ARG, "render_view: spp must be >= 1");
+        if (S.batch_passes) {
+            renderViews({c}, {spp}, {seed});
+            outFb = viewsFb.as<float4>();
+            outLdr = viewsLdr.as<float4>();
+            return;
+        }
+        outFb = viewFb.as<float4>();
+        outLdr = viewLdr.as<float4>();
         const uint32_t npx = c.W * c.H;
         ensureCamera(npx);
         viewFb.reserve((size_t)npx * sizeof(float4));
@@ -1595,9 +1654,9 @@ int pale_device_bytes(uint64_t* out) {
     return PALE_OK;
 }
 
-int pale_view_bytes(pale_handle* h, uint32_t width, uint32_t height, int32_t adjoint, uint64_t* out) {
+int pale_view_bytes(pale_handle* h, uint32_t width, uint32_t height, uint32_t spp, int32_t adjoint, uint64_t* out) {
     if (!h || !out) return fail(PALE_ERR_INVALID_ARG, "null argument");
-    return guarded([&] { *out = h->viewGrowth(width * height, adjoint != 0); });
+    return guarded([&] { *out = h->viewGrowth(width, height, spp, adjoint != 0); });
 }
 
 int pale_view_buffer_bytes(pale_handle* h, uint64_t* out) {
@@ -1625,8 +1684,8 @@ int pale_render_view(pale_handle* h, const pale_camera_desc* camera, uint32_t sp
         const HostCamera c = hostCamera(*camera);
         h->renderView(c, spp, seed);
         const size_t bytes = (size_t)c.W * c.H * 4 * sizeof(float);
-        if (raw_out) HIPCHECK(hipMemcpy(raw_out, h->viewFb.p, bytes, hipMemcpyDeviceToHost));
-        if (ldr_out) HIPCHECK(hipMemcpy(ldr_out, h->viewLdr.p, bytes, hipMemcpyDeviceToHost));
+        if (raw_out) HIPCHECK(hipMemcpy(raw_out, h->outFb, bytes, hipMemcpyDeviceToHost));
+        if (ldr_out) HIPCHECK(hipMemcpy(ldr_out, h->outLdr, bytes, hipMemcpyDeviceToHost));
     });
 }
 
@@ -1637,8 +1696,8 @@ int pale_render_view_device(pale_handle* h, const pale_camera_desc* camera, uint
         const HostCamera c = hostCamera(*camera);
         h->renderView(c, spp, seed);
         const size_t bytes = (size_t)c.W * c.H * 4 * sizeof(float);
-        if (raw_dev) HIPCHECK(hipMemcpyAsync(raw_dev, h->viewFb.p, bytes, hipMemcpyDeviceToDevice, h->stream));
-        if (ldr_dev) HIPCHECK(hipMemcpyAsync(ldr_dev, h->viewLdr.p, bytes, hipMemcpyDeviceToDevice, h->stream));
+        if (raw_dev) HIPCHECK(hipMemcpyAsync(raw_dev, h->outFb, bytes, hipMemcpyDeviceToDevice, h->stream));
+        if (ldr_dev) HIPCHECK(hipMemcpyAsync(ldr_dev, h->outLdr, bytes, hipMemcpyDeviceToDevice, h->stream));
         HIPCHECK(hipStreamSynchronize(h->stream));
     });
 }
@@ -1649,7 +1708,10 @@ int pale_render_view_backward(pale_handle* h, const pale_camera_desc* camera, ui
     return guarded([&] {
         if (spp == 0) throw StatusError(PALE_ERR_INVALID_ARG, "render_view_backward: spp must be >= 1");
         const HostCamera c = hostCamera(*camera);
-        h->renderBackwardCams({&c}, {dloss_rgb}, false, spp);
+        if (h->S.batch_passes)
+            h->renderViewsBackward({c}, {spp}, {dloss_rgb}, false);
+        else
+            h->renderBackwardCams({&c}, {dloss_rgb}, false, spp);
         copyGradsHost(h, grads);
     });
 }

@@ -229,10 +229,12 @@ class Renderer:
         check(load().pale_device_bytes(C.byref(v)))
         return v.value
 
-    def view_bytes(self, width: int, height: int, adjoint: bool) -> int:
-        """Device bytes a width x height view (with its adjoint) would add."""
+    def view_bytes(self, width: int, height: int, spp: int, adjoint: bool) -> int:
+        """Device bytes a width x height view of spp passes (with its
+        adjoint) would add."""
         v = C.c_uint64()
-        check(self._L.pale_view_bytes(self._h, int(width), int(height), 1 if adjoint else 0, C.byref(v)))
+        check(self._L.pale_view_bytes(self._h, int(width), int(height), int(spp), 1 if adjoint else 0,
+                                      C.byref(v)))
         return v.value
 
     def view_buffer_bytes(self) -> int:

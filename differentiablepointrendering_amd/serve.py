@@ -291,7 +291,7 @@ class SceneServer:
             if not idle:
                 continue
             # The idle replica whose buffers need the least growth.
-            need, rep = min(((r.renderer.view_bytes(req.camera.width, req.camera.height, req.adjoint), r)
+            need, rep = min(((r.renderer.view_bytes(req.camera.width, req.camera.height, req.spp, req.adjoint), r)
                              for r in idle), key=lambda x: (x[0], x[1].index))
             if self.budget is not None and not self._make_room(rep, need):
                 if self._running == 0:

@@ -294,10 +294,12 @@ int pale_set_opacity_device(pale_handle* h, const float* opacity_dev,
  * and are kept until pale_destroy): the measure of HBM admission. */
 int pale_device_bytes(uint64_t* out);
 
-/* Device bytes that serving a width x height view (adjoint != 0: with its
- * adjoint) would add to this renderer's buffers; 0 when they already fit. */
+/* Device bytes that serving a width x height view of spp passes (adjoint
+ * != 0: with its adjoint) would add to this renderer's buffers; 0 when they
+ * already fit (with batch_passes, the passes of a batch share the buffers:
+ * the batch tuning's slots per batch bound them). */
 int pale_view_bytes(pale_handle* h, uint32_t width, uint32_t height,
-                    int32_t adjoint, uint64_t* out);
+                    uint32_t spp, int32_t adjoint, uint64_t* out);
 
 /* Device bytes of this renderer's image-sized buffers (camera gather, view
  * images, adjoint): what pale_release_view_buffers frees. */

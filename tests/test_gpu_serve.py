@@ -93,14 +93,14 @@ def test_view_memory_accounting_is_exact(pair):
     gpu.release_view_buffers()
     assert gpu.view_buffer_bytes() == 0
     cam = _cam((0.0, 2.0, 3.0), 96, 72)
-    for adjoint in (False, True):
-        want = gpu.view_bytes(96, 72, adjoint)
+    for spp, adjoint in ((1, False), (1, True), (5, False), (5, True)):
+        want = gpu.view_bytes(96, 72, spp, adjoint)
         before = Renderer.device_bytes()
-        gpu.render_view(cam, 1, 3)
+        gpu.render_view(cam, spp, 3)
         if adjoint:
-            gpu.render_view_backward(cam, 1, np.zeros((72, 96, 3), np.float32))
-        assert Renderer.device_bytes() - before == want
-        assert gpu.view_bytes(96, 72, adjoint) == 0
+            gpu.render_view_backward(cam, spp, np.zeros((72, 96, 3), np.float32))
+        assert Renderer.device_bytes() - before == want, (spp, adjoint)
+        assert gpu.view_bytes(96, 72, spp, adjoint) == 0
     held = gpu.view_buffer_bytes()
     img = gpu.render_view(cam, 2, 9)["raw"]
     before = Renderer.device_bytes()

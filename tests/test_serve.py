@@ -53,7 +53,7 @@ class StandIn:
         a = max(self.adj, 20 * w * h) if adjoint else self.adj
         return v, a
 
-    def view_bytes(self, w, h, adjoint):
+    def view_bytes(self, w, h, spp, adjoint):
         v, a = self._need(w, h, adjoint)
         return (v - self.view) + (a - self.adj)
 
