@@ -28,7 +28,7 @@ EXPORTED = (
     "pale_set_opacity_device", "pale_build_photon_map", "pale_render_view", "pale_render_view_device",
     "pale_render_view_backward", "pale_device_bytes", "pale_view_bytes", "pale_view_buffer_bytes",
     "pale_release_view_buffers", "pale_set_batch_tuning", "pale_get_batch_tuning", "pale_render_views",
-    "pale_render_views_backward",
+    "pale_render_views_backward", "pale_render_view_backward_device",
 )
 
 
@@ -78,6 +78,7 @@ def load(path: str = LIB_PATH):
     L.pale_get_batch_tuning.argtypes = [H, C.POINTER(_abi.BatchTuning)]
     L.pale_render_views.argtypes = [H, C.c_uint32, C.POINTER(_abi.CameraDesc), _u32p, _u32p, C.POINTER(_fp),
                                     C.POINTER(_fp), _u32p]
+    L.pale_render_view_backward_device.argtypes = [H, C.POINTER(_abi.CameraDesc), C.c_uint32, _fp, C.POINTER(_fp)]
     L.pale_render_views_backward.argtypes = [H, C.c_uint32, C.POINTER(_abi.CameraDesc), _u32p, C.POINTER(_fp),
                                              C.POINTER(_fp), _u32p]
     _camp = C.POINTER(_abi.CameraDesc)

@@ -1716,6 +1716,20 @@ int pale_render_view_backward(pale_handle* h, const pale_camera_desc* camera, ui
     });
 }
 
+int pale_render_view_backward_device(pale_handle* h, const pale_camera_desc* camera, uint32_t spp,
+                                     const float* dloss_rgb_dev, float* const* grads_dev) {
+    if (!h || !camera || !dloss_rgb_dev) return fail(PALE_ERR_INVALID_ARG, "null argument");
+    return guarded([&] {
+        if (spp == 0) throw StatusError(PALE_ERR_INVALID_ARG, "render_view_backward: spp must be >= 1");
+        const HostCamera c = hostCamera(*camera);
+        if (h->S.batch_passes)
+            h->renderViewsBackward({c}, {spp}, {dloss_rgb_dev}, true);
+        else
+            h->renderBackwardCams({&c}, {dloss_rgb_dev}, true, spp);
+        copyGradsDevice(h, grads_dev);
+    });
+}
+
 int pale_point_count(const pale_handle* h, uint32_t* count) {
     if (!h || !count) return fail(PALE_ERR_INVALID_ARG, "null argument");
     *count = (uint32_t)h->bp.points.size();

@@ -88,6 +88,10 @@ class Renderer:
             check(self._L.pale_camera_info(self._h, i, name, C.byref(w), C.byref(hh), C.byref(adj)))
             self._cams.append((name.value.decode(), w.value, hh.value, bool(adj.value)))
 
+    @property
+    def full_gradients(self) -> bool:
+        return bool(self._settings.full_gradients)
+
     def close(self):
         if getattr(self, "_h", None):
             self._L.pale_destroy(self._h)
@@ -291,6 +295,24 @@ class Renderer:
                                         C.byref(nb)))
         self.last_batches = nb.value
         return [{"ldr": a, "raw": b} for a, b in zip(ldr, raw)]
+
+    def render_view_backward_device(self, camera: _abi.Camera, spp: int, dloss) -> dict:
+        """render_view_backward from an (H,W,3) dL/dI torch tensor on the GPU;
+        the gradient dict as torch tensors on the GPU."""
+        import torch
+
+        if dloss.dim() != 3 or dloss.shape[0] != camera.height or dloss.shape[1] != camera.width:
+            raise RuntimeError(f"render_view_backward: dL/dI must be ({camera.height},{camera.width},3)")
+        dl = dloss[..., :3].to(device="cuda", dtype=torch.float32).contiguous()
+        d = _abi.fill_camera_desc(_abi.CameraDesc(), camera)
+        n = self.point_count
+        grads = {k: torch.empty((n, w) if w else (n,), dtype=torch.float32, device="cuda")
+                 for k, w in _abi.POINT_SHAPES.items()}
+        gp = (_fp * 8)(*[self._dptr(grads[k]) for k in _abi.POINT_KEYS])
+        torch.cuda.current_stream().synchronize()
+        check(self._L.pale_render_view_backward_device(self._h, C.byref(d), int(spp), self._dptr(dl), gp))
+        grads["color"] = grads["albedo"]
+        return grads
 
     def render_views_backward(self, cameras, spp, dloss) -> dict:
         """Adjoint of several views (dL/dI (H,W,3) each) over the current

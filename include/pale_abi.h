@@ -329,6 +329,13 @@ int pale_render_view_backward(pale_handle* h, const pale_camera_desc* camera,
                               uint32_t spp, const float* dloss_rgb,
                               float* const* grads);
 
+/* pale_render_view_backward with device dL/dI (H,W,3) and device gradient
+ * arrays (as pale_render_backward_device). */
+int pale_render_view_backward_device(pale_handle* h,
+                                     const pale_camera_desc* camera,
+                                     uint32_t spp, const float* dloss_rgb_dev,
+                                     float* const* grads_dev);
+
 /* ---- batched views: no reference counterpart (the reference renders one
  * camera after another, SyclBridge.cpp:144-157).  Many views of uneven
  * resolution and spp over the current photon map in one persistent

@@ -86,3 +86,120 @@ def test_device_full_gradients_match_host():
         m = np.abs(gh[k]).max()
         assert m > 0, k
         np.testing.assert_allclose(gd[k].cpu().numpy(), gh[k], rtol=1e-3, atol=1e-3 * m, err_msg=k)
+
+
+def _tensors(r, requires_grad=True):
+    import torch
+
+    from differentiablepointrendering_amd.torch_ops import PARAMS
+
+    p = r.get_point_parameters()
+    return {k: torch.from_numpy(p[k]).cuda().requires_grad_(requires_grad) for k in PARAMS}
+
+
+def test_render_surfels_gradients_are_the_adjoint():
+    import torch
+
+    from differentiablepointrendering_amd.torch_ops import render_surfels
+
+    spec = scene.config_scene(0, n=4096, width=64, height=64)
+    r = _renderer(spec, dict(photons=20_000, full_gradients=1))
+    params = _tensors(r)
+    with torch.no_grad():
+        target = 0.8 * render_surfels(r, params, camera="camera1")[..., :3]
+    img = render_surfels(r, params, camera="camera1")
+    loss = 0.5 * ((img[..., :3] - target) ** 2).mean()
+    loss.backward()
+    raw = r.render_forward()["camera1_raw"][..., :3]
+    dl = ((raw - target.cpu().numpy()) / raw.size).astype(np.float32)
+    g, _ = r.render_backward({"camera1": dl})
+    for k, t in params.items():
+        want = g[k]
+        scale = float(np.abs(want).max())
+        assert scale > 0, k
+        np.testing.assert_allclose(t.grad.cpu().numpy(), want, rtol=1e-3, atol=1e-3 * scale, err_msg=k)
+
+
+def test_render_surfels_needs_full_gradients_for_geometry():
+    from differentiablepointrendering_amd.torch_ops import render_surfels
+
+    spec = scene.config_scene(0, n=256, width=16, height=16)
+    r = _renderer(spec, dict(photons=5_000))
+    params = _tensors(r, requires_grad=False)
+    params["position"].requires_grad_(True)
+    with pytest.raises(RuntimeError, match="full_gradients"):
+        render_surfels(r, params, camera="camera1")
+
+
+def test_render_surfels_frozen_map_finite_differences():
+    # The reference's "empty" gradient scene (one anisotropic surfel with a
+    # smooth rim), the photon map built once (render_view), central
+    # differences of the loss against the autograd gradient for parameters
+    # that keep the hit point in place, as the CPU oracle's frozen-map check.
+    import os
+
+    import torch
+
+    from differentiablepointrendering_amd import assets
+    from differentiablepointrendering_amd.pale import Renderer
+    from differentiablepointrendering_amd.torch_ops import render_surfels
+
+    root = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "reference_scenes")
+    meshes, cams = assets.load_xml(os.path.join(root, "GradientTests", "empty", "empty.xml"), root)
+    ini = assets.load_points(os.path.join(root, "GradientTests", "empty", "initial.ply"))
+    ini["opacity"][:] = 0.6
+    ini["beta"][:] = 1.0
+    ini["scale"][0] = [0.9, 0.6]
+    tgt = {k: v.copy() for k, v in ini.items()}
+    tgt["position"][0] += [0.05, 0.0, -0.04]
+    tgt["scale"][0] *= 0.9
+    tgt["beta"][0] += 0.5
+    tgt["albedo"][0] = [0.5, 0.6, 0.7]
+    base = dict(photons=100_000, bounces=5, forward_passes=5, seed=42, adjoint_bounces=1, full_gradients=1)
+    cam = cams[0]
+    view = (cam, 4, 7)
+    print("case: frozen-map finite differences, reference scene 'empty'", flush=True)
+    t = Renderer.from_scene(meshes, [], tgt, base)
+    t.build_photon_map()
+    target = torch.from_numpy(t.render_view(cam, 4, 7)["raw"][..., :3].copy()).cuda()
+    t.close()
+    r = Renderer.from_scene(meshes, [], ini, base)
+    r.build_photon_map()
+    p0 = _tensors(r, requires_grad=False)
+
+    def loss(p):
+        with torch.no_grad():
+            img = render_surfels(r, p, view=view)
+        return 0.5 * float(((img[..., :3] - target) ** 2).mean())
+
+    params = {k: v.clone().requires_grad_(True) for k, v in p0.items()}
+    img = render_surfels(r, params, view=view)
+    (0.5 * ((img[..., :3] - target) ** 2).mean()).backward()
+    g = {k: v.grad.cpu().numpy() for k, v in params.items()}
+    tu, tv = p0["tangent_u"][0].clone(), p0["tangent_v"][0].clone()
+
+    def moved(key, comp, dv):
+        q = {k: v.clone() for k, v in p0.items()}
+        if key == "rotation":
+            c, s = np.cos(dv), np.sin(dv)
+            q["tangent_u"][0] = c * tu + s * tv
+            q["tangent_v"][0] = -s * tu + c * tv
+        elif comp is None:
+            q[key][0] += dv
+        else:
+            q[key][0, comp] += dv
+        return q
+
+    cases = [("opacity", None, 5e-3, 0.05), ("beta", None, 1e-2, 0.05), ("position", 0, 2e-3, 0.05),
+             ("position", 2, 2e-3, 0.05), ("scale", 0, 2e-3, 0.05), ("scale", 1, 2e-3, 0.05),
+             ("albedo", 1, 1e-2, 0.05), ("rotation", None, 2e-3, 0.3)]
+    tu_n, tv_n = tu.cpu().numpy(), tv.cpu().numpy()
+    for key, comp, h, tol in cases:
+        fd = (loss(moved(key, comp, h)) - loss(moved(key, comp, -h))) / (2 * h)
+        if key == "rotation":
+            an = float(g["tangent_u"][0] @ tv_n - g["tangent_v"][0] @ tu_n)
+        else:
+            an = float(g[key][0] if comp is None else g[key][0, comp])
+        print(key, comp, an, fd, flush=True)
+        assert np.sign(an) == np.sign(fd), (key, comp, an, fd)
+        assert abs(an - fd) <= tol * abs(fd), (key, comp, an, fd)
