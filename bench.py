@@ -424,6 +424,7 @@ def served_leg(a):
                                "pixels_differing": int(np.sum(np.any(x.raw != y.raw, axis=-1))),
                                "max_abs_pixel_diff": float(np.abs(x.raw - y.raw).max()),
                                "grad_max_err_over_max": gerr})
+    groups = srv.groups_served
     srv.close()
     dev_bytes_after = Renderer.device_bytes()
     out = {
@@ -447,6 +448,7 @@ def served_leg(a):
                                 "total_s": round(plain_s, 3)},
         "queries": served["queries"],
         "same_results_alone_and_mixed": bool(same_images and same_grads),
+        "groups_served": groups,
         "same_images": bool(same_images),
         "same_gradients": bool(same_grads),
         "mismatches": mismatches[:8],

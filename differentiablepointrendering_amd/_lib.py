@@ -26,7 +26,7 @@ EXPORTED = (
     "pale_set_point_opacity", "pale_set_point_beta", "pale_trace_rays", "pale_gather_irradiance", "pale_get_surfel_bvh", "pale_get_photons",
     "pale_get_stats", "pale_set_count_work", "pale_render_forward_device", "pale_render_backward_device",
     "pale_set_opacity_device", "pale_build_photon_map", "pale_render_view", "pale_render_view_device",
-    "pale_render_view_backward", "pale_device_bytes", "pale_view_bytes", "pale_view_buffer_bytes",
+    "pale_render_view_backward", "pale_device_bytes", "pale_view_bytes", "pale_views_bytes", "pale_view_buffer_bytes",
     "pale_release_view_buffers", "pale_set_batch_tuning", "pale_get_batch_tuning", "pale_render_views",
     "pale_render_views_backward", "pale_render_view_backward_device",
 )
@@ -74,6 +74,7 @@ def load(path: str = LIB_PATH):
     L.pale_view_bytes.argtypes = [H, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int32, C.POINTER(C.c_uint64)]
     L.pale_release_view_buffers.argtypes = [H]
     L.pale_view_buffer_bytes.argtypes = [H, C.POINTER(C.c_uint64)]
+    L.pale_views_bytes.argtypes = [H, C.c_uint32, _u32p, _u32p, _u32p, _i32p, C.POINTER(C.c_uint64)]
     L.pale_set_batch_tuning.argtypes = [H, C.POINTER(_abi.BatchTuning)]
     L.pale_get_batch_tuning.argtypes = [H, C.POINTER(_abi.BatchTuning)]
     L.pale_render_views.argtypes = [H, C.c_uint32, C.POINTER(_abi.CameraDesc), _u32p, _u32p, C.POINTER(_fp),

@@ -430,15 +430,6 @@ struct pale_handle {
         sizeAdjoint(npx, a);
     }
 
-    // Passes of one view per batch (whole passes; a pass larger than the
-    // batch size is a batch of its own), as planForward / renderViewsBackward
-    // cut them.
-    uint32_t passesPerBatch(uint32_t npx, uint32_t spp) const {
-        const uint64_t budget = tune.slots_per_batch ? tune.slots_per_batch : (4u << 20);
-        const uint64_t k = npx >= budget ? 1u : budget / npx;
-        return (uint32_t)std::min<uint64_t>(std::max<uint64_t>(k, 1u), spp);
-    }
-
     // Device bytes a W x H view of spp passes (with its adjoint) would add:
     // the buffer sizing of renderView / renderViewBackward, run dry.
     uint64_t viewGrowth(uint32_t W, uint32_t H, uint32_t spp, bool adjoint) {
@@ -456,24 +447,24 @@ struct pale_handle {
             }
             return p.add;
         }
-        const uint32_t k = passesPerBatch(npx, std::max(spp, 1u));
-        const uint32_t slots = k * npx, tiles = k * ((W + 7u) / 8u) * ((H + 7u) / 8u);
-        sizeCamera(std::max(slots, 1u), p, tune.entries_per_slot);
-        p(cursors, sizeof(uint32_t));
-        p(bTiles, (size_t)std::max(tiles, 1u) * sizeof(uint2));
-        p(bSegs, (size_t)k * sizeof(BatchSeg));
-        p(bRuns, sizeof(BatchRun));
-        p(bHdr, sizeof(BatchHeader));
-        p(bCams, sizeof(DevCamera));
-        p(viewsFb, (size_t)std::max(npx, 1u) * sizeof(float4));
-        p(viewsLdr, (size_t)std::max(npx, 1u) * sizeof(float4));
-        if (adjoint) {
-            sizeAdjoint(std::max(slots, 1u), p);
-            p(bTiles, (size_t)tiles * sizeof(uint2));
-            p(bSegs, (size_t)k * sizeof(BatchSeg));
-            p(bSegFirst, (size_t)k * sizeof(uint32_t));
-            p(dloss, (size_t)std::max(npx, 1u) * 3 * sizeof(float));
-            if (S.full_gradients) p(gradFull, std::max<size_t>(1, bp.points.size() * 16) * sizeof(float));
+        HostCamera c{};
+        c.W = W;
+        c.H = H;
+        return viewsGrowth({c}, {std::max(spp, 1u)}, {adjoint});
+    }
+
+    // Device bytes a render_views of these views (then, for those marked,
+    // each one's adjoint) would add: the batch cutting and buffer sizing of
+    // the real calls, run dry.
+    uint64_t viewsGrowth(const std::vector<HostCamera>& vc, const std::vector<uint32_t>& spp,
+                         const std::vector<bool>& adjoint) {
+        Probe p;
+        const ForwardPlan fp = cutForward(vc, spp, [](uint32_t, uint32_t) { return 0u; });
+        sizeForward(fp, vc.size(), p);
+        for (size_t v = 0; v < vc.size(); ++v) {
+            if (!adjoint[v]) continue;
+            const BackwardPlan bpl = cutBackward({vc[v]}, {spp[v]});
+            sizeBackward(bpl, 1, p);
         }
         return p.add;
     }
@@ -864,11 +855,46 @@ struct pale_handle {
         std::vector<HostBatch> batches;
         std::vector<size_t> fbOff;
         uint32_t maxTiles = 0, maxSlots = 0, rounds = 0;
+        size_t maxSegs = 1, maxRuns = 1;
         float fraction = 1.0f;
     };
 
+    // Buffers of a forward plan; A = Allocate, or Probe for the dry run of
+    // admission (the same calls, so the two cannot disagree).
+    template <typename A>
+    void sizeForward(const ForwardPlan& plan, size_t nViews, A& a) {
+        // Entries per slot: the tuning's, else the settings' per-pixel budget.
+        sizeCamera(std::max(plan.maxSlots, 1u), a, tune.entries_per_slot);
+        a(cursors, sizeof(uint32_t));
+        a(bTiles, (size_t)std::max(plan.maxTiles, 1u) * sizeof(uint2));
+        a(bSegs, plan.maxSegs * sizeof(BatchSeg));
+        a(bRuns, plan.maxRuns * sizeof(BatchRun));
+        a(bHdr, sizeof(BatchHeader));
+        a(bCams, std::max<size_t>(nViews, 1) * sizeof(DevCamera));
+        a(viewsFb, std::max<size_t>(plan.fbOff.back(), 1) * sizeof(float4));
+        a(viewsLdr, std::max<size_t>(plan.fbOff.back(), 1) * sizeof(float4));
+    }
+
     template <typename SeedFn>
     ForwardPlan planForward(const std::vector<HostCamera>& vc, const std::vector<uint32_t>& spp, SeedFn seedOf) {
+        ForwardPlan plan = cutForward(vc, spp, seedOf);
+        Allocate a;
+        sizeForward(plan, vc.size(), a);
+        ensureCamera(std::max(plan.maxSlots, 1u), tune.entries_per_slot);  // and the pinned round flag
+        const uint32_t n = (uint32_t)vc.size();
+        std::vector<DevCamera> dcs(n);
+        for (uint32_t v = 0; v < n; ++v) dcs[v] = devCam(vc[v]);
+        upload(stream, bCams, dcs);
+        HIPCHECK(hipStreamSynchronize(stream));
+        if (plan.rounds) ensureBatchGraph(plan.maxTiles, plan.maxSlots, plan.rounds, plan.fraction);
+        return plan;
+    }
+
+    // Segments (view, pass) in pass-major order cut into batches of at most
+    // slots_per_batch pixel slots (a pass larger than that is a batch of its
+    // own), with their descriptors.
+    template <typename SeedFn>
+    ForwardPlan cutForward(const std::vector<HostCamera>& vc, const std::vector<uint32_t>& spp, SeedFn seedOf) const {
         ForwardPlan plan;
         const uint32_t n = (uint32_t)vc.size();
         plan.fbOff.assign(n + 1, 0);
@@ -903,7 +929,6 @@ struct pale_handle {
                 cur += npx;
             }
         }
-        size_t maxSegs = 1, maxRuns = 1;
         for (HostBatch& b : batches) {
             // One run per view present, its segments in pass order (in
             // pass-major order a view's segments of a batch are not adjacent).
@@ -937,25 +962,9 @@ struct pale_handle {
                                 resolve, {0, 0, 0}};
             plan.maxTiles = std::max(plan.maxTiles, b.hdr.nTiles);
             plan.maxSlots = std::max(plan.maxSlots, slots);
-            maxSegs = std::max(maxSegs, b.segs.size());
-            maxRuns = std::max(maxRuns, b.runs.size());
+            plan.maxSegs = std::max(plan.maxSegs, b.segs.size());
+            plan.maxRuns = std::max(plan.maxRuns, b.runs.size());
         }
-        // Buffers (all before any capture).
-        // Entries per slot: the tuning's, else the settings' per-pixel budget.
-        ensureCamera(std::max(plan.maxSlots, 1u), tune.entries_per_slot);
-        cursors.reserve(sizeof(uint32_t));
-        bTiles.reserve((size_t)std::max(plan.maxTiles, 1u) * sizeof(uint2));
-        bSegs.reserve(maxSegs * sizeof(BatchSeg));
-        bRuns.reserve(maxRuns * sizeof(BatchRun));
-        bHdr.reserve(sizeof(BatchHeader));
-        bCams.reserve(std::max<size_t>(n, 1) * sizeof(DevCamera));
-        viewsFb.reserve(std::max<size_t>(plan.fbOff[n], 1) * sizeof(float4));
-        viewsLdr.reserve(std::max<size_t>(plan.fbOff[n], 1) * sizeof(float4));
-        std::vector<DevCamera> dcs(n);
-        for (uint32_t v = 0; v < n; ++v) dcs[v] = devCam(vc[v]);
-        upload(stream, bCams, dcs);
-        HIPCHECK(hipStreamSynchronize(stream));
-        if (plan.rounds) ensureBatchGraph(plan.maxTiles, plan.maxSlots, plan.rounds, plan.fraction);
         return plan;
     }
 
@@ -1090,26 +1099,45 @@ struct pale_handle {
         readStats(false);
     }
 
-    // Adjoint of several views (gradients summed): (view, sample) segments in
-    // sample-major order cut into batches like the forward's; each batch is
-    // one adjoint wavefront over all its rays.  A ray seeds its draws with
-    // its index among its segment's rays and its events weigh its view's
-    // 1/spp, so each view contributes what renderBackwardCams gives it.
-    void renderViewsBackward(const std::vector<HostCamera>& vc, const std::vector<uint32_t>& spp,
-                             const std::vector<const float*>& src, bool srcOnDevice) {
-        if (!mapValid) throw StatusError(PALE_ERR_INVALID_ARG, "render_views_backward: no photon map");
+    struct BackwardPlan {
+        std::vector<HostBatch> batches;
+        std::vector<size_t> pixOff, off;  // pixel and dL/dI float offsets of the views
+        uint32_t maxTiles = 0, maxSlots = 1;
+        size_t maxSegs = 1;
+    };
+
+    // Buffers of a backward plan (A = Allocate or Probe, as sizeForward).
+    template <typename A>
+    void sizeBackward(const BackwardPlan& plan, size_t nViews, A& a) {
+        sizeAdjoint(plan.maxSlots, a);
+        a(bTiles, (size_t)plan.maxTiles * sizeof(uint2));
+        a(bSegs, plan.maxSegs * sizeof(BatchSeg));
+        a(bRuns, sizeof(BatchRun));
+        a(bHdr, sizeof(BatchHeader));
+        a(bSegFirst, plan.maxSegs * sizeof(uint32_t));
+        a(bCams, std::max<size_t>(nViews, 1) * sizeof(DevCamera));
+        a(dloss, std::max<size_t>(plan.off.back(), 1) * sizeof(float));
+        if (S.full_gradients) a(gradFull, std::max<size_t>(1, bp.points.size() * 16) * sizeof(float));
+    }
+
+    // (view, sample) segments in sample-major order cut into batches like
+    // the forward's.
+    BackwardPlan cutBackward(const std::vector<HostCamera>& vc, const std::vector<uint32_t>& spp) const {
+        BackwardPlan plan;
         const uint32_t n = (uint32_t)vc.size();
-        std::vector<size_t> pixOff(n + 1, 0), off(n + 1, 0);
+        std::vector<size_t>& pixOff = plan.pixOff;
+        pixOff.assign(n + 1, 0);
+        plan.off.assign(n + 1, 0);
         uint32_t maxSpp = 0;
         for (uint32_t v = 0; v < n; ++v) {
             if (spp[v] == 0) throw StatusError(PALE_ERR_INVALID_ARG, "render_views_backward: spp must be >= 1");
             pixOff[v + 1] = pixOff[v] + (size_t)vc[v].W * vc[v].H;
-            off[v + 1] = pixOff[v + 1] * 3;
+            plan.off[v + 1] = pixOff[v + 1] * 3;
             maxSpp = std::max(maxSpp, spp[v]);
         }
         if (pixOff[n] > 0x7FFFFFFFull) throw StatusError(PALE_ERR_INVALID_ARG, "render_views_backward: too many pixels");
         const uint64_t budget = tune.slots_per_batch ? tune.slots_per_batch : (4u << 20);
-        std::vector<HostBatch> batches;
+        std::vector<HostBatch>& batches = plan.batches;
         uint64_t cur = budget + 1;
         for (uint32_t p = 0; p < maxSpp; ++p) {
             for (uint32_t v = 0; v < n; ++v) {
@@ -1129,27 +1157,33 @@ struct pale_handle {
                 cur += npx;
             }
         }
-        uint32_t maxTiles = 0, maxSlots = 1;
-        size_t maxSegs = 1;
         for (HostBatch& b : batches) {
             uint32_t slots = 0;
             for (const BatchSeg& s : b.segs) slots += s.W * s.H;
             b.hdr = BatchHeader{(uint32_t)b.tiles.size(), slots, (uint32_t)b.segs.size(), 0u, 0u, {0, 0, 0}};
-            maxTiles = std::max(maxTiles, b.hdr.nTiles);
-            maxSlots = std::max(maxSlots, slots);
-            maxSegs = std::max(maxSegs, b.segs.size());
+            plan.maxTiles = std::max(plan.maxTiles, b.hdr.nTiles);
+            plan.maxSlots = std::max(plan.maxSlots, slots);
+            plan.maxSegs = std::max(plan.maxSegs, b.segs.size());
         }
-        ensureAdjoint(maxSlots);
-        bTiles.reserve((size_t)maxTiles * sizeof(uint2));
-        bSegs.reserve(maxSegs * sizeof(BatchSeg));
-        bRuns.reserve(sizeof(BatchRun));
-        bHdr.reserve(sizeof(BatchHeader));
-        bSegFirst.reserve(maxSegs * sizeof(uint32_t));
-        bCams.reserve(std::max<size_t>(n, 1) * sizeof(DevCamera));
+        return plan;
+    }
+
+    // Adjoint of several views (gradients summed): each batch of the plan is
+    // one adjoint wavefront over all its rays.  A ray seeds its draws with
+    // its index among its segment's rays and its events weigh its view's
+    // 1/spp, so each view contributes what renderBackwardCams gives it.
+    void renderViewsBackward(const std::vector<HostCamera>& vc, const std::vector<uint32_t>& spp,
+                             const std::vector<const float*>& src, bool srcOnDevice) {
+        if (!mapValid) throw StatusError(PALE_ERR_INVALID_ARG, "render_views_backward: no photon map");
+        BackwardPlan plan = cutBackward(vc, spp);
+        std::vector<HostBatch>& batches = plan.batches;
+        const uint32_t n = (uint32_t)vc.size();
+        Allocate a;
+        sizeBackward(plan, n, a);
         std::vector<DevCamera> dcs(n);
         for (uint32_t v = 0; v < n; ++v) dcs[v] = devCam(vc[v]);
         upload(stream, bCams, dcs);
-        beginBackward(off, src, srcOnDevice);
+        beginBackward(plan.off, src, srcOnDevice);
         DevState* st = this->st();
         uint32_t* qCount = stField(offsetof(DevState, qCount));
         uint32_t* extTotal = stField(offsetof(DevState, extTotal));
@@ -1657,6 +1691,25 @@ int pale_device_bytes(uint64_t* out) {
 int pale_view_bytes(pale_handle* h, uint32_t width, uint32_t height, uint32_t spp, int32_t adjoint, uint64_t* out) {
     if (!h || !out) return fail(PALE_ERR_INVALID_ARG, "null argument");
     return guarded([&] { *out = h->viewGrowth(width, height, spp, adjoint != 0); });
+}
+
+int pale_views_bytes(pale_handle* h, uint32_t n, const uint32_t* widths, const uint32_t* heights,
+                     const uint32_t* spp, const int32_t* adjoint, uint64_t* out) {
+    if (!h || !out || (n && (!widths || !heights || !spp || !adjoint)))
+        return fail(PALE_ERR_INVALID_ARG, "null argument");
+    return guarded([&] {
+        if (!h->S.batch_passes) throw StatusError(PALE_ERR_INVALID_ARG, "views_bytes: needs batch_passes = 1");
+        std::vector<HostCamera> vc(n);
+        std::vector<uint32_t> s(n);
+        std::vector<bool> adj(n);
+        for (uint32_t i = 0; i < n; ++i) {
+            vc[i].W = widths[i];
+            vc[i].H = heights[i];
+            s[i] = std::max(spp[i], 1u);
+            adj[i] = adjoint[i] != 0;
+        }
+        *out = h->viewsGrowth(vc, s, adj);
+    });
 }
 
 int pale_view_buffer_bytes(pale_handle* h, uint64_t* out) {

@@ -241,6 +241,20 @@ class Renderer:
                                       C.byref(v)))
         return v.value
 
+    def views_bytes(self, sizes, spp, adjoint) -> int:
+        """Device bytes render_views of views of `sizes` ((w, h) each) with
+        `spp`, then render_view_backward of those with adjoint[i], would add."""
+        n = len(sizes)
+        w = np.ascontiguousarray([s[0] for s in sizes], dtype=np.uint32)
+        h = np.ascontiguousarray([s[1] for s in sizes], dtype=np.uint32)
+        sp = np.ascontiguousarray(spp, dtype=np.uint32)
+        ad = np.ascontiguousarray([1 if x else 0 for x in adjoint], dtype=np.int32)
+        v = C.c_uint64()
+        u32 = C.POINTER(C.c_uint32)
+        check(self._L.pale_views_bytes(self._h, n, w.ctypes.data_as(u32), h.ctypes.data_as(u32),
+                                       sp.ctypes.data_as(u32), ad.ctypes.data_as(C.POINTER(C.c_int32)), C.byref(v)))
+        return v.value
+
     def view_buffer_bytes(self) -> int:
         """Device bytes of the image-sized buffers (what release_view_buffers frees)."""
         v = C.c_uint64()

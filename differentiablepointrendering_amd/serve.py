@@ -9,7 +9,10 @@ its camera gather (PrimalKernels.cpp:500-653) and adjoint pass
 
 Scheduling is continuous: a request is started as soon as its scene has an
 idle replica and its device memory fits the HBM budget, whatever else is
-running; it never waits for a batch to fill.  Every replica is a renderer
+running; it never waits for a batch to fill.  Small requests of one scene
+that are waiting when a replica frees up go with it as a group: their
+forward passes are one batched render_views call (its images equal the
+single views' bit for bit), their adjoints follow one by one.  Every replica is a renderer
 with its own HIP stream and its own host thread, so the work of views of
 different scenes (and of several replicas of one scene) overlaps on the
 device.  A view's image depends only on (scene, settings, camera, spp,
@@ -117,34 +120,49 @@ class _Replica:
                 job = self.inbox.popleft()
             if job is None:
                 return
-            req, fut, reserve = job
+            group, reserve = job
             try:
-                res = self._serve(req)
-                ok = True
-            except BaseException as e:  # reported through the future
-                res, ok = e, False
-            self.server._finished(self, reserve)
-            if ok:
-                fut.set_result(res)
-            else:
-                fut.set_exception(res)
+                results = self._serve(group)
+                errors = None
+            except BaseException as e:  # reported through the futures
+                results, errors = None, e
+            self.server._finished(self, reserve, len(group))
+            for k, (req, fut) in enumerate(group):
+                if errors is None:
+                    fut.set_result(results[k])
+                else:
+                    fut.set_exception(errors)
 
-    def _serve(self, req: ViewRequest) -> ViewResult:
+    def _serve(self, group) -> List[ViewResult]:
+        """The forward passes of every request of the group in one batched
+        render_views call (a group of one: render_view), then each adjoint.
+        The group's forward queries and device time are booked on its first
+        result."""
         r = self.renderer
+        reqs = [g[0] for g in group]
         started = time.perf_counter()
-        img = r.render_view(req.camera, req.spp, req.seed)
+        if len(reqs) == 1:
+            imgs = [r.render_view(reqs[0].camera, reqs[0].spp, reqs[0].seed)]
+        else:
+            imgs = r.render_views([q.camera for q in reqs], [q.spp for q in reqs], [q.seed for q in reqs])
         st = r.stats()
-        queries = int(st["gather_queries"])
-        device_ms = float(st["forward_ms"])
-        grads = loss = None
-        if req.target is not None:
-            loss, dl = l2_adjoint(img["raw"][..., :3], req.target)
-            grads = r.render_view_backward(req.camera, req.spp, dl)
-            st = r.stats()
-            queries += int(st["adjoint_queries"])
-            device_ms += float(st["backward_ms"])
-        return ViewResult(req, img["ldr"], img["raw"], grads, loss, queries, device_ms, started,
-                          time.perf_counter(), self.index)
+        fwd_q, fwd_ms = int(st["gather_queries"]), float(st["forward_ms"])
+        done = time.perf_counter()
+        out = []
+        for k, (req, img) in enumerate(zip(reqs, imgs)):
+            queries, device_ms = (fwd_q, fwd_ms) if k == 0 else (0, 0.0)
+            grads = loss = None
+            finished = done
+            if req.target is not None:
+                loss, dl = l2_adjoint(img["raw"][..., :3], req.target)
+                grads = r.render_view_backward(req.camera, req.spp, dl)
+                st = r.stats()
+                queries += int(st["adjoint_queries"])
+                device_ms += float(st["backward_ms"])
+                finished = time.perf_counter()
+            out.append(ViewResult(req, img["ldr"], img["raw"], grads, loss, queries, device_ms, started, finished,
+                                  self.index))
+        return out
 
 
 @dataclass
@@ -165,7 +183,8 @@ class SceneServer:
 
     def __init__(self, hbm_budget_bytes: Optional[int] = None,
                  renderer_factory: Optional[Callable] = None,
-                 device_bytes: Optional[Callable[[], int]] = None):
+                 device_bytes: Optional[Callable[[], int]] = None,
+                 max_group: int = 8, group_slots: int = 4 << 20):
         if renderer_factory is None or device_bytes is None:
             from .pale import Renderer
 
@@ -181,6 +200,12 @@ class SceneServer:
         self._reserved = 0
         self._running = 0
         self._closed = False
+        # Small requests of one scene that wait together are served by one
+        # batched call: at most max_group requests of at most group_slots
+        # pixel samples (width x height x spp) each and in all.
+        self.max_group = max(1, int(max_group))
+        self.group_slots = int(group_slots)
+        self.groups_served = 0
 
     # ------------------------------------------------------------- scenes --
     def add_scene(self, name: str, meshes, points: dict, settings: dict, replicas: int = 1) -> int:
@@ -252,14 +277,39 @@ class SceneServer:
     def render(self, req: ViewRequest) -> ViewResult:
         return self.submit(req).result()
 
-    def _finished(self, rep: _Replica, reserve: int) -> None:
+    def _finished(self, rep: _Replica, reserve: int, count: int = 1) -> None:
         with self._lock:
             rep.busy = False
             rep.last_used = time.perf_counter()
             rep.has_view_buffers = True
             self._reserved -= reserve
-            self._running -= 1
+            self._running -= count
             self._schedule()
+
+    @staticmethod
+    def _slots(req: ViewRequest) -> int:
+        return req.camera.width * req.camera.height * req.spp
+
+    def _companions(self, head: ViewRequest, item) -> list:
+        """Younger waiting requests of head's scene to serve with it."""
+        if self.max_group <= 1 or self._slots(head) > self.group_slots:
+            return []
+        out, total, after = [], self._slots(head), False
+        for other in self._pending:
+            if other is item:
+                after = True
+                continue
+            if not after or len(out) + 1 >= self.max_group:
+                continue
+            req, fut = other
+            if req.scene != head.scene or fut.cancelled():
+                continue
+            s = self._slots(req)
+            if s > self.group_slots or total + s > self.group_slots:
+                continue
+            out.append(other)
+            total += s
+        return out
 
     def _make_room(self, keep: _Replica, need: int) -> bool:
         """True if ``need`` more bytes fit the budget, after releasing image
@@ -302,11 +352,22 @@ class SceneServer:
                         f"device bytes than the HBM budget ({self.budget}) leaves"))
                     continue
                 return  # the oldest request blocked on memory holds back younger ones
-            self._pending.remove(item)
+            group = [item]
+            companions = self._companions(req, item)
+            if companions:
+                cand = group + companions
+                gneed = rep.renderer.views_bytes([(q.camera.width, q.camera.height) for q, _ in cand],
+                                                 [q.spp for q, _ in cand], [q.adjoint for q, _ in cand])
+                if self.budget is None or self._make_room(rep, gneed):
+                    group, need = cand, gneed
+            for g in group:
+                self._pending.remove(g)
+            if len(group) > 1:
+                self.groups_served += 1
             rep.busy = True
             self._reserved += need
-            self._running += 1
-            rep.inbox.append((req, fut, need))
+            self._running += len(group)
+            rep.inbox.append((group, need))
             rep.wake.notify()
 
     # -------------------------------------------------------------- close --

@@ -301,6 +301,13 @@ int pale_device_bytes(uint64_t* out);
 int pale_view_bytes(pale_handle* h, uint32_t width, uint32_t height,
                     uint32_t spp, int32_t adjoint, uint64_t* out);
 
+/* Device bytes that pale_render_views of n views (then pale_render_view_
+ * backward of each view with adjoint[i] != 0) would add; needs batch_passes
+ * = 1 (render_views always batches; single views then do too). */
+int pale_views_bytes(pale_handle* h, uint32_t n, const uint32_t* widths,
+                     const uint32_t* heights, const uint32_t* spp,
+                     const int32_t* adjoint, uint64_t* out);
+
 /* Device bytes of this renderer's image-sized buffers (camera gather, view
  * images, adjoint): what pale_release_view_buffers frees. */
 int pale_view_buffer_bytes(pale_handle* h, uint64_t* out);

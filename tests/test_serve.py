@@ -42,6 +42,7 @@ class StandIn:
         self.adj = 0
         self.closed = False
         self.last = {}
+        self.group_calls = []
         mem.add(1000 * n)
 
     def build_photon_map(self):
@@ -81,6 +82,25 @@ class StandIn:
         self.last.update(adjoint_queries=cam.width * cam.height * spp, backward_ms=self.delay * 1e3)
         return {"opacity": np.full(self.n, float(dl.sum()), np.float32)}
 
+    def render_views(self, cams, spp, seeds):
+        self.group_calls.append(len(cams))
+        total = sum(c.width * c.height for c in cams)
+        grow = max(self.view, 10 * total) - self.view
+        self.mem.add(grow)
+        self.view += grow
+        out = []
+        for c, s, sd in zip(cams, spp, seeds):
+            out.append(self.render_view(c, s, sd))
+        self.last = dict(gather_queries=sum(c.width * c.height * s for c, s in zip(cams, spp)),
+                         forward_ms=self.delay * 1e3)
+        return out
+
+    def views_bytes(self, sizes, spp, adjoint):
+        # A group's views share one batch: its buffers hold all their pixels.
+        v = max(self.view, 10 * sum(w * h for w, h in sizes))
+        a = max([self.adj] + [20 * w * h for (w, h), ad in zip(sizes, adjoint) if ad])
+        return (v - self.view) + (a - self.adj)
+
     def stats(self):
         return dict(self.last)
 
@@ -98,7 +118,7 @@ class StandIn:
             self.closed = True
 
 
-def make_server(budget=None, delay=0.01):
+def make_server(budget=None, delay=0.01, **kw):
     mem = Memory()
     made = []
 
@@ -107,7 +127,7 @@ def make_server(budget=None, delay=0.01):
         made.append(r)
         return r
 
-    return SceneServer(budget, renderer_factory=factory, device_bytes=mem), mem, made
+    return SceneServer(budget, renderer_factory=factory, device_bytes=mem, **kw), mem, made
 
 
 def pts(n):
@@ -238,3 +258,39 @@ def test_latency_summary():
         s = latency_summary(res, time.perf_counter() - t0)
     assert s["requests"] == 20 and s["p50_ms"] <= s["p99_ms"]
     assert s["queries"] == 20 * 16
+
+
+def test_waiting_small_requests_of_a_scene_are_served_as_a_group():
+    srv, mem, made = make_server(delay=0.05, max_group=4, group_slots=10_000)
+    with srv:
+        srv.add_scene("a", [], pts(10), {})
+        first = srv.submit(ViewRequest("a", cam(20, 20), 1, 0))  # runs alone
+        time.sleep(0.01)
+        small = [srv.submit(ViewRequest("a", cam(10 + i, 10), 2, 10 + i,
+                                        target=np.zeros((10, 10 + i, 3), np.float32) if i == 1 else None))
+                 for i in range(5)]
+        big = srv.submit(ViewRequest("a", cam(200, 100), 1, 99))  # too large to join a group
+        res = [f.result(timeout=10) for f in [first] + small + [big]]
+    assert made[0].group_calls == [4], made[0].group_calls   # 4 of the 5 small ones, then the fifth alone
+    assert srv.groups_served == 1
+    # Same results as alone.
+    for r in res:
+        want = np.float32((r.request.seed * 7919 + r.request.camera.width * 31 + r.request.spp) % 1000) / 1000
+        assert np.all(r.raw == want)
+    assert res[2].grads is not None and res[2].loss is not None
+    assert sum(r.queries for r in res) == sum(r.request.camera.width * r.request.camera.height * r.request.spp * (2 if r.request.target is not None else 1) for r in res)
+
+
+def test_group_is_admitted_by_its_memory():
+    srv, mem, made = make_server(budget=100_000, delay=0.05, max_group=4, group_slots=20_000)
+    with srv:
+        srv.add_scene("a", [], pts(10), {})                        # 15,000
+        first = srv.submit(ViewRequest("a", cam(50, 50), 1, 0))    # +25,000 of view buffers
+        time.sleep(0.01)
+        # One 80x80 view: 64,000 of buffers (79,000 in all) fits; the two
+        # together would hold 128,000: the head goes alone, then the other.
+        fs = [srv.submit(ViewRequest("a", cam(80, 80), 1, i)) for i in range(1, 3)]
+        for f in [first] + fs:
+            f.result(timeout=10)
+    assert made[0].group_calls == [], made[0].group_calls
+    assert mem() <= 100_000
