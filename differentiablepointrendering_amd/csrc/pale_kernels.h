@@ -10,6 +10,11 @@ namespace pale {
 
 constexpr uint32_t kTile = 1024;       // compaction / scan tile (256 threads x 4)
 constexpr uint32_t kTraceBlock = 128;  // threads per traversal workgroup
+// The layer walk's tiles are split into this many contiguous regions, one
+// cursor each; a workgroup starts in region blockIdx.x % kTraceRegions (its
+// XCD under round-robin dispatch) so each XCD's L2 serves one part of the
+// image, and moves on when its region is used up.
+constexpr uint32_t kTraceRegions = 8;
 
 // Device-side progress of one render call.  Counts live on the device so the
 // whole forward or backward pass runs without host round trips.

@@ -640,6 +640,8 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(4))
     const LaneStack stk = laneStack(s_stack, S.stackDepth);
     const uint32_t lane = laneId();
     const uint32_t slots = map.slots();
+    const uint32_t tiles = slots >> 6;
+    uint32_t region = blockIdx.x % kTraceRegions, emptyRegions = 0;
     uint32_t layers = 0;
     Work w{};
     bool active = false, exhausted = false;
@@ -650,17 +652,26 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(4))
         if (!exhausted) {
             const unsigned long long need = __ballot(!active);
             if (need) {
+                // Slots [lo, hi) of the current region; cursor[region] counts
+                // the slots drawn from it this round.
+                const uint32_t lo = (uint32_t)(((uint64_t)region * tiles) / kTraceRegions) * 64u;
+                const uint32_t hi = (uint32_t)(((uint64_t)(region + 1u) * tiles) / kTraceRegions) * 64u;
                 const uint32_t leader = (uint32_t)__ffsll((long long)need) - 1u;
                 uint32_t base = 0;
-                if (lane == leader) base = atomicAdd(cursor, (uint32_t)__popcll(need));
+                if (lane == leader) base = lo + atomicAdd(cursor + region, (uint32_t)__popcll(need));
                 base = __shfl(base, (int)leader, 64);
-                if (base + (uint32_t)__popcll(need) >= slots ||
-                    *reinterpret_cast<volatile const uint32_t*>(&st->overflow) != 0u)
+                const bool regionDone = base >= hi;
+                if (regionDone) {
+                    // Used up (it stays so): the next region, done after all.
+                    if (++emptyRegions == kTraceRegions) exhausted = true;
+                    region = (region + 1u) % kTraceRegions;
+                } else if (*reinterpret_cast<volatile const uint32_t*>(&st->overflow) != 0u) {
                     exhausted = true;
-                if (!active) {
+                }
+                if (!active && !regionDone) {
                     const uint32_t slot = base + (uint32_t)__popcll(need & ((1ull << lane) - 1ull));
                     uint32_t p = 0;
-                    if (slot < slots && map.locate(slot, p) && f2u(px.cs[p].w) == kPixActive) {
+                    if (slot < hi && map.locate(slot, p) && f2u(px.cs[p].w) == kPixActive) {
                         pixel = p;
                         active = true;
                         const float4 a = px.ot[p], b = px.dl[p];

@@ -437,7 +437,7 @@ struct pale_handle {
         const uint32_t npx = W * H;
         if (!S.batch_passes) {
             sizeCamera(npx, p);
-            p(cursors, sizeof(uint32_t));
+            p(cursors, kTraceRegions * sizeof(uint32_t));
             p(viewFb, (size_t)npx * sizeof(float4));
             p(viewLdr, (size_t)npx * sizeof(float4));
             if (adjoint) {
@@ -591,7 +591,7 @@ struct pale_handle {
         launchCamInit(stream, dc, st, pass, decorrelate, px);
         for (uint32_t round = 0;; ++round) {
             launchFill(stream, entries, 2, 0u);  // entries, overflow
-            launchFill(stream, cursors.as<uint32_t>(), 1, 0u);
+            launchFill(stream, cursors.as<uint32_t>(), kTraceRegions, 0u);
             const size_t tt = tBegin(kTCamTrace);
             launchCamTrace(stream, scene, st, npx, dc.W, px, ent, entryCap, cursors.as<uint32_t>(), countWork);
             tEnd(tt);
@@ -657,7 +657,7 @@ struct pale_handle {
         }
         DevState* st = this->st();
         const float invSpp = 1.0f / (float)S.gather_passes;
-        cursors.reserve(sizeof(uint32_t));
+        cursors.reserve(kTraceRegions * sizeof(uint32_t));
         for (size_t k = 0; k < camIdx.size(); ++k) {
             const HostCamera& c = cams[camIdx[k]];
             const uint32_t npx = c.W * c.H;
@@ -781,7 +781,7 @@ struct pale_handle {
         const EntryArrays ent = enta();
         const bool count = timed && countWork;
         launchFill(stream, stField(offsetof(DevState, entries)), 2, 0u);  // entries, overflow
-        launchFill(stream, cursors.as<uint32_t>(), 1, 0u);
+        launchFill(stream, cursors.as<uint32_t>(), kTraceRegions, 0u);
         const size_t tt = timed ? tBegin(kTCamTrace) : 0;
         launchCamTraceBatch(stream, scene, st(), batchDesc(), maxSlots, px, ent, entryCap, cursors.as<uint32_t>(),
                             fraction, count);
@@ -865,7 +865,7 @@ struct pale_handle {
     void sizeForward(const ForwardPlan& plan, size_t nViews, A& a) {
         // Entries per slot: the tuning's, else the settings' per-pixel budget.
         sizeCamera(std::max(plan.maxSlots, 1u), a, tune.entries_per_slot);
-        a(cursors, sizeof(uint32_t));
+        a(cursors, kTraceRegions * sizeof(uint32_t));
         a(bTiles, (size_t)std::max(plan.maxTiles, 1u) * sizeof(uint2));
         a(bSegs, plan.maxSegs * sizeof(BatchSeg));
         a(bRuns, plan.maxRuns * sizeof(BatchRun));
@@ -935,7 +935,6 @@ struct pale_handle {
             std::vector<std::vector<uint32_t>> byView(n);
             for (uint32_t s2 = 0; s2 < b.segs.size(); ++s2) byView[b.segs[s2].view].push_back(s2);
             std::vector<BatchSeg> ordered;
-            std::vector<uint32_t> remap(b.segs.size());
             uint32_t resolve = 0;
             for (uint32_t v = 0; v < n; ++v) {
                 if (byView[v].empty()) continue;
@@ -947,15 +946,23 @@ struct pale_handle {
                 r.fbOffset = (uint32_t)plan.fbOff[v];
                 r.resolveBase = resolve;
                 r.invSpp = 1.0f / (float)spp[v];
-                for (uint32_t s2 : byView[v]) {
-                    remap[s2] = (uint32_t)ordered.size();
-                    ordered.push_back(b.segs[s2]);
-                }
+                for (uint32_t s2 : byView[v]) ordered.push_back(b.segs[s2]);
                 resolve += r.npx;
                 b.runs.push_back(r);
             }
-            for (uint2& t : b.tiles) t.x = remap[t.x];
             b.segs.swap(ordered);
+            // Tiles view by view, then tile by tile (row-major), then pass by
+            // pass: a contiguous range of slots (a layer-walk region) covers
+            // a band of one image across its passes.
+            std::vector<uint2> tiles;
+            tiles.reserve(b.tiles.size());
+            for (const BatchRun& r : b.runs) {
+                const BatchSeg& s0 = b.segs[r.firstSeg];
+                const uint32_t nt = s0.tilesX * ((s0.H + 7u) / 8u);
+                for (uint32_t t = 0; t < nt; ++t)
+                    for (uint32_t k = 0; k < r.numSegs; ++k) tiles.push_back(make_uint2(r.firstSeg + k, t));
+            }
+            b.tiles.swap(tiles);
             uint32_t slots = 0;
             for (const BatchSeg& sg : b.segs) slots += sg.W * sg.H;
             b.hdr = BatchHeader{(uint32_t)b.tiles.size(), slots, (uint32_t)b.segs.size(), (uint32_t)b.runs.size(),
@@ -1041,7 +1048,7 @@ struct pale_handle {
         launchFillF(stream, viewFb.as<float>(), npx * 4, 0.0f);
         const DevCamera dc = devCam(c);
         const float invSpp = 1.0f / (float)spp;
-        cursors.reserve(sizeof(uint32_t));
+        cursors.reserve(kTraceRegions * sizeof(uint32_t));
         for (uint32_t pass = 0; pass < spp; ++pass) cameraPass(dc, pass, invSpp, viewFb.as<float4>(), 1);
         launchPostprocess(stream, viewFb.as<float4>(), viewLdr.as<float4>(), npx);
         finishForward();
