@@ -1180,7 +1180,9 @@ __global__ __launch_bounds__(256) void k_event_scatter(const DevState* st, uint3
 // Full-gradient contribution of one event: d(alpha_geom)/d(geometry) of
 // surfel `prim` for the recorded ray, times eta * w (w = the event's weight
 // sum_rgb(T * Lo) * invSpp with its sign).
-__device__ void addGeometryGradient(const DevScene& S, uint32_t prim, float4 ro, float4 rd, float w, float* full) {
+// c[] = the surfel's 16 accumulator columns (position 0-2, t_u 3-5, t_v 6-8,
+// scale 9-10, albedo 11-13, beta 15).
+__device__ void addGeometryGradient(const DevScene& S, uint32_t prim, float4 ro, float4 rd, float w, float c[16]) {
     const DevInstance& I = S.inst[S.cloudInst];
     float hv[4];
     mulv4(I.w2o, ro.x, ro.y, ro.z, 1.0f, hv);
@@ -1190,10 +1192,30 @@ __device__ void addGeometryGradient(const DevScene& S, uint32_t prim, float4 ro,
     float g[12];
     if (!alphaGradient(S.geo[prim], oo, od, g)) return;
     const float f = S.shade[prim].opacity * w;
-    float* out = full + (size_t)prim * 16;
 #pragma unroll
-    for (int k = 0; k < 11; ++k) atomicAdd(out + k, g[k] * f);
-    atomicAdd(out + 15, g[11] * f);
+    for (int k = 0; k < 11; ++k) c[k] += g[k] * f;
+    c[15] += g[11] * f;
+}
+
+// Sums v over each run of equal keys among consecutive lanes of the wave
+// into the run's first lane (a segmented suffix reduction by shuffles);
+// returns whether this lane heads its run.  Every lane of the wave calls it.
+template <int kN>
+__device__ __forceinline__ bool runReduce(uint32_t key, float (&v)[kN]) {
+    const uint32_t lane = laneId();
+    const uint32_t next = __shfl_down(key, 1, 64);
+    const uint32_t prev = __shfl_up(key, 1, 64);
+    const unsigned long long ends = __ballot(lane == 63u || next != key);
+    const uint32_t end = lane + (uint32_t)__ffsll((long long)(ends >> lane)) - 1u;
+#pragma unroll
+    for (uint32_t o = 1; o < 64; o <<= 1) {
+#pragma unroll
+        for (int k = 0; k < kN; ++k) {
+            const float t = __shfl_down(v[k], o, 64);
+            if (lane + o <= end) v[k] += t;
+        }
+    }
+    return lane == 0u || prev != key;
 }
 
 // launchAdjointTransportKernel (AdjointKernels.cpp:389-509), one lane per
@@ -1204,14 +1226,23 @@ __global__ __launch_bounds__(256) void k_adj_transport(DevScene S, DevGrid G, De
                                                        EventArrays ev, const uint32_t* sorted, float* grad,
                                                        float* full) {
     const uint32_t n = min(st->evPool, poolCap);
+    if (blockIdx.x * blockDim.x >= n) return;  // the grid covers the pool's capacity
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const bool active = i < n;
     Work w{};
+    // Contributions of this lane's event: opacity in c[0], the full
+    // gradient's columns in c[1 + column]; lanes of a wave that follow one
+    // another with the same target surfel (events are cell-sorted, so runs
+    // are common) add theirs up before one lane issues the atomics.
+    float c[17];
+#pragma unroll
+    for (int k = 0; k < 17; ++k) c[k] = 0.0f;
+    uint32_t target = 0xFFFFFFFFu;
     if (active) {
         const uint32_t j = sorted[i];
         const float4 e0 = ev.e0[j], e1 = ev.e1[j], e2 = ev.e2[j], e3 = ev.e3[j];
         const uint32_t kind = f2u(e0.w);
-        const uint32_t target = f2u(e1.w);
+        target = f2u(e1.w);
         const float invSpp = e3.w;
         const F3 T = ld3(e2);
         const float a = e2.w;
@@ -1224,12 +1255,11 @@ __global__ __launch_bounds__(256) void k_adj_transport(DevScene S, DevGrid G, De
             const F3 g = (a * T) * Lr;
             val = fsum(g) * invSpp;
             if (full) {
-                addGeometryGradient(S, target, ev.e4[j], ev.e5[j], fsum(T * Lr) * invSpp, full);
+                addGeometryGradient(S, target, ev.e4[j], ev.e5[j], fsum(T * Lr) * invSpp, c + 1);
                 const F3 ga = ((a * sh.opacity * sh.alpha_r * kInvPi) * (T * E)) * invSpp;
-                float* out = full + (size_t)target * 16;
-                atomicAdd(out + 11, ga.x);
-                atomicAdd(out + 12, ga.y);
-                atomicAdd(out + 13, ga.z);
+                c[1 + 11] += ga.x;
+                c[1 + 12] += ga.y;
+                c[1 + 13] += ga.z;
             }
         } else if (kind == kNullT) {
             F3 base;
@@ -1249,16 +1279,33 @@ __global__ __launch_bounds__(256) void k_adj_transport(DevScene S, DevGrid G, De
             const F3 Lo = Le + Lr;
             const F3 g = ((-a) * T) * Lo;
             val = fsum(g) * invSpp;
-            if (full) addGeometryGradient(S, target, ev.e4[j], ev.e5[j], -(fsum(T * Lo) * invSpp), full);
+            if (full) addGeometryGradient(S, target, ev.e4[j], ev.e5[j], -(fsum(T * Lo) * invSpp), c + 1);
         } else if (kind == kReflectS) {
             const DevMaterial m = S.mats[f2u(e3.x)];
             const F3 Lo = (E * mk3(m.r, m.g, m.b)) * kInvPi;
             const F3 p = divs(T, e3.y);
             const F3 g = (p * Lo) * a;
             val = fsum(g) * invSpp;
-            if (full) addGeometryGradient(S, target, ev.e4[j], ev.e5[j], fsum(p * Lo) * invSpp, full);
+            if (full) addGeometryGradient(S, target, ev.e4[j], ev.e5[j], fsum(p * Lo) * invSpp, c + 1);
         }
-        atomicAdd(&grad[target], val);
+        c[0] = val;
+    }
+    bool head;
+    if (full) {
+        head = runReduce(target, c);
+    } else {
+        float v[1] = {c[0]};
+        head = runReduce(target, v);
+        c[0] = v[0];
+    }
+    if (head && target != 0xFFFFFFFFu) {
+        atomicAdd(&grad[target], c[0]);
+        if (full) {
+            float* out = full + (size_t)target * 16;
+#pragma unroll
+            for (int k = 0; k < 16; ++k)
+                if (k != 14 && c[1 + k] != 0.0f) atomicAdd(out + k, c[1 + k]);
+        }
     }
     waveCount(active, &st->events);
     if (kCount) flushWork(w, st, kWorkAdjoint);
