@@ -137,6 +137,13 @@ struct EntryArrays {
 // carries its segment in T.w, weighs that segment's invSpp, and seeds with
 // its index among its segment's rays (queue index - segFirst[segment]), the
 // index it has when its view is traced alone.
+//
+// In a batch the lanes also take the rays in another order than the queue's:
+// the rays of one view's segments (its samples) interleave, lane t of the
+// view's range taking rank t / K of segment t % K (K = the view's segments in
+// the batch), so neighbouring lanes trace the same pixel of different
+// samples.  groups[g] = (first entry in groupSegs, K, first lane, rays per
+// segment at most) for the views of the batch; segEnd[s] ends segment s.
 struct TraceParams {
     uint32_t rrStart;
     float qNull, qReflect, qTransmit;
@@ -144,6 +151,10 @@ struct TraceParams {
     float invSpp;
     const BatchSeg* segs;
     const uint32_t* segFirst;
+    const uint32_t* segEnd;
+    const uint4* groups;
+    const uint32_t* groupSegs;
+    uint32_t nGroups;
 };
 
 // Photon-map reservoir (depositPhotonSurface past capacity): per queue item
@@ -224,7 +235,8 @@ void launchPostprocess(hipStream_t s, const float4* fb, float4* ldr, uint32_t n)
 // slot, T.w = segment); first queue index of every segment present.
 void launchAdjRaygenBatch(hipStream_t s, BatchDesc b, uint32_t maxTiles, const float* dloss, RayQueue q,
                           PendingArrays pend, DevState* st);
-void launchSegFirst(hipStream_t s, const DevState* st, uint32_t maxRays, RayQueue q, uint32_t* segFirst);
+void launchSegFirst(hipStream_t s, const DevState* st, uint32_t maxRays, RayQueue q, uint32_t* segFirst,
+                    uint32_t* segEnd, uint32_t nSegs, uint4* groups, uint32_t nGroups, const uint32_t* groupSegs);
 void launchAdjRaygen(hipStream_t s, const DevCamera& cam, const float* dloss, RayQueue q, PendingArrays pend,
                      DevState* st);
 void launchAdjTrace(hipStream_t s, const DevScene& S, DevState* st, uint32_t maxRays, uint32_t bounce, RayQueue in,

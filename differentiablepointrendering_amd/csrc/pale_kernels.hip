@@ -979,13 +979,41 @@ __global__ __launch_bounds__(256) void k_adj_raygen_batch(BatchDesc b, const flo
     pend.p0[slot] = make_float4(0.0f, 0.0f, 0.0f, u2f(kNone));
 }
 
-// First queue index of every segment present in a batch's queue (the queue
-// stays in slot order: raygen writes slot order, compaction is stable).
-__global__ __launch_bounds__(256) void k_seg_first(const DevState* st, RayQueue q, uint32_t* segFirst) {
+// First and end queue index of every segment present in a batch's queue
+// (the queue stays in slot order: raygen writes slot order, compaction is
+// stable); absent segments keep first = end = 0 from k_seg_clear.
+__global__ __launch_bounds__(256) void k_seg_clear(uint32_t* segFirst, uint32_t* segEnd, uint32_t nSegs) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= st->qCount) return;
+    if (i < nSegs) segFirst[i] = segEnd[i] = 0u;
+}
+
+__global__ __launch_bounds__(256) void k_seg_first(const DevState* st, RayQueue q, uint32_t* segFirst,
+                                                   uint32_t* segEnd) {
+    const uint32_t n = st->qCount;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
     const uint32_t seg = f2u(q.T[i].w);
     if (i == 0 || f2u(q.T[i - 1].w) != seg) segFirst[seg] = i;
+    if (i + 1 == n || f2u(q.T[i + 1].w) != seg) segEnd[seg] = i + 1;
+}
+
+// Lanes of each view group: K segments x the most rays of one of them.
+__global__ __launch_bounds__(64) void k_group_lanes(const uint32_t* segFirst, const uint32_t* segEnd, uint4* groups,
+                                                    uint32_t nGroups, const uint32_t* groupSegs) {
+    if (threadIdx.x != 0) return;
+    uint32_t base = 0;
+    for (uint32_t g = 0; g < nGroups; ++g) {
+        uint4 gr = groups[g];
+        uint32_t most = 0;
+        for (uint32_t k = 0; k < gr.y; ++k) {
+            const uint32_t s = groupSegs[gr.x + k];
+            most = max(most, segEnd[s] - segFirst[s]);
+        }
+        gr.z = base;
+        gr.w = most;
+        groups[g] = gr;
+        base += gr.y * most;
+    }
 }
 
 // launchAdjointIntersectKernel (AdjointKernels.cpp:87-386) for one queue.
@@ -996,9 +1024,31 @@ __global__ __launch_bounds__(kTraceBlock) void k_adj_trace(DevScene S, DevState*
                                                            uint32_t pendCap, TraceParams tp) {
     extern __shared__ uint2 s_stack[];
     const LaneStack stk = laneStack(s_stack, S.stackDepth);
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t n = st->qCount;
-    const bool active = i < n;
+    uint32_t i = t;
+    bool active = t < n;
+    if (tp.groups) {
+        // Lane t -> (view group, segment, rank) -> queue index.
+        active = false;
+        uint32_t lo = 0, hi = tp.nGroups;  // last group whose first lane <= t
+        while (hi - lo > 1u) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (tp.groups[mid].z <= t)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        const uint4 gr = tp.nGroups ? tp.groups[lo] : make_uint4(0u, 0u, 0u, 0u);
+        const uint32_t local = t - gr.z;
+        if (t >= gr.z && local < gr.y * gr.w) {
+            const uint32_t s = tp.groupSegs[gr.x + local % gr.y], r = local / gr.y;
+            if (r < tp.segEnd[s] - tp.segFirst[s]) {
+                i = tp.segFirst[s] + r;
+                active = true;
+            }
+        }
+    }
     Work w{};
     if (active) {
         uint8_t ef = 0, vf = 0;
@@ -1576,8 +1626,12 @@ void launchAdjRaygenBatch(hipStream_t s, BatchDesc b, uint32_t maxTiles, const f
                           PendingArrays pend, DevState* st) {
     k_adj_raygen_batch<<<blocks(max(maxTiles, 1u), 4), 256, 0, s>>>(b, dloss, q, pend, st);
 }
-void launchSegFirst(hipStream_t s, const DevState* st, uint32_t maxRays, RayQueue q, uint32_t* segFirst) {
-    if (maxRays) k_seg_first<<<blocks(maxRays, 256), 256, 0, s>>>(st, q, segFirst);
+void launchSegFirst(hipStream_t s, const DevState* st, uint32_t maxRays, RayQueue q, uint32_t* segFirst,
+                    uint32_t* segEnd, uint32_t nSegs, uint4* groups, uint32_t nGroups, const uint32_t* groupSegs) {
+    if (!maxRays) return;
+    k_seg_clear<<<blocks(max(nSegs, 1u), 256), 256, 0, s>>>(segFirst, segEnd, nSegs);
+    k_seg_first<<<blocks(maxRays, 256), 256, 0, s>>>(st, q, segFirst, segEnd);
+    k_group_lanes<<<1, 64, 0, s>>>(segFirst, segEnd, groups, nGroups, groupSegs);
 }
 void launchAdjRaygen(hipStream_t s, const DevCamera& cam, const float* dloss, RayQueue q, PendingArrays pend,
                      DevState* st) {

@@ -133,7 +133,7 @@ struct pale_handle {
     float4* outLdr = nullptr;
     // Batched views (pale_render_views): descriptors of the current batch,
     // the framebuffers of all views, the captured graph of a batch's rounds.
-    DevMem bTiles, bSegs, bRuns, bHdr, bCams, bSegFirst, viewsFb, viewsLdr;
+    DevMem bTiles, bSegs, bRuns, bHdr, bCams, bSegFirst, bSegEnd, bGroups, bGroupSegs, viewsFb, viewsLdr;
     pale_batch_tuning tune{};
     hipGraphExec_t bGraph = nullptr;
     std::vector<uint8_t> bGraphKey;
@@ -213,7 +213,7 @@ struct pale_handle {
 
     TraceParams tp(float invSpp = 1.0f, const BatchSeg* segs = nullptr, const uint32_t* segFirst = nullptr) const {
         return TraceParams{S.rr_start, S.q_null, S.q_reflect, S.q_transmit, S.full_gradients ? 1u : 0u,
-                           invSpp, segs, segFirst};
+                           invSpp, segs, segFirst, nullptr, nullptr, nullptr, 0u};
     }
 
     DevGrid devGrid() const {
@@ -1122,6 +1122,9 @@ struct pale_handle {
         a(bRuns, sizeof(BatchRun));
         a(bHdr, sizeof(BatchHeader));
         a(bSegFirst, plan.maxSegs * sizeof(uint32_t));
+        a(bSegEnd, plan.maxSegs * sizeof(uint32_t));
+        a(bGroups, std::max<size_t>(nViews, 1) * sizeof(uint4));
+        a(bGroupSegs, plan.maxSegs * sizeof(uint32_t));
         a(bCams, std::max<size_t>(nViews, 1) * sizeof(DevCamera));
         a(dloss, std::max<size_t>(plan.off.back(), 1) * sizeof(float));
         if (S.full_gradients) a(gradFull, std::max<size_t>(1, bp.points.size() * 16) * sizeof(float));
@@ -1195,7 +1198,10 @@ struct pale_handle {
         uint32_t* qCount = stField(offsetof(DevState, qCount));
         uint32_t* extTotal = stField(offsetof(DevState, extTotal));
         const BatchDesc bd = batchDesc();
-        const TraceParams tpb = tp(1.0f, bSegs.as<BatchSeg>(), bSegFirst.as<uint32_t>());
+        TraceParams tpb = tp(1.0f, bSegs.as<BatchSeg>(), bSegFirst.as<uint32_t>());
+        tpb.segEnd = bSegEnd.as<uint32_t>();
+        tpb.groups = bGroups.as<uint4>();
+        tpb.groupSegs = bGroupSegs.as<uint32_t>();
         const PendingArrays pend{pend0.as<float4>(), pend1.as<float4>(), pend2.as<uint2>(), pend3.as<float4>(),
                                  pend4.as<float4>()};
         uint64_t pooledBound = 0;
@@ -1205,6 +1211,21 @@ struct pale_handle {
             upload(stream, bTiles, b.tiles);
             upload(stream, bSegs, b.segs);
             HIPCHECK(hipMemcpyAsync(bHdr.p, &b.hdr, sizeof(BatchHeader), hipMemcpyHostToDevice, stream));
+            // The views' groups of segments (their samples in this batch),
+            // for the interleaved lanes of the adjoint trace.
+            std::vector<uint4> groups;
+            std::vector<uint32_t> groupSegs;
+            for (uint32_t v = 0; v < n; ++v) {
+                const uint32_t first = (uint32_t)groupSegs.size();
+                for (uint32_t s2 = 0; s2 < b.segs.size(); ++s2)
+                    if (b.segs[s2].view == v) groupSegs.push_back(s2);
+                if (groupSegs.size() > first)
+                    groups.push_back(make_uint4(first, (uint32_t)groupSegs.size() - first, 0u, 0u));
+            }
+            upload(stream, bGroups, groups);
+            upload(stream, bGroupSegs, groupSegs);
+            HIPCHECK(hipStreamSynchronize(stream));  // the host vectors go out of scope
+            tpb.nGroups = (uint32_t)groups.size();
             const uint32_t nr = b.hdr.nSlots;
             DevMem* in = qA;
             DevMem* out = qB;
@@ -1214,7 +1235,8 @@ struct pale_handle {
                     flushEvents();
                     pooledBound = 0;
                 }
-                launchSegFirst(stream, st, nr, rq(in), bSegFirst.as<uint32_t>());
+                launchSegFirst(stream, st, nr, rq(in), bSegFirst.as<uint32_t>(), bSegEnd.as<uint32_t>(), b.hdr.nSegs,
+                               bGroups.as<uint4>(), tpb.nGroups, bGroupSegs.as<uint32_t>());
                 const size_t ta = tBegin(kTAdjTrace);
                 launchAdjTrace(stream, scene, st, nr, k, rq(in), rq(stg), extFlags.as<uint8_t>(), ea(evStg),
                                evFlags.as<uint8_t>(), pend, tpb, countWork);
