@@ -110,7 +110,12 @@ struct BatchHeader {
     uint32_t nTiles, nSlots, nSegs, nRuns, nResolve, pad[3];
 };
 struct BatchDesc {
-    const uint2* tiles;  // (segment, tile within the segment's image)
+    // Units of 64 slots: (first segment, tile within the image, first unit
+    // of the group, K).  A group is the K units of one tile of one view's K
+    // segments of the batch; its 64 K slots take the tile's pixels K at a
+    // time, segment fastest (slot j of the group: segment first + j % K,
+    // pixel j / K of the 8x8 tile).
+    const uint4* tiles;
     const BatchSeg* segs;
     const BatchRun* runs;
     const BatchHeader* hdr;

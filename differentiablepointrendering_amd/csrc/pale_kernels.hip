@@ -617,19 +617,32 @@ struct ImageMap {
     }
 };
 
+// Segment and pixel of a slot of a batch (see BatchDesc::tiles); false if
+// the pixel lies outside the image (right and bottom tile edges).
+__device__ __forceinline__ bool batchSlot(const uint4* tiles, uint32_t slot, uint32_t& seg, uint32_t& pxx,
+                                          uint32_t& pxy, const BatchSeg*& s, const BatchSeg* segs) {
+    const uint4 e = tiles[slot >> 6];
+    const uint32_t j = slot - (e.z << 6);
+    const uint32_t q = j / e.w;
+    seg = e.x + (j - q * e.w);
+    s = segs + seg;
+    const uint32_t tilesX = s->tilesX;
+    pxx = (e.y % tilesX) * 8u + (q & 7u);
+    pxy = (e.y / tilesX) * 8u + (q >> 3);
+    return pxx < s->W && pxy < s->H;
+}
+
 struct BatchMap {
-    const uint2* tiles;
+    const uint4* tiles;
     const BatchSeg* segs;
     const BatchHeader* hdr;
     __device__ __forceinline__ uint32_t slots() const { return hdr->nTiles * 64u; }
     __device__ __forceinline__ bool locate(uint32_t slot, uint32_t& p) const {
-        const uint2 t = tiles[slot >> 6];
-        const BatchSeg& s = segs[t.x];
-        const uint32_t within = slot & 63u;
-        const uint32_t pxx = (t.y % s.tilesX) * 8u + (within & 7u);
-        const uint32_t pxy = (t.y / s.tilesX) * 8u + (within >> 3);
-        p = s.slotBase + pxy * s.W + pxx;
-        return pxx < s.W && pxy < s.H;
+        uint32_t seg, pxx, pxy;
+        const BatchSeg* s;
+        if (!batchSlot(tiles, slot, seg, pxx, pxy, s, segs)) return false;
+        p = s->slotBase + pxy * s->W + pxx;
+        return true;
     }
 };
 
@@ -874,12 +887,10 @@ __global__ __launch_bounds__(256) void k_cam_composite(uint32_t npx, const uint3
 __global__ __launch_bounds__(256) void k_batch_init(BatchDesc b, PixelArrays px) {
     const uint32_t t = blockIdx.x * 4u + (threadIdx.x >> 6);
     if (t >= b.hdr->nTiles) return;
-    const uint2 td = b.tiles[t];
-    const BatchSeg s = b.segs[td.x];
-    const uint32_t within = threadIdx.x & 63u;
-    const uint32_t pxx = (td.y % s.tilesX) * 8u + (within & 7u);
-    const uint32_t pxy = (td.y / s.tilesX) * 8u + (within >> 3);
-    if (pxx >= s.W || pxy >= s.H) return;
+    uint32_t seg, pxx, pxy;
+    const BatchSeg* sp;
+    if (!batchSlot(b.tiles, t * 64u + (threadIdx.x & 63u), seg, pxx, pxy, sp, b.segs)) return;
+    const BatchSeg s = *sp;
     const uint32_t pixel = pxy * s.W + pxx, slot = s.slotBase + pixel;
     Rng rng(perItemSeed(s.seed, pixel));
     const float jx = rng.nextFloat() - 0.5f;
@@ -960,12 +971,10 @@ __global__ __launch_bounds__(256) void k_adj_raygen_batch(BatchDesc b, const flo
     const uint32_t t = blockIdx.x * 4u + (threadIdx.x >> 6);
     if (blockIdx.x == 0 && threadIdx.x == 0) st->qCount = b.hdr->nSlots;
     if (t >= b.hdr->nTiles) return;
-    const uint2 td = b.tiles[t];
-    const BatchSeg s = b.segs[td.x];
-    const uint32_t within = threadIdx.x & 63u;
-    const uint32_t pxx = (td.y % s.tilesX) * 8u + (within & 7u);
-    const uint32_t pxy = (td.y / s.tilesX) * 8u + (within >> 3);
-    if (pxx >= s.W || pxy >= s.H) return;
+    uint32_t seg, pxx, pxy;
+    const BatchSeg* sp;
+    if (!batchSlot(b.tiles, t * 64u + (threadIdx.x & 63u), seg, pxx, pxy, sp, b.segs)) return;
+    const BatchSeg s = *sp;
     const uint32_t pixel = pxy * s.W + pxx, slot = s.slotBase + pixel;
     Rng rng(perItemSeed(42ull, pixel));
     const float jx = rng.nextFloat() - 0.5f;
@@ -975,7 +984,7 @@ __global__ __launch_bounds__(256) void k_adj_raygen_batch(BatchDesc b, const flo
     const size_t src = 3 * ((size_t)s.pixOffset + pixel);
     q.o[slot] = f4(o, u2f(slot));
     q.d[slot] = f4(d, u2f(0u));
-    q.T[slot] = make_float4(dloss[src], dloss[src + 1], dloss[src + 2], u2f(td.x));
+    q.T[slot] = make_float4(dloss[src], dloss[src + 1], dloss[src + 2], u2f(seg));
     pend.p0[slot] = make_float4(0.0f, 0.0f, 0.0f, u2f(kNone));
 }
 

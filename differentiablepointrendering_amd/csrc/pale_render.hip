@@ -763,14 +763,14 @@ struct pale_handle {
     // round over all its tiles, and a view's framebuffer is updated pass by
     // pass in order (k_batch_resolve), so each image equals renderView's.
     struct HostBatch {
-        std::vector<uint2> tiles;
+        std::vector<uint4> tiles;
         std::vector<BatchSeg> segs;
         std::vector<BatchRun> runs;
         BatchHeader hdr{};
     };
 
     BatchDesc batchDesc() const {
-        return BatchDesc{bTiles.as<uint2>(), bSegs.as<BatchSeg>(), bRuns.as<BatchRun>(), bHdr.as<BatchHeader>(),
+        return BatchDesc{bTiles.as<uint4>(), bSegs.as<BatchSeg>(), bRuns.as<BatchRun>(), bHdr.as<BatchHeader>(),
                          bCams.as<DevCamera>()};
     }
 
@@ -866,7 +866,7 @@ struct pale_handle {
         // Entries per slot: the tuning's, else the settings' per-pixel budget.
         sizeCamera(std::max(plan.maxSlots, 1u), a, tune.entries_per_slot);
         a(cursors, kTraceRegions * sizeof(uint32_t));
-        a(bTiles, (size_t)std::max(plan.maxTiles, 1u) * sizeof(uint2));
+        a(bTiles, (size_t)std::max(plan.maxTiles, 1u) * sizeof(uint4));
         a(bSegs, plan.maxSegs * sizeof(BatchSeg));
         a(bRuns, plan.maxRuns * sizeof(BatchRun));
         a(bHdr, sizeof(BatchHeader));
@@ -925,7 +925,8 @@ struct pale_handle {
                 BatchSeg sg{v, p, c.W, c.H, tx, (uint32_t)cur, seedOf(v, p), (uint32_t)plan.fbOff[v],
                             1.0f / (float)spp[v], {0, 0, 0}};
                 b.segs.push_back(sg);
-                for (uint32_t t = 0; t < tx * ty; ++t) b.tiles.push_back(make_uint2(si, t));
+                for (uint32_t t = 0; t < tx * ty; ++t)
+                    b.tiles.push_back(make_uint4(si, t, (uint32_t)b.tiles.size(), 1u));
                 cur += npx;
             }
         }
@@ -951,16 +952,21 @@ struct pale_handle {
                 b.runs.push_back(r);
             }
             b.segs.swap(ordered);
-            // Tiles view by view, then tile by tile (row-major), then pass by
-            // pass: a contiguous range of slots (a layer-walk region) covers
-            // a band of one image across its passes.
-            std::vector<uint2> tiles;
+            // Tiles view by view, then tile by tile (row-major), each tile as
+            // one group of K units whose slots take its pixels K at a time,
+            // pass fastest: a contiguous range of slots (a layer-walk region)
+            // covers a band of one image across its passes, and the lanes of
+            // a wave walk a few pixels in all their passes side by side.
+            std::vector<uint4> tiles;
             tiles.reserve(b.tiles.size());
             for (const BatchRun& r : b.runs) {
                 const BatchSeg& s0 = b.segs[r.firstSeg];
                 const uint32_t nt = s0.tilesX * ((s0.H + 7u) / 8u);
-                for (uint32_t t = 0; t < nt; ++t)
-                    for (uint32_t k = 0; k < r.numSegs; ++k) tiles.push_back(make_uint2(r.firstSeg + k, t));
+                for (uint32_t t = 0; t < nt; ++t) {
+                    const uint32_t first = (uint32_t)tiles.size();
+                    for (uint32_t k = 0; k < r.numSegs; ++k)
+                        tiles.push_back(make_uint4(r.firstSeg, t, first, r.numSegs));
+                }
             }
             b.tiles.swap(tiles);
             uint32_t slots = 0;
@@ -1117,7 +1123,7 @@ struct pale_handle {
     template <typename A>
     void sizeBackward(const BackwardPlan& plan, size_t nViews, A& a) {
         sizeAdjoint(plan.maxSlots, a);
-        a(bTiles, (size_t)plan.maxTiles * sizeof(uint2));
+        a(bTiles, (size_t)plan.maxTiles * sizeof(uint4));
         a(bSegs, plan.maxSegs * sizeof(BatchSeg));
         a(bRuns, sizeof(BatchRun));
         a(bHdr, sizeof(BatchHeader));
@@ -1163,7 +1169,8 @@ struct pale_handle {
                 const uint32_t si = (uint32_t)b.segs.size();
                 BatchSeg sg{v, p, c.W, c.H, tx, (uint32_t)cur, 42u, (uint32_t)pixOff[v], 1.0f / (float)spp[v], {0, 0, 0}};
                 b.segs.push_back(sg);
-                for (uint32_t t = 0; t < tx * ty; ++t) b.tiles.push_back(make_uint2(si, t));
+                for (uint32_t t = 0; t < tx * ty; ++t)
+                    b.tiles.push_back(make_uint4(si, t, (uint32_t)b.tiles.size(), 1u));
                 cur += npx;
             }
         }
