@@ -71,6 +71,8 @@ def parse():
     ap.add_argument("--cpu-pixel-stride", type=int, default=80)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-count", action="store_true", help="skip the work-counting step (roofline)")
+    ap.add_argument("--decorrelated", type=int, default=2,
+                    help="timed steps of the same workload with an independent seed per gather pass (0: skip)")
     ap.add_argument("--served", type=int, default=40, help="requests of the served stream (0: skip)")
     ap.add_argument("--served-rate", type=float, default=1.0, help="Poisson arrival rate, requests per second")
     ap.add_argument("--served-replicas", type=int, default=2)
@@ -464,6 +466,54 @@ def served_leg(a):
     return out
 
 
+def decorrelated_leg(a, spec, settings, target, kernels):
+    """The plain workload with spp_decorrelate = 1: every gather pass draws
+    its own camera rays.  With the reference's default (one seed for all
+    passes) the passes of a pixel trace identical rays, which the batched
+    layer walk places side by side in a wave, so their node and surfel loads
+    coalesce; here every lane's ray is distinct.  Same surfels, target and
+    tuning; the work per step differs from the shared-seed one only in which
+    rays are traced."""
+    from differentiablepointrendering_amd.pale import Renderer
+
+    s2 = dict(settings, spp_decorrelate=1)
+    r = Renderer.from_spec(spec, s2)
+    if a.tuning:
+        sl, en, gr, fr = a.tuning.split(",")
+        r.set_batch_tuning(int(sl), int(en), int(gr), float(fr))
+    cam = spec.cameras[0]
+
+    def step():
+        raw = r.render_forward()[cam.name + "_raw"][..., :3]
+        r.render_backward({cam.name: ((raw - target) / raw.size).astype(np.float32)})
+        return r.stats()
+
+    step()
+    dev, q, tr, trl = [], [], 0.0, 0
+    for _ in range(a.decorrelated):
+        st = step()
+        dev.append(st["forward_ms"] + st["backward_ms"])
+        q.append(st["photon_queries"] + st["gather_queries"] + st["adjoint_queries"])
+        tr += st["camera_trace_ms"]
+        trl += st["camera_trace_launches"]
+    r.close()
+    ms = float(np.mean(dev))
+    out = {"value": round(float(np.mean(q)) / (ms * 1e-3) / 1e6, 2), "unit": "Mrays/s", "steps": a.decorrelated,
+           "ms_per_step": round(ms, 3),
+           "phase_ms": {k: round(st[k], 3) for k in ("camera_trace_ms", "camera_sort_ms", "gather_ms",
+                                                     "adjoint_trace_ms", "adjoint_transport_ms")}}
+    if kernels is not None and trl > 0:
+        # Algorithmic bytes of the layer walk as counted for the shared-seed
+        # step (the decorrelated walk's count differs by well under 0.1 %).
+        per_launch = kernels["k_cam_trace"]["algorithmic_bytes_per_launch"]
+        avg_s = tr / trl * 1e-3
+        out["k_cam_trace"] = {"avg_launch_ms": round(avg_s * 1e3, 3),
+                              "achieved_gbs": round(per_launch / avg_s / 1e9, 1),
+                              "frac": round(per_launch / avg_s / 1e9 / HBM_PEAK_GBS, 4)}
+    print(f"[bench] decorrelated passes {out['value']} Mrays/s", file=sys.stderr, flush=True)
+    return out
+
+
 def main():
     a = parse()
     if a.copy_calibration:
@@ -644,6 +694,8 @@ def main():
         out["kernels"] = kernels
     r.close()
     del r
+    if a.decorrelated > 0 and not a.decorrelate:
+        out["decorrelated_passes"] = decorrelated_leg(a, spec, settings, target, kernels)
     if a.batched > 0:
         out["batched"] = batched_leg(a)
     if a.differentiation > 0:
