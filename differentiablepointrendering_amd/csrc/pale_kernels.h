@@ -31,6 +31,8 @@ struct DevState {
     uint32_t overflow;     // the entry buffer filled this round
     uint32_t evPool;       // adjoint events waiting for the transport
     uint32_t resv;         // this bounce may pass the photon capacity
+    uint32_t bigCells;     // photon cells too large for the per-wave sort
+    uint32_t pad32;
     unsigned long long photonQ, gatherQ, adjointQ, deposits, events, pad;
     // Counting mode, per kernel class (photon, gather, adjoint): popped
     // nodes, AABB tests, primitive tests, gathers, gather cells, non-empty
@@ -127,12 +129,18 @@ struct BatchDesc {
 // top bit = no gather); comp = colour factor k + weight w; E = irradiance;
 // ord = entries in (pixel, layer) order; cell / sorted = counting sort of
 // the gather entries by grid cell.
+// Camera-gather entries of one round, in append order: qa = gather point +
+// normal.x, qb = normal.yz, slot, layer tag; comp = colour factor k and
+// weight w.  pos[i] = place of entry i in its slot's layer-ordered list
+// (slot offset + layer); contrib holds each entry's w * (E * k) at that place,
+// so a slot's layers are contiguous for the in-order sum.  cell / sorted:
+// the counting sort of the gather entries by photon-grid cell.
 struct EntryArrays {
     float4* qa;
     float4* qb;
     float4* comp;
-    float4* E;
-    uint32_t* ord;
+    float4* contrib;
+    uint32_t* pos;
     uint32_t* cell;
     uint32_t* sorted;
 };

@@ -533,19 +533,73 @@ __global__ __launch_bounds__(256) void k_cell_scatter(const DevState* st, uint32
 }
 
 // Order each cell by deposit index: the reference's sequential scatter
-// order, which makes every gather sum in the same order.
-__global__ __launch_bounds__(256) void k_cell_sort(uint32_t total, const uint32_t* cellStart, uint32_t* sortedIdx) {
+// order, which makes every gather sum in the same order.  A wave takes 64
+// consecutive cells and sorts each one of up to 64 photons together: lane j
+// holds photon j of the cell and its place is the number of smaller indices
+// (indices are distinct).  Larger cells go to a list for k_cell_sort_big.
+constexpr uint32_t kWaveCell = 64;
+constexpr uint32_t kBlockCell = 4096;
+
+__global__ __launch_bounds__(256) void k_cell_sort(uint32_t total, const uint32_t* cellStart, uint32_t* sortedIdx,
+                                                   DevState* st, uint32_t* bigList) {
     const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= total) return;
-    const uint32_t a = cellStart[c], b = cellStart[c + 1];
-    for (uint32_t j = a + 1; j < b; ++j) {
-        const uint32_t v = sortedIdx[j];
-        uint32_t k = j;
-        while (k > a && sortedIdx[k - 1] > v) {
-            sortedIdx[k] = sortedIdx[k - 1];
-            --k;
+    const uint32_t lane = laneId();
+    uint32_t a = 0, n = 0;
+    if (c < total) {
+        a = cellStart[c];
+        n = cellStart[c + 1] - a;
+    }
+    const unsigned long long big = __ballot(n > kWaveCell);
+    if (big) {
+        const uint32_t leader = (uint32_t)__ffsll((long long)big) - 1u;
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(&st->bigCells, (uint32_t)__popcll(big));
+        base = __shfl(base, (int)leader, 64);
+        if (n > kWaveCell) bigList[base + (uint32_t)__popcll(big & ((1ull << lane) - 1ull))] = c;
+    }
+    unsigned long long todo = __ballot(n >= 2u && n <= kWaveCell);
+    while (todo) {
+        const uint32_t j = (uint32_t)__ffsll((long long)todo) - 1u;
+        todo &= todo - 1ull;
+        const uint32_t ca = __shfl(a, (int)j, 64), cn = __shfl(n, (int)j, 64);
+        const uint32_t x = lane < cn ? sortedIdx[ca + lane] : 0xFFFFFFFFu;
+        uint32_t r = 0;
+        for (uint32_t m = 0; m < cn; ++m) r += __shfl(x, (int)m, 64) < x ? 1u : 0u;
+        if (lane < cn) sortedIdx[ca + r] = x;
+    }
+}
+
+// Cells of more than 64 photons, one workgroup each: the cell in LDS, each
+// photon's place = the number of smaller indices.  A cell beyond the LDS
+// tile (none at the benchmark scenes' densities) is sorted by one thread.
+__global__ __launch_bounds__(256) void k_cell_sort_big(const DevState* st, const uint32_t* bigList,
+                                                       const uint32_t* cellStart, uint32_t* sortedIdx) {
+    __shared__ uint32_t v[kBlockCell];
+    const uint32_t nb = st->bigCells;
+    for (uint32_t q = blockIdx.x; q < nb; q += gridDim.x) {
+        const uint32_t c = bigList[q];
+        const uint32_t a = cellStart[c], n = cellStart[c + 1] - a;
+        if (n <= kBlockCell) {
+            for (uint32_t j = threadIdx.x; j < n; j += blockDim.x) v[j] = sortedIdx[a + j];
+            __syncthreads();
+            for (uint32_t j = threadIdx.x; j < n; j += blockDim.x) {
+                const uint32_t x = v[j];
+                uint32_t r = 0;
+                for (uint32_t m = 0; m < n; ++m) r += v[m] < x ? 1u : 0u;
+                sortedIdx[a + r] = x;
+            }
+            __syncthreads();
+        } else if (threadIdx.x == 0) {
+            for (uint32_t j = a + 1; j < a + n; ++j) {
+                const uint32_t x = sortedIdx[j];
+                uint32_t k = j;
+                while (k > a && sortedIdx[k - 1] > x) {
+                    sortedIdx[k] = sortedIdx[k - 1];
+                    --k;
+                }
+                sortedIdx[k] = x;
+            }
         }
-        sortedIdx[k] = v;
     }
 }
 
@@ -773,7 +827,6 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(4))
                     ent.qa[e] = f4(q, n.x);
                     ent.qb[e] = make_float4(n.y, n.z, u2f(pixel), u2f(r | (gather ? 0u : kEntryNoGather)));
                     ent.comp[e] = f4(k, wgt);
-                    if (!gather) ent.E[e] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
                     ++r;
                     ++layer;
                 }
@@ -797,42 +850,83 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(4))
     if (kCount) flushWork(w, st, kWorkGather);
 }
 
-// Slot of every entry in its pixel's ordered list; grid cell and per-cell
-// count of the gather entries (the cell of worldToCellClamped,
-// KernelHelpers.h:920-934).
+// Runs of equal keys among consecutive lanes of a wave: the lane that heads
+// this lane's run and the run's length.  Every lane of the wave calls it.
+struct KeyRun {
+    uint32_t head, len;
+};
+__device__ __forceinline__ KeyRun keyRun(uint32_t key) {
+    const uint32_t lane = laneId();
+    const uint32_t next = __shfl_down(key, 1, 64);
+    const uint32_t prev = __shfl_up(key, 1, 64);
+    const unsigned long long heads = __ballot(lane == 0u || prev != key);
+    const unsigned long long ends = __ballot(lane == 63u || next != key);
+    const unsigned long long upTo = lane == 63u ? ~0ull : ((2ull << lane) - 1ull);
+    const uint32_t head = 63u - (uint32_t)__clzll((long long)(heads & upTo));
+    const uint32_t end = lane + (uint32_t)__ffsll((long long)(ends >> lane)) - 1u;
+    return KeyRun{head, end - head + 1u};
+}
+
+constexpr uint32_t kNoCell = 0xFFFFFFFFu;
+
+// Place of every entry in its slot's layer-ordered list; grid cell and
+// per-cell count of the gather entries (the cell of worldToCellClamped,
+// KernelHelpers.h:920-934).  An entry that gathers nothing (an emitter) has
+// its contribution w * k written here (the gather's E is one).  Neighbouring
+// entries come from neighbouring lanes of the layer walk, so equal cells
+// arrive in runs: one atomic per run.
 // The entry kernels stride over the entries with a bounded grid: the count
-// is known only on the device and the capacity is far above it.
+// is known only on the device and the capacity is far above it.  The loop is
+// uniform over the block, so whole waves take part in the run detection.
 __global__ __launch_bounds__(256) void k_entry_key(GridParams g, const DevState* st, uint32_t cap, PixelArrays px,
                                                    EntryArrays ent, uint32_t* cellCount) {
     const uint32_t n = min(st->entries, cap);
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const float4 b = ent.qb[i];
-    const uint32_t tag = f2u(b.w);
-    ent.ord[px.off[f2u(b.z)] + (tag & ~kEntryNoGather)] = i;
-    uint32_t c = 0xFFFFFFFFu;
-    if (!(tag & kEntryNoGather)) {
-        const float4 a = ent.qa[i];
-        const F3 f = divv(mk3(a.x, a.y, a.z) - mk3(g.ox, g.oy, g.oz), mk3(g.cell, g.cell, g.cell));
-        const int x = min(max((int)floorf(f.x), 0), g.rx - 1);
-        const int y = min(max((int)floorf(f.y), 0), g.ry - 1);
-        const int z = min(max((int)floorf(f.z), 0), g.rz - 1);
-        c = (uint32_t)((z * g.ry + y) * g.rx + x);
-        atomicAdd(&cellCount[c], 1u);
-    }
-    ent.cell[i] = c;
+    for (uint32_t i0 = blockIdx.x * blockDim.x; i0 < n; i0 += gridDim.x * blockDim.x) {
+        const uint32_t i = i0 + threadIdx.x;
+        uint32_t c = kNoCell;
+        if (i < n) {
+            const float4 b = ent.qb[i];
+            const uint32_t tag = f2u(b.w);
+            const uint32_t pos = px.off[f2u(b.z)] + (tag & ~kEntryNoGather);
+            ent.pos[i] = pos;
+            if (tag & kEntryNoGather) {
+                const float4 k = ent.comp[i];
+                ent.contrib[pos] = f4(k.w * mk3(k.x, k.y, k.z), 0.0f);
+            } else {
+                const float4 a = ent.qa[i];
+                const F3 f = divv(mk3(a.x, a.y, a.z) - mk3(g.ox, g.oy, g.oz), mk3(g.cell, g.cell, g.cell));
+                const int x = min(max((int)floorf(f.x), 0), g.rx - 1);
+                const int y = min(max((int)floorf(f.y), 0), g.ry - 1);
+                const int z = min(max((int)floorf(f.z), 0), g.rz - 1);
+                c = (uint32_t)((z * g.ry + y) * g.rx + x);
+            }
+            ent.cell[i] = c;
+        }
+        const KeyRun r = keyRun(c);
+        if (laneId() == r.head && c != kNoCell) atomicAdd(&cellCount[c], r.len);
     }
 }
 
-// Counting-sort scatter of the gather entries; leaves cellCount zeroed.
+// Counting-sort scatter of the gather entries; leaves cellCount zeroed.  A
+// run of equal cells takes its places with one atomic.
 __global__ __launch_bounds__(256) void k_entry_scatter(const DevState* st, uint32_t cap, const uint32_t* cellStart,
                                                        uint32_t* cellCount, EntryArrays ent) {
     const uint32_t n = min(st->entries, cap);
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const uint32_t c = ent.cell[i];
-        if (c != 0xFFFFFFFFu) ent.sorted[cellStart[c] + atomicSub(&cellCount[c], 1u) - 1u] = i;
+    for (uint32_t i0 = blockIdx.x * blockDim.x; i0 < n; i0 += gridDim.x * blockDim.x) {
+        const uint32_t i = i0 + threadIdx.x;
+        const uint32_t c = i < n ? ent.cell[i] : kNoCell;
+        const KeyRun r = keyRun(c);
+        const uint32_t lane = laneId();
+        uint32_t base = 0;
+        if (lane == r.head && c != kNoCell) base = cellStart[c] + atomicSub(&cellCount[c], r.len) - r.len;
+        base = __shfl(base, (int)r.head, 64);
+        if (c != kNoCell) ent.sorted[base + (lane - r.head)] = i;
     }
 }
 
+// One irradiance gather per entry in cell order; the entry's contribution
+// w * (E * k) goes to its place in the slot's layer list (the products the
+// reference's loop forms, PrimalKernels.cpp:583-585, :620, :626-628).
 template <bool kCount>
 __global__ __launch_bounds__(256) void k_entry_gather(DevGrid G, DevState* st, const uint32_t* nGather,
                                                       EntryArrays ent) {
@@ -844,15 +938,18 @@ __global__ __launch_bounds__(256) void k_entry_gather(DevGrid G, DevState* st, c
     Work w{};
     if (i < n) {
         const uint32_t e = ent.sorted[i];
-        const float4 a = ent.qa[e], b = ent.qb[e];
+        const float4 a = ent.qa[e], b = ent.qb[e], c = ent.comp[e];
+        const uint32_t pos = ent.pos[e];
         const F3 E = gatherIrradiance<kCount>(G, mk3(a.x, a.y, a.z), mk3(a.w, b.x, b.y), w);
-        ent.E[e] = f4(E, 0.0f);
+        const F3 Lo = E * mk3(c.x, c.y, c.z);
+        ent.contrib[pos] = f4(c.w * Lo, 0.0f);
     }
     if (kCount) flushWork(w, st, kWorkGather);
 }
 
 // fb == nullptr (batches): a finished pixel keeps its colour in cs and the
 // framebuffer update is left to k_batch_resolve.  nDev: device-side count.
+// C = C + w * (E * k) over the pixel's layers in order.
 __global__ __launch_bounds__(256) void k_cam_composite(uint32_t npx, const uint32_t* nDev, PixelArrays px,
                                                        EntryArrays ent, float invSpp, float4* fb) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -862,12 +959,7 @@ __global__ __launch_bounds__(256) void k_cam_composite(uint32_t npx, const uint3
     if (status == kPixFlushed) return;
     F3 C = mk3(cs.x, cs.y, cs.z);
     const uint32_t first = px.off[p], cnt = px.cnt[p];
-    for (uint32_t j = 0; j < cnt; ++j) {
-        const uint32_t e = ent.ord[first + j];
-        const float4 c = ent.comp[e], E = ent.E[e];
-        const F3 Lo = mk3(E.x, E.y, E.z) * mk3(c.x, c.y, c.z);
-        C = C + c.w * Lo;
-    }
+    for (uint32_t j = 0; j < cnt; ++j) C = C + ld3(ent.contrib[first + j]);
     // The next round's slot scan must see zero for every pixel it did not
     // touch, finished ones included.
     px.cnt[p] = 0u;
@@ -1522,7 +1614,11 @@ void launchBuildGrid(hipStream_t s, const GridParams& g, DevState* st, uint32_t 
     k_tile_scan<<<1, 1024, 0, s>>>(tiles, nullptr, g.total, g.total, scratchTotal, 0xFFFFFFFFu);
     k_scan_add<<<blocks(g.total, 256), 256, 0, s>>>(cellStart, g.total, tiles, scratchTotal);
     k_cell_scatter<<<pb, 256, 0, s>>>(st, cap, cellId, cellStart, cellCount, sortedIdx);
-    k_cell_sort<<<blocks(g.total, 256), 256, 0, s>>>(g.total, cellStart, sortedIdx);
+    // cellId is free once the photons are scattered: it holds the list of
+    // large cells.
+    launchFill(s, reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(st) + offsetof(DevState, bigCells)), 1, 0u);
+    k_cell_sort<<<blocks(g.total, 256), 256, 0, s>>>(g.total, cellStart, sortedIdx, st, cellId);
+    k_cell_sort_big<<<min(pb, 2048u), 256, 0, s>>>(st, cellId, cellStart, sortedIdx);
     k_cell_payload<<<pb, 256, 0, s>>>(st, cap, sortedIdx, raw, sorted);
 }
 

@@ -137,7 +137,7 @@ struct pale_handle {
     pale_batch_tuning tune{};
     hipGraphExec_t bGraph = nullptr;
     std::vector<uint8_t> bGraphKey;
-    DevMem pxOt, pxDl, pxCs, pxCnt, pxOff, pxTiles, entQa, entQb, entComp, entE, entOrd, entCell, entSorted,
+    DevMem pxOt, pxDl, pxCs, pxCnt, pxOff, pxTiles, entQa, entQb, entComp, entContrib, entPos, entCell, entSorted,
         entStart, flagHost;
     // Device state and seed table.
     DevMem state, seedTable, cursors;
@@ -474,7 +474,7 @@ struct pale_handle {
     // stay.  They are allocated again by the next view.
     std::vector<DevMem*> viewBuffers() {
         std::vector<DevMem*> v = {&pxOt,    &pxDl,    &pxCs,    &pxCnt,  &pxOff, &pxTiles, &entQa,  &entQb,
-                                  &entComp, &entE,    &entOrd,  &entCell, &entSorted, &viewFb, &viewLdr, &pend0,
+                                  &entComp, &entContrib, &entPos, &entCell, &entSorted, &viewFb, &viewLdr, &pend0,
                                   &pend1,   &pend2,   &pend3,   &pend4,  &evCell, &evSorted, &evFlags, &dloss,
                                   &bTiles,  &bSegs,   &bRuns,   &bHdr,   &bCams, &bSegFirst, &viewsFb, &viewsLdr};
         for (int k = 0; k < 6; ++k) {
@@ -551,8 +551,8 @@ struct pale_handle {
             a(entQa, (size_t)want * 16);
             a(entQb, (size_t)want * 16);
             a(entComp, (size_t)want * 16);
-            a(entE, (size_t)want * 16);
-            a(entOrd, (size_t)want * 4);
+            a(entContrib, (size_t)want * 16);
+            a(entPos, (size_t)want * 4);
             a(entCell, (size_t)want * 4);
             a(entSorted, (size_t)want * 4);
             if (!A::kDry) entryCap = want;
@@ -575,7 +575,7 @@ struct pale_handle {
     }
     EntryArrays enta() const {
         return EntryArrays{entQa.as<float4>(),    entQb.as<float4>(),    entComp.as<float4>(),
-                           entE.as<float4>(),     entOrd.as<uint32_t>(), entCell.as<uint32_t>(),
+                           entContrib.as<float4>(), entPos.as<uint32_t>(), entCell.as<uint32_t>(),
                            entSorted.as<uint32_t>()};
     }
 
