@@ -195,6 +195,10 @@ __device__ bool traverseCloud(const DevScene& S, const DevInstance& I, F3 o, F3 
     return true;
 }
 
+// Words of a leaf's surfels loaded before its tests: 2 (plane words) or 4
+// (the whole 64-byte records).
+constexpr int kLeafWords = 4;
+
 // The same traversal without the counters, and without a stack round trip
 // for the nearer child: the reference pushes the farther child, then the
 // nearer, and pops the nearer at once with an unchanged best, so descending
@@ -213,18 +217,42 @@ __device__ bool traverseCloudFast(const DevScene& S, const DevInstance& I, F3 o,
     bool any = false;
     while (true) {
         if (ref & kLeafBit) {
+            // A leaf holds at most four surfels: their plane words (center,
+            // normal, scales) are loaded together before the first test, the
+            // tangents of a surfel only when its plane hit is in range.  The
+            // tests run in leaf order as before.
             const uint32_t first = leafFirst(ref), cnt = leafCount(ref);
-            for (uint32_t k = 0; k < cnt; ++k) {
-                const uint32_t si = first + k;
-                const float4* g = reinterpret_cast<const float4*>(S.geo + si);
-                const float4 g0 = g[0], g1 = g[1];
+            const float4* g = reinterpret_cast<const float4*>(S.geo + first);
+            float4 w0[4], w1[4], w2[4], w3[4];
+#pragma unroll
+            for (uint32_t k = 0; k < 4; ++k) {
+                if (k < cnt) {
+                    w0[k] = g[4 * k];
+                    w1[k] = g[4 * k + 1];
+                    if (kLeafWords == 4) {
+                        w2[k] = g[4 * k + 2];
+                        w3[k] = g[4 * k + 3];
+                    }
+                }
+            }
+#pragma unroll
+            for (uint32_t k = 0; k < 4; ++k) {
+                if (k >= cnt) break;
+                const float4 g0 = w0[k], g1 = w1[k];
                 const F3 n = mk3(g1.x, g1.y, g1.z);
                 const float nd = dot(n, d);
                 if (fabsf(nd) < 1e-6f) continue;
                 const F3 p = mk3(g0.x, g0.y, g0.z);
                 const float t = dot(n, p - o) / nd;
                 if (t <= 1e-5f || t >= best) continue;
-                const float4 g2 = g[2], g3 = g[3];
+                float4 g2, g3;
+                if (kLeafWords == 4) {
+                    g2 = w2[k];
+                    g3 = w3[k];
+                } else {
+                    g2 = g[4 * k + 2];
+                    g3 = g[4 * k + 3];
+                }
                 const F3 x = o + d * t;
                 const F3 r = x - p;
                 const float uu = dot(mk3(g2.x, g2.y, g2.z), r) / g0.w;
@@ -233,7 +261,7 @@ __device__ bool traverseCloudFast(const DevScene& S, const DevInstance& I, F3 o,
                 if (r2 >= 1.0f) continue;
                 any = true;
                 best = t;
-                bestIdx = si;
+                bestIdx = first + k;
                 bestX = x;
                 bestR2 = r2;
             }

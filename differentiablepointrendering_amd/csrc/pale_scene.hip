@@ -7,10 +7,12 @@
 // left-then-right), so this build uses the same libstdc++ selection
 // algorithm and recursion order.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <functional>
 #include <limits>
 #include <stdexcept>
+#include <thread>
 
 #include "pale_scene.h"
 
@@ -42,14 +44,33 @@ Box surfelBox(const HostPoint& s) {
     return Box{s.p - h, s.p + h};
 }
 
+// f(k) for k < n on up to 16 threads (in any order).
+template <typename F>
+void parallelFor(size_t n, F&& f) {
+    const size_t hw = std::max<unsigned>(1u, std::thread::hardware_concurrency());
+    const size_t nt = std::min<size_t>(std::min<size_t>(hw, 16), n);
+    if (nt <= 1) {
+        for (size_t k = 0; k < n; ++k) f(k);
+        return;
+    }
+    std::atomic<size_t> next{0};
+    std::vector<std::thread> th;
+    for (size_t t = 0; t < nt; ++t)
+        th.emplace_back([&] {
+            for (size_t k; (k = next.fetch_add(1)) < n;) f(k);
+        });
+    for (auto& x : th) x.join();
+}
+
 // Generic median-split builder (BVH.ixx:49-133 for boxes, :26-45/:141-220
 // for triangles).  `bounds(i)` is the box of primitive i, `key(i, axis)` its
 // centroid coordinate.
+template <typename BoundsF, typename KeyF>
 struct MedianSplit {
     std::vector<RefNode>& nodes;
     std::vector<uint32_t>& order;
-    std::function<Box(uint32_t)> bounds;
-    std::function<float(uint32_t, int)> key;
+    BoundsF bounds;
+    KeyF key;
     uint32_t maxLeaf;
 
     void fit(RefNode& nd) {
@@ -63,9 +84,10 @@ struct MedianSplit {
         nd.mx = bmax;
     }
 
-    void subdivide(uint32_t ni) {
-        const RefNode nd = nodes[ni];
-        if (nd.count <= maxLeaf) return;
+    // One split of the reference's recursion: false if nd stays a leaf,
+    // else the two children (fitted) over the halves of nd's range.
+    bool split(const RefNode& nd, RefNode& l, RefNode& r) {
+        if (nd.count <= maxLeaf) return false;
         const F3 e = nd.mx - nd.mn;
         const int axis = (e.y > e.x && e.y > e.z) ? 1 : (e.z > e.x && e.z > e.y) ? 2 : 0;
         auto b = order.begin() + nd.leftFirst;
@@ -73,20 +95,30 @@ struct MedianSplit {
         auto mid = b + (nd.count >> 1);
         std::nth_element(b, mid, en, [&](uint32_t p, uint32_t q) { return key(p, axis) < key(q, axis); });
         const uint32_t leftCount = (uint32_t)(mid - b);
-        if (leftCount == 0 || leftCount == nd.count) return;
-        const uint32_t L = (uint32_t)nodes.size(), R = L + 1;
-        nodes.push_back(RefNode{});
-        nodes.push_back(RefNode{});
-        nodes[L].leftFirst = nd.leftFirst;
-        nodes[L].count = leftCount;
-        nodes[R].leftFirst = nd.leftFirst + leftCount;
-        nodes[R].count = nd.count - leftCount;
+        if (leftCount == 0 || leftCount == nd.count) return false;
+        l = RefNode{};
+        r = RefNode{};
+        l.leftFirst = nd.leftFirst;
+        l.count = leftCount;
+        r.leftFirst = nd.leftFirst + leftCount;
+        r.count = nd.count - leftCount;
+        fit(l);
+        fit(r);
+        return true;
+    }
+
+    // Depth first, left before right; the children of a node are appended
+    // as a pair when it splits.
+    void subdivide(uint32_t ni) {
+        RefNode l, r;
+        if (!split(nodes[ni], l, r)) return;
+        const uint32_t L = (uint32_t)nodes.size();
+        nodes.push_back(l);
+        nodes.push_back(r);
         nodes[ni].leftFirst = L;
         nodes[ni].count = 0;
-        fit(nodes[L]);
-        fit(nodes[R]);
         subdivide(L);
-        subdivide(R);
+        subdivide(L + 1);
     }
 
     void build(uint32_t n) {
@@ -98,9 +130,105 @@ struct MedianSplit {
         nodes[0].leftFirst = 0;
         nodes[0].count = n;
         fit(nodes[0]);
-        subdivide(0);
+        if (n >= kParallelBuild)
+            buildParallel();
+        else
+            subdivide(0);
+    }
+
+    // The same tree built by several threads.  Every split works on its own
+    // range of `order` and its subtree only depends on that range, so the
+    // subtrees below the first kTaskDepth levels are built independently,
+    // each into a local node list in the recursion's order, and then copied
+    // into place by a walk that appends the pairs in the sequential
+    // recursion's order: nodes, bounds and permutation come out identical.
+    static constexpr uint32_t kParallelBuild = 1u << 16;
+    static constexpr int kTaskDepth = 6;
+    struct Part {
+        RefNode nd;
+        int32_t left = -1;              // index of the left child Part (right = left + 1)
+        std::vector<RefNode> subtree;  // local tree below a task node (nodes[0] = the node)
+        bool task = false;
+    };
+
+    void buildParallel() {
+        std::vector<Part> parts;
+        parts.reserve(size_t(2) << kTaskDepth);
+        parts.push_back(Part{nodes[0]});
+        // The top levels, breadth first (their splits touch disjoint ranges).
+        std::vector<uint32_t> level{0u};
+        for (int depth = 0; depth < kTaskDepth && !level.empty(); ++depth) {
+            std::vector<std::pair<RefNode, RefNode>> kids(level.size());
+            std::vector<char> did(level.size(), 0);
+            parallelFor(level.size(), [&](size_t k) { did[k] = split(parts[level[k]].nd, kids[k].first, kids[k].second); });
+            std::vector<uint32_t> next;
+            for (size_t k = 0; k < level.size(); ++k) {
+                if (!did[k]) continue;
+                const uint32_t li = (uint32_t)parts.size();
+                parts[level[k]].left = (int32_t)li;
+                parts.push_back(Part{kids[k].first});
+                parts.push_back(Part{kids[k].second});
+                next.push_back(li);
+                next.push_back(li + 1);
+            }
+            level.swap(next);
+        }
+        for (uint32_t pi : level) parts[pi].task = true;
+        parallelFor(level.size(), [&](size_t k) {
+            Part& p = parts[level[k]];
+            MedianSplit sub{p.subtree, order, bounds, key, maxLeaf};
+            p.subtree.reserve(std::max<uint32_t>(1u, p.nd.count / 2));
+            p.subtree.push_back(p.nd);
+            sub.subdivide(0);
+        });
+        nodes.clear();
+        nodes.push_back(RefNode{});
+        place(parts, 0, 0);
+    }
+
+    void place(const std::vector<Part>& parts, uint32_t pi, uint32_t gi) {
+        const Part& p = parts[pi];
+        if (p.task) {
+            placeLocal(p.subtree, 0, gi);
+            return;
+        }
+        RefNode nd = p.nd;
+        if (p.left < 0) {
+            nodes[gi] = nd;
+            return;
+        }
+        const uint32_t L = (uint32_t)nodes.size();
+        nodes.push_back(RefNode{});
+        nodes.push_back(RefNode{});
+        nd.leftFirst = L;
+        nd.count = 0;
+        nodes[gi] = nd;
+        place(parts, (uint32_t)p.left, L);
+        place(parts, (uint32_t)p.left + 1, L + 1);
+    }
+
+    void placeLocal(const std::vector<RefNode>& loc, uint32_t li, uint32_t gi) {
+        RefNode nd = loc[li];
+        if (nd.count != 0) {
+            nodes[gi] = nd;
+            return;
+        }
+        const uint32_t L = (uint32_t)nodes.size();
+        nodes.push_back(RefNode{});
+        nodes.push_back(RefNode{});
+        const uint32_t lc = nd.leftFirst;
+        nd.leftFirst = L;
+        nodes[gi] = nd;
+        placeLocal(loc, lc, L);
+        placeLocal(loc, lc + 1, L + 1);
     }
 };
+
+template <typename BoundsF, typename KeyF>
+MedianSplit<BoundsF, KeyF> medianSplit(std::vector<RefNode>& nodes, std::vector<uint32_t>& order, BoundsF bounds,
+                                       KeyF key, uint32_t maxLeaf) {
+    return MedianSplit<BoundsF, KeyF>{nodes, order, bounds, key, maxLeaf};
+}
 
 uint32_t maxDepth(const std::vector<RefNode>& nodes, uint32_t first, uint32_t count) {
     if (count == 0) return 0;
@@ -224,18 +352,19 @@ BuildProducts buildScene(const std::vector<HostMesh>& meshes, const std::vector<
         std::vector<HostTri> lt(bp.tris.begin() + mr.first, bp.tris.begin() + mr.first + mr.count);
         std::vector<RefNode> nodes;
         std::vector<uint32_t> order;
-        MedianSplit ms{nodes, order,
-                       [&](uint32_t i) {
-                           const HostTri& T = lt[i];
-                           Box b{mk3(kInf, kInf, kInf), mk3(-kInf, -kInf, -kInf)};
-                           const F3 v[3] = {bp.verts[T.v0], bp.verts[T.v1], bp.verts[T.v2]};
-                           for (const F3& p : v) {
-                               b.mn = vmin(b.mn, p);
-                               b.mx = vmax(b.mx, p);
-                           }
-                           return b;
-                       },
-                       [&](uint32_t i, int a) { return comp(lt[i].centroid, a); }, kMaxLeafPrims};
+        auto ms = medianSplit(
+            nodes, order,
+            [&](uint32_t i) {
+                const HostTri& T = lt[i];
+                Box b{mk3(kInf, kInf, kInf), mk3(-kInf, -kInf, -kInf)};
+                const F3 v[3] = {bp.verts[T.v0], bp.verts[T.v1], bp.verts[T.v2]};
+                for (const F3& p : v) {
+                    b.mn = vmin(b.mn, p);
+                    b.mx = vmax(b.mx, p);
+                }
+                return b;
+            },
+            [&](uint32_t i, int a) { return comp(lt[i].centroid, a); }, kMaxLeafPrims);
         (void)vb;
         ms.build(mr.count);
         for (uint32_t k = 0; k < order.size(); ++k) bp.tris[mr.first + k] = lt[order[k]];
@@ -248,11 +377,23 @@ BuildProducts buildScene(const std::vector<HostMesh>& meshes, const std::vector<
     }
     {
         std::vector<Box> boxes(pts.size());
-        for (size_t i = 0; i < pts.size(); ++i) boxes[i] = surfelBox(pts[i]);
+        const size_t chunk = 1u << 14;
+        parallelFor((pts.size() + chunk - 1) / chunk, [&](size_t c) {
+            for (size_t i = c * chunk; i < std::min(pts.size(), (c + 1) * chunk); ++i) boxes[i] = surfelBox(pts[i]);
+        });
         std::vector<RefNode> nodes;
         std::vector<uint32_t> order;
-        MedianSplit ms{nodes, order, [&](uint32_t i) { return boxes[i]; },
-                       [&](uint32_t i, int a) { return comp(pts[i].p, a); }, kMaxLeafPrims};
+        // Centroid coordinates per axis, contiguous for the selections.
+        std::vector<float> kx(pts.size()), ky(pts.size()), kz(pts.size());
+        for (size_t i = 0; i < pts.size(); ++i) {
+            kx[i] = pts[i].p.x;
+            ky[i] = pts[i].p.y;
+            kz[i] = pts[i].p.z;
+        }
+        const float* keys[3] = {kx.data(), ky.data(), kz.data()};
+        auto ms = medianSplit(
+            nodes, order, [&](uint32_t i) { return boxes[i]; }, [&](uint32_t i, int a) { return keys[a][i]; },
+            kMaxLeafPrims);
         ms.build((uint32_t)pts.size());
         bp.points.resize(pts.size());
         for (size_t k = 0; k < order.size(); ++k) bp.points[k] = pts[order[k]];

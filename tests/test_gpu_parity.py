@@ -45,6 +45,21 @@ def test_bvh_topology_bit_exact(c0_small):
         assert np.array_equal(pa[k], pb[k]), k
 
 
+def test_bvh_topology_bit_exact_threaded_build():
+    """Above 65,536 surfels the host build splits the top levels and builds
+    the subtrees on several threads; nodes, bounds and permutation must still
+    be the sequential recursion's (the oracle's)."""
+    spec = scene.config_scene(1, n=150_000, width=32, height=32)
+    gpu, orc = _pair(spec, dict(photons=1000))
+    a, b = gpu.surfel_bvh(), orc.surfel_bvh()
+    assert len(a["count"]) > 65_536 // 2
+    for k in ("count", "left_first", "permutation"):
+        assert np.array_equal(a[k], b[k]), k
+    assert np.array_equal(a["bounds"].view(np.uint32), b["bounds"].view(np.uint32))
+    gpu.close()
+    orc.close()
+
+
 def _random_rays(n, seed):
     rng = np.random.default_rng(seed)
     o = rng.uniform(-1.5, 1.5, size=(n, 3)).astype(np.float32)
