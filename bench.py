@@ -70,6 +70,8 @@ def parse():
     ap.add_argument("--cpu-photon-stride", type=int, default=8)
     ap.add_argument("--cpu-pixel-stride", type=int, default=80)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=16,
+                    help="concurrent oracle instances of the CPU baseline (capped at the host's CPU count)")
     ap.add_argument("--no-count", action="store_true", help="skip the work-counting step (roofline)")
     ap.add_argument("--decorrelated", type=int, default=2,
                     help="timed steps of the same workload with an independent seed per gather pass (0: skip)")
@@ -466,6 +468,49 @@ def served_leg(a):
     return out
 
 
+def cpu_leg(a, spec, settings, target, cam):
+    """The oracle (the sequential CPU restatement) on a bounded sample of the
+    same workload: photons/stride photons, every stride-th pixel, all passes
+    and samples, forward + adjoint.  `threads` instances run the same sample
+    at once, one per host thread (ctypes releases the GIL inside the oracle),
+    and the rate is their queries over the wall time from the common start to
+    the last finish: the throughput of the host's cores on this work."""
+    import threading
+    from concurrent.futures import ThreadPoolExecutor
+
+    from oracle.oracle import Oracle
+
+    threads = max(1, min(a.cpu_threads, os.cpu_count() or 1))
+    start = threading.Barrier(threads)
+
+    def one(_):
+        o = Oracle.from_spec(spec, settings)
+        o.set_sample(a.cpu_photon_stride, a.cpu_pixel_stride)
+        start.wait()
+        t0 = time.time()
+        raw = o.render_forward()[cam.name + "_raw"][..., :3]
+        o.render_backward({cam.name: ((raw - target) / raw.size).astype(np.float32)})
+        t1 = time.time()
+        cs = o.stats()
+        o.close()
+        return cs["photon_queries"] + cs["gather_queries"] + cs["adjoint_queries"], t0, t1
+
+    with ThreadPoolExecutor(threads) as ex:
+        runs = list(ex.map(one, range(threads)))
+    q = sum(r[0] for r in runs)
+    wall = max(r[2] for r in runs) - min(r[1] for r in runs)
+    per = [r[0] / (r[2] - r[1]) / 1e6 for r in runs]
+    out = {"value": round(q / wall / 1e6, 4), "unit": "Mrays/s", "cores": threads, "kind": "port",
+           "per_thread_value": round(float(np.mean(per)), 4),
+           "sample": f"{threads} concurrent oracle instances (one per host thread), each on the same scene with "
+                     f"photons/{a.cpu_photon_stride} and every {a.cpu_pixel_stride}th pixel, "
+                     f"{settings['gather_passes']} spp, depth {settings['bounces']}: {runs[0][0]} queries each, "
+                     f"{wall:.1f}s wall"}
+    print(f"[bench] cpu baseline {out['value']} Mrays/s on {threads} threads ({wall:.1f}s)", file=sys.stderr,
+          flush=True)
+    return out
+
+
 def decorrelated_leg(a, spec, settings, target, kernels):
     """The plain workload with spp_decorrelate = 1: every gather pass draws
     its own camera rays.  With the reference's default (one seed for all
@@ -636,22 +681,7 @@ def main():
 
     cpu = None
     if not a.no_cpu:
-        from oracle.oracle import Oracle
-
-        o = Oracle.from_spec(spec, settings)
-        o.set_sample(a.cpu_photon_stride, a.cpu_pixel_stride)
-        tc = time.time()
-        img = o.render_forward()
-        raw = img[cam.name + "_raw"][..., :3]
-        o.render_backward({cam.name: ((raw - target) / raw.size).astype(np.float32)})
-        cpu_s = time.time() - tc
-        cs = o.stats()
-        cq = cs["photon_queries"] + cs["gather_queries"] + cs["adjoint_queries"]
-        cpu = {"value": round(cq / cpu_s / 1e6, 4), "unit": "Mrays/s", "cores": 1, "kind": "port",
-               "sample": f"oracle (single thread) on the same scene: photons/{a.cpu_photon_stride}, every "
-                         f"{a.cpu_pixel_stride}th pixel, {settings['gather_passes']} spp, depth 6; "
-                         f"{cq} queries in {cpu_s:.1f}s"}
-        print(f"[bench] cpu baseline {cpu['value']} Mrays/s ({cpu_s:.1f}s)", file=sys.stderr, flush=True)
+        cpu = cpu_leg(a, spec, settings, target, cam)
 
     out = {
         "metric": METRIC,

@@ -57,7 +57,7 @@ def lib():
         L.orc_get_stats.argtypes = [C.c_void_p, C.POINTER(_abi.Stats)]
         L.orc_apply_points.argtypes = [C.c_void_p, C.POINTER(_abi.PointsDesc)]
         L.orc_rebuild_bvh.argtypes = [C.c_void_p]
-        L.orc_set_sample.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32]
+        L.orc_set_sample.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32]
         L.orc_full_gradients.argtypes = [C.c_void_p, _fp]
         L.orc_freeze_map.argtypes = [C.c_void_p, C.c_int32]
         L.orc_build_photon_map.argtypes = [C.c_void_p]
@@ -237,7 +237,7 @@ class Oracle:
         map of the previous render (camera seeds unchanged)."""
         lib().orc_freeze_map(self.h, 1 if on else 0)
 
-    def set_sample(self, photon_stride: int, pixel_stride: int):
+    def set_sample(self, photon_stride: int, pixel_stride: int, pixel_offset: int = 0):
         """Bounded-sample mode (CPU baseline): photons/photon_stride photons,
-        pixels with index % pixel_stride == 0."""
-        lib().orc_set_sample(self.h, int(photon_stride), int(pixel_stride))
+        pixels with index % pixel_stride == pixel_offset."""
+        lib().orc_set_sample(self.h, int(photon_stride), int(pixel_stride), int(pixel_offset))

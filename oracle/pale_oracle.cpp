@@ -306,8 +306,9 @@ struct Renderer {
     std::vector<float> gradFull;
     pale_stats stats{};
     // Bounded-sample mode of the CPU baseline: trace photons/photonStride
-    // photons and only pixels with index % pixelStride == 0.
-    uint32_t photonStride = 1, pixelStride = 1;
+    // photons and only pixels with index % pixelStride == pixelOffset
+    // (several handles with different offsets split one sample).
+    uint32_t photonStride = 1, pixelStride = 1, pixelOffset = 0;
 };
 
 static thread_local std::string g_err;
@@ -1291,7 +1292,7 @@ static void cameraGather(Renderer& R, const Camera& c, uint32_t baseSeed, std::v
     for (uint32_t pass = 0; pass < passes; ++pass) {
         const uint64_t seed = decorrelate ? (uint64_t)(uint32_t)(baseSeed + pass) : (uint64_t)baseSeed;
         for (uint32_t pi = 0; pi < npx; ++pi) {
-            if (pi % R.pixelStride) continue;
+            if (pi % R.pixelStride != R.pixelOffset) continue;
             const uint32_t px = pi % W, py = pi / W;
             Xorshift128 rng(perItemSeed(seed, pi));
             const float jx = rng.nextFloat() - 0.5f;
@@ -1629,7 +1630,7 @@ static void renderBackward(Renderer& R, const std::vector<const Camera*>& camLis
             std::vector<ARay> q, nq;
             q.reserve(npx / R.pixelStride + 1);
             for (uint32_t pi = 0; pi < npx; ++pi) {
-                if (pi % R.pixelStride) continue;
+                if (pi % R.pixelStride != R.pixelOffset) continue;
                 Xorshift128 rng(perItemSeed(number, pi));
                 const float jx = rng.nextFloat() - 0.5f;
                 const float jy = rng.nextFloat() - 0.5f;
@@ -2167,10 +2168,11 @@ int orc_freeze_map(void* h, int32_t on) {
     return 0;
 }
 
-int orc_set_sample(void* h, uint32_t photonStride, uint32_t pixelStride) {
+int orc_set_sample(void* h, uint32_t photonStride, uint32_t pixelStride, uint32_t pixelOffset) {
     Renderer& R = *static_cast<Renderer*>(h);
     R.photonStride = std::max(1u, photonStride);
     R.pixelStride = std::max(1u, pixelStride);
+    R.pixelOffset = pixelOffset % R.pixelStride;
     return 0;
 }
 
