@@ -571,7 +571,8 @@ __global__ __launch_bounds__(256) void k_cell_sort(uint32_t total, const uint32_
 
 // Cells of more than 64 photons, one workgroup each: the cell in LDS, each
 // photon's place = the number of smaller indices.  A cell beyond the LDS
-// tile (none at the benchmark scenes' densities) is sorted by one thread.
+// tile (more than 4096 photons, which clustered scenes of millions of
+// photons do not reach) is sorted by one thread.
 __global__ __launch_bounds__(256) void k_cell_sort_big(const DevState* st, const uint32_t* bigList,
                                                        const uint32_t* cellStart, uint32_t* sortedIdx) {
     __shared__ uint32_t v[kBlockCell];
@@ -624,10 +625,11 @@ __global__ __launch_bounds__(256) void k_cell_payload(const DevState* st, uint32
 //                   queries only) and append one entry per layer: the gather
 //                   point and normal, the layer's weight w and colour factor
 //                   k, in layer order within the pixel;
-//   k_entry_key     entry -> slot of its pixel's ordered list, and a
+//   k_entry_key     entry -> place in its pixel's layer list, and a
 //                   counting sort of the gather entries by photon-grid cell;
 //   k_entry_gather  one irradiance gather per entry, neighbouring lanes on
 //                   neighbouring points (shared photon rows, similar loops);
+//                   w * (E * k) goes to the entry's place in the layer list;
 //   k_cam_composite per pixel, C = C + w * (E * k) over the layers in order,
 //                   then the reference's framebuffer update.
 //
