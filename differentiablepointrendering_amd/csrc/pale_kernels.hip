@@ -702,6 +702,9 @@ struct BatchMap {
     }
 };
 
+// Slots a wave of the layer walk claims at a time from its region.
+constexpr uint32_t kWalkUnit = 64;
+
 template <bool kCount, typename Map>
 __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_cam_trace(
     DevScene S, DevState* st, Map map, PixelArrays px, EntryArrays ent, uint32_t cap, uint32_t* cursor) {
@@ -711,6 +714,7 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(4))
     const uint32_t slots = map.slots();
     const uint32_t tiles = slots >> 6;
     uint32_t region = blockIdx.x % kTraceRegions, emptyRegions = 0;
+    uint32_t chunkNext = 0, chunkEnd = 0;  // the wave's claimed, untaken slots
     uint32_t layers = 0;
     Work w{};
     bool active = false, exhausted = false;
@@ -725,31 +729,43 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(4))
                 // the slots drawn from it this round.
                 const uint32_t lo = (uint32_t)(((uint64_t)region * tiles) / kTraceRegions) * 64u;
                 const uint32_t hi = (uint32_t)(((uint64_t)(region + 1u) * tiles) / kTraceRegions) * 64u;
-                const uint32_t leader = (uint32_t)__ffsll((long long)need) - 1u;
-                uint32_t base = 0;
-                if (lane == leader) base = lo + atomicAdd(cursor + region, (uint32_t)__popcll(need));
-                base = __shfl(base, (int)leader, 64);
-                const bool regionDone = base >= hi;
-                if (regionDone) {
-                    // Used up (it stays so): the next region, done after all.
-                    if (++emptyRegions == kTraceRegions) exhausted = true;
-                    region = (region + 1u) % kTraceRegions;
-                } else if (*reinterpret_cast<volatile const uint32_t*>(&st->overflow) != 0u) {
-                    exhausted = true;
-                }
-                if (!active && !regionDone) {
-                    const uint32_t slot = base + (uint32_t)__popcll(need & ((1ull << lane) - 1ull));
-                    uint32_t p = 0;
-                    if (slot < hi && map.locate(slot, p) && f2u(px.cs[p].w) == kPixActive) {
-                        pixel = p;
-                        active = true;
-                        const float4 a = px.ot[p], b = px.dl[p];
-                        o = mk3(a.x, a.y, a.z);
-                        tr = a.w;
-                        d = mk3(b.x, b.y, b.z);
-                        layer = f2u(b.w);
-                        r = 0;
+                // The wave claims a whole 64-slot unit (a few pixels in all
+                // their passes) and keeps what its lanes have not taken for
+                // its next draws: its lanes stay on neighbouring rays.
+                // Claiming only what the lanes need interleaves the waves'
+                // draws and scatters each wave over many pixels (the walk
+                // took 17 % longer, 28 % with decorrelated passes).
+                if (chunkNext >= chunkEnd) {
+                    const uint32_t leader = (uint32_t)__ffsll((long long)need) - 1u;
+                    uint32_t base = 0;
+                    if (lane == leader) base = lo + atomicAdd(cursor + region, kWalkUnit);
+                    base = __shfl(base, (int)leader, 64);
+                    if (base >= hi) {
+                        // Used up (it stays so): the next region, done after all.
+                        if (++emptyRegions == kTraceRegions) exhausted = true;
+                        region = (region + 1u) % kTraceRegions;
+                    } else {
+                        chunkNext = base;
+                        chunkEnd = min(base + kWalkUnit, hi);
+                        if (*reinterpret_cast<volatile const uint32_t*>(&st->overflow) != 0u) exhausted = true;
                     }
+                }
+                if (chunkNext < chunkEnd) {
+                    if (!active) {
+                        const uint32_t slot = chunkNext + (uint32_t)__popcll(need & ((1ull << lane) - 1ull));
+                        uint32_t p = 0;
+                        if (slot < chunkEnd && map.locate(slot, p) && f2u(px.cs[p].w) == kPixActive) {
+                            pixel = p;
+                            active = true;
+                            const float4 a = px.ot[p], b = px.dl[p];
+                            o = mk3(a.x, a.y, a.z);
+                            tr = a.w;
+                            d = mk3(b.x, b.y, b.z);
+                            layer = f2u(b.w);
+                            r = 0;
+                        }
+                    }
+                    chunkNext = min(chunkNext + (uint32_t)__popcll(need), chunkEnd);
                 }
             }
         }
