@@ -353,7 +353,9 @@ int pale_render_view_backward_device(pale_handle* h,
  * tuning. ---- */
 
 typedef struct pale_batch_tuning {
-    uint32_t slots_per_batch;  /* pixel slots (pixels x passes) per batch; 0 = 4Mi       */
+    uint32_t slots_per_batch;  /* pixel slots (pixels x passes) per batch; 0 = forward:
+                                  16 passes of the largest view within [4Mi, 32Mi],
+                                  adjoint: 4Mi                                         */
     uint32_t entries_per_slot; /* camera-gather entries per slot and round; 0 = 24        */
     uint32_t graph_rounds;     /* rounds in the captured graph; 0 = eager launches        */
     float grid_fraction;       /* layer-walk grid / full occupancy grid, (0,1]; 0 = 1     */
