@@ -21,7 +21,7 @@ CANDIDATES: Dict[str, Sequence] = {
     "graph_rounds": (0, 1, 2),
     "grid_fraction": (1.0, 0.75),
 }
-DEFAULT = {"slots_per_batch": 4 << 20, "entries_per_slot": 24, "graph_rounds": 1, "grid_fraction": 1.0}
+DEFAULT = {"slots_per_batch": 16 << 20, "entries_per_slot": 24, "graph_rounds": 1, "grid_fraction": 1.0}
 
 
 def measure(renderer, tuning: dict, cameras, spp, seeds, dloss=None, repeats: int = 1) -> float:
