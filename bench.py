@@ -223,12 +223,12 @@ def batched_leg(a):
         for k in keys:
             p_grad[k] = p_grad[k] + g[k]
     plain_wall = time.time() - tw
-    # Autotuner on a sample: the first 8 views at their spp (how many passes
-    # of a view share a batch is what the batch size decides).
+    # Autotuner on the workload itself (how many passes of which views share
+    # a batch is what the batch size decides, so a reduced sample misleads it).
     tune = None
     if not a.no_tune:
-        ns = min(8, len(cams))
-        sc, ss = cams[:ns], spp[:ns]
+        ns = len(cams)
+        sc, ss = cams, spp
         sd = [dloss(np.zeros((c.height, c.width, 4), np.float32), tg) for c, tg in zip(sc, targets[:ns])]
         tune = autotune.tune(r, sc, ss, seeds[:ns], sd, total_slots=slots)
         print(f"[bench] batched tuning {tune['tuning']} ({tune['seconds']}s)", file=sys.stderr, flush=True)
@@ -266,7 +266,7 @@ def batched_leg(a):
         "same_images_as_plain": bool(same_images),
         "gradient_max_err_over_max": grad_err,
         "tuning": tune["tuning"] if tune else autotune.DEFAULT,
-        "autotune": {"sample": "first 8 views at their spp", "seconds": tune["seconds"], "table": tune["table"]}
+        "autotune": {"sample": f"the {len(cams)} views at their spp", "seconds": tune["seconds"], "table": tune["table"]}
         if tune else None,
     }
     print(f"[bench] batched {res['mrays_per_s']} Mrays/s vs one view at a time "

@@ -16,7 +16,7 @@ import time
 from typing import Dict, List, Optional, Sequence
 
 CANDIDATES: Dict[str, Sequence] = {
-    "slots_per_batch": (1 << 20, 2 << 20, 4 << 20, 8 << 20, 16 << 20, 32 << 20),
+    "slots_per_batch": (1 << 20, 2 << 20, 4 << 20, 8 << 20, 16 << 20, 32 << 20, 64 << 20),
     "entries_per_slot": (16, 24, 32),
     "graph_rounds": (0, 1, 2),
     "grid_fraction": (1.0, 0.75),
