@@ -226,6 +226,33 @@ def test_gather_points_bit_exact(c0_small):
     assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
 
 
+def _cell_counts(orc):
+    ph, g = orc.photons(), orc.grid_info()
+    f = np.floor((ph["position"] - g["origin"]) / g["cell"]).astype(np.int64)
+    f = np.clip(f, 0, g["res"] - 1)
+    cid = (f[:, 2] * g["res"][1] + f[:, 1]) * g["res"][0] + f[:, 0]
+    return np.bincount(cid)
+
+
+@pytest.mark.parametrize("radius,photons,least", [(0.3, 60_000, 65), (1.0, 300_000, 4097)])
+def test_gather_bit_exact_with_crowded_cells(radius, photons, least):
+    """Cells of more than 64 photons are ordered by a workgroup in LDS, cells
+    of more than 4096 by one thread; the gathers over them must still add in
+    deposit order (bit-exact against the oracle)."""
+    spec = scene.config_scene(0, n=4096, width=32, height=32)
+    gpu, orc = _pair(spec, dict(photons=photons, gather_radius=radius))
+    gpu.render_forward()
+    orc.render_forward()
+    assert _cell_counts(orc).max() >= least
+    q, n = _gather_probe_points(orc, np.random.default_rng(7))
+    a = gpu.gather_irradiance(q, n)
+    b = orc.gather(q, n)
+    assert np.count_nonzero(b[:, 0]) > 1000
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    gpu.close()
+    orc.close()
+
+
 def test_camera_gather_rounds_bit_exact():
     # A one-entry-per-pixel budget forces several rounds per pass (about
     # seven layers per pixel here): pixels stop at a layer that does not fit
