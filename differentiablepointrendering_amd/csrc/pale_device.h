@@ -86,19 +86,31 @@ __device__ __forceinline__ bool slab(F3 o, F3 inv, float mnx, float mny, float m
     return true;
 }
 
+// Entries below inLds live in LDS (entry k at base[k * stride]); deeper
+// ones, if any, in a per-lane global area (entry k at
+// spill[(k - inLds) * spillStride]).
 struct LaneStack {
     uint2* base;
     uint32_t stride;
     uint32_t cap;
+    uint2* spill;
+    uint32_t spillStride;
+    uint32_t inLds;
     // SmallStack::push (GPUDataTypes.h:398-402): a push beyond capacity is
     // dropped, so sp never exceeds cap and every pop reads a written slot.
     __device__ __forceinline__ void push(uint32_t& sp, uint32_t ref, float t) const {
         if (sp < cap) {
-            base[sp * stride] = make_uint2(ref, __float_as_uint(t));
+            const uint2 v = make_uint2(ref, __float_as_uint(t));
+            if (sp < inLds)
+                base[sp * stride] = v;
+            else
+                spill[(sp - inLds) * spillStride] = v;
             ++sp;
         }
     }
-    __device__ __forceinline__ uint2 get(uint32_t k) const { return base[k * stride]; }
+    __device__ __forceinline__ uint2 get(uint32_t k) const {
+        return k < inLds ? base[k * stride] : spill[(k - inLds) * spillStride];
+    }
 };
 
 struct LocalHit {

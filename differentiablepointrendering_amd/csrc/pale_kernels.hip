@@ -83,7 +83,17 @@ __device__ __forceinline__ uint32_t loadCount(const uint32_t* count, uint32_t ma
 }
 
 __device__ __forceinline__ LaneStack laneStack(uint2* lds, uint32_t depth) {
-    return LaneStack{lds + threadIdx.x, blockDim.x, depth};
+    return LaneStack{lds + threadIdx.x, blockDim.x, depth, nullptr, 0u, depth};
+}
+
+// The layer walk's stack: its first kWalkLdsEntries entries in LDS, the rest
+// in the spill area (entry k of lane l at spill[(k - kWalkLdsEntries) * lanes
+// + l]), so that more waves fit on a CU; spill == nullptr: all in LDS.
+__device__ __forceinline__ LaneStack walkStack(uint2* lds, uint32_t depth, uint2* spill) {
+    if (!spill) return laneStack(lds, depth);
+    const uint32_t lanes = gridDim.x * blockDim.x;
+    return LaneStack{lds + threadIdx.x, blockDim.x, depth, spill + blockIdx.x * blockDim.x + threadIdx.x, lanes,
+                     min(depth, kWalkLdsEntries)};
 }
 
 __device__ __forceinline__ float4 f4(F3 v, float w) { return make_float4(v.x, v.y, v.z, w); }
@@ -707,9 +717,10 @@ constexpr uint32_t kWalkUnit = 64;
 
 template <bool kCount, typename Map>
 __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_cam_trace(
-    DevScene S, DevState* st, Map map, PixelArrays px, EntryArrays ent, uint32_t cap, uint32_t* cursor) {
+    DevScene S, DevState* st, Map map, PixelArrays px, EntryArrays ent, uint32_t cap, uint32_t* cursor,
+    uint2* spill) {
     extern __shared__ uint2 s_stack[];
-    const LaneStack stk = laneStack(s_stack, S.stackDepth);
+    const LaneStack stk = walkStack(s_stack, S.stackDepth, spill);
     const uint32_t lane = laneId();
     const uint32_t slots = map.slots();
     const uint32_t tiles = slots >> 6;
@@ -1674,17 +1685,31 @@ void launchCamInit(hipStream_t s, const DevCamera& cam, const DevState* st, uint
     if (npx) k_cam_init<<<blocks(npx, 256), 256, 0, s>>>(cam, st, pass, decorrelate, px);
 }
 
+// LDS of one layer-walk workgroup: the in-LDS part of its lanes' stacks.
+static size_t walkLdsBytes(const DevScene& S, const uint2* spill) {
+    return spill ? (size_t)std::min(S.stackDepth, kWalkLdsEntries) * sizeof(uint2) * kTraceBlock : stackBytes(S);
+}
+
+size_t walkSpillBytes(const DevScene& S) {
+    if (S.stackDepth <= kWalkLdsEntries) return 0;
+    const size_t lds = (size_t)kWalkLdsEntries * sizeof(uint2) * kTraceBlock;
+    const uint32_t grid = std::max(std::max(camTraceGrid<false, BatchMap>(lds), camTraceGrid<true, BatchMap>(lds)),
+                                   std::max(camTraceGrid<false, ImageMap>(lds), camTraceGrid<true, ImageMap>(lds)));
+    return (size_t)(S.stackDepth - kWalkLdsEntries) * grid * kTraceBlock * sizeof(uint2);
+}
+
 void launchCamTrace(hipStream_t s, const DevScene& S, DevState* st, uint32_t npx, uint32_t width, PixelArrays px,
-                    EntryArrays ent, uint32_t cap, uint32_t* cursor, bool count) {
+                    EntryArrays ent, uint32_t cap, uint32_t* cursor, uint2* spill, bool count) {
     if (!npx) return;
-    const size_t lds = stackBytes(S);
+    if (S.stackDepth <= kWalkLdsEntries) spill = nullptr;
+    const size_t lds = walkLdsBytes(S, spill);
     const ImageMap map{width, npx / width, (width + 7u) / 8u};
     if (count) {
         const uint32_t g = min(camTraceGrid<true, ImageMap>(lds), blocks(npx, kTraceBlock));
-        k_cam_trace<true><<<g, kTraceBlock, lds, s>>>(S, st, map, px, ent, cap, cursor);
+        k_cam_trace<true><<<g, kTraceBlock, lds, s>>>(S, st, map, px, ent, cap, cursor, spill);
     } else {
         const uint32_t g = min(camTraceGrid<false, ImageMap>(lds), blocks(npx, kTraceBlock));
-        k_cam_trace<false><<<g, kTraceBlock, lds, s>>>(S, st, map, px, ent, cap, cursor);
+        k_cam_trace<false><<<g, kTraceBlock, lds, s>>>(S, st, map, px, ent, cap, cursor, spill);
     }
 }
 
@@ -1693,19 +1718,20 @@ void launchBatchInit(hipStream_t s, BatchDesc b, uint32_t maxTiles, PixelArrays 
 }
 
 void launchCamTraceBatch(hipStream_t s, const DevScene& S, DevState* st, BatchDesc b, uint32_t maxSlots,
-                         PixelArrays px, EntryArrays ent, uint32_t cap, uint32_t* cursor, float gridFraction,
-                         bool count) {
+                         PixelArrays px, EntryArrays ent, uint32_t cap, uint32_t* cursor, uint2* spill,
+                         float gridFraction, bool count) {
     if (!maxSlots) return;
-    const size_t lds = stackBytes(S);
+    if (S.stackDepth <= kWalkLdsEntries) spill = nullptr;
+    const size_t lds = walkLdsBytes(S, spill);
     const BatchMap map{b.tiles, b.segs, b.hdr};
     if (count) {
         const uint32_t full = camTraceGrid<true, BatchMap>(lds);
         const uint32_t g = min(max(1u, (uint32_t)((float)full * gridFraction)), blocks(maxSlots, kTraceBlock));
-        k_cam_trace<true><<<g, kTraceBlock, lds, s>>>(S, st, map, px, ent, cap, cursor);
+        k_cam_trace<true><<<g, kTraceBlock, lds, s>>>(S, st, map, px, ent, cap, cursor, spill);
     } else {
         const uint32_t full = camTraceGrid<false, BatchMap>(lds);
         const uint32_t g = min(max(1u, (uint32_t)((float)full * gridFraction)), blocks(maxSlots, kTraceBlock));
-        k_cam_trace<false><<<g, kTraceBlock, lds, s>>>(S, st, map, px, ent, cap, cursor);
+        k_cam_trace<false><<<g, kTraceBlock, lds, s>>>(S, st, map, px, ent, cap, cursor, spill);
     }
 }
 

@@ -121,6 +121,8 @@ struct pale_handle {
     // Ray queues and staging.
     uint32_t queueCap = 0;
     DevMem qA[3], qB[3], stg[3], extFlags, depStg[3], depFlags, tiles, counts;
+    DevMem walkSpill;  // layer-walk stack entries beyond the LDS part (deep BVHs)
+    uint2* walkSpillPtr() const { return walkSpill.p ? walkSpill.as<uint2>() : nullptr; }
     // Adjoint.
     uint32_t adjCap = 0, poolCap = 0;
     DevMem pend0, pend1, pend2, pend3, pend4, evStg[6], evCmp[6], evCell, evSorted, evFlags, dloss, gradOpacity,
@@ -276,6 +278,7 @@ struct pale_handle {
         scene.stackDepth = img.stackDepth;
         if ((size_t)img.stackDepth * sizeof(uint2) * kTraceBlock > 65536)
             throw StatusError(PALE_ERR_INVALID_ARG, "BVH too deep for the LDS traversal stack");
+        if (const size_t sb = walkSpillBytes(scene)) walkSpill.reserve(sb);
         gradOpacity.reserve(std::max<size_t>(1, bp.points.size()) * sizeof(float));
         HIPCHECK(hipStreamSynchronize(stream));
     }
@@ -593,7 +596,8 @@ struct pale_handle {
             launchFill(stream, entries, 2, 0u);  // entries, overflow
             launchFill(stream, cursors.as<uint32_t>(), kTraceRegions, 0u);
             const size_t tt = tBegin(kTCamTrace);
-            launchCamTrace(stream, scene, st, npx, dc.W, px, ent, entryCap, cursors.as<uint32_t>(), countWork);
+            launchCamTrace(stream, scene, st, npx, dc.W, px, ent, entryCap, cursors.as<uint32_t>(), walkSpillPtr(),
+                           countWork);
             tEnd(tt);
             const size_t ts = tBegin(kTCamSort);
             launchExclusiveScan(stream, px.cnt, px.off, npx, pxTiles.as<uint32_t>(), gridTotal.as<uint32_t>());
@@ -784,7 +788,7 @@ struct pale_handle {
         launchFill(stream, cursors.as<uint32_t>(), kTraceRegions, 0u);
         const size_t tt = timed ? tBegin(kTCamTrace) : 0;
         launchCamTraceBatch(stream, scene, st(), batchDesc(), maxSlots, px, ent, entryCap, cursors.as<uint32_t>(),
-                            fraction, count);
+                            walkSpillPtr(), fraction, count);
         if (timed) tEnd(tt);
         const size_t ts = timed ? tBegin(kTCamSort) : 0;
         launchExclusiveScan(stream, px.cnt, px.off, maxSlots, pxTiles.as<uint32_t>(), gridTotal.as<uint32_t>());
@@ -822,6 +826,7 @@ struct pale_handle {
         keyAdd(key, grid);
         keyAdd(key, state.p);
         keyAdd(key, cursors.p);
+        keyAdd(key, walkSpill.p);
         keyAdd(key, entStart.p);
         keyAdd(key, pxTiles.p);
         keyAdd(key, gridTiles.p);
