@@ -89,8 +89,10 @@ __device__ __forceinline__ bool slab(F3 o, F3 inv, float mnx, float mny, float m
 // Entries below inLds live in LDS (entry k at base[k * stride]); deeper
 // ones, if any, in a per-lane global area (entry k at
 // spill[(k - inLds) * spillStride]).
+typedef __attribute__((address_space(3))) uint2 LdsUint2;  // an LDS word pair
+
 struct LaneStack {
-    uint2* base;
+    LdsUint2* base;
     uint32_t stride;
     uint32_t cap;
     uint2* spill;
@@ -108,8 +110,13 @@ struct LaneStack {
             ++sp;
         }
     }
+    // The LDS read is unconditional (an LDS instruction, not a generic one
+    // that could reach either memory); a spilled entry is then read again
+    // from the spill area.
     __device__ __forceinline__ uint2 get(uint32_t k) const {
-        return k < inLds ? base[k * stride] : spill[(k - inLds) * spillStride];
+        uint2 v = base[min(k, inLds - 1u) * stride];
+        if (k >= inLds) v = spill[(k - inLds) * spillStride];
+        return v;
     }
 };
 

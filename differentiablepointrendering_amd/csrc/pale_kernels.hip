@@ -83,7 +83,7 @@ __device__ __forceinline__ uint32_t loadCount(const uint32_t* count, uint32_t ma
 }
 
 __device__ __forceinline__ LaneStack laneStack(uint2* lds, uint32_t depth) {
-    return LaneStack{lds + threadIdx.x, blockDim.x, depth, nullptr, 0u, depth};
+    return LaneStack{(LdsUint2*)(lds + threadIdx.x), blockDim.x, depth, nullptr, 0u, depth};
 }
 
 // The layer walk's stack: its first kWalkLdsEntries entries in LDS, the rest
@@ -92,8 +92,8 @@ __device__ __forceinline__ LaneStack laneStack(uint2* lds, uint32_t depth) {
 __device__ __forceinline__ LaneStack walkStack(uint2* lds, uint32_t depth, uint2* spill) {
     if (!spill) return laneStack(lds, depth);
     const uint32_t lanes = gridDim.x * blockDim.x;
-    return LaneStack{lds + threadIdx.x, blockDim.x, depth, spill + blockIdx.x * blockDim.x + threadIdx.x, lanes,
-                     min(depth, kWalkLdsEntries)};
+    return LaneStack{(LdsUint2*)(lds + threadIdx.x), blockDim.x, depth, spill + blockIdx.x * blockDim.x + threadIdx.x,
+                     lanes, min(depth, kWalkLdsEntries)};
 }
 
 __device__ __forceinline__ float4 f4(F3 v, float w) { return make_float4(v.x, v.y, v.z, w); }
