@@ -89,7 +89,13 @@ __device__ __forceinline__ bool slab(F3 o, F3 inv, float mnx, float mny, float m
 // Entries below inLds live in LDS (entry k at base[k * stride]); deeper
 // ones, if any, in a per-lane global area (entry k at
 // spill[(k - inLds) * spillStride]).
-typedef __attribute__((address_space(3))) uint2 LdsUint2;  // an LDS word pair
+// An LDS word pair.  The device pass types it as LDS memory so that stack
+// accesses compile to LDS instructions; the host pass only parses the code.
+#if defined(__HIP_DEVICE_COMPILE__)
+typedef __attribute__((address_space(3))) uint2 LdsUint2;
+#else
+typedef uint2 LdsUint2;
+#endif
 
 struct LaneStack {
     LdsUint2* base;
