@@ -501,8 +501,15 @@ def cpu_leg(a, spec, settings, target, cam):
     q = sum(r[0] for r in runs)
     wall = max(r[2] for r in runs) - min(r[1] for r in runs)
     per = [r[0] / (r[2] - r[1]) / 1e6 for r in runs]
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), model)
+    except OSError:
+        pass
     out = {"value": round(q / wall / 1e6, 4), "unit": "Mrays/s", "cores": threads, "kind": "port",
            "per_thread_value": round(float(np.mean(per)), 4),
+           "host": {"cpu": model, "cpu_count": os.cpu_count()},
            "sample": f"{threads} concurrent oracle instances (one per host thread), each on the same scene with "
                      f"photons/{a.cpu_photon_stride} and every {a.cpu_pixel_stride}th pixel, "
                      f"{settings['gather_passes']} spp, depth {settings['bounces']}: {runs[0][0]} queries each, "
