@@ -58,6 +58,7 @@ class BatchTuning(C.Structure):
         ("entries_per_slot", C.c_uint32),
         ("graph_rounds", C.c_uint32),
         ("grid_fraction", C.c_float),
+        ("walk_lds_entries", C.c_uint32),
     ]
 
     def as_dict(self):

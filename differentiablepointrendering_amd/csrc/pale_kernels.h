@@ -224,12 +224,13 @@ void launchExclusiveScan(hipStream_t s, const uint32_t* in, uint32_t* out, uint3
 // Camera gather of one pass (see pale_kernels.hip for the stages).
 void launchCamInit(hipStream_t s, const DevCamera& cam, const DevState* st, uint32_t pass, int decorrelate,
                    PixelArrays px);
-// Layer walk: the first kWalkLdsEntries stack entries of a lane in LDS, the
-// rest in a spill area of walkSpillBytes(S) (null: the whole stack in LDS).
+// Layer walk: the first inLds stack entries of a lane (default
+// kWalkLdsEntries) in LDS, the rest in a spill area of
+// walkSpillBytes(S, inLds) (null: the whole stack in LDS).
 constexpr uint32_t kWalkLdsEntries = 16;
-size_t walkSpillBytes(const DevScene& S);
+size_t walkSpillBytes(const DevScene& S, uint32_t inLds);
 void launchCamTrace(hipStream_t s, const DevScene& S, DevState* st, uint32_t npx, uint32_t width, PixelArrays px,
-                    EntryArrays ent, uint32_t cap, uint32_t* cursor, uint2* spill, bool count);
+                    EntryArrays ent, uint32_t cap, uint32_t* cursor, uint2* spill, uint32_t inLds, bool count);
 void launchEntrySort(hipStream_t s, const GridParams& g, DevState* st, uint32_t cap, PixelArrays px,
                      EntryArrays ent, uint32_t* cellCount, uint32_t* cellStart, uint32_t* tiles,
                      uint32_t* scratchTotal);
@@ -243,7 +244,7 @@ void launchCamComposite(hipStream_t s, uint32_t npx, PixelArrays px, EntryArrays
 void launchBatchInit(hipStream_t s, BatchDesc b, uint32_t maxTiles, PixelArrays px);
 void launchCamTraceBatch(hipStream_t s, const DevScene& S, DevState* st, BatchDesc b, uint32_t maxSlots,
                          PixelArrays px, EntryArrays ent, uint32_t cap, uint32_t* cursor, uint2* spill,
-                         float gridFraction, bool count);
+                         uint32_t inLds, float gridFraction, bool count);
 void launchCamCompositeBatch(hipStream_t s, uint32_t maxSlots, const BatchHeader* hdr, PixelArrays px,
                              EntryArrays ent);
 void launchBatchResolve(hipStream_t s, BatchDesc b, uint32_t maxResolve, PixelArrays px, float4* fb);

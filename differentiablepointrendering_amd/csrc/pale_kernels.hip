@@ -86,14 +86,14 @@ __device__ __forceinline__ LaneStack laneStack(uint2* lds, uint32_t depth) {
     return LaneStack{(LdsUint2*)(lds + threadIdx.x), blockDim.x, depth, nullptr, 0u, depth};
 }
 
-// The layer walk's stack: its first kWalkLdsEntries entries in LDS, the rest
-// in the spill area (entry k of lane l at spill[(k - kWalkLdsEntries) * lanes
-// + l]), so that more waves fit on a CU; spill == nullptr: all in LDS.
-__device__ __forceinline__ LaneStack walkStack(uint2* lds, uint32_t depth, uint2* spill) {
+// The layer walk's stack: its first inLds entries in LDS, the rest in the
+// spill area (entry k of lane l at spill[(k - inLds) * lanes + l]), so that
+// more waves fit on a CU; spill == nullptr: all in LDS.
+__device__ __forceinline__ LaneStack walkStack(uint2* lds, uint32_t depth, uint2* spill, uint32_t inLds) {
     if (!spill) return laneStack(lds, depth);
     const uint32_t lanes = gridDim.x * blockDim.x;
     return LaneStack{(LdsUint2*)(lds + threadIdx.x), blockDim.x, depth, spill + blockIdx.x * blockDim.x + threadIdx.x,
-                     lanes, min(depth, kWalkLdsEntries)};
+                     lanes, inLds};
 }
 
 __device__ __forceinline__ float4 f4(F3 v, float w) { return make_float4(v.x, v.y, v.z, w); }
@@ -718,9 +718,9 @@ constexpr uint32_t kWalkUnit = 64;
 template <bool kCount, typename Map>
 __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_cam_trace(
     DevScene S, DevState* st, Map map, PixelArrays px, EntryArrays ent, uint32_t cap, uint32_t* cursor,
-    uint2* spill) {
+    uint2* spill, uint32_t inLds) {
     extern __shared__ uint2 s_stack[];
-    const LaneStack stk = walkStack(s_stack, S.stackDepth, spill);
+    const LaneStack stk = walkStack(s_stack, S.stackDepth, spill, inLds);
     const uint32_t lane = laneId();
     const uint32_t slots = map.slots();
     const uint32_t tiles = slots >> 6;
@@ -1686,30 +1686,31 @@ void launchCamInit(hipStream_t s, const DevCamera& cam, const DevState* st, uint
 }
 
 // LDS of one layer-walk workgroup: the in-LDS part of its lanes' stacks.
-static size_t walkLdsBytes(const DevScene& S, const uint2* spill) {
-    return spill ? (size_t)std::min(S.stackDepth, kWalkLdsEntries) * sizeof(uint2) * kTraceBlock : stackBytes(S);
+static size_t walkLdsBytes(const DevScene& S, const uint2* spill, uint32_t inLds) {
+    return spill ? (size_t)inLds * sizeof(uint2) * kTraceBlock : stackBytes(S);
 }
 
-size_t walkSpillBytes(const DevScene& S) {
-    if (S.stackDepth <= kWalkLdsEntries) return 0;
-    const size_t lds = (size_t)kWalkLdsEntries * sizeof(uint2) * kTraceBlock;
+size_t walkSpillBytes(const DevScene& S, uint32_t inLds) {
+    if (S.stackDepth <= inLds) return 0;
+    const size_t lds = (size_t)inLds * sizeof(uint2) * kTraceBlock;
     const uint32_t grid = std::max(std::max(camTraceGrid<false, BatchMap>(lds), camTraceGrid<true, BatchMap>(lds)),
                                    std::max(camTraceGrid<false, ImageMap>(lds), camTraceGrid<true, ImageMap>(lds)));
-    return (size_t)(S.stackDepth - kWalkLdsEntries) * grid * kTraceBlock * sizeof(uint2);
+    return (size_t)(S.stackDepth - inLds) * grid * kTraceBlock * sizeof(uint2);
 }
 
 void launchCamTrace(hipStream_t s, const DevScene& S, DevState* st, uint32_t npx, uint32_t width, PixelArrays px,
-                    EntryArrays ent, uint32_t cap, uint32_t* cursor, uint2* spill, bool count) {
+                    EntryArrays ent, uint32_t cap, uint32_t* cursor, uint2* spill, uint32_t inLds, bool count) {
     if (!npx) return;
-    if (S.stackDepth <= kWalkLdsEntries) spill = nullptr;
-    const size_t lds = walkLdsBytes(S, spill);
+    inLds = std::max(1u, inLds);
+    if (S.stackDepth <= inLds) spill = nullptr;
+    const size_t lds = walkLdsBytes(S, spill, inLds);
     const ImageMap map{width, npx / width, (width + 7u) / 8u};
     if (count) {
         const uint32_t g = min(camTraceGrid<true, ImageMap>(lds), blocks(npx, kTraceBlock));
-        k_cam_trace<true><<<g, kTraceBlock, lds, s>>>(S, st, map, px, ent, cap, cursor, spill);
+        k_cam_trace<true><<<g, kTraceBlock, lds, s>>>(S, st, map, px, ent, cap, cursor, spill, inLds);
     } else {
         const uint32_t g = min(camTraceGrid<false, ImageMap>(lds), blocks(npx, kTraceBlock));
-        k_cam_trace<false><<<g, kTraceBlock, lds, s>>>(S, st, map, px, ent, cap, cursor, spill);
+        k_cam_trace<false><<<g, kTraceBlock, lds, s>>>(S, st, map, px, ent, cap, cursor, spill, inLds);
     }
 }
 
@@ -1719,19 +1720,20 @@ void launchBatchInit(hipStream_t s, BatchDesc b, uint32_t maxTiles, PixelArrays 
 
 void launchCamTraceBatch(hipStream_t s, const DevScene& S, DevState* st, BatchDesc b, uint32_t maxSlots,
                          PixelArrays px, EntryArrays ent, uint32_t cap, uint32_t* cursor, uint2* spill,
-                         float gridFraction, bool count) {
+                         uint32_t inLds, float gridFraction, bool count) {
     if (!maxSlots) return;
-    if (S.stackDepth <= kWalkLdsEntries) spill = nullptr;
-    const size_t lds = walkLdsBytes(S, spill);
+    inLds = std::max(1u, inLds);
+    if (S.stackDepth <= inLds) spill = nullptr;
+    const size_t lds = walkLdsBytes(S, spill, inLds);
     const BatchMap map{b.tiles, b.segs, b.hdr};
     if (count) {
         const uint32_t full = camTraceGrid<true, BatchMap>(lds);
         const uint32_t g = min(max(1u, (uint32_t)((float)full * gridFraction)), blocks(maxSlots, kTraceBlock));
-        k_cam_trace<true><<<g, kTraceBlock, lds, s>>>(S, st, map, px, ent, cap, cursor, spill);
+        k_cam_trace<true><<<g, kTraceBlock, lds, s>>>(S, st, map, px, ent, cap, cursor, spill, inLds);
     } else {
         const uint32_t full = camTraceGrid<false, BatchMap>(lds);
         const uint32_t g = min(max(1u, (uint32_t)((float)full * gridFraction)), blocks(maxSlots, kTraceBlock));
-        k_cam_trace<false><<<g, kTraceBlock, lds, s>>>(S, st, map, px, ent, cap, cursor, spill);
+        k_cam_trace<false><<<g, kTraceBlock, lds, s>>>(S, st, map, px, ent, cap, cursor, spill, inLds);
     }
 }
 

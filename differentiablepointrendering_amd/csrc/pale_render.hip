@@ -123,6 +123,13 @@ struct pale_handle {
     DevMem qA[3], qB[3], stg[3], extFlags, depStg[3], depFlags, tiles, counts;
     DevMem walkSpill;  // layer-walk stack entries beyond the LDS part (deep BVHs)
     uint2* walkSpillPtr() const { return walkSpill.p ? walkSpill.as<uint2>() : nullptr; }
+    uint32_t walkLds() const { return tune.walk_lds_entries ? tune.walk_lds_entries : kWalkLdsEntries; }
+    // The spill area of the layer walk for the current scene and tuning
+    // (before any launch: a captured graph holds its address).
+    void ensureWalkSpill() {
+        if (!scene.nTlas) return;
+        if (const size_t sb = walkSpillBytes(scene, walkLds())) walkSpill.reserve(sb);
+    }
     // Adjoint.
     uint32_t adjCap = 0, poolCap = 0;
     DevMem pend0, pend1, pend2, pend3, pend4, evStg[6], evCmp[6], evCell, evSorted, evFlags, dloss, gradOpacity,
@@ -278,7 +285,7 @@ struct pale_handle {
         scene.stackDepth = img.stackDepth;
         if ((size_t)img.stackDepth * sizeof(uint2) * kTraceBlock > 65536)
             throw StatusError(PALE_ERR_INVALID_ARG, "BVH too deep for the LDS traversal stack");
-        if (const size_t sb = walkSpillBytes(scene)) walkSpill.reserve(sb);
+        ensureWalkSpill();
         gradOpacity.reserve(std::max<size_t>(1, bp.points.size()) * sizeof(float));
         HIPCHECK(hipStreamSynchronize(stream));
     }
@@ -597,7 +604,7 @@ struct pale_handle {
             launchFill(stream, cursors.as<uint32_t>(), kTraceRegions, 0u);
             const size_t tt = tBegin(kTCamTrace);
             launchCamTrace(stream, scene, st, npx, dc.W, px, ent, entryCap, cursors.as<uint32_t>(), walkSpillPtr(),
-                           countWork);
+                           walkLds(), countWork);
             tEnd(tt);
             const size_t ts = tBegin(kTCamSort);
             launchExclusiveScan(stream, px.cnt, px.off, npx, pxTiles.as<uint32_t>(), gridTotal.as<uint32_t>());
@@ -788,7 +795,7 @@ struct pale_handle {
         launchFill(stream, cursors.as<uint32_t>(), kTraceRegions, 0u);
         const size_t tt = timed ? tBegin(kTCamTrace) : 0;
         launchCamTraceBatch(stream, scene, st(), batchDesc(), maxSlots, px, ent, entryCap, cursors.as<uint32_t>(),
-                            walkSpillPtr(), fraction, count);
+                            walkSpillPtr(), walkLds(), fraction, count);
         if (timed) tEnd(tt);
         const size_t ts = timed ? tBegin(kTCamSort) : 0;
         launchExclusiveScan(stream, px.cnt, px.off, maxSlots, pxTiles.as<uint32_t>(), gridTotal.as<uint32_t>());
@@ -827,6 +834,7 @@ struct pale_handle {
         keyAdd(key, state.p);
         keyAdd(key, cursors.p);
         keyAdd(key, walkSpill.p);
+        keyAdd(key, walkLds());
         keyAdd(key, entStart.p);
         keyAdd(key, pxTiles.p);
         keyAdd(key, gridTiles.p);
@@ -1690,8 +1698,11 @@ int pale_set_batch_tuning(pale_handle* h, const pale_batch_tuning* t) {
     if (!(t->grid_fraction >= 0.0f && t->grid_fraction <= 1.0f))
         return fail(PALE_ERR_INVALID_ARG, "grid_fraction must be in [0, 1]");
     if (t->graph_rounds > 64) return fail(PALE_ERR_INVALID_ARG, "graph_rounds must be <= 64");
-    h->tune = *t;
-    return PALE_OK;
+    if (t->walk_lds_entries > 64) return fail(PALE_ERR_INVALID_ARG, "walk_lds_entries must be <= 64");
+    return guarded([&] {
+        h->tune = *t;
+        h->ensureWalkSpill();
+    });
 }
 
 int pale_get_batch_tuning(const pale_handle* h, pale_batch_tuning* t) {

@@ -277,9 +277,10 @@ class Renderer:
         return {"ldr": ldr, "raw": raw}
 
     def set_batch_tuning(self, slots_per_batch: int = 0, entries_per_slot: int = 0, graph_rounds: int = 0,
-                         grid_fraction: float = 0.0) -> None:
+                         grid_fraction: float = 0.0, walk_lds_entries: int = 0) -> None:
         """Launch shape of render_views (0 = default); never changes results."""
-        t = _abi.BatchTuning(int(slots_per_batch), int(entries_per_slot), int(graph_rounds), float(grid_fraction))
+        t = _abi.BatchTuning(int(slots_per_batch), int(entries_per_slot), int(graph_rounds), float(grid_fraction),
+                             int(walk_lds_entries))
         check(self._L.pale_set_batch_tuning(self._h, C.byref(t)))
 
     def batch_tuning(self) -> dict:

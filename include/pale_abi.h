@@ -359,6 +359,8 @@ typedef struct pale_batch_tuning {
     uint32_t entries_per_slot; /* camera-gather entries per slot and round; 0 = 24        */
     uint32_t graph_rounds;     /* rounds in the captured graph; 0 = eager launches        */
     float grid_fraction;       /* layer-walk grid / full occupancy grid, (0,1]; 0 = 1     */
+    uint32_t walk_lds_entries; /* layer-walk stack entries per lane held in LDS (the rest
+                                  in a global spill area); 0 = 16, at most 64          */
 } pale_batch_tuning;
 
 int pale_set_batch_tuning(pale_handle* h, const pale_batch_tuning* tuning);

@@ -253,6 +253,26 @@ def test_gather_bit_exact_with_crowded_cells(radius, photons, least):
     orc.close()
 
 
+@pytest.mark.parametrize("batch_passes", [1, 0])
+def test_walk_stack_spill_bit_exact(batch_passes):
+    """The layer walk keeps the first walk_lds_entries stack entries of a
+    lane in LDS and the deeper ones in a global spill area; with one or four
+    entries in LDS nearly every traversal spills, and the image must still be
+    the oracle's bit for bit (batched and pass-by-pass walks)."""
+    spec = scene.config_scene(0, n=8192, width=32, height=32)
+    gpu, orc = _pair(spec, dict(SMALL, batch_passes=batch_passes))
+    b = orc.render_forward()["camera1_raw"]
+    for lds in (1, 4, 0):
+        gpu.set_batch_tuning(walk_lds_entries=lds)
+        assert gpu.batch_tuning()["walk_lds_entries"] == lds
+        a = gpu.render_forward()["camera1_raw"]
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), lds
+    with pytest.raises(RuntimeError):
+        gpu.set_batch_tuning(walk_lds_entries=65)
+    gpu.close()
+    orc.close()
+
+
 def test_camera_gather_rounds_bit_exact():
     # A one-entry-per-pixel budget forces several rounds per pass (about
     # seven layers per pixel here): pixels stop at a layer that does not fit
