@@ -1332,28 +1332,45 @@ __global__ __launch_bounds__(kTraceBlock) void k_adj_trace(DevScene S, DevState*
 
 // Counting sort of the pooled events by the photon-grid cell of their
 // gather point, so the transport's irradiance gathers run in cell order.
+// Cell of every pooled event and per-cell counts; neighbouring events come
+// from neighbouring pixels, so equal cells arrive in runs: one atomic per
+// run (the loop is uniform over the block, as in k_entry_key).
 __global__ __launch_bounds__(256) void k_event_key(GridParams g, const DevState* st, uint32_t cap, EventArrays ev,
                                                    uint32_t* cell, uint32_t* cellCount) {
     const uint32_t n = min(st->evPool, cap);
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const float4 a = ev.e0[i];
-        const F3 f = divv(mk3(a.x, a.y, a.z) - mk3(g.ox, g.oy, g.oz), mk3(g.cell, g.cell, g.cell));
-        const int x = min(max((int)floorf(f.x), 0), g.rx - 1);
-        const int y = min(max((int)floorf(f.y), 0), g.ry - 1);
-        const int z = min(max((int)floorf(f.z), 0), g.rz - 1);
-        const uint32_t c = (uint32_t)((z * g.ry + y) * g.rx + x);
-        cell[i] = c;
-        atomicAdd(&cellCount[c], 1u);
+    for (uint32_t i0 = blockIdx.x * blockDim.x; i0 < n; i0 += gridDim.x * blockDim.x) {
+        const uint32_t i = i0 + threadIdx.x;
+        uint32_t c = kNoCell;
+        if (i < n) {
+            const float4 a = ev.e0[i];
+            const F3 f = divv(mk3(a.x, a.y, a.z) - mk3(g.ox, g.oy, g.oz), mk3(g.cell, g.cell, g.cell));
+            const int x = min(max((int)floorf(f.x), 0), g.rx - 1);
+            const int y = min(max((int)floorf(f.y), 0), g.ry - 1);
+            const int z = min(max((int)floorf(f.z), 0), g.rz - 1);
+            c = (uint32_t)((z * g.ry + y) * g.rx + x);
+            cell[i] = c;
+        }
+        const KeyRun r = keyRun(c);
+        if (laneId() == r.head && c != kNoCell) atomicAdd(&cellCount[c], r.len);
     }
 }
 
+// Counting-sort scatter of the events (order within a cell: any; the
+// transport's FP32 atomics sum in device order anyway); leaves cellCount
+// zeroed.
 __global__ __launch_bounds__(256) void k_event_scatter(const DevState* st, uint32_t cap, const uint32_t* cell,
                                                        const uint32_t* cellStart, uint32_t* cellCount,
                                                        uint32_t* sorted) {
     const uint32_t n = min(st->evPool, cap);
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const uint32_t c = cell[i];
-        sorted[cellStart[c] + atomicSub(&cellCount[c], 1u) - 1u] = i;
+    for (uint32_t i0 = blockIdx.x * blockDim.x; i0 < n; i0 += gridDim.x * blockDim.x) {
+        const uint32_t i = i0 + threadIdx.x;
+        const uint32_t c = i < n ? cell[i] : kNoCell;
+        const KeyRun r = keyRun(c);
+        const uint32_t lane = laneId();
+        uint32_t base = 0;
+        if (lane == r.head && c != kNoCell) base = cellStart[c] + atomicSub(&cellCount[c], r.len) - r.len;
+        base = __shfl(base, (int)r.head, 64);
+        if (c != kNoCell) sorted[base + (lane - r.head)] = i;
     }
 }
 
