@@ -109,6 +109,7 @@ class Settings(C.Structure):
         ("photon_map_capacity", C.c_uint32),
         ("full_gradients", C.c_int32),
         ("batch_passes", C.c_int32),
+        ("adjoint_decorrelate", C.c_int32),
     ]
 
 
@@ -170,6 +171,7 @@ DEFAULT_SETTINGS = dict(
     photon_map_capacity=0,
     full_gradients=0,
     batch_passes=1,
+    adjoint_decorrelate=0,
 )
 
 
@@ -185,7 +187,7 @@ def make_settings(d: Optional[dict] = None) -> Settings:
     for k, v in vals.items():
         if k in ("gather_radius", "q_null", "q_reflect", "q_transmit"):
             setattr(s, k, float(v))
-        elif k in ("spp_decorrelate", "debug_images", "full_gradients", "batch_passes"):
+        elif k in ("spp_decorrelate", "debug_images", "full_gradients", "batch_passes", "adjoint_decorrelate"):
             setattr(s, k, int(bool(v)))
         else:
             setattr(s, k, int(v))

@@ -168,6 +168,7 @@ struct TraceParams {
     const uint4* groups;
     const uint32_t* groupSegs;
     uint32_t nGroups;
+    uint32_t seed;  // adjoint base seed of an unbatched queue (batched: BatchSeg::seed)
 };
 
 // Photon-map reservoir (depositPhotonSurface past capacity): per queue item
@@ -256,7 +257,7 @@ void launchAdjRaygenBatch(hipStream_t s, BatchDesc b, uint32_t maxTiles, const f
 void launchSegFirst(hipStream_t s, const DevState* st, uint32_t maxRays, RayQueue q, uint32_t* segFirst,
                     uint32_t* segEnd, uint32_t nSegs, uint4* groups, uint32_t nGroups, const uint32_t* groupSegs);
 void launchAdjRaygen(hipStream_t s, const DevCamera& cam, const float* dloss, RayQueue q, PendingArrays pend,
-                     DevState* st);
+                     DevState* st, uint32_t seed);
 void launchAdjTrace(hipStream_t s, const DevScene& S, DevState* st, uint32_t maxRays, uint32_t bounce, RayQueue in,
                     RayQueue staged, uint8_t* extFlags, EventArrays ev, uint8_t* evFlags, PendingArrays pend,
                     TraceParams tp, bool count);
@@ -272,6 +273,15 @@ void launchAdjTransport(hipStream_t s, const DevScene& S, const DevGrid& G, DevS
 void launchSplitGradients(hipStream_t s, const float* gradFull, uint32_t n, float* position, float* tangentU,
                           float* tangentV, float* scale, float* albedo, float* beta);
 void launchSetOpacity(hipStream_t s, SurfelShade* shade, const float* opacity, uint32_t n);
+// The seven parameter arrays of apply_point_optimization in device memory
+// (BVH order): position, tangent_u, tangent_v (3 floats each), scale (2),
+// albedo (3), opacity, beta.
+struct DevPointArrays {
+    const float *position, *tangent_u, *tangent_v, *scale, *albedo, *opacity, *beta;
+};
+// Surfel records from device parameter arrays, and their raw copy (16 floats
+// per surfel: p, t_u, t_v, s_u, s_v, albedo, opacity, beta).
+void launchApplyPoints(hipStream_t s, SurfelGeo* geo, SurfelShade* shade, float* raw, DevPointArrays a, uint32_t n);
 void launchGatherPoints(hipStream_t s, const DevGrid& G, uint32_t n, const float* q, const float* nrm, float* out);
 void launchTraceRays(hipStream_t s, const DevScene& S, uint32_t n, const float* o, const float* d, int32_t* inst,
                      int32_t* prim, float* t, float* pos, float* alpha);

@@ -1067,14 +1067,15 @@ __global__ __launch_bounds__(256) void k_post(const float4* fb, float4* ldr, uin
 
 // ---------------------------------------------------------- adjoint --
 // launchRayGenAdjointKernel (AdjointKernels.cpp:15-84).  The adjoint seeds
-// from PathTracerSettings::random.number, which keeps its default 42.
+// from PathTracerSettings::random.number, which keeps its default 42; with
+// adjoint_decorrelate the host passes 42 + sample instead.
 __global__ __launch_bounds__(256) void k_adj_raygen(DevCamera cam, const float* dloss, RayQueue q,
-                                                    PendingArrays pend, DevState* st) {
+                                                    PendingArrays pend, DevState* st, uint32_t seed) {
     const uint32_t npx = cam.W * cam.H;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0) st->qCount = npx;
     if (i >= npx) return;
-    Rng rng(perItemSeed(42ull, i));
+    Rng rng(perItemSeed((uint64_t)seed, i));
     const float jx = rng.nextFloat() - 0.5f;
     const float jy = rng.nextFloat() - 0.5f;
     F3 o, d;
@@ -1097,7 +1098,7 @@ __global__ __launch_bounds__(256) void k_adj_raygen_batch(BatchDesc b, const flo
     if (!batchSlot(b.tiles, t * 64u + (threadIdx.x & 63u), seg, pxx, pxy, sp, b.segs)) return;
     const BatchSeg s = *sp;
     const uint32_t pixel = pxy * s.W + pxx, slot = s.slotBase + pixel;
-    Rng rng(perItemSeed(42ull, pixel));
+    Rng rng(perItemSeed((uint64_t)s.seed, pixel));
     const float jx = rng.nextFloat() - 0.5f;
     const float jy = rng.nextFloat() - 0.5f;
     F3 o, d;
@@ -1186,7 +1187,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_adj_trace(DevScene S, DevState*
         const uint32_t seg = f2u(T4.w);
         const uint32_t rank = tp.segFirst ? i - tp.segFirst[seg] : i;
         const float invS = tp.segs ? tp.segs[seg].invSpp : tp.invSpp;
-        Rng rng(perItemSeed(42ull, rank));
+        Rng rng(perItemSeed((uint64_t)(tp.segs ? tp.segs[seg].seed : tp.seed), rank));
         const F3 o = ld3(o4), d = ld3(d4), T = ld3(T4);
         const uint32_t path = f2u(o4.w);
         const uint32_t bounce = f2u(d4.w);
@@ -1556,6 +1557,46 @@ __global__ __launch_bounds__(256) void k_set_opacity(SurfelShade* shade, const f
     if (i < n) shade[i].opacity = opacity[i];
 }
 
+// apply_point_optimization with the parameters already in device memory:
+// each surfel's intersection and shading records from the same binary32
+// expressions as the host's refreshSurfels (orthonormal frame of
+// intersectSurfel, KernelHelpers.h:646-648; shading normal
+// normalize(cross(t_u, t_v)), :497-501; 4 e^beta), so that both paths give
+// identical records.  The values are written as given (no clamp or
+// normalisation), as the reference's apply_point_optimization writes them
+// (pale_bindings.cpp:853-1056); alpha_r is kept.  `raw` receives the
+// parameters for the host mirror.
+__global__ __launch_bounds__(256) void k_apply_points(SurfelGeo* geo, SurfelShade* shade, float* raw,
+                                                      DevPointArrays a, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const F3 p = mk3(a.position[3 * i], a.position[3 * i + 1], a.position[3 * i + 2]);
+    const F3 tu = mk3(a.tangent_u[3 * i], a.tangent_u[3 * i + 1], a.tangent_u[3 * i + 2]);
+    const F3 tv = mk3(a.tangent_v[3 * i], a.tangent_v[3 * i + 1], a.tangent_v[3 * i + 2]);
+    const float su = a.scale[2 * i], sv = a.scale[2 * i + 1];
+    const F3 alb = mk3(a.albedo[3 * i], a.albedo[3 * i + 1], a.albedo[3 * i + 2]);
+    const float op = a.opacity[i], beta = a.beta[i];
+    const F3 u = normalize(tu);
+    const F3 v = normalize(tv - u * dot(u, tv));
+    const F3 nrm = normalize(cross(u, v));
+    SurfelGeo g;
+    g.px = p.x; g.py = p.y; g.pz = p.z; g.su = su;
+    g.nx = nrm.x; g.ny = nrm.y; g.nz = nrm.z; g.sv = sv;
+    g.tux = tu.x; g.tuy = tu.y; g.tuz = tu.z; g.e4b = 4.0f * exp_r(beta);
+    g.tvx = tv.x; g.tvy = tv.y; g.tvz = tv.z; g.pad = 0.0f;
+    geo[i] = g;
+    const F3 ns = normalize(cross(tu, tv));
+    SurfelShade h;
+    h.nsx = ns.x; h.nsy = ns.y; h.nsz = ns.z; h.opacity = op;
+    h.ar = alb.x; h.ag = alb.y; h.ab = alb.z; h.alpha_r = shade[i].alpha_r;
+    shade[i] = h;
+    float4* r = reinterpret_cast<float4*>(raw) + 4 * (size_t)i;
+    r[0] = make_float4(p.x, p.y, p.z, tu.x);
+    r[1] = make_float4(tu.y, tu.z, tv.x, tv.y);
+    r[2] = make_float4(tv.z, su, sv, alb.x);
+    r[3] = make_float4(alb.y, alb.z, op, beta);
+}
+
 // Irradiance estimate at given points (gatherDiffuseIrradianceAtPoint,
 // KernelHelpers.h:990-1037) for tests of the gather on its own.
 __global__ __launch_bounds__(256) void k_gather_points(DevGrid G, uint32_t n, const float* q, const float* nrm,
@@ -1802,9 +1843,9 @@ void launchSegFirst(hipStream_t s, const DevState* st, uint32_t maxRays, RayQueu
     k_group_lanes<<<1, 64, 0, s>>>(segFirst, segEnd, groups, nGroups, groupSegs);
 }
 void launchAdjRaygen(hipStream_t s, const DevCamera& cam, const float* dloss, RayQueue q, PendingArrays pend,
-                     DevState* st) {
+                     DevState* st, uint32_t seed) {
     const uint32_t npx = cam.W * cam.H;
-    k_adj_raygen<<<blocks(npx > 0 ? npx : 1, 256), 256, 0, s>>>(cam, dloss, q, pend, st);
+    k_adj_raygen<<<blocks(npx > 0 ? npx : 1, 256), 256, 0, s>>>(cam, dloss, q, pend, st, seed);
 }
 void launchAdjTrace(hipStream_t s, const DevScene& S, DevState* st, uint32_t maxRays, uint32_t bounce, RayQueue in,
                     RayQueue staged, uint8_t* extFlags, EventArrays ev, uint8_t* evFlags, PendingArrays pend,
@@ -1854,6 +1895,10 @@ void launchGatherPoints(hipStream_t s, const DevGrid& G, uint32_t n, const float
 
 void launchSetOpacity(hipStream_t s, SurfelShade* shade, const float* opacity, uint32_t n) {
     if (n) k_set_opacity<<<blocks(n, 256), 256, 0, s>>>(shade, opacity, n);
+}
+
+void launchApplyPoints(hipStream_t s, SurfelGeo* geo, SurfelShade* shade, float* raw, DevPointArrays a, uint32_t n) {
+    if (n) k_apply_points<<<blocks(n, 256), 256, 0, s>>>(geo, shade, raw, a, n);
 }
 
 }  // namespace pale

@@ -150,6 +150,41 @@ struct pale_handle {
         entStart, flagHost;
     // Device state and seed table.
     DevMem state, seedTable, cursors;
+    // Parameters applied from device memory (pale_apply_points_device): their
+    // raw copy, 16 floats per surfel; the host mirror (bp.points, asset,
+    // img) is refreshed from it when host code next needs it.
+    DevMem rawPts;
+    bool hostStale = false;
+    void syncHostPoints() {
+        if (!hostStale) return;
+        const size_t n = bp.points.size();
+        std::vector<float> r(n * 16);
+        if (n) HIPCHECK(hipMemcpyAsync(r.data(), rawPts.p, n * 16 * sizeof(float), hipMemcpyDeviceToHost, stream));
+        HIPCHECK(hipStreamSynchronize(stream));
+        for (size_t i = 0; i < n; ++i) {
+            const float* v = &r[16 * i];
+            HostPoint& q = bp.points[i];
+            q.p = mk3(v[0], v[1], v[2]);
+            q.tu = mk3(v[3], v[4], v[5]);
+            q.tv = mk3(v[6], v[7], v[8]);
+            q.su = v[9];
+            q.sv = v[10];
+            q.albedo = mk3(v[11], v[12], v[13]);
+            q.opacity = v[14];
+            q.beta = v[15];
+            AssetPoint& a = asset[i];
+            a.p = q.p;
+            a.tu = q.tu;
+            a.tv = q.tv;
+            a.su = q.su;
+            a.sv = q.sv;
+            a.albedo = q.albedo;
+            a.opacity = q.opacity;
+            a.beta = q.beta;
+        }
+        refreshSurfels(bp.points, img);
+        hostStale = false;
+    }
     // Per-camera framebuffers.
     std::vector<DevMem> fb, ldr;
     pale_stats stats{};
@@ -220,10 +255,14 @@ struct pale_handle {
                            full ? m[4].as<float4>() : nullptr, full ? m[5].as<float4>() : nullptr};
     }
 
-    TraceParams tp(float invSpp = 1.0f, const BatchSeg* segs = nullptr, const uint32_t* segFirst = nullptr) const {
+    TraceParams tp(float invSpp = 1.0f, const BatchSeg* segs = nullptr, const uint32_t* segFirst = nullptr,
+                   uint32_t seed = 42u) const {
         return TraceParams{S.rr_start, S.q_null, S.q_reflect, S.q_transmit, S.full_gradients ? 1u : 0u,
-                           invSpp, segs, segFirst, nullptr, nullptr, nullptr, 0u};
+                           invSpp, segs, segFirst, nullptr, nullptr, nullptr, 0u, seed};
     }
+    // Base seed of adjoint sample k: the reference's constant 42
+    // (AdjointKernels.cpp:27,94), or 42 + k with adjoint_decorrelate.
+    uint32_t adjointSeed(uint32_t k) const { return S.adjoint_decorrelate ? 42u + k : 42u; }
 
     DevGrid devGrid() const {
         DevGrid g{};
@@ -1192,7 +1231,8 @@ struct pale_handle {
                 HostBatch& b = batches.back();
                 const uint32_t tx = (c.W + 7u) / 8u, ty = (c.H + 7u) / 8u;
                 const uint32_t si = (uint32_t)b.segs.size();
-                BatchSeg sg{v, p, c.W, c.H, tx, (uint32_t)cur, 42u, (uint32_t)pixOff[v], 1.0f / (float)spp[v], {0, 0, 0}};
+                BatchSeg sg{v, p, c.W, c.H, tx, (uint32_t)cur, adjointSeed(p), (uint32_t)pixOff[v], 1.0f / (float)spp[v],
+                            {0, 0, 0}};
                 b.segs.push_back(sg);
                 for (uint32_t t = 0; t < tx * ty; ++t)
                     b.tiles.push_back(make_uint4(si, t, (uint32_t)b.tiles.size(), 1u));
@@ -1331,12 +1371,13 @@ struct pale_handle {
             for (uint32_t spp = 0; spp < samples; ++spp) {
                 DevMem* in = qA;
                 DevMem* out = qB;
-                launchAdjRaygen(stream, dc, dloss.as<float>() + off[k], rq(in), pend, st);
+                launchAdjRaygen(stream, dc, dloss.as<float>() + off[k], rq(in), pend, st, adjointSeed(spp));
                 for (uint32_t b = 0; b < S.adjoint_bounces; ++b) {
                     if (pooledBound + npx > poolCap) flush();
                     const size_t ta = tBegin(kTAdjTrace);
                     launchAdjTrace(stream, scene, st, npx, b, rq(in), rq(stg), extFlags.as<uint8_t>(), ea(evStg),
-                                   evFlags.as<uint8_t>(), pend, tp(invSpp), countWork);
+                                   evFlags.as<uint8_t>(), pend, tp(invSpp, nullptr, nullptr, adjointSeed(spp)),
+                                   countWork);
                     tEnd(ta);
                     launchCompactEvents(stream, evFlags.as<uint8_t>(), qCount, npx, tiles.as<uint32_t>(), st,
                                         ea(evStg), ea(evCmp), poolCap);
@@ -1436,6 +1477,7 @@ void pale_default_settings(pale_settings* out) {
     s.photon_map_capacity = 0;
     s.full_gradients = 0;
     s.batch_passes = 1;
+    s.adjoint_decorrelate = 0;
     *out = s;
 }
 
@@ -1680,6 +1722,7 @@ int pale_set_opacity_device(pale_handle* h, const float* opacity_dev, uint32_t n
         if (n != h->bp.points.size())
             throw StatusError(PALE_ERR_SIZE, "opacity: expected " + std::to_string(h->bp.points.size()) + " values");
         if (!n) return;
+        h->syncHostPoints();
         launchSetOpacity(h->stream, h->shade.as<SurfelShade>(), opacity_dev, n);
         std::vector<float> v(n);
         HIPCHECK(hipMemcpyAsync(v.data(), opacity_dev, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, h->stream));
@@ -1690,6 +1733,30 @@ int pale_set_opacity_device(pale_handle* h, const float* opacity_dev, uint32_t n
             h->asset[i].opacity = v[i];
             h->img.surfShade[i].opacity = v[i];
         }
+    });
+}
+
+int pale_apply_points_device(pale_handle* h, uint32_t n, const float* position, const float* tangent_u,
+                             const float* tangent_v, const float* scale, const float* albedo, const float* opacity,
+                             const float* beta) {
+    if (!h) return fail(PALE_ERR_INVALID_ARG, "null handle");
+    if (n && (!position || !tangent_u || !tangent_v || !scale || !albedo || !opacity || !beta))
+        return fail(PALE_ERR_INVALID_ARG, "points: missing array");
+    return guarded([&] {
+        if (n == 0) return;  // ignored, as pale_apply_points ignores an empty dict
+        if (n != h->bp.points.size())
+            throw StatusError(PALE_ERR_SIZE,
+                              "apply_point_optimization expects consistent point count; use densification API for "
+                              "adding/removing points.");
+        // Host fields that the device arrays do not carry (shape, alpha_r)
+        // must be current before the mirror is marked stale again.
+        h->syncHostPoints();
+        h->rawPts.reserve((size_t)n * 16 * sizeof(float));
+        launchApplyPoints(h->stream, h->geo.as<SurfelGeo>(), h->shade.as<SurfelShade>(), h->rawPts.as<float>(),
+                          DevPointArrays{position, tangent_u, tangent_v, scale, albedo, opacity, beta}, n);
+        HIPCHECK(hipGetLastError());
+        h->hostStale = true;
+        HIPCHECK(hipStreamSynchronize(h->stream));
     });
 }
 
@@ -1856,6 +1923,10 @@ int pale_point_count(const pale_handle* h, uint32_t* count) {
 int pale_get_points(const pale_handle* h, float* position, float* tangent_u, float* tangent_v, float* scale,
                     float* albedo, float* opacity, float* beta, float* shape) {
     if (!h) return fail(PALE_ERR_INVALID_ARG, "null handle");
+    if (h->hostStale) {
+        const int rc = guarded([&] { const_cast<pale_handle*>(h)->syncHostPoints(); });
+        if (rc != PALE_OK) return rc;
+    }
     const auto& P = h->bp.points;
     for (size_t i = 0; i < P.size(); ++i) {
         const HostPoint& p = P[i];
@@ -1880,6 +1951,7 @@ int pale_apply_points(pale_handle* h, const pale_points_desc* points) {
                               "apply_point_optimization expects consistent point count; use densification API for "
                               "adding/removing points.");
         const std::vector<AssetPoint> in = toAsset(*points);
+        h->syncHostPoints();
         for (size_t i = 0; i < in.size(); ++i) {
             HostPoint& q = h->bp.points[i];
             q.p = in[i].p;
@@ -1917,6 +1989,7 @@ int pale_add_points(pale_handle* h, const pale_points_desc* points) {
     return guarded([&] {
         if (points->count == 0) return;
         std::vector<AssetPoint> in = toAsset(*points);
+        h->syncHostPoints();
         for (AssetPoint& a : in) a.shape = 0.0f;  // new points get shape 0
         h->asset.insert(h->asset.end(), in.begin(), in.end());
         h->rebuild();
@@ -1927,6 +2000,7 @@ int pale_remove_points(pale_handle* h, const int64_t* indices, size_t n) {
     if (!h) return fail(PALE_ERR_INVALID_ARG, "null handle");
     if (n && !indices) return fail(PALE_ERR_INVALID_ARG, "null indices");
     return guarded([&] {
+        h->syncHostPoints();
         if (n == 0 || h->asset.empty()) return;
         std::vector<char> keep(h->asset.size(), 1);
         for (size_t k = 0; k < n; ++k) {
@@ -1945,13 +2019,17 @@ int pale_remove_points(pale_handle* h, const int64_t* indices, size_t n) {
 
 int pale_rebuild_bvh(pale_handle* h) {
     if (!h) return fail(PALE_ERR_INVALID_ARG, "null handle");
-    return guarded([&] { h->rebuild(); });
+    return guarded([&] {
+        h->syncHostPoints();
+        h->rebuild();
+    });
 }
 
 int pale_set_point_properties(pale_handle* h, const float translation[3], const float rotation_xyzw[4],
                               const float scale[3], const float albedo[3], float opacity, float beta, int32_t index) {
     if (!h || !translation || !rotation_xyzw || !scale || !albedo) return fail(PALE_ERR_INVALID_ARG, "null argument");
     return guarded([&] {
+        h->syncHostPoints();
         auto& A = h->asset;
         const F3 t = ld3h(translation), c = ld3h(albedo);
         if (index < 0) {
@@ -1988,6 +2066,7 @@ int pale_set_point_properties(pale_handle* h, const float translation[3], const 
 int pale_set_point_opacity(pale_handle* h, float opacity, int32_t index) {
     if (!h) return fail(PALE_ERR_INVALID_ARG, "null handle");
     return guarded([&] {
+        h->syncHostPoints();
         if (index < 0 || (size_t)index >= h->asset.size())
             throw StatusError(PALE_ERR_INVALID_ARG, "set_gaussian_opacity: index out of range");
         h->asset[(size_t)index].opacity = opacity;
@@ -1998,6 +2077,7 @@ int pale_set_point_opacity(pale_handle* h, float opacity, int32_t index) {
 int pale_set_point_beta(pale_handle* h, float beta, int32_t index) {
     if (!h) return fail(PALE_ERR_INVALID_ARG, "null handle");
     return guarded([&] {
+        h->syncHostPoints();
         if (index < 0 || (size_t)index >= h->asset.size())
             throw StatusError(PALE_ERR_INVALID_ARG, "set_point_beta: index out of range");
         h->asset[(size_t)index].beta = beta;

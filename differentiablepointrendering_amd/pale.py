@@ -20,6 +20,7 @@ Compatibility extensions (supersets of the reference):
 from __future__ import annotations
 
 import ctypes as C
+import math
 import os
 from typing import Dict, List, Optional, Tuple
 
@@ -220,6 +221,28 @@ class Renderer:
         t = opacity.detach().to(device="cuda", dtype=torch.float32).contiguous().reshape(-1)
         torch.cuda.current_stream().synchronize()
         check(self._L.pale_set_opacity_device(self._h, self._dptr(t), t.numel()))
+
+    def apply_point_optimization_device(self, parameters: dict) -> None:
+        """apply_point_optimization from torch tensors on the GPU (keys
+        position, tangent_u, tangent_v, scale, albedo, opacity, beta; BVH
+        order; same N): the parameters never leave device memory and the BVH
+        is kept."""
+        import torch
+
+        n = self.point_count
+        shapes = {"position": (n, 3), "tangent_u": (n, 3), "tangent_v": (n, 3), "scale": (n, 2),
+                  "albedo": (n, 3), "opacity": (n,), "beta": (n,)}
+        ts = []
+        for k, shp in shapes.items():
+            if k not in parameters:
+                raise RuntimeError(f"Missing key '{k}' in parameters dict")
+            t = parameters[k].detach()
+            if t.numel() != math.prod(shp):
+                raise RuntimeError("apply_point_optimization expects consistent point count; use densification "
+                                   "API for adding/removing points.")
+            ts.append(t.to(device="cuda", dtype=torch.float32).contiguous().reshape(shp))
+        torch.cuda.current_stream().synchronize()
+        check(self._L.pale_apply_points_device(self._h, n, *[C.c_void_p(t.data_ptr()) for t in ts]))
 
     # ------------------------------------------------------- served views --
     def build_photon_map(self) -> None:

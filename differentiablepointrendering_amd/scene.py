@@ -44,6 +44,25 @@ def light_quad(y: float = 1.5, size: float = 0.5, radiance: float = 10.0,
                 tuple(color), float(np.float32(power)), name)
 
 
+def quad_mesh(name: str, center, rotation, half_size: float, base_color, power: float = 0.0) -> Mesh:
+    """The unit quad (normal +z in object space) scaled by half_size,
+    rotated by the 3x3 `rotation` (its third column is the world normal) and
+    moved to `center`: a floor, wall or ceiling of test scenes."""
+    R = np.asarray(rotation, dtype=np.float64).reshape(3, 3)
+    o2w = np.eye(4)
+    o2w[:3, :3] = R * half_size
+    o2w[:3, 3] = np.asarray(center, dtype=np.float64)
+    w2o = np.linalg.inv(o2w)
+    return Mesh(QUAD_POSITIONS.copy(), QUAD_INDICES.copy(), o2w.astype(np.float32), w2o.astype(np.float32),
+                tuple(float(c) for c in base_color), float(power), name)
+
+
+# Rotations whose third column (the quad normal) is +y, -y or +z.
+FACING_UP = ((1, 0, 0), (0, 0, 1), (0, -1, 0))
+FACING_DOWN = ((1, 0, 0), (0, 0, -1), (0, 1, 0))
+FACING_PLUS_Z = ((1, 0, 0), (0, 1, 0), (0, 0, 1))
+
+
 def look_at_camera(name: str, eye, target, up, width: int, height: int, fovy_deg: float,
                    use_for_adjoint: bool = True) -> Camera:
     """Pinhole camera from a look-at (right-handed, camera looks down -z)."""
@@ -176,3 +195,56 @@ def single_surfel_scene(position=(0.0, 0.0, 0.0), tangent_u=(1.0, 0.0, 0.0), tan
                albedo=[albedo], opacity=[opacity], beta=[beta], shape=[0.0])
     pts = {k: np.ascontiguousarray(np.asarray(v, dtype=np.float32)) for k, v in pts.items()}
     return SceneSpec([light_quad()], [default_camera(width, height)], pts, {}, None, name="single")
+
+
+# ------------------------------------------------- batched and served sets --
+def batched_cameras(count: int = 32, seed: int = 2):
+    """Views of the batched configuration: eyes uniform on the sphere of
+    radius 4 looking at the origin, pixel area log-uniform between 128x96 and
+    1024x768 (aspect 4:3), spp uniform in {4, 8, 16}.  Returns (cameras,
+    spp)."""
+    rng = np.random.default_rng(seed)
+    lo, hi = math.log(128 * 96), math.log(1024 * 768)
+    cams, spp = [], []
+    for i in range(count):
+        area = math.exp(rng.uniform(lo, hi))
+        w = max(8, int(round(math.sqrt(area * 4.0 / 3.0))))
+        h = max(6, int(round(w * 0.75)))
+        d = rng.normal(size=3)
+        eye = 4.0 * d / np.linalg.norm(d)
+        up = (0.0, 1.0, 0.0) if abs(eye[1]) < 3.9 else (1.0, 0.0, 0.0)
+        cams.append(look_at_camera(f"v{i}", eye, (0.0, 0.0, 0.0), up, w, h, 45.0))
+        spp.append(int(rng.choice([4, 8, 16])))
+    return cams, spp
+
+
+# The served configuration's resident scenes: (clustered surfels, photons).
+SERVED_SCENES = ((100_000, 1_000_000), (500_000, 2_000_000), (1_000_000, 3_000_000), (3_000_000, 4_000_000))
+
+
+def served_requests(count: int, rate: float, seed: int = 1):
+    """The request stream of the served configuration: Poisson arrivals at
+    `rate` per second; each request picks a resident scene uniformly, is
+    forward-only with p = 0.7 (else forward + adjoint), has an area
+    log-uniform between 256x256 and 1920x1080 (aspect in [1, 16/9]), spp
+    uniform in {1, 4, 16, 64} and an eye uniform on the radius-4 sphere.
+    Returns [(arrival s, scene index, camera, spp, adjoint)]."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    lo, hi = math.log(256 * 256), math.log(1920 * 1080)
+    for i in range(count):
+        t += rng.exponential(1.0 / rate)
+        sc = int(rng.integers(0, len(SERVED_SCENES)))
+        adjoint = bool(rng.random() >= 0.7)
+        area = math.exp(rng.uniform(lo, hi))
+        aspect = rng.uniform(1.0, 16.0 / 9.0)
+        w = max(16, int(round(math.sqrt(area * aspect))))
+        h = max(16, int(round(area / w)))
+        spp = int(rng.choice([1, 4, 16, 64]))
+        d = rng.normal(size=3)
+        eye = 4.0 * d / np.linalg.norm(d)
+        up = (0.0, 1.0, 0.0) if abs(eye[1]) < 3.9 else (1.0, 0.0, 0.0)
+        cam = look_at_camera(f"r{i}", eye, (0.0, 0.0, 0.0), up, w, h, 45.0)
+        out.append((t, sc, cam, spp, adjoint))
+    return out

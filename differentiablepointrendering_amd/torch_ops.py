@@ -16,7 +16,7 @@ tangent_u, tangent_v (or a rotation through quaternion_tangents, in torch),
 scale, albedo, opacity, beta -- with the full-gradient adjoint (the renderer
 must be made with full_gradients = 1).  The forward applies the parameters
 the way the reference's apply_point_optimization does (render_hooks.py:
-306-316: in place, the BVH kept) and renders either a camera of the scene
+306-316: in place, the BVH kept), device to device, and renders either a camera of the scene
 (render_forward: the photon map is traced again) or any view over the
 current photon map (render_view: the map stays fixed, so a forward and the
 next are the same function of the parameters, which finite differences
@@ -44,13 +44,14 @@ class _RenderRaw(torch.autograd.Function):
         img = renderer.render_forward_device(camera)[camera + "_raw"]
         ctx.renderer = renderer
         ctx.camera = camera
+        ctx.device = opacity.device
         return img
 
     @staticmethod
     def backward(ctx, grad_img: torch.Tensor):
         dl = grad_img[..., :3].contiguous()
         grads = ctx.renderer.render_backward_device({ctx.camera: dl})
-        return grads["opacity"], None, None
+        return grads["opacity"].to(ctx.device), None, None
 
 
 def render_raw(renderer, opacity: torch.Tensor, camera: str) -> torch.Tensor:
@@ -64,14 +65,15 @@ def render_raw(renderer, opacity: torch.Tensor, camera: str) -> torch.Tensor:
 class _RenderSurfels(torch.autograd.Function):
     @staticmethod
     def forward(ctx, renderer, view, *params):
-        host = {k: p.detach().to("cpu", torch.float32).contiguous().numpy() for k, p in zip(PARAMS, params)}
-        renderer.apply_point_optimization(host)
+        # The parameters go to the renderer device to device (no host copy).
+        renderer.apply_point_optimization_device(dict(zip(PARAMS, params)))
         if view[0] == "camera":
             img = renderer.render_forward_device(view[1])[view[1] + "_raw"]
         else:
             img = renderer.render_view_device(view[1], view[2], view[3])["raw"]
         ctx.renderer = renderer
         ctx.view = view
+        ctx.devices = tuple(p.device for p in params)
         return img
 
     @staticmethod
@@ -82,13 +84,16 @@ class _RenderSurfels(torch.autograd.Function):
             grads = ctx.renderer.render_backward_device({v[1]: dl})
         else:
             grads = ctx.renderer.render_view_backward_device(v[1], v[2], dl)
-        return (None, None) + tuple(grads[k] for k in PARAMS)
+        # Each gradient on its parameter's device (autograd requires it).
+        return (None, None) + tuple(grads[k].to(d) for k, d in zip(PARAMS, ctx.devices))
 
 
 def render_surfels(renderer, params: Dict[str, torch.Tensor], camera: Optional[str] = None,
                    view: Optional[Tuple] = None) -> torch.Tensor:
     """(H,W,4) HDR image as a differentiable function of every surfel
-    parameter in `params` (keys of PARAMS, BVH order, any device).
+    parameter in `params` (keys of PARAMS, BVH order).  The image is on the
+    renderer's GPU; parameters may live on any device and receive their
+    gradients on it (GPU parameters never pass through host memory).
 
     camera="name": render_forward of a scene camera (photon map traced
     again).  view=(Camera, spp, seed): render_view over the current photon

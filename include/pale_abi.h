@@ -142,6 +142,13 @@ typedef struct pale_settings {
                                   0 = one pass / sample after another.
                                   Same images either way; gradients equal
                                   up to the order of float atomics          */
+    int32_t adjoint_decorrelate; /* "adjoint_decorrelate" (default 0): 0 =
+                                  every adjoint sample seeds from 42, as the
+                                  reference (AdjointKernels.cpp:27,94: its
+                                  random.number is never updated), so the
+                                  samples of a pixel trace one path; 1 =
+                                  sample k seeds from 42 + k (independent
+                                  samples)                                  */
 } pale_settings;
 
 /* Work counters of the last render calls (no reference counterpart; the
@@ -279,6 +286,18 @@ int pale_render_backward_device(pale_handle* h, uint32_t n,
  * rebuild (apply_point_optimization semantics for one parameter). */
 int pale_set_opacity_device(pale_handle* h, const float* opacity_dev,
                             uint32_t n);
+
+/* apply_point_optimization(parameters) (pale_bindings.cpp:853-1056) with the
+ * seven parameter arrays in device memory (BVH order; position, tangent_u,
+ * tangent_v: n*3 floats, scale: n*2, albedo: n*3, opacity, beta: n): same N
+ * required, values written as given, no BVH rebuild -- the pale_apply_points
+ * result without the host round trip.  Host copies of the parameters
+ * (pale_get_points and the point-editing calls) are refreshed from the
+ * device when next needed. */
+int pale_apply_points_device(pale_handle* h, uint32_t n, const float* position,
+                             const float* tangent_u, const float* tangent_v,
+                             const float* scale, const float* albedo,
+                             const float* opacity, const float* beta);
 
 /* ---- served views: no reference counterpart (the reference renders only
  * the cameras of its scene, pale_bindings.cpp:180-277).  For a server that

@@ -17,7 +17,9 @@ from differentiablepointrendering_amd import _abi
 from differentiablepointrendering_amd._abi import SceneHolder, make_settings
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libpale_oracle.so")
+# PALE_ORACLE_LIB names another build of the oracle (for example one with a
+# planted mistake, to show that the pin tests catch it).
+LIB_PATH = os.environ.get("PALE_ORACLE_LIB") or os.path.join(HERE, "libpale_oracle.so")
 
 _fp = C.POINTER(C.c_float)
 _lib = None
@@ -63,8 +65,17 @@ def lib():
         L.orc_build_photon_map.argtypes = [C.c_void_p]
         L.orc_render_view.argtypes = [C.c_void_p, C.POINTER(_abi.CameraDesc), C.c_uint32, C.c_uint32, _fp, _fp]
         L.orc_render_view_backward.argtypes = [C.c_void_p, C.POINTER(_abi.CameraDesc), C.c_uint32, _fp, _fp]
+        L.orc_set_threads.argtypes = [C.c_int32]
+        L.orc_russian_roulette.argtypes = [C.c_uint32, _fp, C.c_uint32, C.c_uint32, C.c_uint64,
+                                           C.POINTER(C.c_int32), _fp, _fp]
         _lib = L
     return _lib
+
+
+def set_threads(n: int) -> None:
+    """Threads of the oracle's parallel loops (0 = all); results do not
+    depend on it."""
+    lib().orc_set_threads(int(n))
 
 
 def _p(a: np.ndarray):

@@ -47,6 +47,8 @@
 #include <string>
 #include <vector>
 
+#include <omp.h>
+
 namespace orc {
 
 // ------------------------------------------------------------------ math --
@@ -246,6 +248,7 @@ struct Settings {
     bool decorrelate;
     uint32_t mapCapacity;
     bool fullGradients;
+    bool adjointDecorrelate;
 };
 
 struct Grid {
@@ -323,6 +326,41 @@ static thread_local Work g_workClass[3];
 static thread_local Work* g_workPtr = &g_workClass[0];
 #define g_work (*g_workPtr)
 static void workClass(int c) { g_workPtr = &g_workClass[c]; }
+static void addWork(Work& a, const Work& b) {
+    a.pops += b.pops;
+    a.aabb += b.aabb;
+    a.prims += b.prims;
+    a.gathers += b.gathers;
+    a.cells += b.cells;
+    a.cellsNE += b.cellsNE;
+    a.photons += b.photons;
+}
+
+// Threads of the oracle's parallel loops (0 = OpenMP's default).  Only work
+// that the reference runs as independent work-items without shared state is
+// spread over threads: closest-hit queries, camera pixels and irradiance
+// gathers.  Everything whose result depends on the sequential item order
+// (queue slots, deposit slots, the reservoir, gradient sums) stays in one
+// sequential loop, so the results do not depend on the thread count.
+static int g_threads = 0;
+template <class F>
+static void parallelItems(size_t n, F&& body) {
+    Work* caller = g_workPtr;
+    Work total{};
+    const int nt = g_threads > 0 ? g_threads : omp_get_max_threads();
+#pragma omp parallel num_threads(nt)
+    {
+        Work local{};
+        Work* saved = g_workPtr;
+        g_workPtr = &local;
+#pragma omp for schedule(dynamic, 16)
+        for (long long i = 0; i < (long long)n; ++i) body((size_t)i);
+        g_workPtr = saved;
+#pragma omp critical
+        addWork(total, local);
+    }
+    addWork(*caller, total);
+}
 
 // ------------------------------------------------------- BVH (BVH.ixx) --
 struct AABB {
@@ -1090,14 +1128,21 @@ static bool sampleLight(const Renderer& R, Xorshift128& rng, F3& pos, F3& nrm, F
 }
 
 // One photon bounce (launchIntersectKernel, PrimalKernels.cpp:65-321).
+// The closest hits of a bounce are computed for all rays first (in
+// parallel: each depends only on its ray), then the rays are shaded in
+// queue order, where deposits, the reservoir and the next queue depend on
+// the order.
 static void photonBounce(Renderer& R, const std::vector<RayState>& in, std::vector<RayState>& out, uint32_t number) {
     const Settings& S = R.S;
+    std::vector<WorldHit> hits(in.size());
+    std::vector<char> hit(in.size());
+    parallelItems(in.size(), [&](size_t i) { hit[i] = intersectScene(R, in[i].ray, hits[i]) ? 1 : 0; });
+    R.stats.photon_queries += in.size();
     for (uint32_t i = 0; i < in.size(); ++i) {
         Xorshift128 rng(perItemSeed(number, i));
         const RayState& rs = in[i];
-        WorldHit h;
-        R.stats.photon_queries++;
-        if (!intersectScene(R, rs.ray, h)) continue;
+        if (!hit[i]) continue;
+        const WorldHit& h = hits[i];
         const Instance& inst = R.inst[h.inst];
         if (inst.type == kMesh) {
             deposit(R, h.pos, rs.ray.d, rs.T, rng);
@@ -1289,10 +1334,16 @@ static void cameraGather(Renderer& R, const Camera& c, uint32_t baseSeed, std::v
     const CamConst k = camConst(c);
     const float spp = (float)passes;
     const float invSpp = 1.0f / spp;
+    std::vector<uint32_t> pix;
+    for (uint32_t pi = 0; pi < npx; ++pi)
+        if (pi % R.pixelStride == R.pixelOffset) pix.push_back(pi);
+    std::vector<uint64_t> queries(pix.size());
     for (uint32_t pass = 0; pass < passes; ++pass) {
         const uint64_t seed = decorrelate ? (uint64_t)(uint32_t)(baseSeed + pass) : (uint64_t)baseSeed;
-        for (uint32_t pi = 0; pi < npx; ++pi) {
-            if (pi % R.pixelStride != R.pixelOffset) continue;
+        // Pixels are independent: each adds to its own framebuffer entry,
+        // once per pass, in pass order.
+        parallelItems(pix.size(), [&](size_t ix) {
+            const uint32_t pi = pix[ix];
             const uint32_t px = pi % W, py = pi / W;
             Xorshift128 rng(perItemSeed(seed, pi));
             const float jx = rng.nextFloat() - 0.5f;
@@ -1300,9 +1351,10 @@ static void cameraGather(Renderer& R, const Camera& c, uint32_t baseSeed, std::v
             Ray ray = primaryRay(c, k, (float)px, (float)py, jx, jy);
             F3 C = mk(0, 0, 0);
             float tr = 1.0f;
+            uint64_t nq = 0;
             while (true) {
                 WorldHit h;
-                R.stats.gather_queries++;
+                nq++;
                 if (!intersectScene(R, ray, h)) break;
                 const Instance& in = R.inst[h.inst];
                 if (in.type == kCloud) {
@@ -1334,8 +1386,10 @@ static void cameraGather(Renderer& R, const Camera& c, uint32_t baseSeed, std::v
             f[1] = C.y * invSpp + f[1];
             f[2] = C.z * invSpp + f[2];
             f[3] = 1.0f * invSpp + f[3];
-        }
+            queries[ix] += nq;
+        });
     }
+    for (uint64_t q : queries) R.stats.gather_queries += q;
 }
 
 // launchPostProcessKernel (PostprocessKernel.cpp:11-106), LDR float output.
@@ -1544,14 +1598,26 @@ static bool alphaGradient(const Point& s, F3 o, F3 d, float grad[12]) {
 // Adds the full gradient of one event: `w` is the event's scalar weight
 // sum_rgb(T * Lo) * invSpp (the factor of alpha_geom * eta in its
 // contribution), `sgn` its sign.
-static void addGeometryGradient(Renderer& R, uint32_t prim, F3 ro, F3 rd, float w) {
+// What an event's gradient needs besides its own record: the irradiance at
+// its shading point and, with full gradients, d(alpha_geom)/d(geometry) of
+// the surfel it weighs.  Computed for all events of a bounce in parallel.
+struct EventPre {
+    F3 E{0, 0, 0};
+    float g[12] = {};
+    bool gv = false;
+};
+static void geometryGradient(const Renderer& R, uint32_t prim, F3 ro, F3 rd, EventPre& pre) {
     const Point& s = R.points[prim];
     Ray wr;
     wr.o = ro;
     wr.d = rd;
     const Ray obj = toObjectSpace(wr, R.xfs[R.inst[R.cloudInstance].xf]);
-    float g[12];
-    if (!alphaGradient(s, obj.o, obj.d, g)) return;
+    pre.gv = alphaGradient(s, obj.o, obj.d, pre.g);
+}
+static void addGeometryGradient(Renderer& R, uint32_t prim, const EventPre& pre, float w) {
+    const Point& s = R.points[prim];
+    if (!pre.gv) return;
+    const float* g = pre.g;
     float* out = &R.gradFull[(size_t)prim * 16];
     const float f = s.opacity * w;
     for (int i = 0; i < 11; ++i) out[i] += g[i] * f;
@@ -1559,16 +1625,18 @@ static void addGeometryGradient(Renderer& R, uint32_t prim, F3 ro, F3 rd, float 
 }
 
 // launchAdjointTransportKernel (AdjointKernels.cpp:389-509), one event.
-static void transport(Renderer& R, const Completed& c, float invSpp) {
+// E is the irradiance gathered at the event's shading point (the surfel hit
+// for projection events, the endpoint otherwise).
+static void transport(Renderer& R, const Completed& c, float invSpp, const EventPre& pre) {
+    const F3 E = pre.E;
     if (c.kind == kProjection) {
         const Point& s = R.points[c.prim];
-        const F3 E = gather(R, c.pos, c.normal);
         const F3 fr = (s.alpha_r * s.albedo) * kInvPi;
         const F3 Lr = fr * E;
         const F3 g = (c.alpha * c.T) * Lr;
         R.gradOpacity[c.prim] += fsum(g) * invSpp;
         if (R.S.fullGradients) {
-            addGeometryGradient(R, c.prim, c.ro, c.rd, fsum(c.T * Lr) * invSpp);
+            addGeometryGradient(R, c.prim, pre, fsum(c.T * Lr) * invSpp);
             // d/d(albedo_c) of alpha * eta * (alpha_r * albedo_c / pi) * E_c
             const F3 ga = ((c.alpha * s.opacity * s.alpha_r * kInvPi) * (c.T * E)) * invSpp;
             float* out = &R.gradFull[(size_t)c.prim * 16];
@@ -1589,7 +1657,7 @@ static void transport(Renderer& R, const Completed& c, float invSpp) {
             base = e.alpha_r * e.albedo;
             power = 0.0f;
         }
-        const F3 Lr = (gather(R, c.endPos, c.endNormal) * base) * kInvPi;
+        const F3 Lr = (E * base) * kInvPi;
         F3 Le = mk(0, 0, 0);
         if (power > 0.0f) {
             const Light& L0 = R.lights[0];
@@ -1598,25 +1666,26 @@ static void transport(Renderer& R, const Completed& c, float invSpp) {
         const F3 Lo = Le + Lr;
         const F3 g = ((-c.alpha) * c.T) * Lo;
         R.gradOpacity[c.prim] += fsum(g) * invSpp;
-        if (R.S.fullGradients) addGeometryGradient(R, c.prim, c.ro, c.rd, -(fsum(c.T * Lo) * invSpp));
+        if (R.S.fullGradients) addGeometryGradient(R, c.prim, pre, -(fsum(c.T * Lo) * invSpp));
     }
     if (c.kind == kReflectS) {
         const Material& m = R.materials[R.inst[c.inst].material];
-        const F3 Lo = (gather(R, c.endPos, c.endNormal) * m.base) * kInvPi;
+        const F3 Lo = (E * m.base) * kInvPi;
         const F3 p = divs(c.T, c.endPdf);
         const F3 g = (p * Lo) * c.endAlpha;
         R.gradOpacity[c.endPrim] += fsum(g) * invSpp;
-        if (R.S.fullGradients) addGeometryGradient(R, c.endPrim, c.ro, c.rd, fsum(p * Lo) * invSpp);
+        if (R.S.fullGradients) addGeometryGradient(R, c.endPrim, pre, fsum(p * Lo) * invSpp);
     }
 }
 
 // submitAdjointKernel (SyclBridge.cpp:169-294) for the given cameras.  The
 // adjoint kernels seed from PathTracerSettings::random.number, which keeps
 // its default 42 (AdjointKernels.cpp:27,94; GPUDataStructures.h:395-398).
+// With adjoint_decorrelate (no reference counterpart) sample k seeds from
+// 42 + k instead, so that the samples of a pixel are independent.
 static void renderBackward(Renderer& R, const std::vector<const Camera*>& camList,
                            const std::vector<const float*>& dloss, uint32_t passes) {
     const Settings& S = R.S;
-    const uint64_t number = 42;
     workClass(2);
     R.gradOpacity.assign(R.points.size(), 0.0f);
     R.gradFull.assign(S.fullGradients ? R.points.size() * 16 : 0, 0.0f);
@@ -1626,6 +1695,7 @@ static void renderBackward(Renderer& R, const std::vector<const Camera*>& camLis
         const CamConst kc = camConst(c);
         const uint32_t W = c.W, H = c.H, npx = W * H;
         for (uint32_t spp = 0; spp < passes; ++spp) {
+            const uint64_t number = S.adjointDecorrelate ? (uint64_t)(uint32_t)(42u + spp) : 42ull;
             // launchRayGenAdjointKernel (AdjointKernels.cpp:15-84)
             std::vector<ARay> q, nq;
             q.reserve(npx / R.pixelStride + 1);
@@ -1646,13 +1716,18 @@ static void renderBackward(Renderer& R, const std::vector<const Camera*>& camLis
             for (uint32_t b = 0; b < S.adjointBounces && !q.empty(); ++b) {
                 nq.clear();
                 std::vector<Completed> done;
-                // launchAdjointIntersectKernel (AdjointKernels.cpp:87-386)
+                // launchAdjointIntersectKernel (AdjointKernels.cpp:87-386):
+                // closest hits of all rays (in parallel), then each ray in
+                // queue order.
+                std::vector<WorldHit> hits(q.size());
+                std::vector<char> hit(q.size());
+                parallelItems(q.size(), [&](size_t i) { hit[i] = intersectScene(R, q[i].ray, hits[i]) ? 1 : 0; });
+                R.stats.adjoint_queries += q.size();
                 for (uint32_t ri = 0; ri < q.size(); ++ri) {
                     Xorshift128 rng(perItemSeed(number, ri));
                     const ARay& rs = q[ri];
-                    WorldHit h;
-                    R.stats.adjoint_queries++;
-                    if (!intersectScene(R, rs.ray, h)) continue;
+                    if (!hit[ri]) continue;
+                    const WorldHit& h = hits[ri];
                     const Instance& ein = R.inst[h.inst];
                     F3 no = mk(0, 0, 0), dir = mk(0, 0, 0);
                     float pdf = 0.0f;
@@ -1789,7 +1864,16 @@ static void renderBackward(Renderer& R, const std::vector<const Camera*>& camLis
                     }
                 }
                 R.stats.gradient_events += done.size();
-                for (const Completed& e : done) transport(R, e, invSpp);
+                // Irradiance at every event (in parallel), then the gradient
+                // sums in event order.
+                std::vector<EventPre> pre(done.size());
+                parallelItems(done.size(), [&](size_t i) {
+                    const Completed& e = done[i];
+                    pre[i].E = e.kind == kProjection ? gather(R, e.pos, e.normal) : gather(R, e.endPos, e.endNormal);
+                    if (S.fullGradients)
+                        geometryGradient(R, e.kind == kReflectS ? e.endPrim : e.prim, e.ro, e.rd, pre[i]);
+                });
+                for (size_t i = 0; i < done.size(); ++i) transport(R, done[i], invSpp, pre[i]);
                 q.swap(nq);
             }
         }
@@ -1814,6 +1898,7 @@ static Settings toSettings(const pale_settings& s) {
     S.decorrelate = s.spp_decorrelate != 0;
     S.mapCapacity = s.photon_map_capacity;
     S.fullGradients = s.full_gradients != 0;
+    S.adjointDecorrelate = s.adjoint_decorrelate != 0;
     return S;
 }
 
@@ -2165,6 +2250,31 @@ int orc_apply_points(void* h, const pale_points_desc* p) {
 
 int orc_freeze_map(void* h, int32_t on) {
     static_cast<Renderer*>(h)->freezeMap = on != 0;
+    return 0;
+}
+
+// Threads of the oracle's parallel loops in this process (0 = OpenMP's
+// default); results do not depend on it.
+int orc_set_threads(int32_t n) {
+    g_threads = n > 0 ? n : 0;
+    return 0;
+}
+
+// Diagnostic for the tests: applyRussianRoulette (KernelHelpers.h:1612-1657)
+// on n throughputs T[3*i..], each with the per-item stream (seed, i).  Out:
+// alive[i], the throughput after the call, and next[i] = the stream's next
+// float after the call (which tells how many draws the call consumed).
+int orc_russian_roulette(uint32_t n, const float* T, uint32_t bounce, uint32_t rrStart, uint64_t seed,
+                         int32_t* alive, float* Tout, float* next) {
+    for (uint32_t i = 0; i < n; ++i) {
+        Xorshift128 rng(perItemSeed(seed, i));
+        F3 t = mk(T[3 * i], T[3 * i + 1], T[3 * i + 2]);
+        alive[i] = russianRoulette(rng, bounce, t, rrStart) ? 1 : 0;
+        Tout[3 * i] = t.x;
+        Tout[3 * i + 1] = t.y;
+        Tout[3 * i + 2] = t.z;
+        next[i] = rng.nextFloat();
+    }
     return 0;
 }
 

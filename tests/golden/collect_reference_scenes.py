@@ -6,7 +6,8 @@ Copied: Assets/GradientTests/<case>/{<case>.xml, initial.ply, target.ply}
 for the cases listed in python/finite_difference/tests.json, and the mesh
 files those XMLs name (Assets/meshes/rectangle.obj).  These are scene
 descriptions and point clouds the reference's own FD tests load; nothing
-executable is copied.
+executable is copied.  They are CC BY-NC 4.0 data of the reference;
+reference_scenes/NOTICE.md names their sources and the licence.
 
 Usage: python tests/golden/collect_reference_scenes.py [/root/reference]
 """

@@ -370,3 +370,96 @@ def test_full_gradients_match_oracle():
         assert m > 0, k
         np.testing.assert_allclose(g[k], ref, rtol=1e-3, atol=1e-3 * m, err_msg=k)
     assert not np.any(g["shape"])
+
+
+def _photons_match(a, b):
+    """Photon maps as in test_photon_map_bit_exact: same counts, positions
+    and directions bit for bit (up to a rare flip of a final rounding of a
+    transcendental evaluated in binary64 on both sides), powers to 1e-5."""
+    assert a["count"] == b["count"] > 0 and a["attempted"] == b["attempted"]
+    same = np.all(a["position"] == b["position"], axis=1) & np.all(a["direction"] == b["direction"], axis=1)
+    assert same.mean() > 0.999, same.mean()
+    np.testing.assert_allclose(a["power"], b["power"], rtol=1e-5)
+
+
+def _grads_close(g, go, full=None, got_full=None):
+    scale = np.abs(go).max()
+    assert scale > 0
+    np.testing.assert_allclose(g["opacity"], go, rtol=1e-4, atol=1e-4 * scale)
+    if full is not None:
+        for k in ("position", "tangent_u", "tangent_v", "scale", "albedo", "beta"):
+            m = np.abs(full[k]).max()
+            assert m > 0, k
+            np.testing.assert_allclose(got_full[k], full[k], rtol=1e-3, atol=1e-3 * m, err_msg=k)
+
+
+def test_russian_roulette_parity():
+    """rr_start = 2 with 4 photon and 4 adjoint bounces, so Russian roulette
+    (KernelHelpers.h:1612-1657) decides paths from the second extension on:
+    the photon map and the image bit for bit, opacity and full gradients
+    within tolerance; and roulette really acts (fewer photon and adjoint
+    queries than without it)."""
+    spec = scene.config_scene(0, n=4096, width=64, height=64)
+    st = dict(SMALL, bounces=4, adjoint_bounces=4, full_gradients=1)
+    gpu, orc = _pair(spec, dict(st, rr_start=2))
+    ia, ib = gpu.render_forward(), orc.render_forward()
+    _photons_match(gpu.photons(), orc.photons())
+    assert np.array_equal(ia["camera1_raw"].view(np.uint32), ib["camera1_raw"].view(np.uint32))
+    raw = ib["camera1_raw"][..., :3]
+    dl = ((raw - 0.5 * raw) / raw.size).astype(np.float32)
+    g, _ = gpu.render_backward({"camera1": dl})
+    go = orc.render_backward({"camera1": dl})
+    _grads_close(g, go, orc.full_gradients(), g)
+    s_rr = gpu.stats()
+    no_rr, _ = _pair(spec, dict(st, rr_start=6))
+    no_rr.render_forward()
+    no_rr.render_backward({"camera1": dl})
+    s_no = no_rr.stats()
+    assert s_rr["photon_queries"] < s_no["photon_queries"]
+    assert s_rr["adjoint_queries"] < s_no["adjoint_queries"]
+
+
+@pytest.mark.parametrize("batch_passes", [1, 0])
+def test_adjoint_decorrelate_parity(batch_passes):
+    """adjoint_decorrelate = 1: sample k seeds from 42 + k (independent
+    adjoint samples); GPU = oracle, batched and sample by sample, and the
+    result differs from the reference's repeated seed."""
+    spec = scene.config_scene(0, n=4096, width=64, height=64)
+    st = dict(SMALL, adjoint_passes=4, full_gradients=1, batch_passes=batch_passes)
+    gpu, orc = _pair(spec, dict(st, adjoint_decorrelate=1))
+    ib = orc.render_forward()
+    gpu.render_forward()
+    raw = ib["camera1_raw"][..., :3]
+    dl = ((raw - 0.5 * raw) / raw.size).astype(np.float32)
+    g, _ = gpu.render_backward({"camera1": dl})
+    go = orc.render_backward({"camera1": dl})
+    _grads_close(g, go, orc.full_gradients(), g)
+    same, _ = _pair(spec, st)
+    same.render_forward()
+    g0, _ = same.render_backward({"camera1": dl})
+    assert not np.allclose(g0["opacity"], g["opacity"], rtol=1e-3, atol=0)
+    assert gpu.stats()["adjoint_queries"] != same.stats()["adjoint_queries"]
+
+
+def test_transport_pin_scenes_bit_exact():
+    """The scenes that pin the oracle's transport weights (a surfel between
+    light, floor and ceiling; two surfels over a wall with q_null != q_reflect)
+    give the oracle's photon maps bit for bit and its gradients on the GPU."""
+    import sys
+    import os
+
+    sys.path.insert(0, os.path.dirname(__file__))
+    import test_oracle_transport_pins as T
+
+    gpu, orc = _pair(T._photon_scene(), T.PH)
+    gpu.render_forward()
+    orc.render_forward()
+    _photons_match(gpu.photons(), orc.photons())
+    spec = T._adjoint_scene()
+    gpu, orc = _pair(spec, dict(T.ADJ, full_gradients=1))
+    ia, ib = gpu.render_forward(), orc.render_forward()
+    assert np.array_equal(ia["camera1_raw"].view(np.uint32), ib["camera1_raw"].view(np.uint32))
+    dl = np.random.default_rng(8).uniform(-1, 1, (T.H_IMG, T.W_IMG, 3)).astype(np.float32) / (T.H_IMG * T.W_IMG)
+    g, _ = gpu.render_backward({"camera1": dl})
+    go = orc.render_backward({"camera1": dl})
+    _grads_close(g, go, orc.full_gradients(), g)

@@ -398,3 +398,32 @@ def test_sample_offsets_partition_the_image():
         acc += img
     assert np.array_equal(seen, np.ones_like(seen))
     assert np.array_equal(acc, ref)
+
+
+def test_oracle_thread_count_does_not_change_results():
+    """The oracle spreads closest-hit queries, camera pixels and event gathers
+    over threads; everything order-dependent stays sequential, so one thread
+    and many give identical photon maps, images and gradients."""
+    spec = scene.config_scene(0, n=2048, width=24, height=24, seed=2)
+    st = dict(photons=8000, bounces=4, gather_passes=2, adjoint_passes=2, adjoint_bounces=4, rr_start=2,
+              full_gradients=1)
+    out = []
+    for nt in (1, 4):
+        L.orc_set_threads(nt)
+        o = Oracle.from_spec(spec, st)
+        img = o.render_forward()["camera1_raw"]
+        dl = (img[..., :3] / img[..., :3].size).astype(np.float32)
+        g = o.render_backward({"camera1": dl})
+        out.append((img, o.photons(), g, o.full_gradients(), o.stats()))
+        o.close()
+    L.orc_set_threads(0)
+    (a, pa, ga, fa, sa), (b, pb, gb, fb, sb) = out
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    assert np.array_equal(pa["position"], pb["position"]) and np.array_equal(pa["power"], pb["power"])
+    assert np.array_equal(ga.view(np.uint32), gb.view(np.uint32))
+    for k in fa:
+        assert np.array_equal(fa[k].view(np.uint32), fb[k].view(np.uint32)), k
+    for k in ("photon_queries", "gather_queries", "adjoint_queries", "gradient_events", "work_photon",
+              "work_gather", "work_adjoint"):
+        assert sa[k] == sb[k], k
+    assert np.count_nonzero(ga) > 0
