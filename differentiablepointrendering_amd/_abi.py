@@ -110,6 +110,7 @@ class Settings(C.Structure):
         ("full_gradients", C.c_int32),
         ("batch_passes", C.c_int32),
         ("adjoint_decorrelate", C.c_int32),
+        ("adjoint_seed", C.c_uint32),
     ]
 
 
@@ -172,6 +173,7 @@ DEFAULT_SETTINGS = dict(
     full_gradients=0,
     batch_passes=1,
     adjoint_decorrelate=0,
+    adjoint_seed=42,
 )
 
 

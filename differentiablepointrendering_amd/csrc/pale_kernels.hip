@@ -1067,8 +1067,8 @@ __global__ __launch_bounds__(256) void k_post(const float4* fb, float4* ldr, uin
 
 // ---------------------------------------------------------- adjoint --
 // launchRayGenAdjointKernel (AdjointKernels.cpp:15-84).  The adjoint seeds
-// from PathTracerSettings::random.number, which keeps its default 42; with
-// adjoint_decorrelate the host passes 42 + sample instead.
+// from PathTracerSettings::random.number (default 42, setting adjoint_seed);
+// with adjoint_decorrelate the host passes adjoint_seed + sample instead.
 __global__ __launch_bounds__(256) void k_adj_raygen(DevCamera cam, const float* dloss, RayQueue q,
                                                     PendingArrays pend, DevState* st, uint32_t seed) {
     const uint32_t npx = cam.W * cam.H;

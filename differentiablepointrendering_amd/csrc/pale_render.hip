@@ -260,9 +260,10 @@ struct pale_handle {
         return TraceParams{S.rr_start, S.q_null, S.q_reflect, S.q_transmit, S.full_gradients ? 1u : 0u,
                            invSpp, segs, segFirst, nullptr, nullptr, nullptr, 0u, seed};
     }
-    // Base seed of adjoint sample k: the reference's constant 42
-    // (AdjointKernels.cpp:27,94), or 42 + k with adjoint_decorrelate.
-    uint32_t adjointSeed(uint32_t k) const { return S.adjoint_decorrelate ? 42u + k : 42u; }
+    // Base seed of adjoint sample k: Random::number, the reference's
+    // constant 42 (AdjointKernels.cpp:27,94; setting adjoint_seed), or
+    // adjoint_seed + k with adjoint_decorrelate.
+    uint32_t adjointSeed(uint32_t k) const { return S.adjoint_decorrelate ? S.adjoint_seed + k : S.adjoint_seed; }
 
     DevGrid devGrid() const {
         DevGrid g{};
@@ -1478,6 +1479,7 @@ void pale_default_settings(pale_settings* out) {
     s.full_gradients = 0;
     s.batch_passes = 1;
     s.adjoint_decorrelate = 0;
+    s.adjoint_seed = 42;
     *out = s;
 }
 

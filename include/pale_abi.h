@@ -143,12 +143,18 @@ typedef struct pale_settings {
                                   Same images either way; gradients equal
                                   up to the order of float atomics          */
     int32_t adjoint_decorrelate; /* "adjoint_decorrelate" (default 0): 0 =
-                                  every adjoint sample seeds from 42, as the
+                                  every adjoint sample seeds from
+                                  adjoint_seed (42), as the
                                   reference (AdjointKernels.cpp:27,94: its
                                   random.number is never updated), so the
                                   samples of a pixel trace one path; 1 =
                                   sample k seeds from 42 + k (independent
                                   samples)                                  */
+    uint32_t adjoint_seed;     /* "adjoint_seed" (default 42): the adjoint's
+                                  base seed, Random::number
+                                  (GPUDataStructures.h:395-398); with
+                                  adjoint_decorrelate sample k seeds from
+                                  adjoint_seed + k                          */
 } pale_settings;
 
 /* Work counters of the last render calls (no reference counterpart; the

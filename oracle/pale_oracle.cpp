@@ -249,6 +249,7 @@ struct Settings {
     uint32_t mapCapacity;
     bool fullGradients;
     bool adjointDecorrelate;
+    uint32_t adjointSeed;
 };
 
 struct Grid {
@@ -1679,10 +1680,11 @@ static void transport(Renderer& R, const Completed& c, float invSpp, const Event
 }
 
 // submitAdjointKernel (SyclBridge.cpp:169-294) for the given cameras.  The
-// adjoint kernels seed from PathTracerSettings::random.number, which keeps
-// its default 42 (AdjointKernels.cpp:27,94; GPUDataStructures.h:395-398).
-// With adjoint_decorrelate (no reference counterpart) sample k seeds from
-// 42 + k instead, so that the samples of a pixel are independent.
+// adjoint kernels seed from PathTracerSettings::random.number, default 42
+// (AdjointKernels.cpp:27,94; GPUDataStructures.h:395-398; setting
+// adjoint_seed).  With adjoint_decorrelate (no reference counterpart)
+// sample k seeds from adjoint_seed + k instead, so that the samples of a
+// pixel are independent.
 static void renderBackward(Renderer& R, const std::vector<const Camera*>& camList,
                            const std::vector<const float*>& dloss, uint32_t passes) {
     const Settings& S = R.S;
@@ -1695,7 +1697,8 @@ static void renderBackward(Renderer& R, const std::vector<const Camera*>& camLis
         const CamConst kc = camConst(c);
         const uint32_t W = c.W, H = c.H, npx = W * H;
         for (uint32_t spp = 0; spp < passes; ++spp) {
-            const uint64_t number = S.adjointDecorrelate ? (uint64_t)(uint32_t)(42u + spp) : 42ull;
+            const uint64_t number = S.adjointDecorrelate ? (uint64_t)(uint32_t)(S.adjointSeed + spp)
+                                                         : (uint64_t)S.adjointSeed;
             // launchRayGenAdjointKernel (AdjointKernels.cpp:15-84)
             std::vector<ARay> q, nq;
             q.reserve(npx / R.pixelStride + 1);
@@ -1899,6 +1902,7 @@ static Settings toSettings(const pale_settings& s) {
     S.mapCapacity = s.photon_map_capacity;
     S.fullGradients = s.full_gradients != 0;
     S.adjointDecorrelate = s.adjoint_decorrelate != 0;
+    S.adjointSeed = s.adjoint_seed;
     return S;
 }
 

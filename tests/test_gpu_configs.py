@@ -80,7 +80,7 @@ def test_c0_full_config_against_oracle():
     _bvh_match(gpu, orc)
     ia, ib = gpu.render_forward(), orc.render_forward()
     _photons_match(gpu.photons(), orc.photons())
-    assert gpu.photons()["attempted"] > 100_000
+    assert gpu.stats()["photon_queries"] >= 100_000  # every emitted photon traced at least once
     for k in ib:
         assert np.array_equal(ia[k].view(np.uint32), ib[k].view(np.uint32)), k
     dl = _dloss(ib["camera1_raw"], 1)

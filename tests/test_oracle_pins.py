@@ -427,3 +427,35 @@ def test_oracle_thread_count_does_not_change_results():
               "work_gather", "work_adjoint"):
         assert sa[k] == sb[k], k
     assert np.count_nonzero(ga) > 0
+
+
+# ------------------------------------------------- adjoint seed schedule --
+def test_adjoint_seed_schedule():
+    """adjoint_seed is the adjoint's base seed (Random::number, default 42):
+    the default equals 42 given explicitly; with adjoint_decorrelate the
+    2-sample adjoint from seed s is the mean of the 1-sample adjoints from
+    s and s + 1 (sample k seeds from s + k); without it both samples repeat
+    the seed-s sample."""
+    spec = scene.config_scene(0, n=4096, width=32, height=32)
+    st = dict(photons=20_000, gather_passes=1, full_gradients=0)
+
+    def grad(**kw):
+        o = Oracle.from_spec(spec, dict(st, **kw))
+        raw = o.render_forward()["camera1_raw"]
+        dl = (raw[..., :3] / raw[..., :3].size).astype(np.float32)
+        g = o.render_backward({"camera1": dl})
+        o.close()
+        return g
+
+    base = grad(adjoint_passes=1)
+    assert np.abs(base).max() > 0
+    assert np.array_equal(base, grad(adjoint_passes=1, adjoint_seed=42))
+    s = 1000
+    one_a = grad(adjoint_passes=1, adjoint_seed=s)
+    one_b = grad(adjoint_passes=1, adjoint_seed=s + 1)
+    assert not np.array_equal(one_a, one_b)
+    two = grad(adjoint_passes=2, adjoint_seed=s, adjoint_decorrelate=1)
+    scale = np.abs(two).max()
+    np.testing.assert_allclose(two, 0.5 * (one_a + one_b), rtol=1e-5, atol=1e-6 * scale)
+    rep = grad(adjoint_passes=2, adjoint_seed=s)
+    np.testing.assert_allclose(rep, one_a, rtol=1e-5, atol=1e-6 * scale)
