@@ -18,10 +18,20 @@ step's inputs are resident in device memory when the timed region starts;
 the wall-clock rate, which includes the host<->device image copies and the
 loss on the host, is reported beside it (mrays_per_s_wall).
 
-Also reported: the roofline of the dominant kernel by device time, from the
+Independent-samples form ("independent_samples"): the same step with every
+gather pass and adjoint sample seeded on its own (spp_decorrelate and
+adjoint_decorrelate).  The reference seeds all gather passes of a pixel
+alike and all its adjoint samples with 42, so its 16 spp are one sample
+repeated; this form is 16 independent samples per pixel.
+
+Also reported: the roofline of the dominant kernel by device time, priced
+against HBM from its counted traffic (calibrated rocprofv3 FETCH_SIZE +
+WRITE_SIZE of the same workload, profiles/traffic_<form>.json) over the
+launch time measured here, with what the counters say bounds it; the
 reference's algorithmic byte model per work unit (counted in a separate,
-untimed counting step), the same figures for the other camera-gather stage,
-and the CPU oracle timed on a bounded sample of the same workload.
+untimed counting step) beside it as information; the same figures for the
+other large kernels; and the CPU oracle timed on a bounded sample of the
+same workload.
 
 Served form ("served"): four resident clustered scenes (100k, 500k, 1M and
 3M surfels with photon maps of 1, 2, 3 and 4M photons, depth 6, radius
@@ -73,8 +83,9 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=16,
                     help="concurrent oracle instances of the CPU baseline (capped at the host's CPU count)")
     ap.add_argument("--no-count", action="store_true", help="skip the work-counting step (roofline)")
-    ap.add_argument("--decorrelated", type=int, default=2,
-                    help="timed steps of the same workload with an independent seed per gather pass (0: skip)")
+    ap.add_argument("--decorrelated", type=int, default=3,
+                    help="timed steps of the independent-samples form (every gather pass and adjoint sample "
+                         "seeded on its own; 0: skip)")
     ap.add_argument("--served", type=int, default=40, help="requests of the served stream (0: skip)")
     ap.add_argument("--served-rate", type=float, default=1.0, help="Poisson arrival rate, requests per second")
     ap.add_argument("--served-replicas", type=int, default=2)
@@ -92,6 +103,19 @@ def parse():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only an elementwise kernel reading 1 GiB and writing 1 GiB: the known-size kernel "
                          "that scales rocprof FETCH_SIZE / WRITE_SIZE to bytes")
+    ap.add_argument("--adjoint-decorrelate", action="store_true",
+                    help="plain form: adjoint sample k seeds from 42 + k (with --decorrelate: every sample independent)")
+    ap.add_argument("--form", choices=("plain", "independent", "batched", "served"), default=None,
+                    help="run only this form (plain: the headline step; independent: the same step with every "
+                         "gather pass and adjoint sample independently seeded)")
+    ap.add_argument("--marker", action="store_true",
+                    help="launch a marker kernel (torch's spin_kernel) right before the timed steps, so that a "
+                         "profile can tell the timed steps' dispatches from set-up")
+    ap.add_argument("--pmc-summary", nargs="+", metavar="DIR", default=None,
+                    help="summarise rocprofv3 CSV output (kernel trace and/or counter collections) under the given "
+                         "directories: per kernel, the dispatches after the last marker kernel")
+    ap.add_argument("--calibration", nargs="+", metavar="DIR", default=None,
+                    help="with --pmc-summary: rocprofv3 FETCH_SIZE / WRITE_SIZE collections of --copy-calibration")
     return ap.parse_args()
 
 
@@ -102,11 +126,131 @@ def copy_calibration():
     x = torch.ones(n, dtype=torch.float32, device="cuda")
     y = torch.empty_like(x)
     torch.cuda.synchronize()
+    marker()
     # An elementwise kernel (not the runtime's copy engine or blit path), so
     # the counters see an ordinary streaming read and write.
     torch.add(x, 1.0, out=y)
     torch.cuda.synchronize()
     print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n}))
+    return 0
+
+
+MARKER_KERNEL = "spin_kernel"
+
+
+def marker():
+    """A short kernel of a name no step launches (torch's spin_kernel): in a
+    profile, the dispatches after it are the timed region's."""
+    import torch
+
+    torch.cuda.synchronize()
+    torch.cuda._sleep(1000)
+    torch.cuda.synchronize()
+
+
+def _read_csvs(dirs, suffix):
+    import csv
+    import glob
+
+    rows = []
+    for d in dirs:
+        for path in sorted(glob.glob(os.path.join(d, "**", f"*{suffix}"), recursive=True)):
+            with open(path, newline="") as f:
+                rows.extend(csv.DictReader(f))
+    return rows
+
+
+def _short(name: str) -> str:
+    """Kernel name without template arguments and parameter list."""
+    name = name.split("(")[0].split("<")[0].strip()
+    return name.split()[-1].split("::")[-1] if name else name
+
+
+def _after_marker(rows, key_start):
+    if rows and key_start not in rows[0]:
+        key_start = "Dispatch_Id"  # dispatch ids grow with submission order
+    marks = [int(r[key_start]) for r in rows if MARKER_KERNEL in r.get("Kernel_Name", "")]
+    t0 = max(marks) if marks else -1
+    return [r for r in rows if int(r[key_start]) > t0 and MARKER_KERNEL not in r.get("Kernel_Name", "")]
+
+
+def pmc_summary(dirs, cal_dirs=None):
+    """Per-kernel figures of the dispatches after the last marker kernel in
+    rocprofv3 CSV output (--output-format csv): from a kernel trace the
+    launches, durations and the resources of the code object (VGPRs, AGPRs,
+    SGPRs, scratch and LDS bytes); from counter collections every counter's
+    sum over those dispatches.  With calibration directories (the same
+    FETCH_SIZE / WRITE_SIZE collections over --copy-calibration, 1 GiB read
+    and 1 GiB written by one elementwise kernel) FETCH_SIZE and WRITE_SIZE are
+    turned into bytes."""
+    out = {"kernels": {}}
+    kt = _read_csvs(dirs, "kernel_trace.csv")
+    if kt:
+        for r in _after_marker(kt, "Start_Timestamp"):
+            k = out["kernels"].setdefault(_short(r["Kernel_Name"]), {"launches": 0, "total_ms": 0.0})
+            k["launches"] += 1
+            k["total_ms"] += (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-6
+            for col, key in (("Arch_VGPR_Count", "vgpr"), ("VGPR_Count", "vgpr"), ("Accum_VGPR_Count", "agpr"),
+                             ("SGPR_Count", "sgpr"), ("Scratch_Size", "scratch_bytes_per_lane"),
+                             ("Private_Segment_Size", "scratch_bytes_per_lane"), ("LDS_Block_Size", "lds_bytes"),
+                             ("Group_Segment_Size", "lds_bytes")):
+                if col in r and r[col] not in ("", None):
+                    k[key] = int(float(r[col]))
+        for k in out["kernels"].values():
+            k["avg_ms"] = k["total_ms"] / k["launches"]
+    # Counter collections: one directory per rocprofv3 --pmc pass (separate
+    # processes running the same workload); per kernel and counter, the mean
+    # per dispatch over that pass's dispatches after the marker.
+    for d in dirs:
+        cc = _read_csvs([d], "counter_collection.csv")
+        if not cc:
+            continue
+        acc = {}
+        for r in _after_marker(cc, "Start_Timestamp"):
+            a = acc.setdefault((_short(r["Kernel_Name"]), r["Counter_Name"]), [0.0, set()])
+            a[0] += float(r["Counter_Value"])
+            a[1].add(r["Dispatch_Id"])
+        for (name, counter), (total, ids) in acc.items():
+            k = out["kernels"].setdefault(name, {})
+            k.setdefault("counters_per_launch", {})[counter] = total / len(ids)
+            k.setdefault("counter_launches", {})[counter] = len(ids)
+    if cal_dirs:
+        cal = {}
+        for r in _after_marker(_read_csvs(cal_dirs, "counter_collection.csv"), "Start_Timestamp"):
+            if r["Counter_Name"] in ("FETCH_SIZE", "WRITE_SIZE"):
+                cal[r["Counter_Name"]] = max(cal.get(r["Counter_Name"], 0.0), float(r["Counter_Value"]))
+        gib = float(1 << 30)
+        f_fac = gib / (cal["FETCH_SIZE"] * 1024.0) if cal.get("FETCH_SIZE") else None
+        w_fac = gib / (cal["WRITE_SIZE"] * 1024.0) if cal.get("WRITE_SIZE") else None
+        out["calibration"] = {"fetch_kib_for_1GiB_read": cal.get("FETCH_SIZE"),
+                              "write_kib_for_1GiB_written": cal.get("WRITE_SIZE"),
+                              "fetch_factor": f_fac, "write_factor": w_fac}
+        for k in out["kernels"].values():
+            c = k.get("counters_per_launch", {})
+            if "FETCH_SIZE" in c and "WRITE_SIZE" in c and f_fac and w_fac:
+                k["hbm_bytes_per_launch"] = c["FETCH_SIZE"] * 1024.0 * f_fac + c["WRITE_SIZE"] * 1024.0 * w_fac
+    # What the counters say bounds each kernel, as fractions of the kernel's
+    # cycles on every CU (GRBM_GUI_ACTIVE sums the 8 XCDs' GPU-busy cycles;
+    # the _sum counters sum 256 CUs).
+    for k in out["kernels"].values():
+        c = k.get("counters_per_launch", {})
+        if c.get("GRBM_GUI_ACTIVE"):
+            cyc = c["GRBM_GUI_ACTIVE"] / 8.0
+            u = {}
+            for name, key in (("TD_TD_BUSY_sum", "td_busy"), ("TA_TA_BUSY_sum", "ta_busy"),
+                              ("TCP_PENDING_STALL_CYCLES_sum", "l1_stalled_on_l2")):
+                if name in c:
+                    u[key] = c[name] / 256.0 / cyc
+            if "TCP_TOTAL_CACHE_ACCESSES_sum" in c and "TCP_TCC_READ_REQ_sum" in c:
+                u["l1_hit"] = 1.0 - c["TCP_TCC_READ_REQ_sum"] / max(c["TCP_TOTAL_CACHE_ACCESSES_sum"], 1.0)
+            k["utilisation"] = u
+        if c.get("SQ_WAVE_CYCLES"):
+            u = k.setdefault("utilisation", {})
+            for name, key in (("SQ_WAIT_ANY", "wave_waiting"), ("SQ_ACTIVE_INST_VALU", "wave_valu_issue"),
+                              ("SQ_ACTIVE_INST_VMEM", "wave_vmem_issue")):
+                if name in c:
+                    u[key] = c[name] / c["SQ_WAVE_CYCLES"]
+    print(json.dumps(out, indent=1, sort_keys=True))
     return 0
 
 
@@ -431,7 +575,7 @@ def served_leg(a):
     return out
 
 
-def cpu_leg(a, spec, settings, target, cam):
+def cpu_leg(a, spec, settings, target, cam, step_queries=None):
     """The oracle (the sequential CPU restatement) on a bounded sample of the
     same workload: photons/stride photons, every stride-th pixel, all passes
     and samples, forward + adjoint.  `threads` instances run the same sample
@@ -475,72 +619,189 @@ def cpu_leg(a, spec, settings, target, cam):
            "sample": f"{threads} concurrent oracle instances (one per host thread), each on the same scene with "
                      f"photons/{a.cpu_photon_stride} and every {a.cpu_pixel_stride}th pixel, "
                      f"{settings['gather_passes']} spp, depth {settings['bounces']}: {runs[0][0]} queries each, "
-                     f"{wall:.1f}s wall"}
+                     f"{wall:.1f}s wall",
+           "rate_is": "queries of the sample over its wall time (measured on the sample, not extrapolated)",
+           "sample_fraction_of_step": {"photons": 1.0 / a.cpu_photon_stride, "pixels": 1.0 / a.cpu_pixel_stride}}
+    if step_queries:
+        # What one full step would take on these cores at the sample's rate.
+        out["full_step_s_at_this_rate"] = round(step_queries / (out["value"] * 1e6), 1)
     print(f"[bench] cpu baseline {out['value']} Mrays/s on {threads} threads ({wall:.1f}s)", file=sys.stderr,
           flush=True)
     return out
 
 
-def decorrelated_leg(a, spec, settings, target, kernels):
-    """The plain workload with spp_decorrelate = 1: every gather pass draws
-    its own camera rays.  With the reference's default (one seed for all
-    passes) the passes of a pixel trace identical rays, which the batched
-    layer walk places side by side in a wave, so their node and surfel loads
-    coalesce; here every lane's ray is distinct.  Same surfels, target and
-    tuning; the work per step differs from the shared-seed one only in which
-    rays are traced."""
+def counter_roofline(form, kernel, avg_launch_ms, algorithmic_bytes=None):
+    """Roofline of `kernel` priced against HBM from its counted traffic: the
+    calibrated FETCH_SIZE + WRITE_SIZE bytes per launch of the same workload
+    (profiles/traffic_<form>.json, rocprofv3 --pmc) over the launch duration
+    measured here with HIP events.  What bounds the kernel comes from the
+    same profile's unit counters (TD / TA busy, L1 hit rate, issue), with
+    HBM's own fraction beside them.  The reference byte model per work unit
+    (every lane charged for its own node and surfel loads, most of which
+    the L1 serves) is kept as an informational rate."""
+    prof = load_traffic(os.path.join(ROOT, "profiles", f"traffic_{form}.json"))
+    out = {"kernel": kernel, "bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None,
+           "traffic": None, "avg_launch_ms": round(avg_launch_ms, 3)}
+    k = (prof or {}).get("kernels", {}).get(kernel, {})
+    if "hbm_bytes_per_launch" in k and avg_launch_ms > 0:
+        b = float(k["hbm_bytes_per_launch"])
+        ach = b / (avg_launch_ms * 1e-3) / 1e9
+        out.update(achieved=round(ach, 1), frac=round(ach / HBM_PEAK_GBS, 4), traffic=b,
+                   traffic_source=f"profiles/traffic_{form}.json",
+                   traffic_profile_avg_launch_ms=round(k.get("avg_ms", 0.0), 3))
+        if ach > HBM_PEAK_GBS:
+            out["invalid"] = "counted HBM bytes over the measured time exceed the HBM peak"
+    else:
+        out["note"] = f"no counter profile of {kernel} for this form (profiles/traffic_{form}.json)"
+    u = k.get("utilisation")
+    if u:
+        lim = {kk: round(v, 3) for kk, v in u.items()}
+        if out["frac"] is not None:
+            lim["hbm"] = out["frac"]
+        busy = {kk: v for kk, v in lim.items() if kk in ("td_busy", "ta_busy", "hbm")}
+        top = max(busy, key=busy.get) if busy else None
+        names = {"td_busy": "vector-memory data return (TD)", "ta_busy": "vector-memory addressing (TA)",
+                 "hbm": "HBM bandwidth"}
+        out["limiter"] = {"unit": names.get(top, top), "fractions": lim}
+    for key in ("vgpr", "agpr", "sgpr", "scratch_bytes_per_lane", "lds_bytes"):
+        if key in k:
+            out.setdefault("resources", {})[key] = k[key]
+    if algorithmic_bytes:
+        out["algorithmic"] = {"bytes_per_launch": algorithmic_bytes,
+                              "gbs": round(algorithmic_bytes / (avg_launch_ms * 1e-3) / 1e9, 1),
+                              "model": "reference layouts per work unit: 96 B per query, 24 B per slab test, 8 B "
+                                       "per popped node, 52 B per surfel test (per lane, cache hits included)"}
+    return out
+
+
+def run_plain(a, spec, settings, target, steps, warmup, form, count=True):
+    """Timed forward + adjoint steps of one renderer on the plain workload.
+    Device time per step comes from the renderer's HIP events (its own
+    stream); wall time brackets each step from the host."""
     from differentiablepointrendering_amd.pale import Renderer
 
-    s2 = dict(settings, spp_decorrelate=1)
-    r = Renderer.from_spec(spec, s2)
+    cam = spec.cameras[0]
+    t_build = time.time()
+    r = Renderer.from_spec(spec, settings)
     if a.tuning:
         sl, en, gr, fr = a.tuning.split(",")
         r.set_batch_tuning(int(sl), int(en), int(gr), float(fr))
-    cam = spec.cameras[0]
+    build_s = time.time() - t_build
+    print(f"[bench] {form}: renderer ready (host BVH build {build_s:.2f}s)", file=sys.stderr, flush=True)
 
     def step():
-        raw = r.render_forward()[cam.name + "_raw"][..., :3]
-        r.render_backward({cam.name: ((raw - target) / raw.size).astype(np.float32)})
+        img = r.render_forward()
+        raw = img[cam.name + "_raw"][..., :3]
+        dl = ((raw - target) / raw.size).astype(np.float32)
+        r.render_backward({cam.name: dl})
         return r.stats()
 
-    step()
-    dev, q, tr, trl = [], [], 0.0, 0
-    for _ in range(a.decorrelated):
+    for i in range(warmup):
+        step()
+        print(f"[bench] {form}: warmup {i + 1}/{warmup} done", file=sys.stderr, flush=True)
+    if a.marker:
+        marker()
+    dev_ms, wall_ms, queries = [], [], []
+    acc = {}
+    tw0 = time.time()
+    for i in range(steps):
+        ts = time.time()
         st = step()
-        dev.append(st["forward_ms"] + st["backward_ms"])
-        q.append(st["photon_queries"] + st["gather_queries"] + st["adjoint_queries"])
-        tr += st["camera_trace_ms"]
-        trl += st["camera_trace_launches"]
+        wall_ms.append((time.time() - ts) * 1e3)
+        dev_ms.append(st["forward_ms"] + st["backward_ms"])
+        queries.append(st["photon_queries"] + st["gather_queries"] + st["adjoint_queries"])
+        for k in ("gather_ms", "gather_launches", "camera_trace_ms", "camera_trace_launches", "photon_trace_ms",
+                  "photon_trace_launches", "adjoint_trace_ms", "adjoint_trace_launches", "adjoint_transport_ms",
+                  "adjoint_transport_launches"):
+            acc[k] = acc.get(k, 0) + st[k]
+        print(f"[bench] {form}: step {i + 1}/{steps}: device {dev_ms[-1]:.1f} ms, wall {wall_ms[-1]:.1f} ms, "
+              f"{queries[-1] / 1e6:.1f} M queries", file=sys.stderr, flush=True)
+    total_wall = time.time() - tw0
+    last = st
+    ms_step = float(np.mean(dev_ms))
+    q_step = float(np.mean(queries))
+    res = {
+        "value": round(q_step / (ms_step * 1e-3) / 1e6, 2),
+        "ms_per_step": round(ms_step, 3),
+        "queries_per_step": int(q_step),
+        "ms_per_step_wall": round(float(np.mean(wall_ms)), 3),
+        "mrays_per_s_wall": round(q_step / (np.mean(wall_ms) * 1e-3) / 1e6, 2),
+        "camera_samples_per_s": round(cam.width * cam.height * settings["gather_passes"] / (ms_step * 1e-3), 1),
+        "phase_ms": {k: round(last[k], 3) for k in ("photon_trace_ms", "grid_ms", "camera_trace_ms", "camera_sort_ms",
+                                                    "gather_ms", "adjoint_trace_ms", "adjoint_transport_ms",
+                                                    "forward_ms", "backward_ms")},
+        "camera_rounds": last["camera_rounds"],
+        "host_bvh_build_s": round(build_s, 3),
+        "timed_wall_s": round(total_wall, 3),
+        "batch_tuning": r.batch_tuning(),
+    }
+    # Average launch of each large kernel over the timed steps (HIP events on
+    # the renderer's stream).
+    avg = {}
+    for name, key in (("k_cam_trace", "camera_trace"), ("k_entry_gather", "gather"), ("k_photon_trace", "photon_trace"),
+                      ("k_adj_trace", "adjoint_trace"), ("k_adj_transport", "adjoint_transport")):
+        n_l = acc.get(f"{key}_launches", 0)
+        if n_l:
+            avg[name] = {"avg_launch_ms": acc[f"{key}_ms"] / n_l, "ms_per_step": acc[f"{key}_ms"] / steps,
+                         "launches_per_step": n_l / steps}
+    res["kernels"] = avg
+    algorithmic = {}
+    if count:
+        r.set_count_work(True)
+        cst = step()
+        r.set_count_work(False)
+        work = {k: cst[f"work_{k}"] for k in ("photon", "gather", "adjoint")}
+        res["work_per_step"] = work
+        res["counted_step_matches_timed"] = bool(cst["gather_queries"] == last["gather_queries"])
+        algorithmic["k_entry_gather"] = gather_bytes(work["gather"]) / max(1, cst["gather_launches"])
+        algorithmic["k_cam_trace"] = query_bytes(cst["gather_queries"], work["gather"]) / max(
+            1, cst["camera_trace_launches"])
+        # Whole step in the same byte model, for the cross-check against the
+        # step's device time.
+        step_bytes = (query_bytes(cst["photon_queries"], work["photon"]) +
+                      query_bytes(cst["gather_queries"], work["gather"]) + gather_bytes(work["gather"]) +
+                      query_bytes(cst["adjoint_queries"], work["adjoint"]) + gather_bytes(work["adjoint"]))
+        res["algorithmic_step"] = {"bytes": step_bytes, "gbs_over_step": round(step_bytes / (ms_step * 1e-3) / 1e9, 1),
+                                   "note": "per-lane byte model; its rate above the HBM peak counts L1/L2 hits"}
+    dom = max(avg, key=lambda k: avg[k]["ms_per_step"]) if avg else None
+    if dom:
+        res["roofline"] = counter_roofline(form, dom, avg[dom]["avg_launch_ms"], algorithmic.get(dom))
+        res["kernel_rooflines"] = {k: counter_roofline(form, k, v["avg_launch_ms"], algorithmic.get(k))
+                                   for k, v in avg.items() if k != dom}
     r.close()
-    ms = float(np.mean(dev))
-    out = {"value": round(float(np.mean(q)) / (ms * 1e-3) / 1e6, 2), "unit": "Mrays/s", "steps": a.decorrelated,
-           "ms_per_step": round(ms, 3),
-           "phase_ms": {k: round(st[k], 3) for k in ("camera_trace_ms", "camera_sort_ms", "gather_ms",
-                                                     "adjoint_trace_ms", "adjoint_transport_ms")}}
-    if kernels is not None and trl > 0:
-        # Algorithmic bytes of the layer walk as counted for the shared-seed
-        # step (the decorrelated walk's count differs by well under 0.1 %).
-        per_launch = kernels["k_cam_trace"]["algorithmic_bytes_per_launch"]
-        avg_s = tr / trl * 1e-3
-        out["k_cam_trace"] = {"avg_launch_ms": round(avg_s * 1e3, 3),
-                              "achieved_gbs": round(per_launch / avg_s / 1e9, 1),
-                              "frac": round(per_launch / avg_s / 1e9 / HBM_PEAK_GBS, 4)}
-    print(f"[bench] decorrelated passes {out['value']} Mrays/s", file=sys.stderr, flush=True)
-    return out
+    del r
+    return res
+
+
+def independent_leg(a, spec, settings, target):
+    """The plain workload with every sample independently seeded: gather pass
+    k draws seed + k (spp_decorrelate) and adjoint sample k seeds from 42 + k
+    (adjoint_decorrelate).  With the reference's seeding the 16 gather passes
+    of a pixel trace one ray and its 16 adjoint samples one path (the layer
+    walk places a pixel's passes side by side in a wave, so their loads
+    coalesce); here every sample is a distinct ray and path, so "16 spp" is
+    16 independent samples.  Same surfels, target and tuning."""
+    s2 = dict(settings, spp_decorrelate=1, adjoint_decorrelate=1)
+    res = run_plain(a, spec, s2, target, a.decorrelated, 1, "independent", count=not a.no_count)
+    print(f"[bench] independent samples {res['value']} Mrays/s, {res['ms_per_step']} ms per step",
+          file=sys.stderr, flush=True)
+    return res
 
 
 def main():
     a = parse()
     if a.copy_calibration:
         return copy_calibration()
+    if a.pmc_summary:
+        return pmc_summary(a.pmc_summary, a.calibration)
     if a.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {a.gpus}).",
               file=sys.stderr)
         return 2
-    if a.served_only:
+    if a.served_only or a.form == "served":
         print(json.dumps({"served": served_leg(a)}))
         return 0
-    if a.batched_only:
+    if a.batched_only or a.form == "batched":
         print(json.dumps({"batched": batched_leg(a)}))
         return 0
     if a.differentiation_only:
@@ -550,7 +811,6 @@ def main():
     from differentiablepointrendering_amd import scene
     from differentiablepointrendering_amd.pale import Renderer
 
-    t0 = time.time()
     spec = scene.config_scene(1, n=a.surfels, width=a.width, height=a.height)
     settings = dict(spec.settings)
     if a.photons:
@@ -558,6 +818,7 @@ def main():
     if a.spp:
         settings["gather_passes"] = settings["adjoint_passes"] = a.spp
     settings["spp_decorrelate"] = 1 if a.decorrelate else 0
+    settings["adjoint_decorrelate"] = 1 if a.adjoint_decorrelate else 0
     settings["full_gradients"] = 1 if a.gradients == "full" else 0
     settings["batch_passes"] = a.batch_passes
     cam = spec.cameras[0]
@@ -571,96 +832,26 @@ def main():
     target = tgt.render_forward()[cam.name + "_raw"][..., :3].copy()
     tgt.close()
     del tgt
-    t_build = time.time()
-    r = Renderer.from_spec(spec, settings)
-    if a.tuning:
-        sl, en, gr, fr = a.tuning.split(",")
-        r.set_batch_tuning(int(sl), int(en), int(gr), float(fr))
-    build_s = time.time() - t_build
-    r_tuning = r.batch_tuning()
-    print(f"[bench] renderer ready (host BVH build {build_s:.2f}s)", file=sys.stderr, flush=True)
 
-    def step():
-        img = r.render_forward()
-        raw = img[cam.name + "_raw"][..., :3]
-        dl = ((raw - target) / raw.size).astype(np.float32)
-        r.render_backward({cam.name: dl})
-        return r.stats()
-
-    for i in range(a.warmup):
-        step()
-        print(f"[bench] warmup {i + 1}/{a.warmup} done", file=sys.stderr, flush=True)
-
-    dev_ms, wall_ms, queries = [], [], []
-    gather_ms, gather_launches, trace_ms, trace_launches = 0.0, 0, 0.0, 0
-    tw0 = time.time()
-    for i in range(a.steps):
-        ts = time.time()
-        st = step()
-        wall_ms.append((time.time() - ts) * 1e3)
-        dev_ms.append(st["forward_ms"] + st["backward_ms"])
-        queries.append(st["photon_queries"] + st["gather_queries"] + st["adjoint_queries"])
-        gather_ms += st["gather_ms"]
-        gather_launches += st["gather_launches"]
-        trace_ms += st["camera_trace_ms"]
-        trace_launches += st["camera_trace_launches"]
-        print(f"[bench] step {i + 1}/{a.steps}: device {dev_ms[-1]:.1f} ms, wall {wall_ms[-1]:.1f} ms, "
-              f"{queries[-1] / 1e6:.1f} M queries", file=sys.stderr, flush=True)
-    total_wall = time.time() - tw0
-    last = st
-    ms_step = float(np.mean(dev_ms))
-    q_step = float(np.mean(queries))
-    value = q_step / (ms_step * 1e-3) / 1e6
-
-    roofline = None
-    work = None
-    kernels = None
-    if not a.no_count:
-        r.set_count_work(True)
-        cst = step()
-        r.set_count_work(False)
-        work = {k: cst[f"work_{k}"] for k in ("photon", "gather", "adjoint")}
-        same = cst["gather_queries"] == last["gather_queries"]
-        wg = work["gather"]
-        # Camera gather stages of the timed steps: the irradiance gathers over
-        # cell-sorted entries and the closest-hit layer walk.
-        kernels = {}
-        for name, ms, launches, nbytes, counted_launches in (
-                ("k_entry_gather", gather_ms, gather_launches, gather_bytes(wg), cst["gather_launches"]),
-                ("k_cam_trace", trace_ms, trace_launches, query_bytes(cst["gather_queries"], wg),
-                 cst["camera_trace_launches"])):
-            per_launch = nbytes / max(1, counted_launches)
-            avg_s = ms / max(1, launches) * 1e-3
-            kernels[name] = {"algorithmic_bytes_per_launch": per_launch, "avg_launch_ms": avg_s * 1e3,
-                             "achieved_gbs": per_launch / avg_s / 1e9, "total_ms_per_step": ms / a.steps}
-        dom = max(kernels, key=lambda k: kernels[k]["total_ms_per_step"])
-        kd = kernels[dom]
-        traffic = load_traffic(os.path.join(ROOT, "profiles", f"{dom}_1m_surfels_800px_traffic.json"))
-        roofline = {
-            "kernel": dom,
-            "bound": "hbm",
-            "achieved": round(kd["achieved_gbs"], 1),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(kd["achieved_gbs"] / HBM_PEAK_GBS, 4),
-            "traffic": traffic.get("bytes_per_launch") if traffic else None,
-            "algorithmic_bytes_per_launch": kd["algorithmic_bytes_per_launch"],
-            "avg_launch_ms": kd["avg_launch_ms"],
-            "counted_step_matches_timed": bool(same),
-        }
+    if a.form == "independent":
+        print(json.dumps({"independent_samples": independent_leg(a, spec, settings, target)}))
+        return 0
+    independent = bool(a.decorrelate and a.adjoint_decorrelate)
+    form = "independent" if independent else ("plain" if not (a.decorrelate or a.adjoint_decorrelate) else "custom")
+    p = run_plain(a, spec, settings, target, a.steps, a.warmup, form, count=not a.no_count)
 
     cpu = None
-    if not a.no_cpu:
-        cpu = cpu_leg(a, spec, settings, target, cam)
+    if not a.no_cpu and a.form is None:
+        cpu = cpu_leg(a, spec, settings, target, cam, p["queries_per_step"])
 
     out = {
         "metric": METRIC,
-        "value": round(value, 2),
+        "value": p["value"],
         "unit": "Mrays/s",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": round(ms_step, 3),
+        "ms_per_step": p["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -673,35 +864,27 @@ def main():
             "surfels": n, "width": W, "height": H, "spp": settings["gather_passes"],
             "photons": settings["photons"], "depth": settings["bounces"],
             "gather_passes_share_seed": not a.decorrelate,
+            "adjoint_samples_share_seed": not a.adjoint_decorrelate,
             "batch_passes": a.batch_passes,
-            "batch_tuning": r_tuning,
+            "batch_tuning": p["batch_tuning"],
             "gradients": a.gradients,
         },
-        "queries_per_step": int(q_step),
-        "ms_per_step_wall": round(float(np.mean(wall_ms)), 3),
-        "mrays_per_s_wall": round(q_step / (np.mean(wall_ms) * 1e-3) / 1e6, 2),
-        "phase_ms": {k: round(last[k], 3) for k in ("photon_trace_ms", "grid_ms", "camera_trace_ms", "camera_sort_ms",
-                                                    "gather_ms", "adjoint_trace_ms", "adjoint_transport_ms",
-                                                    "forward_ms", "backward_ms")},
-        "camera_rounds": last["camera_rounds"],
-        "host_bvh_build_s": round(build_s, 3),
-        "timed_wall_s": round(total_wall, 3),
-        "roofline": roofline,
-        "cpu_baseline": cpu,
     }
-    if work is not None:
-        out["work_per_step"] = work
-        out["kernels"] = kernels
-    r.close()
-    del r
-    if a.decorrelated > 0 and not a.decorrelate:
-        out["decorrelated_passes"] = decorrelated_leg(a, spec, settings, target, kernels)
-    if a.batched > 0:
-        out["batched"] = batched_leg(a)
-    if a.differentiation > 0:
-        out["differentiation"] = differentiation_leg(a)
-    if a.served > 0:
-        out["served"] = served_leg(a)
+    for k in ("queries_per_step", "ms_per_step_wall", "mrays_per_s_wall", "camera_samples_per_s", "phase_ms",
+              "camera_rounds", "host_bvh_build_s", "timed_wall_s", "roofline", "kernel_rooflines", "work_per_step",
+              "counted_step_matches_timed", "algorithmic_step"):
+        if k in p:
+            out[k] = p[k]
+    out["cpu_baseline"] = cpu
+    if a.decorrelated > 0 and form == "plain" and a.form is None:
+        out["independent_samples"] = independent_leg(a, spec, settings, target)
+    if a.form is None:
+        if a.batched > 0:
+            out["batched"] = batched_leg(a)
+        if a.differentiation > 0:
+            out["differentiation"] = differentiation_leg(a)
+        if a.served > 0:
+            out["served"] = served_leg(a)
     print(json.dumps(out))
     return 0
 

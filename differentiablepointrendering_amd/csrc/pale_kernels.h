@@ -169,6 +169,12 @@ struct TraceParams {
     const uint32_t* groupSegs;
     uint32_t nGroups;
     uint32_t seed;  // adjoint base seed of an unbatched queue (batched: BatchSeg::seed)
+    // Debug gradient images (setting debug_images): events also record their
+    // image pixel, the index over all adjoint cameras' pixels (camera
+    // offset + pixel) in e5.w; debugBase is the unbatched queue's camera
+    // offset (batched: BatchSeg::pixOffset).
+    uint32_t debug;
+    uint32_t debugBase;
 };
 
 // Photon-map reservoir (depositPhotonSurface past capacity): per queue item
@@ -267,8 +273,12 @@ void launchEventSort(hipStream_t s, const GridParams& g, DevState* st, uint32_t 
                      uint32_t* scratchTotal);
 // gradFull (16 floats per surfel: position 3, tangent_u 3, tangent_v 3,
 // scale 2, albedo 3, unused, beta) is null unless full gradients are on.
+// debugImg (nullable): RGBA per pixel over all adjoint cameras; each event
+// adds its opacity-gradient contribution to RGB and sets alpha to 1
+// (atomicAddFloat4ToImage, KernelHelpers.h:1489-1502).
 void launchAdjTransport(hipStream_t s, const DevScene& S, const DevGrid& G, DevState* st, uint32_t poolCap,
-                        EventArrays ev, const uint32_t* sorted, float* gradOpacity, float* gradFull, bool count);
+                        EventArrays ev, const uint32_t* sorted, float* gradOpacity, float* gradFull, float* debugImg,
+                        bool count);
 // Split gradFull into the ABI's arrays (any may be null).
 void launchSplitGradients(hipStream_t s, const float* gradFull, uint32_t n, float* position, float* tangentU,
                           float* tangentV, float* scale, float* albedo, float* beta);

@@ -1191,6 +1191,9 @@ __global__ __launch_bounds__(kTraceBlock) void k_adj_trace(DevScene S, DevState*
         const F3 o = ld3(o4), d = ld3(d4), T = ld3(T4);
         const uint32_t path = f2u(o4.w);
         const uint32_t bounce = f2u(d4.w);
+        const uint32_t dpx = !tp.debug ? 0u
+                             : tp.segs ? tp.segs[seg].pixOffset + (path - tp.segs[seg].slotBase)
+                                       : tp.debugBase + path;
         Hit h;
         if (intersectScene<kCount>(S, o, d, stk, h, w)) {
             const DevInstance& I = S.inst[h.inst];
@@ -1226,9 +1229,9 @@ __global__ __launch_bounds__(kTraceBlock) void k_adj_trace(DevScene S, DevState*
                         ev.e1[i] = f4(on, u2f(h.prim));
                         ev.e2[i] = f4(Tp, h.alpha);
                         ev.e3[i] = make_float4(u2f(S.inst[p2.y].material), pdf, endCos, invS);
-                        if (tp.fullGradients) {  // this ray hit the endpoint surfel
+                        if (tp.fullGradients || tp.debug) {  // this ray hit the endpoint surfel
                             ev.e4[i] = f4(o, 0.0f);
-                            ev.e5[i] = f4(d, 0.0f);
+                            ev.e5[i] = f4(d, u2f(dpx));
                         }
                     } else {
                         ev.e1[i] = f4(on, u2f(p2.x));
@@ -1237,7 +1240,11 @@ __global__ __launch_bounds__(kTraceBlock) void k_adj_trace(DevScene S, DevState*
                                                endCos, invS);
                         if (tp.fullGradients) {  // the ray that crossed the pending surfel
                             ev.e4[i] = pend.p3[path];
-                            ev.e5[i] = pend.p4[path];
+                            float4 e5 = pend.p4[path];
+                            e5.w = u2f(dpx);
+                            ev.e5[i] = e5;
+                        } else if (tp.debug) {
+                            ev.e5[i] = make_float4(0.0f, 0.0f, 0.0f, u2f(dpx));
                         }
                     }
                     vf = 1;
@@ -1312,9 +1319,9 @@ __global__ __launch_bounds__(kTraceBlock) void k_adj_trace(DevScene S, DevState*
                     ev.e1[i] = f4(on, u2f(h.prim));
                     ev.e2[i] = f4(divs(T, tp.qReflect), h.alpha);
                     ev.e3[i] = make_float4(0.0f, 0.0f, 0.0f, invS);
-                    if (tp.fullGradients) {
+                    if (tp.fullGradients || tp.debug) {
                         ev.e4[i] = f4(o, 0.0f);
-                        ev.e5[i] = f4(d, 0.0f);
+                        ev.e5[i] = f4(d, u2f(dpx));
                     }
                     vf = 1;
                 }
@@ -1422,7 +1429,7 @@ __device__ __forceinline__ bool runReduce(uint32_t key, float (&v)[kN]) {
 template <bool kCount>
 __global__ __launch_bounds__(256) void k_adj_transport(DevScene S, DevGrid G, DevState* st, uint32_t poolCap,
                                                        EventArrays ev, const uint32_t* sorted, float* grad,
-                                                       float* full) {
+                                                       float* full, float* debugImg) {
     const uint32_t n = min(st->evPool, poolCap);
     if (blockIdx.x * blockDim.x >= n) return;  // the grid covers the pool's capacity
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1487,6 +1494,13 @@ __global__ __launch_bounds__(256) void k_adj_transport(DevScene S, DevGrid G, De
             if (full) addGeometryGradient(S, target, ev.e4[j], ev.e5[j], fsum(p * Lo) * invSpp, c + 1);
         }
         c[0] = val;
+        if (debugImg) {
+            float* px = debugImg + 4 * (size_t)f2u(ev.e5[j].w);
+            atomicAdd(px, val);
+            atomicAdd(px + 1, val);
+            atomicAdd(px + 2, val);
+            __hip_atomic_store(px + 3, 1.0f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
     }
     bool head;
     if (full) {
@@ -1868,14 +1882,15 @@ void launchEventSort(hipStream_t s, const GridParams& g, DevState* st, uint32_t 
     k_event_scatter<<<min(blocks(poolCap, 256), kEntryBlocks), 256, 0, s>>>(st, poolCap, cell, cellStart, cellCount, sorted);
 }
 void launchAdjTransport(hipStream_t s, const DevScene& S, const DevGrid& G, DevState* st, uint32_t poolCap,
-                        EventArrays ev, const uint32_t* sorted, float* gradOpacity, float* gradFull, bool count) {
+                        EventArrays ev, const uint32_t* sorted, float* gradOpacity, float* gradFull, float* debugImg,
+                        bool count) {
     if (!poolCap) return;
     if (count)
         k_adj_transport<true><<<blocks(poolCap, 256), 256, 0, s>>>(S, G, st, poolCap, ev, sorted, gradOpacity,
-                                                                  gradFull);
+                                                                  gradFull, debugImg);
     else
         k_adj_transport<false><<<blocks(poolCap, 256), 256, 0, s>>>(S, G, st, poolCap, ev, sorted,
-                                                                   gradOpacity, gradFull);
+                                                                   gradOpacity, gradFull, debugImg);
 }
 void launchSplitGradients(hipStream_t s, const float* gradFull, uint32_t n, float* position, float* tangentU,
                           float* tangentV, float* scale, float* albedo, float* beta) {

@@ -134,7 +134,7 @@ struct pale_handle {
     uint32_t adjCap = 0, poolCap = 0;
     DevMem pend0, pend1, pend2, pend3, pend4, evStg[6], evCmp[6], evCell, evSorted, evFlags, dloss, gradOpacity,
         gradFull;
-    bool adjFull = false;  // event rays allocated (full gradients)
+    bool adjFull = false;  // event rays allocated (full gradients or debug images)
     // Camera gather: pixel state, entries of one round, their cell sort.
     uint32_t pxCap = 0, entryCap = 0;
     DevMem viewFb, viewLdr;  // images of pale_render_view (pass by pass)
@@ -155,6 +155,11 @@ struct pale_handle {
     // img) is refreshed from it when host code next needs it.
     DevMem rawPts;
     bool hostStale = false;
+    // Debug gradient images of the last render_backward (debug_images): RGBA
+    // per pixel over its cameras in call order, and each camera's first pixel.
+    DevMem debugImg;
+    std::vector<size_t> debugOff;
+    std::vector<uint32_t> debugCams;
     void syncHostPoints() {
         if (!hostStale) return;
         const size_t n = bp.points.size();
@@ -249,16 +254,20 @@ struct pale_handle {
     static PhotonArrays pa(DevMem& a, DevMem& b, DevMem& c) {
         return PhotonArrays{a.as<float4>(), b.as<float4>(), c.as<float>()};
     }
+    // Events carry e4 / e5 (the ray of the surfel they weigh, and the image
+    // pixel in e5.w) with full gradients or debug images.
+    bool eventRays() const { return S.full_gradients != 0 || S.debug_images != 0; }
     EventArrays ea(DevMem* m) const {
-        const bool full = S.full_gradients != 0;
+        const bool rays = eventRays();
         return EventArrays{m[0].as<float4>(), m[1].as<float4>(), m[2].as<float4>(), m[3].as<float4>(),
-                           full ? m[4].as<float4>() : nullptr, full ? m[5].as<float4>() : nullptr};
+                           rays ? m[4].as<float4>() : nullptr, rays ? m[5].as<float4>() : nullptr};
     }
 
     TraceParams tp(float invSpp = 1.0f, const BatchSeg* segs = nullptr, const uint32_t* segFirst = nullptr,
                    uint32_t seed = 42u) const {
         return TraceParams{S.rr_start, S.q_null, S.q_reflect, S.q_transmit, S.full_gradients ? 1u : 0u,
-                           invSpp, segs, segFirst, nullptr, nullptr, nullptr, 0u, seed};
+                           invSpp, segs, segFirst, nullptr, nullptr, nullptr, 0u, seed,
+                           S.debug_images ? 1u : 0u, 0u};
     }
     // Base seed of adjoint sample k: Random::number, the reference's
     // constant 42 (AdjointKernels.cpp:27,94; setting adjoint_seed), or
@@ -449,7 +458,7 @@ struct pale_handle {
         sizeQueues(nextPow2(npx), a);
         a(entStart, ((size_t)grid.total + 1) * 4);
         uint32_t have = adjCap;
-        if (S.full_gradients && !adjFull) have = 0;  // also allocate the event rays
+        if (eventRays() && !adjFull) have = 0;  // also allocate the event rays
         if (npx <= have) return;
         a(pend0, (size_t)npx * sizeof(float4));
         a(pend1, (size_t)npx * sizeof(float4));
@@ -457,7 +466,7 @@ struct pale_handle {
         // Event pool: the events of up to 24 bounces of npx paths wait for
         // one cell-sorted transport launch.
         const uint32_t pool = (uint32_t)std::min<uint64_t>((uint64_t)npx * 24u, 1u << 28);
-        const int arrays = S.full_gradients ? 6 : 4;
+        const int arrays = eventRays() ? 6 : 4;
         for (int k = 0; k < arrays; ++k) {
             a(evStg[k], (size_t)npx * sizeof(float4));
             a(evCmp[k], (size_t)pool * sizeof(float4));
@@ -471,7 +480,7 @@ struct pale_handle {
         a(evFlags, npx);
         if (!A::kDry) {
             poolCap = pool;
-            if (S.full_gradients) adjFull = true;
+            if (eventRays()) adjFull = true;
             adjCap = npx;
         }
     }
@@ -1141,6 +1150,14 @@ struct pale_handle {
         HIPCHECK(hipMemcpyAsync(state.p, &h, sizeof(DevState), hipMemcpyHostToDevice, stream));
         HIPCHECK(hipStreamSynchronize(stream));
         dloss.reserve(std::max<size_t>(off.back(), 1) * sizeof(float));
+        debugOff.clear();
+        if (S.debug_images) {
+            // The reference clears its debug framebuffers per call
+            // (SyclBridge.cpp:183-205).
+            for (size_t o : off) debugOff.push_back(o / 3);
+            debugImg.reserve(std::max<size_t>(debugOff.back(), 1) * 4 * sizeof(float));
+            launchFillF(stream, debugImg.as<float>(), (uint32_t)(debugOff.back() * 4), 0.0f);
+        }
         for (size_t k = 0; k + 1 < off.size(); ++k)
             HIPCHECK(hipMemcpyAsync(dloss.as<float>() + off[k], src[k], (off[k + 1] - off[k]) * sizeof(float),
                                     srcOnDevice ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, stream));
@@ -1162,7 +1179,8 @@ struct pale_handle {
                         cellCount.as<uint32_t>(), entStart.as<uint32_t>(), gridTiles.as<uint32_t>(),
                         gridTotal.as<uint32_t>());
         launchAdjTransport(stream, scene, devGrid(), st, poolCap, ea(evCmp), evSorted.as<uint32_t>(),
-                           gradOpacity.as<float>(), S.full_gradients ? gradFull.as<float>() : nullptr, countWork);
+                           gradOpacity.as<float>(), S.full_gradients ? gradFull.as<float>() : nullptr,
+                           S.debug_images ? debugImg.as<float>() : nullptr, countWork);
         tEnd(tt);
         launchFill(stream, stField(offsetof(DevState, evPool)), 1, 0u);
     }
@@ -1330,6 +1348,7 @@ struct pale_handle {
 
     void renderBackward(const std::vector<uint32_t>& camIdx, const std::vector<const float*>& src,
                         bool srcOnDevice = false) {
+        debugCams = camIdx;
         if (S.batch_passes && S.adjoint_passes > 0) {
             std::vector<HostCamera> vc;
             for (uint32_t c : camIdx) vc.push_back(cams[c]);
@@ -1376,9 +1395,10 @@ struct pale_handle {
                 for (uint32_t b = 0; b < S.adjoint_bounces; ++b) {
                     if (pooledBound + npx > poolCap) flush();
                     const size_t ta = tBegin(kTAdjTrace);
+                    TraceParams tpc = tp(invSpp, nullptr, nullptr, adjointSeed(spp));
+                    tpc.debugBase = (uint32_t)(off[k] / 3);
                     launchAdjTrace(stream, scene, st, npx, b, rq(in), rq(stg), extFlags.as<uint8_t>(), ea(evStg),
-                                   evFlags.as<uint8_t>(), pend, tp(invSpp, nullptr, nullptr, adjointSeed(spp)),
-                                   countWork);
+                                   evFlags.as<uint8_t>(), pend, tpc, countWork);
                     tEnd(ta);
                     launchCompactEvents(stream, evFlags.as<uint8_t>(), qCount, npx, tiles.as<uint32_t>(), st,
                                         ea(evStg), ea(evCmp), poolCap);
@@ -1683,6 +1703,24 @@ int pale_render_backward(pale_handle* h, uint32_t n, const char* const* camera_n
                 }
             }
         }
+    });
+}
+
+int pale_get_debug_image(const pale_handle* h, const char* camera_name, float* rgba_out) {
+    if (!h || !camera_name || !rgba_out) return fail(PALE_ERR_INVALID_ARG, "null argument");
+    return guarded([&] {
+        if (!h->S.debug_images) throw StatusError(PALE_ERR_INVALID_ARG, "debug images: setting debug_images is off");
+        for (size_t k = 0; k < h->debugCams.size() && k + 1 < h->debugOff.size(); ++k) {
+            const HostCamera& c = h->cams[h->debugCams[k]];
+            if (c.name != camera_name) continue;
+            const size_t npx = h->debugOff[k + 1] - h->debugOff[k];
+            HIPCHECK(hipMemcpyAsync(rgba_out, h->debugImg.as<float>() + 4 * h->debugOff[k], npx * 4 * sizeof(float),
+                                    hipMemcpyDeviceToHost, h->stream));
+            HIPCHECK(hipStreamSynchronize(h->stream));
+            return;
+        }
+        throw StatusError(PALE_ERR_INVALID_ARG,
+                          std::string("debug images: camera '") + camera_name + "' was not in the last render_backward");
     });
 }
 

@@ -15,7 +15,11 @@ Compatibility extensions (supersets of the reference):
     size) and (H,W,4) images (alpha ignored);
   * the gradient dict also carries "color" (alias of "albedo");
   * render_forward returns a dict whose ``np.asarray`` is the first camera's
-    LDR RGB image.
+    LDR RGB image;
+  * settings key "ldr_channels" (default 4, the reference's (H,W,4) LDR):
+    3 makes render_forward's LDR images (H,W,3), the shape training.py
+    compares with its (H,W,3) targets (io_utils.read_rgb_exr); the "_raw"
+    HDR images stay (H,W,4) either way.
 """
 from __future__ import annotations
 
@@ -30,6 +34,9 @@ from . import _abi
 from ._lib import check, load
 
 _fp = C.POINTER(C.c_float)
+# Keys of images["debug"][camera] in the reference's order
+# (pale_bindings.cpp:646-715).
+DEBUG_IMAGE_KEYS = ("position_x", "position_y", "position_z", "rotation", "scale", "albedo", "opacity", "beta")
 
 
 def _p(a: Optional[np.ndarray]):
@@ -37,7 +44,8 @@ def _p(a: Optional[np.ndarray]):
 
 
 class ForwardImages(dict):
-    """render_forward result: {name: (H,W,4) LDR, name + "_raw": (H,W,4) HDR}."""
+    """render_forward result: {name: (H,W,4) LDR (or (H,W,3) with
+    ldr_channels=3), name + "_raw": (H,W,4) HDR}."""
 
     def __array__(self, dtype=None, copy=None):
         for k, v in self.items():
@@ -75,6 +83,10 @@ class Renderer:
     def _init(self, meshes, cameras, points, settings):
         self._L = load()
         self._settings_dict = dict(settings)
+        ldr_channels = int(settings.get("ldr_channels", 4))
+        if ldr_channels not in (3, 4):
+            raise RuntimeError(f"ldr_channels must be 3 or 4, got {ldr_channels}")
+        self._ldr_channels = ldr_channels
         holder = _abi.SceneHolder(list(meshes), list(cameras), points)
         self._settings = _abi.make_settings(settings)
         h = C.c_void_p()
@@ -120,7 +132,7 @@ class Renderer:
         check(self._L.pale_render_forward(self._h, camera_name.encode(), lp, rp, C.byref(n)))
         out = ForwardImages()
         for c, a, b in zip(cams, ldr, raw):
-            out[c[0]] = a
+            out[c[0]] = a if self._ldr_channels == 4 else np.ascontiguousarray(a[..., :3])
             out[c[0] + "_raw"] = b
         return out
 
@@ -158,6 +170,17 @@ class Renderer:
         check(self._L.pale_render_backward(self._h, len(names), cn, ap, gp, sp))
         grads["color"] = grads["albedo"]
         images = {"adjoint_source": {nm: s for nm, s in zip(names, srcs)}}
+        if self._settings.debug_images:
+            # images["debug"] (pale_bindings.cpp:627-726): per camera the eight
+            # gradient images; the reference's kernels write only "opacity".
+            debug = {}
+            for nm, a in zip(names, arrs):
+                per = {}
+                for key in DEBUG_IMAGE_KEYS:
+                    per[key] = np.zeros((a.shape[0], a.shape[1], 4), np.float32)
+                check(self._L.pale_get_debug_image(self._h, nm.encode(), _p(per["opacity"])))
+                debug[nm] = per
+            images["debug"] = debug
         return grads, images
 
     # ------------------------------------------------ device (torch) tensors --
@@ -181,7 +204,7 @@ class Renderer:
         check(self._L.pale_render_forward_device(self._h, camera_name.encode(), lp, rp, C.byref(n)))
         out = {}
         for c, a, b in zip(cams, ldr, raw):
-            out[c[0]] = a
+            out[c[0]] = a if self._ldr_channels == 4 else a[..., :3].contiguous()
             out[c[0] + "_raw"] = b
         return out
 

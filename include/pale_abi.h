@@ -276,6 +276,18 @@ int pale_set_point_beta(pale_handle* h, float beta, int32_t index);
  * inputs after the caller's own work on them has finished (the caller
  * synchronises its stream first). ---- */
 
+/* images["debug"][camera]["opacity"] of render_backward when the setting
+ * debug_images is on (pale_bindings.cpp:627-726; written by
+ * launchAdjointTransportKernel, AdjointKernels.cpp:434-505): per pixel the
+ * sum of the opacity-gradient contributions of its adjoint events in RGB,
+ * alpha 1 where any event landed (atomicAddFloat4ToImage,
+ * KernelHelpers.h:1489-1502), from the last pale_render_backward.
+ * rgba_out: caller-allocated H*W*4 floats.  The reference's other seven
+ * debug images (position_x/y/z, rotation, scale, albedo, beta) are never
+ * written by its kernels and stay zero. */
+int pale_get_debug_image(const pale_handle* h, const char* camera_name,
+                         float* rgba_out);
+
 /* render_forward with device outputs: H*W*4 floats per rendered camera. */
 int pale_render_forward_device(pale_handle* h, const char* camera_name,
                                float* const* ldr_dev, float* const* raw_dev,

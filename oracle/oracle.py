@@ -61,6 +61,7 @@ def lib():
         L.orc_rebuild_bvh.argtypes = [C.c_void_p]
         L.orc_set_sample.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32]
         L.orc_full_gradients.argtypes = [C.c_void_p, _fp]
+        L.orc_get_debug_image.argtypes = [C.c_void_p, C.c_char_p, _fp]
         L.orc_freeze_map.argtypes = [C.c_void_p, C.c_int32]
         L.orc_build_photon_map.argtypes = [C.c_void_p]
         L.orc_render_view.argtypes = [C.c_void_p, C.POINTER(_abi.CameraDesc), C.c_uint32, C.c_uint32, _fp, _fp]
@@ -203,6 +204,14 @@ class Oracle:
             raise RuntimeError("full gradients were not computed (full_gradients=0)")
         return dict(position=g[:, 0:3].copy(), tangent_u=g[:, 3:6].copy(), tangent_v=g[:, 6:9].copy(),
                     scale=g[:, 9:11].copy(), albedo=g[:, 11:14].copy(), beta=g[:, 15].copy())
+
+    def debug_image(self, camera: str, width: int, height: int) -> np.ndarray:
+        """Debug opacity image (H,W,4) of `camera` from the last render_backward
+        with debug_images=1."""
+        out = np.zeros((height, width, 4), np.float32)
+        if lib().orc_get_debug_image(self.h, camera.encode(), _p(out)) != 0:
+            raise RuntimeError(f"no debug image of camera '{camera}' (debug_images=0 or not in the last backward)")
+        return out
 
     def photons(self):
         cnt = C.c_uint32(0)

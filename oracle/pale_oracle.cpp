@@ -250,6 +250,7 @@ struct Settings {
     bool fullGradients;
     bool adjointDecorrelate;
     uint32_t adjointSeed;
+    bool debugImages;
 };
 
 struct Grid {
@@ -304,6 +305,9 @@ struct Renderer {
     bool gridConfigured = false;
     // Gradients.
     std::vector<float> gradOpacity;
+    std::vector<float> debugImg;  // debug opacity images of the last backward (RGBA per pixel)
+    std::vector<size_t> debugOff;
+    std::vector<std::string> debugNames;
     // Full gradients (this build's extension; the reference returns zeros):
     // per surfel position 3, tangent_u 3, tangent_v 3, scale 2, albedo 3,
     // beta 1.
@@ -1501,6 +1505,7 @@ struct Completed {
     int endType = -1;
     float endAlpha = 0.0f, endPdf = 0.0f, endCos = 0.0f;
     F3 endPos{0, 0, 0}, endNormal{0, 0, 0};
+    uint32_t pixel = 0;  // image pixel of the path (debug images)
 };
 struct ARay {
     Ray ray;
@@ -1628,14 +1633,18 @@ static void addGeometryGradient(Renderer& R, uint32_t prim, const EventPre& pre,
 // launchAdjointTransportKernel (AdjointKernels.cpp:389-509), one event.
 // E is the irradiance gathered at the event's shading point (the surfel hit
 // for projection events, the endpoint otherwise).
-static void transport(Renderer& R, const Completed& c, float invSpp, const EventPre& pre) {
+// Returns the event's opacity-gradient contribution (what the reference also
+// adds to the debug opacity image, AdjointKernels.cpp:434-505).
+static float transport(Renderer& R, const Completed& c, float invSpp, const EventPre& pre) {
     const F3 E = pre.E;
+    float val = 0.0f;
     if (c.kind == kProjection) {
         const Point& s = R.points[c.prim];
         const F3 fr = (s.alpha_r * s.albedo) * kInvPi;
         const F3 Lr = fr * E;
         const F3 g = (c.alpha * c.T) * Lr;
-        R.gradOpacity[c.prim] += fsum(g) * invSpp;
+        val = fsum(g) * invSpp;
+        R.gradOpacity[c.prim] += val;
         if (R.S.fullGradients) {
             addGeometryGradient(R, c.prim, pre, fsum(c.T * Lr) * invSpp);
             // d/d(albedo_c) of alpha * eta * (alpha_r * albedo_c / pi) * E_c
@@ -1666,7 +1675,8 @@ static void transport(Renderer& R, const Completed& c, float invSpp, const Event
         }
         const F3 Lo = Le + Lr;
         const F3 g = ((-c.alpha) * c.T) * Lo;
-        R.gradOpacity[c.prim] += fsum(g) * invSpp;
+        val = fsum(g) * invSpp;
+        R.gradOpacity[c.prim] += val;
         if (R.S.fullGradients) addGeometryGradient(R, c.prim, pre, -(fsum(c.T * Lo) * invSpp));
     }
     if (c.kind == kReflectS) {
@@ -1674,9 +1684,11 @@ static void transport(Renderer& R, const Completed& c, float invSpp, const Event
         const F3 Lo = (E * m.base) * kInvPi;
         const F3 p = divs(c.T, c.endPdf);
         const F3 g = (p * Lo) * c.endAlpha;
-        R.gradOpacity[c.endPrim] += fsum(g) * invSpp;
+        val = fsum(g) * invSpp;
+        R.gradOpacity[c.endPrim] += val;
         if (R.S.fullGradients) addGeometryGradient(R, c.endPrim, pre, fsum(p * Lo) * invSpp);
     }
+    return val;
 }
 
 // submitAdjointKernel (SyclBridge.cpp:169-294) for the given cameras.  The
@@ -1691,6 +1703,15 @@ static void renderBackward(Renderer& R, const std::vector<const Camera*>& camLis
     workClass(2);
     R.gradOpacity.assign(R.points.size(), 0.0f);
     R.gradFull.assign(S.fullGradients ? R.points.size() * 16 : 0, 0.0f);
+    // Debug opacity images of these cameras, cleared per call
+    // (SyclBridge.cpp:183-205).
+    R.debugOff.assign(1, 0);
+    R.debugNames.clear();
+    for (const Camera* c : camList) {
+        R.debugOff.push_back(R.debugOff.back() + (size_t)c->W * c->H);
+        R.debugNames.push_back(c->name);
+    }
+    R.debugImg.assign(S.debugImages ? R.debugOff.back() * 4 : 0, 0.0f);
     const float invSpp = 1.0f / (float)passes;
     for (size_t k = 0; k < camList.size(); ++k) {
         const Camera& c = *camList[k];
@@ -1769,6 +1790,7 @@ static void renderBackward(Renderer& R, const std::vector<const Camera*>& camLis
                             // this ray, which hit the endpoint surfel.
                             e.ro = nullDone ? p.ro : rs.ray.o;
                             e.rd = nullDone ? p.rd : rs.ray.d;
+                            e.pixel = rs.pixel;
                             done.push_back(e);
                             pend[rs.pathId] = Pending{};
                         }
@@ -1841,6 +1863,7 @@ static void renderBackward(Renderer& R, const std::vector<const Camera*>& camLis
                                 e.T = divs(rs.T, S.qReflect);
                                 e.ro = rs.ray.o;
                                 e.rd = rs.ray.d;
+                                e.pixel = rs.pixel;
                                 done.push_back(e);
                             }
                             nq.push_back(nx);
@@ -1876,7 +1899,17 @@ static void renderBackward(Renderer& R, const std::vector<const Camera*>& camLis
                     if (S.fullGradients)
                         geometryGradient(R, e.kind == kReflectS ? e.endPrim : e.prim, e.ro, e.rd, pre[i]);
                 });
-                for (size_t i = 0; i < done.size(); ++i) transport(R, done[i], invSpp, pre[i]);
+                for (size_t i = 0; i < done.size(); ++i) {
+                    const float v = transport(R, done[i], invSpp, pre[i]);
+                    if (S.debugImages) {
+                        // atomicAddFloat4ToImage (KernelHelpers.h:1489-1502)
+                        float* px = &R.debugImg[4 * (R.debugOff[k] + done[i].pixel)];
+                        px[0] += v;
+                        px[1] += v;
+                        px[2] += v;
+                        px[3] = 1.0f;
+                    }
+                }
                 q.swap(nq);
             }
         }
@@ -1903,6 +1936,7 @@ static Settings toSettings(const pale_settings& s) {
     S.fullGradients = s.full_gradients != 0;
     S.adjointDecorrelate = s.adjoint_decorrelate != 0;
     S.adjointSeed = s.adjoint_seed;
+    S.debugImages = s.debug_images != 0;
     return S;
 }
 
@@ -2170,6 +2204,17 @@ int orc_render_view_backward(void* h, const pale_camera_desc* cd, uint32_t spp, 
 
 // Full gradients of the last backward pass, 16 floats per surfel: position
 // (3), tangent_u (3), tangent_v (3), scale (2), albedo (3), unused, beta.
+int orc_get_debug_image(void* h, const char* name, float* out) {
+    const Renderer& R = *static_cast<Renderer*>(h);
+    if (!R.S.debugImages) return -1;
+    for (size_t k = 0; k < R.debugNames.size(); ++k) {
+        if (R.debugNames[k] != name) continue;
+        std::memcpy(out, R.debugImg.data() + 4 * R.debugOff[k], (R.debugOff[k + 1] - R.debugOff[k]) * 4 * sizeof(float));
+        return 0;
+    }
+    return -1;
+}
+
 int orc_full_gradients(void* h, float* out) {
     try {
         Renderer& R = *static_cast<Renderer*>(h);

@@ -1,0 +1,265 @@
+"""The pale.Renderer boundary as the reference's drivers use it (GPU).
+
+Point edits (add_points, remove_points, set_point_properties with an index
+and with index < 0, set_point_opacity, set_point_beta) are checked against
+the oracle built from the point set the reference's binding would hold after
+the edit (pale_bindings.cpp:1081-1587, restated here with numpy float32 in
+the binding's operation order): images bit for bit and the same BVH.  The
+path-based constructor, debug gradient images, ldr_channels=3 and a
+training.py-shaped step (training.py:621-750: forward, L2 gradient of the
+LDR image, backward with a dict, optimizer step, apply_point_optimization,
+rebuild_bvh, refetch) run on the GPU beside the oracle.  The reference's
+Python files do not run here: their call shapes are reproduced.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from differentiablepointrendering_amd import scene
+
+pytestmark = pytest.mark.gpu
+
+SMALL = dict(photons=20_000, gather_passes=2, adjoint_passes=2)
+ROOT = os.path.dirname(os.path.abspath(__file__))
+
+
+def _gpu(spec, settings):
+    from differentiablepointrendering_amd.pale import Renderer
+
+    print(f"case: {spec.name} n={len(spec.points['position'])} settings={settings}", flush=True)
+    return Renderer.from_spec(spec, settings)
+
+
+def _oracle(spec, points, settings):
+    from oracle.oracle import Oracle
+
+    return Oracle(spec.meshes, spec.cameras, points, dict(spec.settings, **settings))
+
+
+def _same_render(gpu, orc):
+    a, b = gpu.render_forward(), orc.render_forward()
+    for k in b:
+        assert np.array_equal(a[k].view(np.uint32), b[k].view(np.uint32)), k
+    ba, bb = gpu.surfel_bvh(), orc.surfel_bvh()
+    for k in ("count", "left_first", "permutation"):
+        assert np.array_equal(ba[k], bb[k]), k
+
+
+def _raw(points):
+    return {k: np.ascontiguousarray(v, dtype=np.float32).copy() for k, v in points.items()}
+
+
+def _rotate(q, v):
+    """glm's quat * vec3 (v + ((u x v) w + u x (u x v)) 2, u = q.xyz) in
+    float32 with the operation order of the binding."""
+    f = np.float32
+    u = np.broadcast_to(np.asarray(q[:3], f), v.shape)
+
+    def cross(a, b):
+        return np.stack([a[..., 1] * b[..., 2] - a[..., 2] * b[..., 1],
+                         a[..., 2] * b[..., 0] - a[..., 0] * b[..., 2],
+                         a[..., 0] * b[..., 1] - a[..., 1] * b[..., 0]], -1).astype(f)
+
+    uv = cross(u, v)
+    uuv = cross(u, uv)
+    return (v + ((uv * f(q[3])) + uuv) * f(2.0)).astype(f)
+
+
+def test_point_edits_match_oracle():
+    spec = scene.config_scene(0, n=2048, width=48, height=48)
+    gpu = _gpu(spec, SMALL)
+    pts = _raw(spec.points)
+    assert gpu.get_camera_names() == ["camera1"]
+    # add_points: appended to the asset with shape 0 (pale_bindings.cpp:1221-1412)
+    rng = np.random.default_rng(5)
+    new, _ = scene.uniform_surfels(64, seed=9)
+    gpu.add_points({"new": {k: v for k, v in new.items() if k != "shape"}})
+    pts = {k: np.concatenate([pts[k], new[k] if k != "shape" else np.zeros(64, np.float32)]) for k in pts}
+    _same_render(gpu, _oracle(spec, pts, SMALL))
+    assert gpu.point_count == 2048 + 64
+    # remove_points: indices into the asset, int32 or int64
+    drop = rng.choice(2048 + 64, size=100, replace=False)
+    gpu.remove_points({"indices": drop.astype(np.int32)})
+    keep = np.setdiff1d(np.arange(2048 + 64), drop)
+    pts = {k: v[keep] for k, v in pts.items()}
+    _same_render(gpu, _oracle(spec, pts, SMALL))
+    # set_point_opacity / set_point_beta (asset index, then rebuild_bvh)
+    gpu.set_point_opacity(0.25, 7)
+    gpu.set_point_beta(-1.5, 11)
+    pts["opacity"][7] = np.float32(0.25)
+    pts["beta"][11] = np.float32(-1.5)
+    _same_render(gpu, _oracle(spec, pts, SMALL))
+    # set_point_properties with an index: translation added, tangents
+    # rotated, albedo / opacity / beta / scale replaced
+    t, q, s, c = (0.01, -0.02, 0.03), (0.0, 0.0, 0.38268343, 0.92387953), (0.04, 0.03, 1.0), (0.3, 0.6, 0.9)
+    gpu.set_point_properties(t, q, s, c, 0.8, 0.25, index=3)
+    f = np.float32
+    pts["position"][3] = (pts["position"][3] + np.asarray(t, f)).astype(f)
+    pts["tangent_u"][3] = _rotate(q, pts["tangent_u"][3])
+    pts["tangent_v"][3] = _rotate(q, pts["tangent_v"][3])
+    pts["albedo"][3] = np.asarray(c, f)
+    pts["opacity"][3] = f(0.8)
+    pts["beta"][3] = f(0.25)
+    pts["scale"][3] = np.asarray(s[:2], f)
+    _same_render(gpu, _oracle(spec, pts, SMALL))
+    # index < 0: the binding's loop advances twice per iteration
+    # (pale_bindings.cpp:1544-1561), so points 0, 2, 4, ... change, by
+    # deltas (added translation, albedo, opacity, beta; scaled scale)
+    t, q, s, c = (0.0, 0.01, 0.0), (0.0, 0.08715574, 0.0, 0.9961947), (1.1, 0.9, 1.0), (0.01, 0.02, 0.03)
+    gpu.set_point_properties(t, q, s, c, 0.05, -0.1, index=-1)
+    e = np.arange(0, len(pts["opacity"]), 2)
+    pts["position"][e] = (pts["position"][e] + np.asarray(t, f)).astype(f)
+    pts["tangent_u"][e] = _rotate(q, pts["tangent_u"][e])
+    pts["tangent_v"][e] = _rotate(q, pts["tangent_v"][e])
+    pts["albedo"][e] = (pts["albedo"][e] + np.asarray(c, f)).astype(f)
+    pts["opacity"][e] = (pts["opacity"][e] + f(0.05)).astype(f)
+    pts["beta"][e] = (pts["beta"][e] + f(-0.1)).astype(f)
+    pts["scale"][e] = (pts["scale"][e] * np.asarray(s[:2], f)).astype(f)
+    _same_render(gpu, _oracle(spec, pts, SMALL))
+    with pytest.raises(RuntimeError):
+        gpu.set_point_opacity(0.5, 10**6)
+    with pytest.raises(RuntimeError):
+        gpu.remove_points({"indices": np.zeros((2, 2), np.int64)})
+    gpu.close()
+
+
+def test_path_constructor_matches_oracle():
+    from differentiablepointrendering_amd import assets
+    from differentiablepointrendering_amd.pale import Renderer
+    from oracle.oracle import Oracle
+
+    root = os.path.join(ROOT, "golden", "reference_scenes")
+    xml = os.path.join("GradientTests", "simple_1", "simple_1.xml")
+    ply = os.path.join(root, "GradientTests", "simple_1", "initial.ply")
+    settings = dict(photons=20_000, bounces=4, forward_passes=2, gather_passes=2, adjoint_passes=2, seed=42)
+    print("case: path constructor, reference scene simple_1", flush=True)
+    gpu = Renderer(root, xml, ply, settings)
+    meshes, cams, pts = assets.load_scene(root, xml, ply)
+    orc = Oracle(meshes, cams, pts, settings)
+    names = gpu.get_camera_names()
+    assert names and names == [c.name for c in cams]
+    a, b = gpu.render_forward(), orc.render_forward()
+    for k in b:
+        assert np.array_equal(a[k].view(np.uint32), b[k].view(np.uint32)), k
+    cam = names[0]
+    raw = b[cam + "_raw"][..., :3]
+    dl = (raw / raw.size).astype(np.float32)
+    g, imgs = gpu.render_backward({cam: dl})
+    go = orc.render_backward({cam: dl})
+    scale = max(float(np.abs(go).max()), 1e-30)
+    np.testing.assert_allclose(g["opacity"], go, rtol=1e-4, atol=1e-4 * scale)
+    assert imgs["adjoint_source"][cam].shape == raw.shape[:2] + (4,)
+    gpu.close()
+    orc.close()
+
+
+def test_debug_images_match_oracle():
+    from differentiablepointrendering_amd.pale import DEBUG_IMAGE_KEYS
+
+    spec = scene.config_scene(0, n=4096, width=48, height=40)
+    st = dict(SMALL, debug_images=1)
+    gpu = _gpu(spec, st)
+    orc = _oracle(spec, spec.points, st)
+    raw = orc.render_forward()["camera1_raw"]
+    gpu.render_forward()
+    dl = (raw[..., :3] / raw[..., :3].size).astype(np.float32)
+    g, imgs = gpu.render_backward({"camera1": dl})
+    go = orc.render_backward({"camera1": dl})
+    dbg = imgs["debug"]["camera1"]
+    assert tuple(dbg) == DEBUG_IMAGE_KEYS
+    for k in DEBUG_IMAGE_KEYS:
+        assert dbg[k].shape == (40, 48, 4), k
+        if k != "opacity":
+            assert not np.any(dbg[k]), k
+    want = orc.debug_image("camera1", 48, 40)
+    m = max(float(np.abs(want).max()), 1e-30)
+    assert (want[..., 3] == 1).sum() > 20
+    assert np.array_equal(dbg["opacity"][..., 3], want[..., 3])
+    np.testing.assert_allclose(dbg["opacity"], want, rtol=1e-4, atol=1e-4 * m)
+    scale = max(float(np.abs(go).max()), 1e-30)
+    np.testing.assert_allclose(g["opacity"], go, rtol=1e-4, atol=1e-4 * scale)
+    # without the setting there is no "debug" entry, as in the reference
+    plain = _gpu(spec, SMALL)
+    plain.render_forward()
+    _, imgs2 = plain.render_backward({"camera1": dl})
+    assert "debug" not in imgs2
+    gpu.close()
+    plain.close()
+    orc.close()
+
+
+def test_ldr_channels_three():
+    spec = scene.config_scene(0, n=2048, width=40, height=32)
+    a = _gpu(spec, SMALL)
+    b = _gpu(spec, dict(SMALL, ldr_channels=3))
+    ia, ib = a.render_forward(), b.render_forward()
+    assert ia["camera1"].shape == (32, 40, 4) and ib["camera1"].shape == (32, 40, 3)
+    assert ib["camera1_raw"].shape == (32, 40, 4)
+    assert np.array_equal(ib["camera1"], ia["camera1"][..., :3])
+    assert np.array_equal(ib["camera1_raw"], ia["camera1_raw"])
+    assert np.array_equal(np.asarray(ib), ia["camera1"][..., :3])
+    a.close()
+    b.close()
+
+
+def test_training_step_replay():
+    """training.py:621-750 with ldr_channels=3: the LDR image against an
+    (H,W,3) target, compute_l2_grad (losses.py:26-45), render_backward with a
+    dict, an Adam step on all seven parameters (numpy here, torch there),
+    apply_point_optimization, rebuild_bvh and the parameters fetched again.
+    The oracle takes the same updated parameters: images, BVH and the
+    refetched parameters stay equal over three iterations."""
+    from oracle.oracle import Oracle
+
+    spec = scene.config_scene(0, n=2048, width=48, height=48)
+    st = dict(SMALL, full_gradients=1, ldr_channels=3)
+    gpu = _gpu(spec, st)
+    orc = Oracle.from_spec(spec, st)
+    tgt = _oracle(spec, spec.target_points, st)
+    target = tgt.render_forward()["camera1"][..., :3].copy()
+    tgt.close()
+    keys = ("position", "tangent_u", "tangent_v", "scale", "albedo", "opacity", "beta")
+    params = gpu.get_point_parameters()
+    m = {k: np.zeros_like(params[k]) for k in keys}
+    v = {k: np.zeros_like(params[k]) for k in keys}
+    losses = []
+    for it in range(1, 4):
+        img = gpu.render_forward()
+        oimg = orc.render_forward()
+        cur = np.asarray(img["camera1"], dtype=np.float32, order="C")
+        assert cur.shape == target.shape
+        assert np.array_equal(cur.view(np.uint32), oimg["camera1"][..., :3].view(np.uint32))
+        diff = cur - target
+        losses.append(0.5 * float(np.mean(diff * diff)))
+        grad = (diff / diff.size).astype(np.float32)
+        grads, images = gpu.render_backward({"camera1": grad})
+        assert images["adjoint_source"]["camera1"].shape == (48, 48, 4)
+        for k in keys:
+            assert grads[k].shape == params[k].shape, k
+        lr = 5e-3
+        for k in keys:
+            g = grads[k].astype(np.float32)
+            m[k] = 0.9 * m[k] + 0.1 * g
+            v[k] = 0.999 * v[k] + 0.001 * g * g
+            mh, vh = m[k] / (1 - 0.9 ** it), v[k] / (1 - 0.999 ** it)
+            params[k] = (params[k] - lr * mh / (np.sqrt(vh) + 1e-8)).astype(np.float32)
+        params["opacity"] = np.clip(params["opacity"], 0.0, 1.0).astype(np.float32)
+        upd = {k: params[k] for k in keys}
+        gpu.apply_point_optimization(upd)
+        orc.apply_points(upd)
+        gpu.rebuild_bvh()
+        orc.rebuild_bvh()
+        params = gpu.get_point_parameters()
+        po = orc.get_points()
+        for k in keys:
+            assert np.array_equal(params[k], po[k]), (it, k)
+        ba, bb = gpu.surfel_bvh(), orc.surfel_bvh()
+        assert np.array_equal(ba["permutation"], bb["permutation"]), it
+        # the optimizer state follows the new BVH order, as a fresh optimizer
+        # over the refetched tensors would start (training.py:741-750)
+        m = {k: np.zeros_like(params[k]) for k in keys}
+        v = {k: np.zeros_like(params[k]) for k in keys}
+    assert np.all(np.isfinite(losses)) and losses[0] > 0, losses
+    gpu.close()
+    orc.close()

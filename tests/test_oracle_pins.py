@@ -459,3 +459,34 @@ def test_adjoint_seed_schedule():
     np.testing.assert_allclose(two, 0.5 * (one_a + one_b), rtol=1e-5, atol=1e-6 * scale)
     rep = grad(adjoint_passes=2, adjoint_seed=s)
     np.testing.assert_allclose(rep, one_a, rtol=1e-5, atol=1e-6 * scale)
+
+
+# --------------------------------------------------------- debug images --
+def test_debug_opacity_image_sums_to_the_opacity_gradient():
+    """debug_images: every adjoint event adds its opacity-gradient
+    contribution to its pixel's R, G and B (atomicAddFloat4ToImage,
+    KernelHelpers.h:1489-1502) and to its surfel's gradient, so the image
+    sums to the gradient's sum; alpha is 1 exactly where events landed, and
+    the setting changes no gradient."""
+    spec = scene.config_scene(0, n=4096, width=32, height=32)
+    st = dict(photons=20_000, gather_passes=1, adjoint_passes=2, adjoint_bounces=3)
+
+    def run(debug):
+        o = Oracle.from_spec(spec, dict(st, debug_images=debug))
+        raw = o.render_forward()["camera1_raw"]
+        dl = (raw[..., :3] / raw[..., :3].size).astype(np.float32)
+        g = o.render_backward({"camera1": dl})
+        img = o.debug_image("camera1", 32, 32) if debug else None
+        o.close()
+        return g, img
+
+    g0, _ = run(0)
+    g, img = run(1)
+    assert np.array_equal(g, g0)
+    assert np.abs(g).sum() > 0
+    assert np.array_equal(img[..., 0], img[..., 1]) and np.array_equal(img[..., 0], img[..., 2])
+    assert set(np.unique(img[..., 3])) <= {0.0, 1.0}
+    assert np.all(img[img[..., 3] == 0.0][:, :3] == 0.0)
+    assert (img[..., 3] == 1.0).sum() > 10
+    np.testing.assert_allclose(img[..., 0].astype(np.float64).sum(), g.astype(np.float64).sum(), rtol=1e-5,
+                               atol=1e-6 * np.abs(g).sum())
