@@ -26,7 +26,7 @@ repeated; this form is 16 independent samples per pixel.
 
 Also reported: the roofline of the dominant kernel by device time, priced
 against HBM from its counted traffic (calibrated rocprofv3 FETCH_SIZE +
-WRITE_SIZE of the same workload, profiles/traffic_<form>.json) over the
+WRITE_SIZE of the same workload, summarised in profiles/) over the
 launch time measured here, with what the counters say bounds it; the
 reference's algorithmic byte model per work unit (counted in a separate,
 untimed counting step) beside it as information; the same figures for the
@@ -86,14 +86,26 @@ def parse():
     ap.add_argument("--decorrelated", type=int, default=3,
                     help="timed steps of the independent-samples form (every gather pass and adjoint sample "
                          "seeded on its own; 0: skip)")
-    ap.add_argument("--served", type=int, default=40, help="requests of the served stream (0: skip)")
-    ap.add_argument("--served-rate", type=float, default=1.0, help="Poisson arrival rate, requests per second")
+    ap.add_argument("--served", type=int, default=400, help="requests of the served stream (0: skip)")
+    ap.add_argument("--served-rate", type=float, default=2.0, help="Poisson arrival rate, requests per second")
+    ap.add_argument("--served-alone", type=int, default=40,
+                    help="head of the stream also run one request at a time and all at once (same results check)")
     ap.add_argument("--served-replicas", type=int, default=2)
+    ap.add_argument("--served-batch-slots", type=int, default=0,
+                    help="pixel slots per layer-walk launch of the served replicas (0: the renderer's default)")
+    ap.add_argument("--served-grid-fraction", type=float, default=0.0,
+                    help="fraction of the GPU a served replica's layer walk occupies (0: all)")
+    ap.add_argument("--hw-queues", type=int, default=0,
+                    help="GPU_MAX_HW_QUEUES for this process (0: leave the environment's; the served form's "
+                         "replicas each own a stream)")
     ap.add_argument("--served-only", action="store_true", help="run only the served form")
     ap.add_argument("--batched", type=int, default=32, help="views of the batched form (0: skip)")
     ap.add_argument("--batched-only", action="store_true", help="run only the batched form")
     ap.add_argument("--differentiation", type=int, default=2, help="autograd steps of the differentiation form (0: skip)")
     ap.add_argument("--differentiation-only", action="store_true", help="run only the differentiation form")
+    ap.add_argument("--training", type=int, default=2,
+                    help="timed iterations of the training-loop form (render, backward, optimizer step, apply, "
+                         "rebuild_bvh, refetch; 0: skip)")
     ap.add_argument("--no-tune", action="store_true", help="batched form with the default tuning")
     ap.add_argument("--batch-passes", type=int, choices=(0, 1), default=1,
                     help="plain form: 1 = all gather passes / adjoint samples of the view as batches")
@@ -105,7 +117,7 @@ def parse():
                          "that scales rocprof FETCH_SIZE / WRITE_SIZE to bytes")
     ap.add_argument("--adjoint-decorrelate", action="store_true",
                     help="plain form: adjoint sample k seeds from 42 + k (with --decorrelate: every sample independent)")
-    ap.add_argument("--form", choices=("plain", "independent", "batched", "served"), default=None,
+    ap.add_argument("--form", choices=("plain", "independent", "batched", "served", "training"), default=None,
                     help="run only this form (plain: the headline step; independent: the same step with every "
                          "gather pass and adjoint sample independently seeded)")
     ap.add_argument("--marker", action="store_true",
@@ -190,7 +202,10 @@ def pmc_summary(dirs, cal_dirs=None):
             k = out["kernels"].setdefault(_short(r["Kernel_Name"]), {"launches": 0, "total_ms": 0.0})
             k["launches"] += 1
             k["total_ms"] += (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-6
-            for col, key in (("Arch_VGPR_Count", "vgpr"), ("VGPR_Count", "vgpr"), ("Accum_VGPR_Count", "agpr"),
+            # VGPR_Count is the profiler's field as it reads it (on gfx950 half
+            # the compiler's VGPR count).
+            for col, key in (("Arch_VGPR_Count", "vgpr_field"), ("VGPR_Count", "vgpr_field"),
+                             ("Accum_VGPR_Count", "agpr"),
                              ("SGPR_Count", "sgpr"), ("Scratch_Size", "scratch_bytes_per_lane"),
                              ("Private_Segment_Size", "scratch_bytes_per_lane"), ("LDS_Block_Size", "lds_bytes"),
                              ("Group_Segment_Size", "lds_bytes")):
@@ -216,9 +231,10 @@ def pmc_summary(dirs, cal_dirs=None):
             k.setdefault("counter_launches", {})[counter] = len(ids)
     if cal_dirs:
         cal = {}
-        for r in _after_marker(_read_csvs(cal_dirs, "counter_collection.csv"), "Start_Timestamp"):
-            if r["Counter_Name"] in ("FETCH_SIZE", "WRITE_SIZE"):
-                cal[r["Counter_Name"]] = max(cal.get(r["Counter_Name"], 0.0), float(r["Counter_Value"]))
+        for d in cal_dirs:  # one run per directory, each with its own marker
+            for r in _after_marker(_read_csvs([d], "counter_collection.csv"), "Start_Timestamp"):
+                if r["Counter_Name"] in ("FETCH_SIZE", "WRITE_SIZE"):
+                    cal[r["Counter_Name"]] = max(cal.get(r["Counter_Name"], 0.0), float(r["Counter_Value"]))
         gib = float(1 << 30)
         f_fac = gib / (cal["FETCH_SIZE"] * 1024.0) if cal.get("FETCH_SIZE") else None
         w_fac = gib / (cal["WRITE_SIZE"] * 1024.0) if cal.get("WRITE_SIZE") else None
@@ -246,10 +262,24 @@ def pmc_summary(dirs, cal_dirs=None):
             k["utilisation"] = u
         if c.get("SQ_WAVE_CYCLES"):
             u = k.setdefault("utilisation", {})
-            for name, key in (("SQ_WAIT_ANY", "wave_waiting"), ("SQ_ACTIVE_INST_VALU", "wave_valu_issue"),
-                              ("SQ_ACTIVE_INST_VMEM", "wave_vmem_issue")):
+            for name, key in (("SQ_WAIT_ANY", "wave_waiting"), ("SQ_ACTIVE_INST_VALU", "wave_valu_issue")):
                 if name in c:
                     u[key] = c[name] / c["SQ_WAVE_CYCLES"]
+    # Keep the kernels that take at least 1 % of the profiled device time
+    # (all of them when there is no kernel trace).
+    tot = sum(k.get("total_ms", 0.0) for k in out["kernels"].values())
+    if tot > 0:
+        out["kernels"] = {n: k for n, k in out["kernels"].items() if k.get("total_ms", 0.0) >= 0.01 * tot}
+    for k in out["kernels"].values():
+        k.pop("counter_launches", None)
+        for c in list(k.get("counters_per_launch", {})):
+            k["counters_per_launch"][c] = round(k["counters_per_launch"][c], 1)
+        if "total_ms" in k:
+            k["total_ms"] = round(k["total_ms"], 4)
+            k["avg_ms"] = round(k["avg_ms"], 4)
+        for key in ("utilisation",):
+            if key in k:
+                k[key] = {u: round(v, 4) for u, v in k[key].items()}
     print(json.dumps(out, indent=1, sort_keys=True))
     return 0
 
@@ -342,10 +372,13 @@ def batched_leg(a):
         r.set_batch_tuning(**autotune.DEFAULT)
     # Batched: all views in one call each way (one warm-up first).
     r.render_views(cams[:2], spp[:2], seeds[:2])
+    if a.marker:
+        marker()
     tw = time.time()
     out = r.render_views(cams, spp, seeds)
     st = r.stats()
     b_ms, b_q, batches = st["forward_ms"], st["gather_queries"], r.last_batches
+    walk_avg = st["camera_trace_ms"] / max(1, st["camera_trace_launches"])
     g = r.render_views_backward(cams, spp, [dloss(o["raw"], tg) for o, tg in zip(out, targets)])
     st = r.stats()
     b_ms += st["backward_ms"]
@@ -371,6 +404,7 @@ def batched_leg(a):
         "same_queries": b_q == p_q,
         "same_images_as_plain": bool(same_images),
         "gradient_max_err_over_max": grad_err,
+        "roofline": counter_roofline("batched", "k_cam_trace", walk_avg),
         "tuning": tune["tuning"] if tune else autotune.DEFAULT,
         "autotune": {"sample": f"the {len(cams)} views at their spp", "seconds": tune["seconds"], "table": tune["table"]}
         if tune else None,
@@ -446,6 +480,89 @@ def differentiation_leg(a):
     return res
 
 
+def training_leg(a):
+    """One iteration of the reference's training loop (training.py:621-750)
+    at the plain workload's scene: render_forward, the L2 gradient of the
+    raw image, render_backward with every gradient, an Adam step on the
+    seven parameter arrays (numpy here, torch there), apply_point_
+    optimization, rebuild_bvh and the parameters fetched again.  Wall time
+    of each phase; the BVH rebuild is host work (the median split with
+    std::nth_element, whose element order the surfel ids depend on) plus the
+    upload of the rebuilt arrays."""
+    from differentiablepointrendering_amd import scene
+    from differentiablepointrendering_amd.pale import Renderer
+
+    spec = scene.config_scene(1)
+    cam = spec.cameras[0]
+    settings = dict(spec.settings, full_gradients=1)
+    tgt = Renderer.from_scene(spec.meshes, spec.cameras, spec.target_points, settings)
+    target = tgt.render_forward()[cam.name + "_raw"][..., :3].copy()
+    tgt.close()
+    r = Renderer.from_spec(spec, settings)
+    keys = ("position", "tangent_u", "tangent_v", "scale", "albedo", "opacity", "beta")
+    params = r.get_point_parameters()
+    m = {k: np.zeros_like(params[k]) for k in keys}
+    v = {k: np.zeros_like(params[k]) for k in keys}
+    phases = []
+    for it in range(1, a.training + 2):
+        ph = {}
+        t = time.perf_counter()
+        raw = r.render_forward()[cam.name + "_raw"][..., :3]
+        ph["render_forward"] = time.perf_counter() - t
+        t = time.perf_counter()
+        grads, _ = r.render_backward({cam.name: ((raw - target) / raw.size).astype(np.float32)})
+        ph["render_backward"] = time.perf_counter() - t
+        t = time.perf_counter()
+        for k in keys:
+            g = grads[k]
+            m[k] = 0.9 * m[k] + 0.1 * g
+            v[k] = 0.999 * v[k] + 0.001 * g * g
+            params[k] = (params[k] - 1e-3 * (m[k] / (1 - 0.9 ** it)) /
+                         (np.sqrt(v[k] / (1 - 0.999 ** it)) + 1e-8)).astype(np.float32)
+        params["opacity"] = np.clip(params["opacity"], 0.0, 1.0)
+        ph["optimizer_step_host"] = time.perf_counter() - t
+        t = time.perf_counter()
+        r.apply_point_optimization({k: params[k] for k in keys})
+        ph["apply_point_optimization"] = time.perf_counter() - t
+        t = time.perf_counter()
+        r.rebuild_bvh()
+        ph["rebuild_bvh"] = time.perf_counter() - t
+        t = time.perf_counter()
+        params = r.get_point_parameters()
+        ph["refetch"] = time.perf_counter() - t
+        m = {k: np.zeros_like(params[k]) for k in keys}  # a fresh optimizer over the new order
+        v = {k: np.zeros_like(params[k]) for k in keys}
+        if it > 1:  # the first iteration warms up
+            phases.append(ph)
+        print(f"[bench] training iteration {it}: {sum(ph.values()) * 1e3:.0f} ms", file=sys.stderr, flush=True)
+    r.close()
+    out = {k: round(float(np.mean([p[k] for p in phases])) * 1e3, 2) for k in phases[0]}
+    return {"workload": f"{len(spec.points['position'])} clustered surfels, 800x800, 16 spp, depth 6: one "
+                        f"training.py iteration (host arrays, as the reference's drivers)",
+            "iterations": len(phases), "ms": out, "total_ms": round(sum(out.values()), 2),
+            "host_threads": os.cpu_count()}
+
+
+def served_hbm():
+    """HBM rate of the served form while its kernels run, from the counted
+    traffic of every kernel of a served run (its summary in profiles/:
+    calibrated FETCH_SIZE + WRITE_SIZE per launch times launches, over their
+    summed durations in the same run's kernel trace)."""
+    prof = load_traffic(profile_path("served"))
+    if not prof:
+        return None
+    b = t = 0.0
+    for k in prof.get("kernels", {}).values():
+        if "hbm_bytes_per_launch" in k and k.get("launches"):
+            b += k["hbm_bytes_per_launch"] * k["launches"]
+            t += k["total_ms"]
+    if t <= 0:
+        return None
+    gbs = b / (t * 1e-3) / 1e9
+    return {"achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
+            "bytes": b, "kernel_ms": round(t, 2), "source": "profiles/" + PROFILES["served"]}
+
+
 def served_leg(a):
     import threading
 
@@ -459,7 +576,12 @@ def served_leg(a):
     budget = int(0.9 * total)
     stream = served_requests(a.served, a.served_rate)
     t0 = time.time()
-    srv = SceneServer(hbm_budget_bytes=budget)
+    tuning = {}
+    if a.served_batch_slots:
+        tuning["slots_per_batch"] = a.served_batch_slots
+    if a.served_grid_fraction:
+        tuning["grid_fraction"] = a.served_grid_fraction
+    srv = SceneServer(hbm_budget_bytes=budget, tuning=tuning or None)
     targets = SceneServer(hbm_budget_bytes=budget)
     names = []
     resident = {}
@@ -475,32 +597,37 @@ def served_leg(a):
             targets.add_scene(name, meshes, tgt, settings, replicas=1)
         print(f"[bench] served scene {name}: {resident[name] / 2**30:.2f} GiB resident "
               f"({a.served_replicas} replicas)", file=sys.stderr, flush=True)
-    # Target images of the forward + adjoint requests (untimed).
+    # Target images of the forward + adjoint requests (untimed; one sample
+    # per pixel of the scene with jittered centers).
     reqs = []
     for i, (t, sc, cam, spp, adjoint) in enumerate(stream):
         target = None
         if adjoint:
-            target = targets.render(ViewRequest(names[sc], cam, spp, 10_000 + i)).raw[..., :3].copy()
+            target = targets.render(ViewRequest(names[sc], cam, 1, 10_000 + i)).raw[..., :3].copy()
         reqs.append((t, ViewRequest(names[sc], cam, spp, 1 + i, target=target)))
     targets.close()
     setup_s = time.time() - t0
     print(f"[bench] served setup {setup_s:.1f}s; {len(reqs)} requests, "
           f"{sum(r.adjoint for _, r in reqs)} with adjoint", file=sys.stderr, flush=True)
+    head = reqs[:max(1, min(a.served_alone, len(reqs)))]
 
-    # One request at a time: the plain single-view path on the same requests.
+    # One request at a time: the plain single-view path on the head of the
+    # stream.
     tp = time.perf_counter()
     alone = []
-    for i, (_, r) in enumerate(reqs):
+    for i, (_, r) in enumerate(head):
         alone.append(srv.render(r))
         if (i + 1) % 10 == 0:
-            print(f"[bench] served alone {i + 1}/{len(reqs)}", file=sys.stderr, flush=True)
+            print(f"[bench] served alone {i + 1}/{len(head)}", file=sys.stderr, flush=True)
     plain_s = time.perf_counter() - tp
     plain = latency_summary(alone, plain_s)
 
     # All at once: the capacity of the served form (views overlap on the
     # device), on the same requests.
+    if a.marker:
+        marker()
     tb = time.perf_counter()
-    burst = [f.result() for f in [srv.submit(r) for _, r in reqs]]
+    burst = [f.result() for f in [srv.submit(r) for _, r in head]]
     burst_s = max(m.finished for m in burst) - tb
     cap = latency_summary(burst, burst_s)
     print(f"[bench] served burst: {cap['mrays_per_s']:.1f} Mrays/s in {burst_s:.1f}s", file=sys.stderr, flush=True)
@@ -508,14 +635,26 @@ def served_leg(a):
     # As they arrive (Poisson), continuously scheduled: the latency.
     futs = []
     ts = time.perf_counter()
-    for t, r in reqs:
+    for i, (t, r) in enumerate(reqs):
         dt = ts + t - time.perf_counter()
         if dt > 0:
             time.sleep(dt)
         futs.append(srv.submit(r))
+        if (i + 1) % 50 == 0:
+            done = sum(f.done() for f in futs)
+            print(f"[bench] served arrivals {i + 1}/{len(reqs)} submitted, {done} done", file=sys.stderr,
+                  flush=True)
     mixed = [f.result() for f in futs]
     makespan = max(m.finished for m in mixed) - ts
     served = latency_summary(mixed, makespan)
+    # Latency by request size (pixel samples = width x height x spp).
+    small = [m for m in mixed if m.request.camera.width * m.request.camera.height * m.request.spp <= (1 << 20)]
+    small_alone = [m for m in alone if m.request.camera.width * m.request.camera.height * m.request.spp <= (1 << 20)]
+    by_size = {"small_max_samples": 1 << 20, "small_requests": len(small),
+               "small_p50_ms": round(float(np.percentile([m.latency_ms for m in small], 50)), 2) if small else None,
+               "small_p99_ms": round(float(np.percentile([m.latency_ms for m in small], 99)), 2) if small else None,
+               "small_alone_p50_ms": round(float(np.percentile([m.latency_ms for m in small_alone], 50)), 2)
+               if small_alone else None}
     same_images = same_grads = True
     mismatches = []
     for i, (x, y) in [(i, p) for i, p in enumerate(zip(alone, mixed))] + [(i, p) for i, p in enumerate(zip(alone, burst))]:
@@ -539,7 +678,8 @@ def served_leg(a):
     srv.close()
     dev_bytes_after = Renderer.device_bytes()
     out = {
-        "workload": f"{len(reqs)} requests (prefix of the seed-1 stream) over {len(names)} resident scenes "
+        "workload": f"{len(reqs)} requests (prefix of the seed-1 stream; the first {len(head)} also alone and all "
+                    f"at once) over {len(names)} resident scenes "
                     f"{'/'.join(names)}, Poisson {a.served_rate}/s, 30% forward+adjoint, 256x256..1920x1080, "
                     f"spp {{1,4,16,64}}, {a.served_replicas} replicas per scene",
         "arrival_rate_per_s": a.served_rate,
@@ -549,6 +689,10 @@ def served_leg(a):
         "mrays_per_s": round(served["mrays_per_s"], 2),
         "requests_per_s": round(served["requests_per_s"], 3),
         "makespan_s": round(makespan, 3),
+        "latency_by_size": by_size,
+        "head_requests_alone_and_all_at_once": len(head),
+        "tuning": tuning or "renderer default",
+        "hw_queues": os.environ.get("GPU_MAX_HW_QUEUES"),
         "capacity_all_at_once": {"mrays_per_s": round(cap["mrays_per_s"], 2),
                                  "requests_per_s": round(cap["requests_per_s"], 3),
                                  "p50_ms": round(cap["p50_ms"], 2), "p99_ms": round(cap["p99_ms"], 2),
@@ -563,6 +707,7 @@ def served_leg(a):
         "same_images": bool(same_images),
         "same_gradients": bool(same_grads),
         "mismatches": mismatches[:8],
+        "hbm_while_busy": served_hbm(),
         "hbm_budget_bytes": budget,
         "resident_bytes": resident,
         "setup_s": round(setup_s, 1),
@@ -630,16 +775,29 @@ def cpu_leg(a, spec, settings, target, cam, step_queries=None):
     return out
 
 
+# Counter summaries in profiles/ per form (written by --pmc-summary).
+PROFILES = {
+    "plain": "step_kernels_1m_surfels_800px_16spp_reference_seeds.json",
+    "independent": "step_kernels_1m_surfels_800px_16spp_independent_samples.json",
+    "batched": "batched_32_views_500k_surfels.json",
+    "served": "served_4_scenes_24_requests.json",
+}
+
+
+def profile_path(form):
+    return os.path.join(ROOT, "profiles", PROFILES.get(form, f"{form}.json"))
+
+
 def counter_roofline(form, kernel, avg_launch_ms, algorithmic_bytes=None):
     """Roofline of `kernel` priced against HBM from its counted traffic: the
     calibrated FETCH_SIZE + WRITE_SIZE bytes per launch of the same workload
-    (profiles/traffic_<form>.json, rocprofv3 --pmc) over the launch duration
+    (its summary in profiles/, rocprofv3 --pmc) over the launch duration
     measured here with HIP events.  What bounds the kernel comes from the
     same profile's unit counters (TD / TA busy, L1 hit rate, issue), with
     HBM's own fraction beside them.  The reference byte model per work unit
     (every lane charged for its own node and surfel loads, most of which
     the L1 serves) is kept as an informational rate."""
-    prof = load_traffic(os.path.join(ROOT, "profiles", f"traffic_{form}.json"))
+    prof = load_traffic(profile_path(form))
     out = {"kernel": kernel, "bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None,
            "traffic": None, "avg_launch_ms": round(avg_launch_ms, 3)}
     k = (prof or {}).get("kernels", {}).get(kernel, {})
@@ -647,12 +805,12 @@ def counter_roofline(form, kernel, avg_launch_ms, algorithmic_bytes=None):
         b = float(k["hbm_bytes_per_launch"])
         ach = b / (avg_launch_ms * 1e-3) / 1e9
         out.update(achieved=round(ach, 1), frac=round(ach / HBM_PEAK_GBS, 4), traffic=b,
-                   traffic_source=f"profiles/traffic_{form}.json",
+                   traffic_source="profiles/" + PROFILES.get(form, ""),
                    traffic_profile_avg_launch_ms=round(k.get("avg_ms", 0.0), 3))
         if ach > HBM_PEAK_GBS:
             out["invalid"] = "counted HBM bytes over the measured time exceed the HBM peak"
     else:
-        out["note"] = f"no counter profile of {kernel} for this form (profiles/traffic_{form}.json)"
+        out["note"] = f"no counter profile of {kernel} for this form (profiles/{PROFILES.get(form)})"
     u = k.get("utilisation")
     if u:
         lim = {kk: round(v, 3) for kk, v in u.items()}
@@ -663,7 +821,7 @@ def counter_roofline(form, kernel, avg_launch_ms, algorithmic_bytes=None):
         names = {"td_busy": "vector-memory data return (TD)", "ta_busy": "vector-memory addressing (TA)",
                  "hbm": "HBM bandwidth"}
         out["limiter"] = {"unit": names.get(top, top), "fractions": lim}
-    for key in ("vgpr", "agpr", "sgpr", "scratch_bytes_per_lane", "lds_bytes"):
+    for key in ("vgpr_field", "agpr", "sgpr", "scratch_bytes_per_lane", "lds_bytes"):
         if key in k:
             out.setdefault("resources", {})[key] = k[key]
     if algorithmic_bytes:
@@ -790,6 +948,10 @@ def independent_leg(a, spec, settings, target):
 
 def main():
     a = parse()
+    if a.hw_queues:
+        # Read by the HIP runtime when it starts (nothing has touched the GPU
+        # yet): one hardware queue per replica stream of the served form.
+        os.environ["GPU_MAX_HW_QUEUES"] = str(a.hw_queues)
     if a.copy_calibration:
         return copy_calibration()
     if a.pmc_summary:
@@ -806,6 +968,9 @@ def main():
         return 0
     if a.differentiation_only:
         print(json.dumps({"differentiation": differentiation_leg(a)}))
+        return 0
+    if a.form == "training":
+        print(json.dumps({"training_iteration": training_leg(a)}))
         return 0
 
     from differentiablepointrendering_amd import scene
@@ -883,6 +1048,8 @@ def main():
             out["batched"] = batched_leg(a)
         if a.differentiation > 0:
             out["differentiation"] = differentiation_leg(a)
+        if a.training > 0:
+            out["training_iteration"] = training_leg(a)
         if a.served > 0:
             out["served"] = served_leg(a)
     print(json.dumps(out))

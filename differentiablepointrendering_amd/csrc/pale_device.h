@@ -193,7 +193,7 @@ __device__ bool traverseCloud(const DevScene& S, const DevInstance& I, F3 o, F3 
         }
         const float4* nd4 = reinterpret_cast<const float4*>(S.nodes + ref);
         const float4 a = nd4[0], b = nd4[1], c = nd4[2];
-        const uint4 refs = reinterpret_cast<const uint4*>(S.nodes + ref)[3];
+        const uint2 refs = reinterpret_cast<const uint2*>(S.nodes + ref)[6];  // left, right (not the padding)
         if (kCount) w.aabb += 2;
         float tl = __builtin_huge_valf(), tr = __builtin_huge_valf();
         const bool hl = slab(o, inv, a.x, a.y, a.z, a.w, b.x, b.y, best, 1e-6f, tl);
@@ -220,14 +220,18 @@ __device__ bool traverseCloud(const DevScene& S, const DevInstance& I, F3 o, F3 
     return true;
 }
 
-// Words of a leaf's surfels loaded before its tests: 2 (plane words) or 4
-// (the whole 64-byte records).
-constexpr int kLeafWords = 4;
+// Words of a leaf's surfels loaded before its tests (kLeafWords): 4 = the
+// whole 64-byte records, which suits queues whose neighbouring lanes trace
+// unrelated rays (photon and adjoint bounces); 2 = the plane words only, the
+// tangents of a surfel loaded when its plane hit is in range, which suits
+// the camera walk (fewer bytes per lane when neighbouring rays diverge).
+// Same tests in the same order either way.
 
 // The same traversal without the counters, and without a stack round trip
 // for the nearer child: the reference pushes the farther child, then the
 // nearer, and pops the nearer at once with an unchanged best, so descending
 // into the nearer child directly visits the same nodes in the same order.
+template <int kLeafWords>
 __device__ bool traverseCloudFast(const DevScene& S, const DevInstance& I, F3 o, F3 d, const LaneStack& st,
                                   uint32_t base, LocalHit& out) {
     const F3 inv = safeInvDir(d);
@@ -293,7 +297,7 @@ __device__ bool traverseCloudFast(const DevScene& S, const DevInstance& I, F3 o,
         } else {
             const float4* nd4 = reinterpret_cast<const float4*>(S.nodes + ref);
             const float4 a = nd4[0], b = nd4[1], c = nd4[2];
-            const uint4 refs = reinterpret_cast<const uint4*>(S.nodes + ref)[3];
+            const uint2 refs = reinterpret_cast<const uint2*>(S.nodes + ref)[6];  // left, right (not the padding)
             float tl = __builtin_huge_valf(), tr = __builtin_huge_valf();
             const bool hl = slab(o, inv, a.x, a.y, a.z, a.w, b.x, b.y, best, 1e-6f, tl);
             const bool hr = slab(o, inv, b.z, b.w, c.x, c.y, c.z, c.w, best, 1e-6f, tr);
@@ -398,7 +402,7 @@ __device__ bool traverseMesh(const DevScene& S, const DevInstance& I, F3 o, F3 d
         }
         const float4* nd4 = reinterpret_cast<const float4*>(S.nodes + ref);
         const float4 a = nd4[0], b = nd4[1], c = nd4[2];
-        const uint4 refs = reinterpret_cast<const uint4*>(S.nodes + ref)[3];
+        const uint2 refs = reinterpret_cast<const uint2*>(S.nodes + ref)[6];  // left, right (not the padding)
         if (kCount) w.aabb += 2;
         float tl = __builtin_huge_valf(), tr = __builtin_huge_valf();
         const bool hl = slab(o, inv, a.x, a.y, a.z, a.w, b.x, b.y, best, 1e-6f, tl);
@@ -427,7 +431,7 @@ __device__ bool traverseMesh(const DevScene& S, const DevInstance& I, F3 o, F3 d
 
 // intersectScene, FirstHit mode (IntersectionKernels.h:954-1097).  The TLAS
 // stack shares the lane's LDS stack below the BLAS entries.
-template <bool kCount>
+template <bool kCount, int kLeafWords = 4>
 __device__ bool intersectScene(const DevScene& S, F3 o, F3 d, const LaneStack& st, Hit& h, Work& w) {
     h.inst = kInvalid;
     if (S.nTlas == 0) return false;
@@ -482,7 +486,7 @@ __device__ bool intersectScene(const DevScene& S, F3 o, F3 d, const LaneStack& s
         else if (kCount)
             acc = traverseCloud<kCount>(S, I, oo, od, st, sp, lh, w);
         else
-            acc = traverseCloudFast(S, I, oo, od, st, sp, lh);
+            acc = traverseCloudFast<kLeafWords>(S, I, oo, od, st, sp, lh);
         if (acc) {
             const float tw = dot(lh.pos - o, d);
             if (tw > 0.0f && tw < bestW) {

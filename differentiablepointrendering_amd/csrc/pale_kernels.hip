@@ -794,7 +794,7 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(4))
         Work wl{};
         if (active) {
             Hit h;
-            if (intersectScene<kCount>(S, o, d, stk, h, wl)) {
+            if (intersectScene<kCount, 2>(S, o, d, stk, h, wl)) {
                 const DevInstance& I = S.inst[h.inst];
                 want = true;
                 if (I.type == kGeoCloud) {

@@ -63,22 +63,46 @@ void parallelFor(size_t n, F&& f) {
 }
 
 // Generic median-split builder (BVH.ixx:49-133 for boxes, :26-45/:141-220
-// for triangles).  `bounds(i)` is the box of primitive i, `key(i, axis)` its
-// centroid coordinate.
-template <typename BoundsF, typename KeyF>
+// for triangles).  `order` holds the primitives being partitioned (an index,
+// or a record that carries its index with its box and centroid); `bounds(e)`
+// is the box of element e, `key(e, axis)` its centroid coordinate.
+// std::nth_element's moves depend only on its comparisons, so partitioning
+// records compared on the same keys leaves their indices in the same order as
+// partitioning the indices themselves.
+template <typename Elem, typename BoundsF, typename KeyF>
 struct MedianSplit {
     std::vector<RefNode>& nodes;
-    std::vector<uint32_t>& order;
+    std::vector<Elem>& order;
     BoundsF bounds;
     KeyF key;
     uint32_t maxLeaf;
 
+    // Bounds of a node's primitives.  min / max are exact in any order, so
+    // large ranges are reduced in parallel chunks.
     void fit(RefNode& nd) {
-        F3 bmin = mk3(kInf, kInf, kInf), bmax = mk3(-kInf, -kInf, -kInf);
-        for (uint32_t i = 0; i < nd.count; ++i) {
-            const Box b = bounds(order[nd.leftFirst + i]);
-            bmin = vmin(bmin, b.mn);
-            bmax = vmax(bmax, b.mx);
+        auto range = [&](uint32_t lo, uint32_t hi, F3& bmin, F3& bmax) {
+            bmin = mk3(kInf, kInf, kInf);
+            bmax = mk3(-kInf, -kInf, -kInf);
+            for (uint32_t i = lo; i < hi; ++i) {
+                const Box b = bounds(order[nd.leftFirst + i]);
+                bmin = vmin(bmin, b.mn);
+                bmax = vmax(bmax, b.mx);
+            }
+        };
+        constexpr uint32_t kChunk = 1u << 16;
+        if (nd.count < 4 * kChunk) {
+            range(0, nd.count, nd.mn, nd.mx);
+            return;
+        }
+        const uint32_t nc = (nd.count + kChunk - 1) / kChunk;
+        std::vector<F3> mins(nc), maxs(nc);
+        parallelFor(nc, [&](size_t c) {
+            range((uint32_t)c * kChunk, std::min(nd.count, (uint32_t)(c + 1) * kChunk), mins[c], maxs[c]);
+        });
+        F3 bmin = mins[0], bmax = maxs[0];
+        for (uint32_t c = 1; c < nc; ++c) {
+            bmin = vmin(bmin, mins[c]);
+            bmax = vmax(bmax, maxs[c]);
         }
         nd.mn = bmin;
         nd.mx = bmax;
@@ -93,7 +117,7 @@ struct MedianSplit {
         auto b = order.begin() + nd.leftFirst;
         auto en = b + nd.count;
         auto mid = b + (nd.count >> 1);
-        std::nth_element(b, mid, en, [&](uint32_t p, uint32_t q) { return key(p, axis) < key(q, axis); });
+        std::nth_element(b, mid, en, [&](const Elem& p, const Elem& q) { return key(p, axis) < key(q, axis); });
         const uint32_t leftCount = (uint32_t)(mid - b);
         if (leftCount == 0 || leftCount == nd.count) return false;
         l = RefNode{};
@@ -121,11 +145,10 @@ struct MedianSplit {
         subdivide(L + 1);
     }
 
+    // order[] must hold the n primitives in index order.
     void build(uint32_t n) {
         nodes.clear();
         nodes.reserve(std::max<uint32_t>(1u, n * 2));
-        order.resize(n);
-        for (uint32_t i = 0; i < n; ++i) order[i] = i;
         nodes.push_back(RefNode{});
         nodes[0].leftFirst = 0;
         nodes[0].count = n;
@@ -224,10 +247,10 @@ struct MedianSplit {
     }
 };
 
-template <typename BoundsF, typename KeyF>
-MedianSplit<BoundsF, KeyF> medianSplit(std::vector<RefNode>& nodes, std::vector<uint32_t>& order, BoundsF bounds,
-                                       KeyF key, uint32_t maxLeaf) {
-    return MedianSplit<BoundsF, KeyF>{nodes, order, bounds, key, maxLeaf};
+template <typename Elem, typename BoundsF, typename KeyF>
+MedianSplit<Elem, BoundsF, KeyF> medianSplit(std::vector<RefNode>& nodes, std::vector<Elem>& order, BoundsF bounds,
+                                             KeyF key, uint32_t maxLeaf) {
+    return MedianSplit<Elem, BoundsF, KeyF>{nodes, order, bounds, key, maxLeaf};
 }
 
 uint32_t maxDepth(const std::vector<RefNode>& nodes, uint32_t first, uint32_t count) {
@@ -286,7 +309,8 @@ BuildProducts buildScene(const std::vector<HostMesh>& meshes, const std::vector<
     }
     // collectPointCloudGeometry (SceneBuild.cpp:181-227)
     std::vector<HostPoint> pts(asset.size());
-    for (size_t i = 0; i < asset.size(); ++i) {
+    parallelFor((asset.size() + 16383) / 16384, [&](size_t c) {
+      for (size_t i = c * 16384; i < std::min(asset.size(), (c + 1) * 16384); ++i) {
         const AssetPoint& a = asset[i];
         HostPoint p{};
         p.p = a.p;
@@ -301,7 +325,8 @@ BuildProducts buildScene(const std::vector<HostMesh>& meshes, const std::vector<
         p.beta = a.beta;
         p.shape = clampf(a.shape, -5.0f, 5.0f);
         pts[i] = p;
-    }
+      }
+    });
     // The "Gaussian" point-cloud instance with identity transform
     // (pale_bindings.cpp:90-101, SceneBuild.cpp:229-251).
     {
@@ -351,7 +376,8 @@ BuildProducts buildScene(const std::vector<HostMesh>& meshes, const std::vector<
         const uint32_t vb = vertBase[mi];
         std::vector<HostTri> lt(bp.tris.begin() + mr.first, bp.tris.begin() + mr.first + mr.count);
         std::vector<RefNode> nodes;
-        std::vector<uint32_t> order;
+        std::vector<uint32_t> order(mr.count);
+        for (uint32_t i = 0; i < mr.count; ++i) order[i] = i;
         auto ms = medianSplit(
             nodes, order,
             [&](uint32_t i) {
@@ -376,28 +402,33 @@ BuildProducts buildScene(const std::vector<HostMesh>& meshes, const std::vector<
         bp.blasRanges.push_back(Range{first, (uint32_t)nodes.size()});
     }
     {
-        std::vector<Box> boxes(pts.size());
+        // Each surfel as one record (centroid, box, index), so that the
+        // selections and the box fits stream through memory instead of
+        // gathering keys and boxes by index.
+        struct Prim {
+            float key[3];
+            uint32_t idx;
+            Box box;
+        };
         const size_t chunk = 1u << 14;
+        std::vector<Prim> prims(pts.size());
         parallelFor((pts.size() + chunk - 1) / chunk, [&](size_t c) {
-            for (size_t i = c * chunk; i < std::min(pts.size(), (c + 1) * chunk); ++i) boxes[i] = surfelBox(pts[i]);
+            for (size_t i = c * chunk; i < std::min(pts.size(), (c + 1) * chunk); ++i)
+                prims[i] = Prim{{pts[i].p.x, pts[i].p.y, pts[i].p.z}, (uint32_t)i, surfelBox(pts[i])};
         });
         std::vector<RefNode> nodes;
-        std::vector<uint32_t> order;
-        // Centroid coordinates per axis, contiguous for the selections.
-        std::vector<float> kx(pts.size()), ky(pts.size()), kz(pts.size());
-        for (size_t i = 0; i < pts.size(); ++i) {
-            kx[i] = pts[i].p.x;
-            ky[i] = pts[i].p.y;
-            kz[i] = pts[i].p.z;
-        }
-        const float* keys[3] = {kx.data(), ky.data(), kz.data()};
         auto ms = medianSplit(
-            nodes, order, [&](uint32_t i) { return boxes[i]; }, [&](uint32_t i, int a) { return keys[a][i]; },
+            nodes, prims, [](const Prim& e) { return e.box; }, [](const Prim& e, int a) { return e.key[a]; },
             kMaxLeafPrims);
         ms.build((uint32_t)pts.size());
         bp.points.resize(pts.size());
-        for (size_t k = 0; k < order.size(); ++k) bp.points[k] = pts[order[k]];
-        bp.perm = order;
+        bp.perm.resize(pts.size());
+        parallelFor((pts.size() + chunk - 1) / chunk, [&](size_t c) {
+            for (size_t k = c * chunk; k < std::min(pts.size(), (c + 1) * chunk); ++k) {
+                bp.perm[k] = prims[k].idx;
+                bp.points[k] = pts[prims[k].idx];
+            }
+        });
         const uint32_t first = (uint32_t)bp.blas.size();
         bp.blas.insert(bp.blas.end(), nodes.begin(), nodes.end());
         const uint32_t bi = (uint32_t)bp.blasRanges.size();
@@ -475,7 +506,9 @@ BuildProducts buildScene(const std::vector<HostMesh>& meshes, const std::vector<
 void refreshSurfels(const std::vector<HostPoint>& pts, DeviceImage& img) {
     img.surfGeo.resize(pts.size());
     img.surfShade.resize(pts.size());
-    for (size_t i = 0; i < pts.size(); ++i) {
+    constexpr size_t kChunk = 1u << 14;
+    parallelFor((pts.size() + kChunk - 1) / kChunk, [&](size_t c) {
+      for (size_t i = c * kChunk; i < std::min(pts.size(), (c + 1) * kChunk); ++i) {
         const HostPoint& s = pts[i];
         // intersectSurfel's frame (KernelHelpers.h:646-648)
         const F3 u = normalize(s.tu);
@@ -492,7 +525,8 @@ void refreshSurfels(const std::vector<HostPoint>& pts, DeviceImage& img) {
         h.nsx = ns.x; h.nsy = ns.y; h.nsz = ns.z; h.opacity = s.opacity;
         h.ar = s.albedo.x; h.ag = s.albedo.y; h.ab = s.albedo.z; h.alpha_r = s.alpha_r;
         img.surfShade[i] = h;
-    }
+      }
+    });
 }
 
 DeviceImage makeDeviceImage(const BuildProducts& bp, const std::vector<HostMesh>& meshes) {
@@ -523,7 +557,8 @@ DeviceImage makeDeviceImage(const BuildProducts& bp, const std::vector<HostMesh>
         return compact[r.first + local];
     };
     for (const Range& r : bp.blasRanges) {
-        for (uint32_t i = 0; i < r.count; ++i) {
+      parallelFor((r.count + 16383u) / 16384u, [&](size_t c) {
+        for (uint32_t i = (uint32_t)c * 16384u; i < std::min(r.count, (uint32_t)(c + 1) * 16384u); ++i) {
             const uint32_t ci = compact[r.first + i];
             if (ci == kInvalid) continue;
             const RefNode& n = bp.blas[r.first + i];
@@ -537,6 +572,7 @@ DeviceImage makeDeviceImage(const BuildProducts& bp, const std::vector<HostMesh>
             w.right = refOf(r, n.leftFirst + 1);
             img.nodes[ci] = w;
         }
+      });
     }
     for (const HostInstance& in : bp.instances) {
         const Range& r = bp.blasRanges[in.blas];

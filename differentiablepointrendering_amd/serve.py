@@ -184,7 +184,7 @@ class SceneServer:
     def __init__(self, hbm_budget_bytes: Optional[int] = None,
                  renderer_factory: Optional[Callable] = None,
                  device_bytes: Optional[Callable[[], int]] = None,
-                 max_group: int = 8, group_slots: int = 4 << 20):
+                 max_group: int = 8, group_slots: int = 4 << 20, tuning: Optional[dict] = None):
         if renderer_factory is None or device_bytes is None:
             from .pale import Renderer
 
@@ -206,6 +206,12 @@ class SceneServer:
         self.max_group = max(1, int(max_group))
         self.group_slots = int(group_slots)
         self.groups_served = 0
+        # Launch shape of every replica (Renderer.set_batch_tuning keys): a
+        # smaller slots_per_batch cuts a large view into more, shorter walk
+        # launches, between which the device takes other requests' kernels;
+        # grid_fraction leaves part of the GPU to them.  Results do not
+        # depend on it.
+        self.tuning = dict(tuning) if tuning else None
 
     # ------------------------------------------------------------- scenes --
     def add_scene(self, name: str, meshes, points: dict, settings: dict, replicas: int = 1) -> int:
@@ -223,6 +229,8 @@ class SceneServer:
                 for _ in range(replicas):
                     r = self._factory(meshes, points, settings)
                     made.append(r)
+                    if self.tuning:
+                        r.set_batch_tuning(**self.tuning)
                     r.build_photon_map()
                     if self.budget is not None and self._device_bytes() + self._reserved > self.budget:
                         raise AdmissionError(

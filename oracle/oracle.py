@@ -62,6 +62,7 @@ def lib():
         L.orc_set_sample.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32]
         L.orc_full_gradients.argtypes = [C.c_void_p, _fp]
         L.orc_get_debug_image.argtypes = [C.c_void_p, C.c_char_p, _fp]
+        L.orc_replace_points.argtypes = [C.c_void_p, C.POINTER(_abi.PointsDesc)]
         L.orc_freeze_map.argtypes = [C.c_void_p, C.c_int32]
         L.orc_build_photon_map.argtypes = [C.c_void_p]
         L.orc_render_view.argtypes = [C.c_void_p, C.POINTER(_abi.CameraDesc), C.c_uint32, C.c_uint32, _fp, _fp]
@@ -251,6 +252,15 @@ class Oracle:
 
     def rebuild_bvh(self):
         lib().orc_rebuild_bvh(self.h)
+
+    def replace_points(self, points: Dict[str, np.ndarray]):
+        """The asset replaced by `points` (any count) and rebuilt, the photon
+        grid kept from the first build (what add_points / remove_points /
+        set_point_* leave behind)."""
+        holder = _abi.PointsHolder(points)
+        rc = lib().orc_replace_points(self.h, C.byref(holder.desc))
+        if rc != 0:
+            raise RuntimeError(lib().orc_last_error().decode())
 
     def freeze_map(self, on: bool):
         """Test mode: render_forward re-renders the cameras over the photon

@@ -2340,6 +2340,23 @@ int orc_rebuild_bvh(void* h) {
     return 0;
 }
 
+// The asset replaced by the given points (any count), then rebuilt, with the
+// photon grid and capacities kept from the first build: what the binding's
+// add_points / remove_points / set_point_* leave behind (they edit the
+// PointAsset and rebuild, pale_bindings.cpp:1081-1587; PathTracer::setScene
+// configures the grid once, PathTracer.cpp:22-53).  Shape is taken as given.
+int orc_replace_points(void* h, const pale_points_desc* p) {
+    try {
+        Renderer& R = *static_cast<Renderer*>(h);
+        R.asset = toAsset(*p);
+        rebuild(R);
+        return 0;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return PALE_ERR_INTERNAL;
+    }
+}
+
 
 // RNG primitives, exposed for known-answer tests.
 void orc_splitmix64(uint64_t seed, uint32_t n, uint64_t* out) {

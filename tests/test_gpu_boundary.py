@@ -2,9 +2,11 @@
 
 Point edits (add_points, remove_points, set_point_properties with an index
 and with index < 0, set_point_opacity, set_point_beta) are checked against
-the oracle built from the point set the reference's binding would hold after
-the edit (pale_bindings.cpp:1081-1587, restated here with numpy float32 in
-the binding's operation order): images bit for bit and the same BVH.  The
+the oracle given the point set the reference's binding would hold after the
+edit (pale_bindings.cpp:1081-1587, restated here with numpy float32 in the
+binding's operation order) and rebuilt with its photon grid kept from the
+first build, as PathTracer::setScene keeps it (PathTracer.cpp:22-53): images
+bit for bit and the same BVH.  The
 path-based constructor, debug gradient images, ldr_channels=3 and a
 training.py-shaped step (training.py:621-750: forward, L2 gradient of the
 LDR image, backward with a dict, optimizer step, apply_point_optimization,
@@ -70,26 +72,33 @@ def test_point_edits_match_oracle():
     spec = scene.config_scene(0, n=2048, width=48, height=48)
     gpu = _gpu(spec, SMALL)
     pts = _raw(spec.points)
+    orc = _oracle(spec, pts, SMALL)
+    _same_render(gpu, orc)
+
+    def check():
+        orc.replace_points(pts)
+        _same_render(gpu, orc)
+
     assert gpu.get_camera_names() == ["camera1"]
     # add_points: appended to the asset with shape 0 (pale_bindings.cpp:1221-1412)
     rng = np.random.default_rng(5)
     new, _ = scene.uniform_surfels(64, seed=9)
     gpu.add_points({"new": {k: v for k, v in new.items() if k != "shape"}})
     pts = {k: np.concatenate([pts[k], new[k] if k != "shape" else np.zeros(64, np.float32)]) for k in pts}
-    _same_render(gpu, _oracle(spec, pts, SMALL))
+    check()
     assert gpu.point_count == 2048 + 64
     # remove_points: indices into the asset, int32 or int64
     drop = rng.choice(2048 + 64, size=100, replace=False)
     gpu.remove_points({"indices": drop.astype(np.int32)})
     keep = np.setdiff1d(np.arange(2048 + 64), drop)
     pts = {k: v[keep] for k, v in pts.items()}
-    _same_render(gpu, _oracle(spec, pts, SMALL))
+    check()
     # set_point_opacity / set_point_beta (asset index, then rebuild_bvh)
     gpu.set_point_opacity(0.25, 7)
     gpu.set_point_beta(-1.5, 11)
     pts["opacity"][7] = np.float32(0.25)
     pts["beta"][11] = np.float32(-1.5)
-    _same_render(gpu, _oracle(spec, pts, SMALL))
+    check()
     # set_point_properties with an index: translation added, tangents
     # rotated, albedo / opacity / beta / scale replaced
     t, q, s, c = (0.01, -0.02, 0.03), (0.0, 0.0, 0.38268343, 0.92387953), (0.04, 0.03, 1.0), (0.3, 0.6, 0.9)
@@ -102,7 +111,7 @@ def test_point_edits_match_oracle():
     pts["opacity"][3] = f(0.8)
     pts["beta"][3] = f(0.25)
     pts["scale"][3] = np.asarray(s[:2], f)
-    _same_render(gpu, _oracle(spec, pts, SMALL))
+    check()
     # index < 0: the binding's loop advances twice per iteration
     # (pale_bindings.cpp:1544-1561), so points 0, 2, 4, ... change, by
     # deltas (added translation, albedo, opacity, beta; scaled scale)
@@ -116,12 +125,13 @@ def test_point_edits_match_oracle():
     pts["opacity"][e] = (pts["opacity"][e] + f(0.05)).astype(f)
     pts["beta"][e] = (pts["beta"][e] + f(-0.1)).astype(f)
     pts["scale"][e] = (pts["scale"][e] * np.asarray(s[:2], f)).astype(f)
-    _same_render(gpu, _oracle(spec, pts, SMALL))
+    check()
     with pytest.raises(RuntimeError):
         gpu.set_point_opacity(0.5, 10**6)
     with pytest.raises(RuntimeError):
         gpu.remove_points({"indices": np.zeros((2, 2), np.int64)})
     gpu.close()
+    orc.close()
 
 
 def test_path_constructor_matches_oracle():
