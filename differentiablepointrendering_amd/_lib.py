@@ -29,7 +29,7 @@ EXPORTED = (
     "pale_render_view_backward", "pale_device_bytes", "pale_view_bytes", "pale_views_bytes", "pale_view_buffer_bytes",
     "pale_release_view_buffers", "pale_set_batch_tuning", "pale_get_batch_tuning", "pale_render_views",
     "pale_render_views_backward", "pale_render_view_backward_device", "pale_apply_points_device",
-    "pale_get_debug_image",
+    "pale_get_debug_image", "pale_render_view_backward_noalloc",
 )
 
 
@@ -72,6 +72,7 @@ def load(path: str = LIB_PATH):
     L.pale_set_opacity_device.argtypes = [H, _fp, C.c_uint32]
     L.pale_apply_points_device.argtypes = [H, C.c_uint32] + [C.c_void_p] * 7
     L.pale_get_debug_image.argtypes = [H, C.c_char_p, _fp]
+    L.pale_render_view_backward_noalloc.argtypes = [H, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]
     L.pale_build_photon_map.argtypes = [H]
     L.pale_device_bytes.argtypes = [C.POINTER(C.c_uint64)]
     L.pale_view_bytes.argtypes = [H, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int32, C.POINTER(C.c_uint64)]

@@ -198,6 +198,7 @@ struct GridParams {
 // Launchers (all asynchronous on `s`).
 void launchFill(hipStream_t s, uint32_t* p, uint32_t n, uint32_t v);
 void launchFillF(hipStream_t s, float* p, uint32_t n, float v);
+void launchIota(hipStream_t s, uint32_t* p, uint32_t n);  // p[i] = i
 void launchSeedNext(hipStream_t s, DevState* st, const uint32_t* table, uint32_t tableLen, int bounce);
 void launchBeginView(hipStream_t s, DevState* st, uint32_t number);
 void launchEmit(hipStream_t s, const DevScene& S, DevState* st, uint32_t photons, float invEmitted,

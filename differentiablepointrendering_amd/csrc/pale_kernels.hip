@@ -109,6 +109,9 @@ __global__ void k_fill(uint32_t* p, uint32_t n, uint32_t v) {
 __global__ void k_fillf(float* p, uint32_t n, float v) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
 }
+__global__ void k_iota(uint32_t* p, uint32_t n) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = i;
+}
 
 // Seed schedule of submitPhotonMappingKernel (SyclBridge.cpp:83-143): one
 // draw per emitter launch, per executed bounce and per camera.  `bounce` < 0
@@ -1638,6 +1641,9 @@ void launchFill(hipStream_t s, uint32_t* p, uint32_t n, uint32_t v) {
 }
 void launchFillF(hipStream_t s, float* p, uint32_t n, float v) {
     if (n) k_fillf<<<min(blocks(n, 256), 65535u), 256, 0, s>>>(p, n, v);
+}
+void launchIota(hipStream_t s, uint32_t* p, uint32_t n) {
+    if (n) k_iota<<<min(blocks(n, 256), 65535u), 256, 0, s>>>(p, n);
 }
 void launchSeedNext(hipStream_t s, DevState* st, const uint32_t* table, uint32_t len, int bounce) {
     k_seed_next<<<1, 64, 0, s>>>(st, table, len, bounce);

@@ -12,6 +12,7 @@
 #include <random>
 #include <stdexcept>
 #include <string>
+#include <mutex>
 #include <vector>
 
 #include "pale_abi.h"
@@ -91,6 +92,45 @@ F3 ld3h(const float* p) { return mk3(p[0], p[1], p[2]); }
 
 }  // namespace
 
+// ------------------------------------------- no-allocation adjoint --
+// pale_render_view_backward_noalloc keeps every intermediate of the adjoint
+// of one view (ray queues, pending states, event staging and pool, flags,
+// compaction tiles) in this file-scope device array instead of buffers the
+// library allocates: views of at most kNoAllocMaxPixels pixels.  Each GPU
+// holds its own copy for the life of the process.  In host code `scratch` is
+// only a symbol: its device address comes from hipGetSymbolAddress, resolved
+// once per device (noAllocScratch) and kept in the handle.
+constexpr uint32_t kNoAllocMaxPixels = 65536;  // 256 x 256
+// Per pixel: 9 queue arrays (16 B), 2 flag bytes, pending 16 + 16 + 8 B,
+// 4 staging + 4 pool event arrays (16 B), 4 B of transport order; plus the
+// compaction tiles.
+constexpr size_t kNoAllocScratchBytes =
+    (size_t)kNoAllocMaxPixels * (9 * 16 + 1 + 1 + 16 + 16 + 8 + 4 * 16 + 4 * 16 + 4) + 512;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+std::mutex g_scratchMu;
+constexpr int kMaxDevices = 64;
+void* g_scratchAddr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device (resolved once per
+// device).
+char* noAllocScratch() {
+    int dev = 0;
+    HIPCHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) throw StatusError(PALE_ERR_DEVICE, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratchMu);
+    if (!g_scratchAddr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            throw StatusError(PALE_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratchAddr[dev] = p;
+    }
+    return static_cast<char*>(g_scratchAddr[dev]);
+}
+}  // namespace
+
 struct pale_handle {
     pale_settings S{};
     std::vector<HostMesh> meshes;
@@ -160,6 +200,8 @@ struct pale_handle {
     DevMem debugImg;
     std::vector<size_t> debugOff;
     std::vector<uint32_t> debugCams;
+    // Device address of this GPU's no-allocation scratch array.
+    char* scratchBase = nullptr;
     void syncHostPoints() {
         if (!hostStale) return;
         const size_t n = bp.points.size();
@@ -1413,6 +1455,67 @@ struct pale_handle {
         flush();
         finishBackward();
     }
+    // The adjoint of one view (opacity gradient) with every intermediate in
+    // the static scratch array: the kernels of renderBackwardCams, with the
+    // events of each bounce transported right away, in compaction order (the
+    // pool holds at most one event per ray), so no buffer is allocated.
+    // gradDev (N floats, device) receives the gradient.
+    void renderViewBackwardNoAlloc(const HostCamera& c, uint32_t samples, const float* dlossDev, float* gradDev) {
+        if (!mapValid) throw StatusError(PALE_ERR_INVALID_ARG, "render_view_backward: no photon map; build it first");
+        const uint32_t npx = c.W * c.H, N = (uint32_t)bp.points.size();
+        const size_t M = kNoAllocMaxPixels;
+        char* p = scratchBase;
+        auto take = [&](size_t bytes) {
+            char* r = p;
+            p += (bytes + 255) & ~(size_t)255;
+            return r;
+        };
+        float4* q[9];
+        for (float4*& x : q) x = reinterpret_cast<float4*>(take(M * sizeof(float4)));
+        uint8_t* ext = reinterpret_cast<uint8_t*>(take(M));
+        uint8_t* evf = reinterpret_cast<uint8_t*>(take(M));
+        const PendingArrays pend{reinterpret_cast<float4*>(take(M * sizeof(float4))),
+                                 reinterpret_cast<float4*>(take(M * sizeof(float4))),
+                                 reinterpret_cast<uint2*>(take(M * sizeof(uint2))), nullptr, nullptr};
+        float4* e[8];
+        for (float4*& x : e) x = reinterpret_cast<float4*>(take(M * sizeof(float4)));
+        const EventArrays evS{e[0], e[1], e[2], e[3], nullptr, nullptr};
+        const EventArrays evP{e[4], e[5], e[6], e[7], nullptr, nullptr};
+        uint32_t* order = reinterpret_cast<uint32_t*>(take(M * sizeof(uint32_t)));
+        uint32_t* tl = reinterpret_cast<uint32_t*>(take((M / kTile + 4) * sizeof(uint32_t)));
+        if ((size_t)(p - scratchBase) > kNoAllocScratchBytes)
+            throw StatusError(PALE_ERR_INTERNAL, "no-allocation scratch layout exceeds the array");
+        const RayQueue A{q[0], q[1], q[2]}, B{q[3], q[4], q[5]}, stgq{q[6], q[7], q[8]};
+        DevState* st = this->st();
+        uint32_t* qCount = stField(offsetof(DevState, qCount));
+        uint32_t* extTotal = stField(offsetof(DevState, extTotal));
+        uint32_t* evPool = stField(offsetof(DevState, evPool));
+        // Fill kernels, not hipMemsetAsync: the calls stay capturable.
+        launchFillF(stream, gradDev, N, 0.0f);
+        launchFill(stream, qCount, 1, 0u);
+        launchFill(stream, evPool, 1, 0u);
+        launchIota(stream, order, npx);
+        const DevCamera dc = devCam(c);
+        const float invSpp = 1.0f / (float)samples;
+        for (uint32_t spp = 0; spp < samples; ++spp) {
+            RayQueue in = A, out = B;
+            launchAdjRaygen(stream, dc, dlossDev, in, pend, st, adjointSeed(spp));
+            TraceParams t = tp(invSpp, nullptr, nullptr, adjointSeed(spp));
+            t.fullGradients = 0;
+            t.debug = 0;
+            for (uint32_t b = 0; b < S.adjoint_bounces; ++b) {
+                launchAdjTrace(stream, scene, st, npx, b, in, stgq, ext, evS, evf, pend, t, false);
+                launchCompactEvents(stream, evf, qCount, npx, tl, st, evS, evP, npx);
+                launchAdjTransport(stream, scene, devGrid(), st, npx, evP, order, gradDev, nullptr, nullptr, false);
+                launchFill(stream, evPool, 1, 0u);
+                launchCompactRays(stream, ext, qCount, npx, tl, extTotal, stgq, out, (uint32_t)M);
+                launchSetQueue(stream, st, extTotal);
+                std::swap(in, out);
+            }
+        }
+        HIPCHECK(hipStreamSynchronize(stream));
+        HIPCHECK(hipGetLastError());
+    }
 };
 
 namespace {
@@ -1548,6 +1651,14 @@ int pale_create(const pale_scene_desc* scene, const pale_settings* settings, pal
         for (uint32_t i = 0; i < scene->camera_count; ++i) h->cams.push_back(hostCamera(scene->cameras[i]));
         h->asset = toAsset(scene->points);
         HIPCHECK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+        // The no-allocation scratch of this device, resolved at set-up; a
+        // failure here is reported by the no-allocation entry point (which
+        // resolves it again), not by every renderer.
+        try {
+            h->scratchBase = noAllocScratch();
+        } catch (const StatusError&) {
+            h->scratchBase = nullptr;
+        }
         HIPCHECK(hipEventCreate(&h->ev0));
         HIPCHECK(hipEventCreate(&h->ev1));
         h->state.reserve(sizeof(DevState));
@@ -1951,6 +2062,22 @@ int pale_render_view_backward_device(pale_handle* h, const pale_camera_desc* cam
         else
             h->renderBackwardCams({&c}, {dloss_rgb_dev}, true, spp);
         copyGradsDevice(h, grads_dev);
+    });
+}
+
+int pale_render_view_backward_noalloc(pale_handle* h, const pale_camera_desc* camera, uint32_t spp,
+                                      const float* dloss_rgb_dev, float* opacity_grad_dev) {
+    if (!h || !camera || !dloss_rgb_dev || !opacity_grad_dev) return fail(PALE_ERR_INVALID_ARG, "null argument");
+    return guarded([&] {
+        if (spp == 0) throw StatusError(PALE_ERR_INVALID_ARG, "render_view_backward_noalloc: spp must be >= 1");
+        const HostCamera c = hostCamera(*camera);
+        const uint64_t npx = (uint64_t)c.W * c.H;
+        if (npx == 0 || npx > kNoAllocMaxPixels)
+            throw StatusError(PALE_ERR_SIZE, "render_view_backward_noalloc: the view has " + std::to_string(npx) +
+                                                 " pixels; at most " + std::to_string(kNoAllocMaxPixels) +
+                                                 " (256 x 256) fit the scratch array");
+        if (!h->scratchBase) h->scratchBase = noAllocScratch();
+        h->renderViewBackwardNoAlloc(c, spp, dloss_rgb_dev, opacity_grad_dev);
     });
 }
 

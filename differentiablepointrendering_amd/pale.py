@@ -413,6 +413,27 @@ class Renderer:
                                               C.c_void_p(ldr.data_ptr()), C.c_void_p(raw.data_ptr())))
         return {"ldr": ldr, "raw": raw}
 
+    def render_view_backward_noalloc(self, camera: _abi.Camera, spp: int, dloss, opacity_grad) -> None:
+        """Opacity gradient of render_view_backward into the caller's device
+        tensor `opacity_grad` ((N,) float32), from an (H,W,3) float32 dL/dI
+        device tensor, with no device allocation anywhere on the way (the
+        library's static scratch array holds the intermediates; views of at
+        most 256 x 256 pixels)."""
+        n = self.point_count
+        if tuple(dloss.shape) != (camera.height, camera.width, 3) or not dloss.is_contiguous():
+            raise RuntimeError(f"render_view_backward_noalloc: dL/dI must be a contiguous ({camera.height},"
+                               f"{camera.width},3) tensor")
+        if tuple(opacity_grad.shape) != (n,) or not opacity_grad.is_contiguous():
+            raise RuntimeError(f"render_view_backward_noalloc: the gradient must be a contiguous ({n},) tensor")
+        if not (dloss.is_cuda and opacity_grad.is_cuda):
+            raise RuntimeError("render_view_backward_noalloc: tensors must be on the GPU")
+        d = _abi.fill_camera_desc(_abi.CameraDesc(), camera)
+        import torch
+
+        torch.cuda.current_stream().synchronize()
+        check(self._L.pale_render_view_backward_noalloc(self._h, C.byref(d), int(spp), C.c_void_p(dloss.data_ptr()),
+                                                        C.c_void_p(opacity_grad.data_ptr())))
+
     def render_view_backward(self, camera: _abi.Camera, spp: int, dloss_rgb: np.ndarray) -> dict:
         """Adjoint of one view: spp adjoint samples of dL/dI (H,W,3) through
         the current photon map; the gradient dict of render_backward."""

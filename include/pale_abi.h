@@ -300,6 +300,22 @@ int pale_render_backward_device(pale_handle* h, uint32_t n,
                                 const float* const* dloss_rgb_dev,
                                 float* const* grads_dev);
 
+/* No counterpart in the reference: pale_render_view_backward_device's
+ * opacity gradient for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  It allocates nothing: every intermediate of the adjoint lives
+ * in a static device array of the library (about 21 MB, one per GPU, held
+ * until the process ends), so views are limited to 65536 pixels (256 x 256);
+ * larger views return PALE_ERR_SIZE.  dloss_rgb_dev: (H,W,3) floats,
+ * opacity_grad_dev: N floats (BVH order), both device memory.  Other
+ * gradients are not computed.  All calls on one GPU share the array: calls
+ * that overlap on one GPU (from several threads or handles) race. */
+int pale_render_view_backward_noalloc(pale_handle* h,
+                                      const pale_camera_desc* camera,
+                                      uint32_t spp,
+                                      const float* dloss_rgb_dev,
+                                      float* opacity_grad_dev);
+
 /* Opacity of all n surfels (BVH order) from device memory, without a BVH
  * rebuild (apply_point_optimization semantics for one parameter). */
 int pale_set_opacity_device(pale_handle* h, const float* opacity_dev,
