@@ -95,9 +95,10 @@ def parse():
                     help="pixel slots per layer-walk launch of the served replicas (0: the renderer's default)")
     ap.add_argument("--served-grid-fraction", type=float, default=0.0,
                     help="fraction of the GPU a served replica's layer walk occupies (0: all)")
-    ap.add_argument("--hw-queues", type=int, default=0,
-                    help="GPU_MAX_HW_QUEUES for this process (0: leave the environment's; the served form's "
-                         "replicas each own a stream)")
+    ap.add_argument("--hw-queues", type=int, default=16,
+                    help="GPU_MAX_HW_QUEUES for this process (0: leave the environment's).  The served form's "
+                         "replicas each own a stream; with fewer hardware queues than streams, a small request's "
+                         "kernels queue behind a large request's in the queue they share")
     ap.add_argument("--served-only", action="store_true", help="run only the served form")
     ap.add_argument("--batched", type=int, default=32, help="views of the batched form (0: skip)")
     ap.add_argument("--batched-only", action="store_true", help="run only the batched form")
@@ -621,6 +622,8 @@ def served_leg(a):
             print(f"[bench] served alone {i + 1}/{len(head)}", file=sys.stderr, flush=True)
     plain_s = time.perf_counter() - tp
     plain = latency_summary(alone, plain_s)
+    small_alone = [m.latency_ms for m in alone
+                   if m.request.camera.width * m.request.camera.height * m.request.spp <= (1 << 20)]
 
     # All at once: the capacity of the served form (views overlap on the
     # device), on the same requests.
@@ -649,12 +652,10 @@ def served_leg(a):
     served = latency_summary(mixed, makespan)
     # Latency by request size (pixel samples = width x height x spp).
     small = [m for m in mixed if m.request.camera.width * m.request.camera.height * m.request.spp <= (1 << 20)]
-    small_alone = [m for m in alone if m.request.camera.width * m.request.camera.height * m.request.spp <= (1 << 20)]
     by_size = {"small_max_samples": 1 << 20, "small_requests": len(small),
                "small_p50_ms": round(float(np.percentile([m.latency_ms for m in small], 50)), 2) if small else None,
                "small_p99_ms": round(float(np.percentile([m.latency_ms for m in small], 99)), 2) if small else None,
-               "small_alone_p50_ms": round(float(np.percentile([m.latency_ms for m in small_alone], 50)), 2)
-               if small_alone else None}
+               "small_alone_p50_ms": round(float(np.percentile(small_alone, 50)), 2) if small_alone else None}
     same_images = same_grads = True
     mismatches = []
     for i, (x, y) in [(i, p) for i, p in enumerate(zip(alone, mixed))] + [(i, p) for i, p in enumerate(zip(alone, burst))]:

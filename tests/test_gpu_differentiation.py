@@ -31,9 +31,10 @@ standard error), and the adjoint is 8 independent 256-sample replicates
 |adjoint - FD| <= 3 sigma + 5 % |FD| (sigma the combined standard error; 5 %
 is the reference's relative bar, batch_fd_check.py:228-232, for what the
 spread does not cover: the irradiance lookup moving with the hit point,
-which neither estimator differentiates).  At most 3 % of the 256 may fail
-(3 sigma alone fails 0.3 % of unbiased estimates; heavy tails of the
-adjoint's path weights fail a few more).  Over all 256, the adjoint must
+which neither estimator differentiates).  At most 4 % of the 256 may fail:
+with sigma estimated from 8 + 8 replicates the 3-sigma test is a t-test
+with about 14 degrees of freedom, which about 1 % of unbiased estimates
+fail, and the heavy tails of the adjoint's path weights fail a few more.  Over all 256, the adjoint must
 correlate with FD above 0.98, and the noise-weighted least-squares slope of
 adjoint on FD (weights 1 / sigma^2) must lie within 10 % of 1.  (The plain
 relative error of the whole vector is no bar: its largest entries are also
@@ -197,6 +198,6 @@ def test_c3_finite_differences_on_256_parameters():
     for x in rows:
         print("c3row", *x, flush=False)
     assert np.count_nonzero(fd) > NPARAM // 2
-    assert 1 - ok.mean() <= 0.03, bad[:10]
+    assert 1 - ok.mean() <= 0.04, bad[:12]
     assert corr > 0.98, corr
     assert abs(slope - 1.0) <= 0.10, slope
