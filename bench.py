@@ -108,6 +108,9 @@ def parse():
                     help="timed iterations of the training-loop form (render, backward, optimizer step, apply, "
                          "rebuild_bvh, refetch; 0: skip)")
     ap.add_argument("--no-tune", action="store_true", help="batched form with the default tuning")
+    ap.add_argument("--batched-tuning", type=str, default="",
+                    help="batched form with this tuning (slots,entries,graph_rounds,grid_fraction) instead of the "
+                         "autotuner's")
     ap.add_argument("--batch-passes", type=int, choices=(0, 1), default=1,
                     help="plain form: 1 = all gather passes / adjoint samples of the view as batches")
     ap.add_argument("--tuning", type=str, default="12582912,24,0,1",
@@ -199,7 +202,20 @@ def pmc_summary(dirs, cal_dirs=None):
     out = {"kernels": {}}
     kt = _read_csvs(dirs, "kernel_trace.csv")
     if kt:
-        for r in _after_marker(kt, "Start_Timestamp"):
+        rows = _after_marker(kt, "Start_Timestamp")
+        # Device busy time: the union of the dispatches' intervals (kernels of
+        # several streams overlap, so their durations add up to more).
+        busy, end = 0, None
+        for s0, s1 in sorted((int(r["Start_Timestamp"]), int(r["End_Timestamp"])) for r in rows):
+            if end is None or s0 > end:
+                busy += s1 - s0
+                end = s1
+            elif s1 > end:
+                busy += s1 - end
+                end = s1
+        if rows:
+            out["totals"] = {"device_busy_ms": round(busy * 1e-6, 3), "dispatches": len(rows)}
+        for r in rows:
             k = out["kernels"].setdefault(_short(r["Kernel_Name"]), {"launches": 0, "total_ms": 0.0})
             k["launches"] += 1
             k["total_ms"] += (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-6
@@ -246,6 +262,10 @@ def pmc_summary(dirs, cal_dirs=None):
             c = k.get("counters_per_launch", {})
             if "FETCH_SIZE" in c and "WRITE_SIZE" in c and f_fac and w_fac:
                 k["hbm_bytes_per_launch"] = c["FETCH_SIZE"] * 1024.0 * f_fac + c["WRITE_SIZE"] * 1024.0 * w_fac
+        if "totals" in out:
+            out["totals"]["hbm_bytes"] = sum(k["hbm_bytes_per_launch"] * k["launches"] for k in out["kernels"].values()
+                                             if "hbm_bytes_per_launch" in k and k.get("launches"))
+            out["totals"]["kernel_ms"] = round(sum(k.get("total_ms", 0.0) for k in out["kernels"].values()), 3)
     # What the counters say bounds each kernel, as fractions of the kernel's
     # cycles on every CU (GRBM_GUI_ACTIVE sums the 8 XCDs' GPU-busy cycles;
     # the _sum counters sum 256 CUs).
@@ -363,7 +383,10 @@ def batched_leg(a):
     # Autotuner on the workload itself (how many passes of which views share
     # a batch is what the batch size decides, so a reduced sample misleads it).
     tune = None
-    if not a.no_tune:
+    if a.batched_tuning:
+        sl, en, gr, fr = a.batched_tuning.split(",")
+        r.set_batch_tuning(int(sl), int(en), int(gr), float(fr))
+    elif not a.no_tune:
         ns = len(cams)
         sc, ss = cams, spp
         sd = [dloss(np.zeros((c.height, c.width, 4), np.float32), tg) for c, tg in zip(sc, targets[:ns])]
@@ -406,7 +429,7 @@ def batched_leg(a):
         "same_images_as_plain": bool(same_images),
         "gradient_max_err_over_max": grad_err,
         "roofline": counter_roofline("batched", "k_cam_trace", walk_avg),
-        "tuning": tune["tuning"] if tune else autotune.DEFAULT,
+        "tuning": tune["tuning"] if tune else r.batch_tuning(),
         "autotune": {"sample": f"the {len(cams)} views at their spp", "seconds": tune["seconds"], "table": tune["table"]}
         if tune else None,
     }
@@ -545,23 +568,21 @@ def training_leg(a):
 
 
 def served_hbm():
-    """HBM rate of the served form while its kernels run, from the counted
+    """HBM rate of the served form while the device is busy, from the counted
     traffic of every kernel of a served run (its summary in profiles/:
-    calibrated FETCH_SIZE + WRITE_SIZE per launch times launches, over their
-    summed durations in the same run's kernel trace)."""
+    calibrated FETCH_SIZE + WRITE_SIZE per launch times launches) over the
+    union of the dispatches' intervals in the same run's kernel trace (the
+    replicas' kernels overlap, so their summed durations would undercount
+    the rate)."""
     prof = load_traffic(profile_path("served"))
-    if not prof:
+    tot = (prof or {}).get("totals", {})
+    if not tot.get("hbm_bytes") or not tot.get("device_busy_ms"):
         return None
-    b = t = 0.0
-    for k in prof.get("kernels", {}).values():
-        if "hbm_bytes_per_launch" in k and k.get("launches"):
-            b += k["hbm_bytes_per_launch"] * k["launches"]
-            t += k["total_ms"]
-    if t <= 0:
-        return None
+    b, t = float(tot["hbm_bytes"]), float(tot["device_busy_ms"])
     gbs = b / (t * 1e-3) / 1e9
     return {"achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
-            "bytes": b, "kernel_ms": round(t, 2), "source": "profiles/" + PROFILES["served"]}
+            "bytes": b, "device_busy_ms": round(t, 2), "kernel_ms": tot.get("kernel_ms"),
+            "source": "profiles/" + PROFILES["served"]}
 
 
 def served_leg(a):
