@@ -413,6 +413,7 @@ def batched_leg(a):
     for k in keys:
         scale = max(float(np.abs(p_grad[k]).max()), 1e-30)
         grad_err[k] = float(np.abs(g[k] - p_grad[k]).max() / scale)
+    used_tuning = r.batch_tuning()
     r.close()
     res = {
         "workload": f"{len(cams)} views (eyes on the radius-4 sphere, 128x96..1024x768 log-uniform area, spp "
@@ -429,7 +430,7 @@ def batched_leg(a):
         "same_images_as_plain": bool(same_images),
         "gradient_max_err_over_max": grad_err,
         "roofline": counter_roofline("batched", "k_cam_trace", walk_avg),
-        "tuning": tune["tuning"] if tune else r.batch_tuning(),
+        "tuning": tune["tuning"] if tune else used_tuning,
         "autotune": {"sample": f"the {len(cams)} views at their spp", "seconds": tune["seconds"], "table": tune["table"]}
         if tune else None,
     }
@@ -510,9 +511,9 @@ def training_leg(a):
     raw image, render_backward with every gradient, an Adam step on the
     seven parameter arrays (numpy here, torch there), apply_point_
     optimization, rebuild_bvh and the parameters fetched again.  Wall time
-    of each phase; the BVH rebuild is host work (the median split with
-    std::nth_element, whose element order the surfel ids depend on) plus the
-    upload of the rebuilt arrays."""
+    of each phase; rebuild_bvh is the whole scene build (the surfel BVH on
+    the GPU, in std::nth_element's element order, with the meshes, TLAS and
+    host mirrors around it) and its upload."""
     from differentiablepointrendering_amd import scene
     from differentiablepointrendering_amd.pale import Renderer
 
@@ -551,6 +552,7 @@ def training_leg(a):
         t = time.perf_counter()
         r.rebuild_bvh()
         ph["rebuild_bvh"] = time.perf_counter() - t
+        on_device = bool(r.stats()["bvh_on_device"])
         t = time.perf_counter()
         params = r.get_point_parameters()
         ph["refetch"] = time.perf_counter() - t
@@ -564,7 +566,7 @@ def training_leg(a):
     return {"workload": f"{len(spec.points['position'])} clustered surfels, 800x800, 16 spp, depth 6: one "
                         f"training.py iteration (host arrays, as the reference's drivers)",
             "iterations": len(phases), "ms": out, "total_ms": round(sum(out.values()), 2),
-            "host_threads": os.cpu_count()}
+            "bvh_rebuilt_on": "GPU" if on_device else "host", "host_threads": os.cpu_count()}
 
 
 def served_hbm():
@@ -867,7 +869,9 @@ def run_plain(a, spec, settings, target, steps, warmup, form, count=True):
         sl, en, gr, fr = a.tuning.split(",")
         r.set_batch_tuning(int(sl), int(en), int(gr), float(fr))
     build_s = time.time() - t_build
-    print(f"[bench] {form}: renderer ready (host BVH build {build_s:.2f}s)", file=sys.stderr, flush=True)
+    bvh = r.stats()
+    print(f"[bench] {form}: renderer ready in {build_s:.2f}s (BVH build {bvh['bvh_build_ms']:.0f} ms, "
+          f"{'GPU' if bvh['bvh_on_device'] else 'host'})", file=sys.stderr, flush=True)
 
     def step():
         img = r.render_forward()
@@ -911,7 +915,9 @@ def run_plain(a, spec, settings, target, steps, warmup, form, count=True):
                                                     "gather_ms", "adjoint_trace_ms", "adjoint_transport_ms",
                                                     "forward_ms", "backward_ms")},
         "camera_rounds": last["camera_rounds"],
-        "host_bvh_build_s": round(build_s, 3),
+        "renderer_setup_s": round(build_s, 3),
+        "scene_build_ms": round(bvh["bvh_build_ms"], 1),
+        "bvh_on_device": bool(bvh["bvh_on_device"]),
         "timed_wall_s": round(total_wall, 3),
         "batch_tuning": r.batch_tuning(),
     }
@@ -1058,7 +1064,7 @@ def main():
         },
     }
     for k in ("queries_per_step", "ms_per_step_wall", "mrays_per_s_wall", "camera_samples_per_s", "phase_ms",
-              "camera_rounds", "host_bvh_build_s", "timed_wall_s", "roofline", "kernel_rooflines", "work_per_step",
+              "camera_rounds", "renderer_setup_s", "scene_build_ms", "bvh_on_device", "timed_wall_s", "roofline", "kernel_rooflines", "work_per_step",
               "counted_step_matches_timed", "algorithmic_step"):
         if k in p:
             out[k] = p[k]

@@ -111,6 +111,7 @@ class Settings(C.Structure):
         ("batch_passes", C.c_int32),
         ("adjoint_decorrelate", C.c_int32),
         ("adjoint_seed", C.c_uint32),
+        ("bvh_build", C.c_uint32),
     ]
 
 
@@ -138,10 +139,13 @@ class Stats(C.Structure):
         ("camera_trace_launches", C.c_uint32),
         ("camera_rounds", C.c_uint32),
         ("work", (C.c_uint64 * 7) * 3),
+        ("bvh_build_ms", C.c_double),
+        ("bvh_on_device", C.c_uint32),
+        ("bvh_pad", C.c_uint32),
     ]
 
     def as_dict(self) -> dict:
-        d = {name: getattr(self, name) for name, _ in self._fields_ if name != "work"}
+        d = {name: getattr(self, name) for name, _ in self._fields_ if name not in ("work", "bvh_pad")}
         for ci, cname in enumerate(WORK_CLASSES):
             d[f"work_{cname}"] = {u: int(self.work[ci][k]) for k, u in enumerate(WORK_UNITS)}
         return d
@@ -174,6 +178,7 @@ DEFAULT_SETTINGS = dict(
     batch_passes=1,
     adjoint_decorrelate=0,
     adjoint_seed=42,
+    bvh_build=0,
 )
 
 

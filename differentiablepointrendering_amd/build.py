@@ -21,7 +21,7 @@ CSRC = os.path.join(HERE, "csrc")
 INCLUDE = os.path.join(ROOT, "include")
 LIB = os.path.join(HERE, "libpale_hip.so")
 BUILD_DIR = os.path.join(HERE, "_build")
-SOURCES = ["pale_scene.hip", "pale_kernels.hip", "pale_render.hip"]
+SOURCES = ["pale_scene.hip", "pale_kernels.hip", "pale_bvh.hip", "pale_render.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"

@@ -293,6 +293,26 @@ struct DevPointArrays {
 // Surfel records from device parameter arrays, and their raw copy (16 floats
 // per surfel: p, t_u, t_v, s_u, s_v, albedo, opacity, beta).
 void launchApplyPoints(hipStream_t s, SurfelGeo* geo, SurfelShade* shade, float* raw, DevPointArrays a, uint32_t n);
+
+// Device build of the surfel BVH (pale_bvh.hip).  boxes / refBox: 6 floats
+// (min xyz, max xyz) per surfel / per reference node; acc: 6 order-
+// preserving integers per node of a fitted level; uint4 tables as described
+// at each kernel.
+struct HostPoint;
+void launchBvhPrims(hipStream_t s, const HostPoint* pts, uint32_t n, float* kx, float* ky, float* kz, float* boxes,
+                    uint32_t* ord, uint32_t* flags);
+void launchBvhInitFit(hipStream_t s, uint32_t* acc, uint32_t nNodes);
+void launchBvhFit(hipStream_t s, const uint4* chunks, uint32_t nChunks, const uint32_t* ord, const float* boxes,
+                  uint32_t* acc);
+void launchBvhSelect(hipStream_t s, const uint4* nodes, uint32_t nNodes, const uint32_t* acc, const float* kx,
+                     const float* ky, const float* kz, float* kv, uint32_t* ord, uint32_t* lpos, uint32_t* rpos,
+                     float* refBox);
+void launchBvhSubtrees(hipStream_t s, const uint4* roots, uint32_t nRoots, const uint32_t* internalOf, const float* kx,
+                       const float* ky, const float* kz, const float* boxes, float* kv, uint32_t* ord, float* refBox);
+void launchBvhWide(hipStream_t s, const uint4* wide, uint32_t nWide, const float* refBox, uint32_t base,
+                   WideNode* out);
+void launchBvhRecords(hipStream_t s, const HostPoint* pts, const uint32_t* ord, uint32_t n, SurfelGeo* geo,
+                      SurfelShade* shade);
 void launchGatherPoints(hipStream_t s, const DevGrid& G, uint32_t n, const float* q, const float* nrm, float* out);
 void launchTraceRays(hipStream_t s, const DevScene& S, uint32_t n, const float* o, const float* d, int32_t* inst,
                      int32_t* prim, float* t, float* pos, float* alpha);

@@ -8,6 +8,7 @@
 // the host synchronises once per call, when results are copied out.
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cstring>
 #include <random>
 #include <stdexcept>
@@ -92,6 +93,128 @@ F3 ld3h(const float* p) { return mk3(p[0], p[1], p[2]); }
 
 }  // namespace
 
+// ---------------------------------------------- device BVH build --
+// The surfel BLAS built on the device (kernels in pale_bvh.hip), equal node
+// for node and slot for slot to the host build.  The tree's shape depends
+// only on the surfel count, so it is computed on the host once per count:
+// every node's range and reference index, the nodes of each level that
+// split with a whole workgroup, the roots of the subtrees one thread builds,
+// and the wide-node table.  Per build the device computes the boxes, the
+// order of the surfels and, from them, the surfel records and wide nodes.
+class DeviceCloudBuild final : public CloudBuild {
+   public:
+    static constexpr uint32_t kSubtree = 64;      // subtrees this small: one thread each
+    static constexpr uint32_t kFitChunk = 16384;  // surfels per fit workgroup
+
+    hipStream_t stream = nullptr;
+    uint32_t minCount = 65536;  // smaller clouds are built on the host
+    DevMem* geo = nullptr;      // the renderer's surfel records and wide nodes
+    DevMem* shade = nullptr;
+    DevMem* nodes = nullptr;
+
+    bool build(const std::vector<HostPoint>& pts, uint32_t wideBase, std::vector<RefNode>& out,
+               std::vector<uint32_t>& perm, uint32_t& depthOut) override {
+        const uint32_t n = (uint32_t)pts.size();
+        if (n < std::max<uint32_t>(minCount, kMaxLeafPrims + 1)) return false;
+        if (n != shapeN) shape(n);
+        // stage the points in pinned memory (host threads), then one copy
+        pinned(stage, (size_t)n * sizeof(HostPoint));
+        HostPoint* hp = stage.as<HostPoint>();
+        constexpr size_t kChunk = 1u << 15;
+        parallelFor((n + kChunk - 1) / kChunk, [&](size_t c) {
+            const size_t a = c * kChunk, b = std::min<size_t>(n, a + kChunk);
+            std::memcpy(hp + a, pts.data() + a, (b - a) * sizeof(HostPoint));
+        });
+        dPts.reserve((size_t)n * sizeof(HostPoint));
+        HIPCHECK(hipMemcpyAsync(dPts.p, hp, (size_t)n * sizeof(HostPoint), hipMemcpyHostToDevice, stream));
+        for (DevMem* m : {&kx, &ky, &kz, &kv}) m->reserve((size_t)n * sizeof(float));
+        for (DevMem* m : {&ord, &lpos, &rpos}) m->reserve((size_t)n * sizeof(uint32_t));
+        boxes.reserve((size_t)n * 6 * sizeof(float));
+        refBox.reserve(sh.topo.size() * 6 * sizeof(float));
+        acc.reserve(std::max<size_t>(1, sh.big.size()) * 6 * sizeof(uint32_t));
+        flags.reserve(sizeof(uint32_t));
+        geo->reserve((size_t)n * sizeof(SurfelGeo));
+        shade->reserve((size_t)n * sizeof(SurfelShade));
+        nodes->reserve((size_t)(wideBase + sh.wide.size()) * sizeof(WideNode));
+        launchFill(stream, flags.as<uint32_t>(), 1, 0u);
+        launchBvhPrims(stream, dPts.as<HostPoint>(), n, kx.as<float>(), ky.as<float>(), kz.as<float>(),
+                       boxes.as<float>(), ord.as<uint32_t>(), flags.as<uint32_t>());
+        launchBvhInitFit(stream, acc.as<uint32_t>(), (uint32_t)sh.big.size());
+        for (size_t d = 0; d + 1 < sh.levelBig.size(); ++d) {
+            launchBvhFit(stream, dChunks.as<uint4>() + sh.levelChunk[d], sh.levelChunk[d + 1] - sh.levelChunk[d],
+                         ord.as<uint32_t>(), boxes.as<float>(), acc.as<uint32_t>());
+            launchBvhSelect(stream, dBig.as<uint4>() + sh.levelBig[d], sh.levelBig[d + 1] - sh.levelBig[d],
+                            acc.as<uint32_t>(), kx.as<float>(), ky.as<float>(), kz.as<float>(), kv.as<float>(),
+                            ord.as<uint32_t>(), lpos.as<uint32_t>(), rpos.as<uint32_t>(), refBox.as<float>());
+        }
+        launchBvhSubtrees(stream, dRoots.as<uint4>(), (uint32_t)sh.roots.size(), dInternal.as<uint32_t>(),
+                          kx.as<float>(), ky.as<float>(), kz.as<float>(), boxes.as<float>(), kv.as<float>(),
+                          ord.as<uint32_t>(), refBox.as<float>());
+        launchBvhWide(stream, dWide.as<uint4>(), (uint32_t)sh.wide.size(), refBox.as<float>(), wideBase,
+                      nodes->as<WideNode>());
+        launchBvhRecords(stream, dPts.as<HostPoint>(), ord.as<uint32_t>(), n, geo->as<SurfelGeo>(),
+                         shade->as<SurfelShade>());
+        HIPCHECK(hipGetLastError());
+        const size_t permB = (size_t)n * sizeof(uint32_t), boxB = sh.topo.size() * 6 * sizeof(float);
+        pinned(back, permB + boxB + 16);
+        char* hb = back.as<char>();
+        HIPCHECK(hipMemcpyAsync(hb, ord.p, permB, hipMemcpyDeviceToHost, stream));
+        HIPCHECK(hipMemcpyAsync(hb + permB, refBox.p, boxB, hipMemcpyDeviceToHost, stream));
+        HIPCHECK(hipMemcpyAsync(hb + permB + boxB, flags.p, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+        HIPCHECK(hipStreamSynchronize(stream));
+        uint32_t fl;
+        std::memcpy(&fl, hb + permB + boxB, sizeof(fl));
+        if (fl) return false;  // non-finite or -0 box coordinates: the host builds this scene
+        out = sh.topo;
+        perm.resize(n);
+        const uint32_t* hperm = reinterpret_cast<const uint32_t*>(hb);
+        const float* hbox = reinterpret_cast<const float*>(hb + permB);
+        parallelFor((n + kChunk - 1) / kChunk, [&](size_t c) {
+            const size_t a = c * kChunk, b = std::min<size_t>(n, a + kChunk);
+            std::memcpy(perm.data() + a, hperm + a, (b - a) * sizeof(uint32_t));
+        });
+        for (size_t i = 0; i < out.size(); ++i) {
+            const float* b = hbox + 6 * i;
+            out[i].mn = mk3(b[0], b[1], b[2]);
+            out[i].mx = mk3(b[3], b[4], b[5]);
+        }
+        depthOut = sh.depth;
+        return true;
+    }
+
+   private:
+    // shape of the tree for shapeN surfels
+    uint32_t shapeN = kInvalid;
+    CloudShape sh;
+    DevMem dBig, dChunks, dRoots, dInternal, dWide;
+    // per build
+    DevMem dPts, kx, ky, kz, kv, ord, lpos, rpos, boxes, refBox, acc, flags;
+    DevMem stage, back;  // pinned host staging
+
+    static void pinned(DevMem& m, size_t b) {
+        if (m.p && m.bytes >= b) return;
+        m.release();
+        HIPCHECK(hipHostMalloc(&m.p, std::max<size_t>(b, 16), hipHostMallocDefault));
+        m.bytes = std::max<size_t>(b, 16);
+        m.pinned = true;
+    }
+
+    void shape(uint32_t n) {
+        sh = cloudShape(n, kSubtree, kFitChunk);
+        auto put = [&](DevMem& m, const auto& v) {
+            m.reserve(std::max<size_t>(1, v.size()) * sizeof(v[0]));
+            if (!v.empty()) HIPCHECK(hipMemcpyAsync(m.p, v.data(), v.size() * sizeof(v[0]), hipMemcpyHostToDevice, stream));
+        };
+        put(dBig, sh.big);
+        put(dChunks, sh.chunks);
+        put(dRoots, sh.roots);
+        put(dInternal, sh.internalOf);
+        put(dWide, sh.wide);
+        HIPCHECK(hipStreamSynchronize(stream));
+        shapeN = n;
+    }
+};
+
 // ------------------------------------------- no-allocation adjoint --
 // pale_render_view_backward_noalloc keeps every intermediate of the adjoint
 // of one view (ray queues, pending states, event staging and pool, flags,
@@ -138,6 +261,7 @@ struct pale_handle {
     std::vector<AssetPoint> asset;
     BuildProducts bp;
     DeviceImage img;
+    DeviceCloudBuild cloudBuild;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
@@ -349,9 +473,11 @@ struct pale_handle {
     // Upload of the build products (SceneUpload::allocateAndUpload,
     // SceneUpload.ixx:321-476).
     void uploadScene() {
-        upload(stream, geo, img.surfGeo);
-        upload(stream, shade, img.surfShade);
-        upload(stream, nodes, img.nodes);
+        if (!bp.cloudOnDevice) {
+            upload(stream, geo, img.surfGeo);
+            upload(stream, shade, img.surfShade);
+        }
+        upload(stream, nodes, img.nodes);  // after a device build: the meshes' part
         upload(stream, tris, img.tris);
         upload(stream, inst, img.instances);
         upload(stream, tlas, img.tlas);
@@ -382,10 +508,18 @@ struct pale_handle {
     }
 
     void rebuild() {
-        bp = buildScene(meshes, asset);
+        const auto t0 = std::chrono::steady_clock::now();
+        cloudBuild.stream = stream;
+        cloudBuild.geo = &geo;
+        cloudBuild.shade = &shade;
+        cloudBuild.nodes = &nodes;
+        cloudBuild.minCount = S.bvh_build == 1 ? 0u : 65536u;
+        bp = buildScene(meshes, asset, S.bvh_build == 2 ? nullptr : &cloudBuild);
         img = makeDeviceImage(bp, meshes);
         uploadScene();
         setScene();
+        stats.bvh_on_device = bp.cloudOnDevice ? 1u : 0u;
+        stats.bvh_build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
 
     // PathTracer::setScene (PathTracer.cpp:22-53): queue capacity, photon map
@@ -1603,6 +1737,7 @@ void pale_default_settings(pale_settings* out) {
     s.batch_passes = 1;
     s.adjoint_decorrelate = 0;
     s.adjoint_seed = 42;
+    s.bvh_build = 0;
     *out = s;
 }
 
@@ -1634,6 +1769,7 @@ int pale_create(const pale_scene_desc* scene, const pale_settings* settings, pal
         // that stage (render_target.py asks for adjoint_passes = 0).
         h->S = *settings;
         if (!(h->S.gather_radius > 0.0f)) throw StatusError(PALE_ERR_INVALID_ARG, "gather_radius must be > 0");
+        if (h->S.bvh_build > 2) throw StatusError(PALE_ERR_INVALID_ARG, "bvh_build must be 0, 1 or 2");
         for (uint32_t i = 0; i < scene->mesh_count; ++i) {
             const pale_mesh_desc& md = scene->meshes[i];
             HostMesh m;
@@ -1879,10 +2015,13 @@ int pale_set_opacity_device(pale_handle* h, const float* opacity_dev, uint32_t n
         HIPCHECK(hipMemcpyAsync(v.data(), opacity_dev, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, h->stream));
         HIPCHECK(hipStreamSynchronize(h->stream));
         HIPCHECK(hipGetLastError());
+        // (the host image is rebuilt before any upload; after a device BVH
+        // build it is not kept)
+        const bool img = h->img.surfShade.size() == n;
         for (uint32_t i = 0; i < n; ++i) {
             h->bp.points[i].opacity = v[i];
             h->asset[i].opacity = v[i];
-            h->img.surfShade[i].opacity = v[i];
+            if (img) h->img.surfShade[i].opacity = v[i];
         }
     });
 }

@@ -1,8 +1,11 @@
 // pale_scene.h -- host-side scene model and build products.
 #pragma once
 
+#include <algorithm>
+#include <atomic>
 #include <cstdint>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "pale_common.h"
@@ -25,6 +28,49 @@ struct HostPoint {
     F3 albedo;
     float alpha_t, alpha_r, opacity, beta, shape;
 };
+
+// The per-surfel records of the host scene build and of the device build,
+// one definition for both so that they agree bit for bit.
+
+// surfelObjectAabbBeta (SceneBuild.cpp:395-444): the surfel's box.
+PALE_HD void surfelBoxOf(const HostPoint& s, F3& mn, F3& mx) {
+    const F3 tu = normalize(s.tu);
+    const F3 tv = normalize(s.tv);
+    const F3 n = normalize(cross(tu, tv));
+    const float rU = 1.0001f * s.su;
+    const float rV = 1.0001f * s.sv;
+    const float cu[3] = {tu.x, tu.y, tu.z}, cv[3] = {tv.x, tv.y, tv.z}, cn[3] = {n.x, n.y, n.z};
+    float h[3];
+    for (int a = 0; a < 3; ++a)
+        h[a] = sqrtf((rU * cu[a]) * (rU * cu[a]) + (rV * cv[a]) * (rV * cv[a])) + 0.001f * fabsf(cn[a]);
+    const F3 hv = mk3(h[0], h[1], h[2]);
+    mn = s.p - hv;
+    mx = s.p + hv;
+}
+
+// Geometry record: intersectSurfel's frame (KernelHelpers.h:646-648) and
+// the opacity exponent 4 e^beta.
+PALE_HD SurfelGeo surfelGeoOf(const HostPoint& s) {
+    const F3 u = normalize(s.tu);
+    const F3 v = normalize(s.tv - u * dot(u, s.tv));
+    const F3 n = normalize(cross(u, v));
+    SurfelGeo g{};
+    g.px = s.p.x; g.py = s.p.y; g.pz = s.p.z; g.su = s.su;
+    g.nx = n.x; g.ny = n.y; g.nz = n.z; g.sv = s.sv;
+    g.tux = s.tu.x; g.tuy = s.tu.y; g.tuz = s.tu.z; g.e4b = 4.0f * exp_r(s.beta);
+    g.tvx = s.tv.x; g.tvy = s.tv.y; g.tvz = s.tv.z; g.pad = 0.0f;
+    return g;
+}
+
+// Shading record: the canonical normal normalize(cross(tanU, tanV))
+// (KernelHelpers.h:497-501), opacity, albedo, alpha_r.
+PALE_HD SurfelShade surfelShadeOf(const HostPoint& s) {
+    const F3 ns = normalize(cross(s.tu, s.tv));
+    SurfelShade h{};
+    h.nsx = ns.x; h.nsy = ns.y; h.nsz = ns.z; h.opacity = s.opacity;
+    h.ar = s.albedo.x; h.ag = s.albedo.y; h.ab = s.albedo.z; h.alpha_r = s.alpha_r;
+    return h;
+}
 
 struct HostMesh {
     std::vector<F3> verts;
@@ -91,10 +137,45 @@ struct BuildProducts {
     std::vector<HostEmTri> emTris;
     uint32_t blasMaxDepth = 0;  // deepest BLAS node (root = 0)
     uint32_t tlasMaxDepth = 0;
+    // The surfel BLAS was built on the device, which already holds the
+    // surfel records and the cloud's wide nodes (after the meshes' ones).
+    bool cloudOnDevice = false;
 };
 
-// Builds everything from the meshes and the asset points.
-BuildProducts buildScene(const std::vector<HostMesh>& meshes, const std::vector<AssetPoint>& asset);
+// A device build of the surfel BLAS.  It returns the reference nodes and the
+// permutation (perm[slot] = index in pts), equal to the host build's, and the
+// tree depth, and leaves the surfel records and the wide nodes (the cloud's
+// first at wide index wideBase) on the device; false = build on the host.
+struct CloudBuild {
+    virtual ~CloudBuild() = default;
+    virtual bool build(const std::vector<HostPoint>& pts, uint32_t wideBase, std::vector<RefNode>& nodes,
+                       std::vector<uint32_t>& perm, uint32_t& depth) = 0;
+};
+
+// The shape of the surfel BLAS over n surfels, which depends on n only (a
+// node of c > 4 surfels splits into c/2 and the rest), in the tables a
+// device build works from (the kernels of pale_bvh.hip).
+struct CloudShape {
+    std::vector<RefNode> topo;         // ranges and child pairs (BVH.ixx numbering), no boxes
+    std::vector<uint4> big;            // {first, count, ref, slot}: nodes of more than `subtree`
+                                       // surfels, level by level (slot = index here)
+    std::vector<uint32_t> levelBig;    // big[levelBig[d] .. levelBig[d + 1]) = level d
+    std::vector<uint4> chunks;         // {slot, first, end, -}: box-fit pieces of the big nodes
+    std::vector<uint32_t> levelChunk;  // chunks of level d, as levelBig
+    std::vector<uint4> roots;          // {first, count, ref, preorder rank}: nodes of at most
+                                       // `subtree` surfels under a big node (or the root)
+    std::vector<uint32_t> internalOf;  // internal nodes of a subtree over c <= subtree surfels
+    std::vector<uint4> wide;           // {children pair, left ref, right ref, -} per internal node
+                                       // in reference order; a ref without the leaf bit is a
+                                       // wide index relative to the first internal node
+    uint32_t depth = 0;                // deepest node (root = 0)
+};
+CloudShape cloudShape(uint32_t n, uint32_t subtree, uint32_t fitChunk);
+
+// Builds everything from the meshes and the asset points (the surfel BLAS
+// with `cloud` when given and it accepts the scene).
+BuildProducts buildScene(const std::vector<HostMesh>& meshes, const std::vector<AssetPoint>& asset,
+                         CloudBuild* cloud = nullptr);
 
 // Device-layout images of the build products (host memory).
 struct DeviceImage {
@@ -108,6 +189,10 @@ struct DeviceImage {
     std::vector<DevLight> lights;
     std::vector<DevEmTri> emTris;
     uint32_t stackDepth = 0;  // LDS stack entries one lane needs
+    // With BuildProducts::cloudOnDevice: surfGeo / surfShade are sized but
+    // not filled, and only nodes[0, meshWide) (the meshes' wide nodes) are;
+    // the device holds the rest.
+    uint32_t meshWide = 0;
 };
 
 DeviceImage makeDeviceImage(const BuildProducts& bp, const std::vector<HostMesh>& meshes);
@@ -116,5 +201,24 @@ DeviceImage makeDeviceImage(const BuildProducts& bp, const std::vector<HostMesh>
 void refreshSurfels(const std::vector<HostPoint>& pts, DeviceImage& img);
 
 void cameraFocal(HostCamera& c);
+
+// f(k) for k < n on up to 16 threads (in any order).
+template <typename F>
+inline void parallelFor(size_t n, F&& f) {
+    const size_t hw = std::max<unsigned>(1u, std::thread::hardware_concurrency());
+    const size_t nt = std::min<size_t>(std::min<size_t>(hw, 16), n);
+    if (nt <= 1) {
+        for (size_t k = 0; k < n; ++k) f(k);
+        return;
+    }
+    std::atomic<size_t> next{0};
+    std::vector<std::thread> th;
+    for (size_t t = 0; t < nt; ++t)
+        th.emplace_back([&] {
+            for (size_t k; (k = next.fetch_add(1)) < n;) f(k);
+        });
+    for (auto& x : th) x.join();
+}
+
 
 }  // namespace pale

@@ -30,36 +30,9 @@ inline float clampf(float v, float lo, float hi) { return std::min(std::max(v, l
 
 // surfelObjectAabbBeta (SceneBuild.cpp:395-444)
 Box surfelBox(const HostPoint& s) {
-    const F3 tu = normalize(s.tu);
-    const F3 tv = normalize(s.tv);
-    const F3 n = normalize(cross(tu, tv));
-    const float rU = 1.0001f * s.su;
-    const float rV = 1.0001f * s.sv;
-    F3 h;
-    float* hp = &h.x;
-    for (int a = 0; a < 3; ++a) {
-        const float cu = comp(tu, a), cv = comp(tv, a), cn = std::fabs(comp(n, a));
-        hp[a] = std::sqrt((rU * cu) * (rU * cu) + (rV * cv) * (rV * cv)) + 0.001f * cn;
-    }
-    return Box{s.p - h, s.p + h};
-}
-
-// f(k) for k < n on up to 16 threads (in any order).
-template <typename F>
-void parallelFor(size_t n, F&& f) {
-    const size_t hw = std::max<unsigned>(1u, std::thread::hardware_concurrency());
-    const size_t nt = std::min<size_t>(std::min<size_t>(hw, 16), n);
-    if (nt <= 1) {
-        for (size_t k = 0; k < n; ++k) f(k);
-        return;
-    }
-    std::atomic<size_t> next{0};
-    std::vector<std::thread> th;
-    for (size_t t = 0; t < nt; ++t)
-        th.emplace_back([&] {
-            for (size_t k; (k = next.fetch_add(1)) < n;) f(k);
-        });
-    for (auto& x : th) x.join();
+    Box b;
+    surfelBoxOf(s, b.mn, b.mx);
+    return b;
 }
 
 // Generic median-split builder (BVH.ixx:49-133 for boxes, :26-45/:141-220
@@ -272,6 +245,81 @@ uint32_t maxDepth(const std::vector<RefNode>& nodes, uint32_t first, uint32_t co
 
 }  // namespace
 
+CloudShape cloudShape(uint32_t n, uint32_t subtree, uint32_t fitChunk) {
+    struct Info {
+        uint32_t first, count, depth;
+        bool parentBig;
+    };
+    CloudShape sh;
+    std::vector<RefNode>& topo = sh.topo;
+    topo.reserve(n);
+    std::vector<Info> info;
+    info.reserve(n);
+    topo.push_back(RefNode{mk3(0, 0, 0), mk3(0, 0, 0), 0, n});
+    info.push_back(Info{0, n, 0, true});
+    // depth first, left before right; children appended as a pair when a
+    // node splits (MedianSplit::subdivide)
+    std::vector<uint32_t> st{0u};
+    while (!st.empty()) {
+        const uint32_t ni = st.back();
+        st.pop_back();
+        const Info in = info[ni];
+        if (in.count <= kMaxLeafPrims) continue;
+        const uint32_t L = (uint32_t)topo.size(), lc = in.count >> 1;
+        const bool isBig = in.count > subtree;
+        topo.push_back(RefNode{mk3(0, 0, 0), mk3(0, 0, 0), in.first, lc});
+        topo.push_back(RefNode{mk3(0, 0, 0), mk3(0, 0, 0), in.first + lc, in.count - lc});
+        info.push_back(Info{in.first, lc, in.depth + 1, isBig});
+        info.push_back(Info{in.first + lc, in.count - lc, in.depth + 1, isBig});
+        topo[ni].leftFirst = L;
+        topo[ni].count = 0;
+        st.push_back(L + 1);
+        st.push_back(L);
+    }
+    uint32_t levels = 0;
+    for (const Info& in : info) {
+        sh.depth = std::max(sh.depth, in.depth);
+        if (in.count > subtree) levels = std::max(levels, in.depth + 1);
+    }
+    std::vector<std::vector<uint32_t>> byLevel(levels);
+    for (uint32_t r = 0; r < (uint32_t)info.size(); ++r) {
+        const Info& in = info[r];
+        if (in.count > subtree) {
+            byLevel[in.depth].push_back(r);
+        } else if (in.parentBig) {
+            const uint32_t rank = topo[r].count == 0 ? (topo[r].leftFirst - 1) / 2 : 0;
+            sh.roots.push_back(uint4{in.first, in.count, r, rank});
+        }
+    }
+    sh.levelBig.assign(1, 0);
+    sh.levelChunk.assign(1, 0);
+    for (const auto& lv : byLevel) {
+        for (uint32_t r : lv) {
+            const uint32_t slot = (uint32_t)sh.big.size(), a0 = info[r].first, a1 = a0 + info[r].count;
+            sh.big.push_back(uint4{a0, info[r].count, r, slot});
+            for (uint32_t a = a0; a < a1; a += fitChunk) sh.chunks.push_back(uint4{slot, a, std::min(a + fitChunk, a1), 0});
+        }
+        sh.levelBig.push_back((uint32_t)sh.big.size());
+        sh.levelChunk.push_back((uint32_t)sh.chunks.size());
+    }
+    sh.internalOf.assign(subtree + 1, 0);
+    for (uint32_t c = kMaxLeafPrims + 1; c <= subtree; ++c)
+        sh.internalOf[c] = 1 + sh.internalOf[c >> 1] + sh.internalOf[c - (c >> 1)];
+    // wide order = internal nodes by reference index
+    std::vector<uint32_t> wideOf(topo.size(), kInvalid);
+    uint32_t w = 0;
+    for (uint32_t r = 0; r < (uint32_t)topo.size(); ++r)
+        if (topo[r].count == 0) wideOf[r] = w++;
+    sh.wide.reserve(w);
+    auto code = [&](uint32_t c) { return topo[c].count == 0 ? wideOf[c] : leafRef(topo[c].leftFirst, topo[c].count); };
+    for (uint32_t r = 0; r < (uint32_t)topo.size(); ++r)
+        if (topo[r].count == 0) {
+            const uint32_t L = topo[r].leftFirst;
+            sh.wide.push_back(uint4{L, code(L), code(L + 1), 0});
+        }
+    return sh;
+}
+
 void cameraFocal(HostCamera& c) {
     // makePrimaryRayFromPixelJitteredFov (KernelHelpers.h:764-796): the focal
     // lengths only depend on the camera, so they are host constants.
@@ -281,7 +329,8 @@ void cameraFocal(HostCamera& c) {
     c.fx = c.fy * (W / H);
 }
 
-BuildProducts buildScene(const std::vector<HostMesh>& meshes, const std::vector<AssetPoint>& asset) {
+BuildProducts buildScene(const std::vector<HostMesh>& meshes, const std::vector<AssetPoint>& asset,
+                         CloudBuild* cloud) {
     BuildProducts bp;
     // collectGeometry (SceneBuild.cpp:72-127)
     const float oneThird = 1.0f / 3.0f;
@@ -401,33 +450,43 @@ BuildProducts buildScene(const std::vector<HostMesh>& meshes, const std::vector<
         meshToBlas[mi] = (uint32_t)bp.blasRanges.size();
         bp.blasRanges.push_back(Range{first, (uint32_t)nodes.size()});
     }
+    uint32_t cloudDepth = 0;
     {
-        // Each surfel as one record (centroid, box, index), so that the
-        // selections and the box fits stream through memory instead of
-        // gathering keys and boxes by index.
-        struct Prim {
-            float key[3];
-            uint32_t idx;
-            Box box;
-        };
-        const size_t chunk = 1u << 14;
-        std::vector<Prim> prims(pts.size());
-        parallelFor((pts.size() + chunk - 1) / chunk, [&](size_t c) {
-            for (size_t i = c * chunk; i < std::min(pts.size(), (c + 1) * chunk); ++i)
-                prims[i] = Prim{{pts[i].p.x, pts[i].p.y, pts[i].p.z}, (uint32_t)i, surfelBox(pts[i])};
-        });
         std::vector<RefNode> nodes;
-        auto ms = medianSplit(
-            nodes, prims, [](const Prim& e) { return e.box; }, [](const Prim& e, int a) { return e.key[a]; },
-            kMaxLeafPrims);
-        ms.build((uint32_t)pts.size());
+        const size_t chunk = 1u << 14;
+        uint32_t wideBase = 0;
+        for (const Range& r : bp.blasRanges)
+            for (uint32_t i = 0; i < r.count; ++i)
+                wideBase += (bp.blas[r.first + i].count == 0 && r.count > 1) ? 1u : 0u;
+        if (cloud && cloud->build(pts, wideBase, nodes, bp.perm, cloudDepth)) {
+            bp.cloudOnDevice = true;
+        } else {
+            // Each surfel as one record (centroid, box, index), so that the
+            // selections and the box fits stream through memory instead of
+            // gathering keys and boxes by index.
+            struct Prim {
+                float key[3];
+                uint32_t idx;
+                Box box;
+            };
+            std::vector<Prim> prims(pts.size());
+            parallelFor((pts.size() + chunk - 1) / chunk, [&](size_t c) {
+                for (size_t i = c * chunk; i < std::min(pts.size(), (c + 1) * chunk); ++i)
+                    prims[i] = Prim{{pts[i].p.x, pts[i].p.y, pts[i].p.z}, (uint32_t)i, surfelBox(pts[i])};
+            });
+            auto ms = medianSplit(
+                nodes, prims, [](const Prim& e) { return e.box; }, [](const Prim& e, int a) { return e.key[a]; },
+                kMaxLeafPrims);
+            ms.build((uint32_t)pts.size());
+            bp.perm.resize(pts.size());
+            parallelFor((pts.size() + chunk - 1) / chunk, [&](size_t c) {
+                for (size_t k = c * chunk; k < std::min(pts.size(), (c + 1) * chunk); ++k) bp.perm[k] = prims[k].idx;
+            });
+            cloudDepth = maxDepth(nodes, 0, (uint32_t)nodes.size());
+        }
         bp.points.resize(pts.size());
-        bp.perm.resize(pts.size());
         parallelFor((pts.size() + chunk - 1) / chunk, [&](size_t c) {
-            for (size_t k = c * chunk; k < std::min(pts.size(), (c + 1) * chunk); ++k) {
-                bp.perm[k] = prims[k].idx;
-                bp.points[k] = pts[prims[k].idx];
-            }
+            for (size_t k = c * chunk; k < std::min(pts.size(), (c + 1) * chunk); ++k) bp.points[k] = pts[bp.perm[k]];
         });
         const uint32_t first = (uint32_t)bp.blas.size();
         bp.blas.insert(bp.blas.end(), nodes.begin(), nodes.end());
@@ -435,7 +494,10 @@ BuildProducts buildScene(const std::vector<HostMesh>& meshes, const std::vector<
         bp.blasRanges.push_back(Range{first, (uint32_t)nodes.size()});
         for (HostInstance& in : bp.instances) in.blas = (in.type == kGeoMesh) ? meshToBlas[in.geom] : bi;
     }
-    for (const Range& r : bp.blasRanges) bp.blasMaxDepth = std::max(bp.blasMaxDepth, maxDepth(bp.blas, r.first, r.count));
+    // the surfel BLAS's depth comes with its build
+    for (size_t i = 0; i + 1 < bp.blasRanges.size(); ++i)
+        bp.blasMaxDepth = std::max(bp.blasMaxDepth, maxDepth(bp.blas, bp.blasRanges[i].first, bp.blasRanges[i].count));
+    bp.blasMaxDepth = std::max(bp.blasMaxDepth, cloudDepth);
     // buildTLAS (SceneBuild.cpp:555-640)
     struct IBox {
         F3 mn, mx;
@@ -509,22 +571,8 @@ void refreshSurfels(const std::vector<HostPoint>& pts, DeviceImage& img) {
     constexpr size_t kChunk = 1u << 14;
     parallelFor((pts.size() + kChunk - 1) / kChunk, [&](size_t c) {
       for (size_t i = c * kChunk; i < std::min(pts.size(), (c + 1) * kChunk); ++i) {
-        const HostPoint& s = pts[i];
-        // intersectSurfel's frame (KernelHelpers.h:646-648)
-        const F3 u = normalize(s.tu);
-        const F3 v = normalize(s.tv - u * dot(u, s.tv));
-        const F3 n = normalize(cross(u, v));
-        SurfelGeo g{};
-        g.px = s.p.x; g.py = s.p.y; g.pz = s.p.z; g.su = s.su;
-        g.nx = n.x; g.ny = n.y; g.nz = n.z; g.sv = s.sv;
-        g.tux = s.tu.x; g.tuy = s.tu.y; g.tuz = s.tu.z; g.e4b = 4.0f * exp_r(s.beta);
-        g.tvx = s.tv.x; g.tvy = s.tv.y; g.tvz = s.tv.z; g.pad = 0.0f;
-        img.surfGeo[i] = g;
-        const F3 ns = normalize(cross(s.tu, s.tv));
-        SurfelShade h{};
-        h.nsx = ns.x; h.nsy = ns.y; h.nsz = ns.z; h.opacity = s.opacity;
-        h.ar = s.albedo.x; h.ag = s.albedo.y; h.ab = s.albedo.z; h.alpha_r = s.alpha_r;
-        img.surfShade[i] = h;
+        img.surfGeo[i] = surfelGeoOf(pts[i]);
+        img.surfShade[i] = surfelShadeOf(pts[i]);
       }
     });
 }
@@ -532,7 +580,7 @@ void refreshSurfels(const std::vector<HostPoint>& pts, DeviceImage& img) {
 DeviceImage makeDeviceImage(const BuildProducts& bp, const std::vector<HostMesh>& meshes) {
     DeviceImage img;
     if (bp.points.size() >= (1u << 29)) throw std::runtime_error("more than 2^29 surfels");
-    refreshSurfels(bp.points, img);
+    if (!bp.cloudOnDevice) refreshSurfels(bp.points, img);
     img.tris.resize(bp.tris.size());
     for (size_t i = 0; i < bp.tris.size(); ++i) {
         const HostTri& t = bp.tris[i];
@@ -550,13 +598,20 @@ DeviceImage makeDeviceImage(const BuildProducts& bp, const std::vector<HostMesh>
             if (internal) compact[r.first + i] = next++;
         }
     }
-    img.nodes.resize(next);
+    // the cloud's wide nodes come last; a device build has written them
+    const Range& cloudRange = bp.blasRanges.back();
+    uint32_t meshWide = next;
+    for (uint32_t i = 0; i < cloudRange.count; ++i)
+        if (compact[cloudRange.first + i] != kInvalid) meshWide = std::min(meshWide, compact[cloudRange.first + i]);
+    img.meshWide = meshWide;
+    img.nodes.resize(bp.cloudOnDevice ? meshWide : next);
     auto refOf = [&](const Range& r, uint32_t local) -> uint32_t {
         const RefNode& n = bp.blas[r.first + local];
         if (n.count > 0) return leafRef(n.leftFirst, n.count);
         return compact[r.first + local];
     };
     for (const Range& r : bp.blasRanges) {
+      if (bp.cloudOnDevice && &r == &cloudRange) continue;
       parallelFor((r.count + 16383u) / 16384u, [&](size_t c) {
         for (uint32_t i = (uint32_t)c * 16384u; i < std::min(r.count, (uint32_t)(c + 1) * 16384u); ++i) {
             const uint32_t ci = compact[r.first + i];

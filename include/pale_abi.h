@@ -155,6 +155,15 @@ typedef struct pale_settings {
                                   (GPUDataStructures.h:395-398); with
                                   adjoint_decorrelate sample k seeds from
                                   adjoint_seed + k                          */
+    uint32_t bvh_build;        /* "bvh_build" (default 0): where the surfel
+                                  BVH is built (SceneBuild.cpp:642-772 on
+                                  the host in the reference); 0 = on the
+                                  GPU from 65,536 surfels, else on the
+                                  host; 1 = on the GPU from 5 surfels; 2 =
+                                  always on the host.  Nodes and
+                                  permutation are the same either way; a
+                                  scene with non-finite or -0 box
+                                  coordinates is always built on the host */
 } pale_settings;
 
 /* Work counters of the last render calls (no reference counterpart; the
@@ -184,6 +193,12 @@ typedef struct pale_stats {
      * nodes, AABB slab tests, primitive tests, irradiance gathers, gather
      * cells visited, non-empty cells, photons scanned. */
     uint64_t work[3][7];
+    /* The last surfel-BVH build (pale_create, rebuild_bvh and the point
+     * edits that rebuild): host wall time of the whole scene build and
+     * upload, and 1 when the surfel BLAS was built on the GPU (setting
+     * bvh_build), else 0. */
+    double bvh_build_ms;
+    uint32_t bvh_on_device, bvh_pad;
 } pale_stats;
 
 enum { PALE_WORK_PHOTON = 0, PALE_WORK_GATHER = 1, PALE_WORK_ADJOINT = 2 };
