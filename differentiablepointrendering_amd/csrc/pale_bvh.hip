@@ -30,10 +30,11 @@ namespace pale {
 
 namespace {
 
-constexpr uint32_t kSelBlock = 512;     // threads of a selecting workgroup
+// A selecting workgroup has 1024, 256 or 64 threads, by the size of the
+// level's nodes (more threads for the few large nodes of the top levels).
 constexpr uint32_t kSelItems = 8;       // consecutive elements per thread and chunk
-constexpr uint32_t kSelChunk = kSelBlock * kSelItems;
-constexpr uint32_t kSelSeq = 256;       // ranges this short finish on one thread in LDS
+constexpr uint32_t kSelSeq = 32;        // ranges this short finish on one thread in LDS
+constexpr uint32_t kSwapIlp = 4;        // swaps per thread and step (independent loads in flight)
 constexpr uint32_t kFitBlock = 256;
 
 // Order-preserving image of a binary32 value in uint32 (for finite values
@@ -72,9 +73,10 @@ __device__ __forceinline__ uint32_t waveSum(uint32_t x) {
 }
 
 // Exclusive prefix of x over the workgroup (thread order) and the total.
-// sw: kSelBlock / 64 + 1 words of LDS.
+// sw: kBlock / 64 + 1 words of LDS.
+template <uint32_t kBlock>
 __device__ uint32_t blockExclusiveScan(uint32_t x, uint32_t& total, uint32_t* sw) {
-    constexpr uint32_t nw = kSelBlock / 64;
+    constexpr uint32_t nw = kBlock / 64;
     const uint32_t incl = waveInclusiveScan(x);
     const uint32_t w = threadIdx.x >> 6;
     if ((threadIdx.x & 63u) == 63u) sw[w] = incl;
@@ -95,9 +97,10 @@ __device__ uint32_t blockExclusiveScan(uint32_t x, uint32_t& total, uint32_t* sw
     return r;
 }
 
+template <uint32_t kBlock>
 __device__ uint32_t blockSum(uint32_t x, uint32_t* sw) {
     uint32_t total;
-    (void)blockExclusiveScan(x, total, sw);
+    (void)blockExclusiveScan<kBlock>(x, total, sw);
     return total;
 }
 
@@ -168,15 +171,30 @@ __global__ __launch_bounds__(kFitBlock) void k_bvh_fit(const uint4* chunks, cons
 }
 
 // nodes[k] = {first, count, ref, slot in acc}: one workgroup per node.
+//
+// One introselect step's partition of [lo + 1, hi) around P, by ranks
+// (pale_select.h): a position p with key >= P and G(p) such positions left
+// of it is the left partner of swap G(p) iff E(p) > G(p), E(p) = keys <= P
+// right of p; a position q with key <= P is the right partner of swap E(q)
+// iff G(q) > E(q).  The first position with key >= P that is not swapped is
+// L_K, and the cut is min(L_K, R_{K-1}).  Pass A counts the keys <= P; pass
+// B walks the range in chunks of kSelChunk (one barrier each: wave scans,
+// wave totals double-buffered in LDS) and records the partners of each swap;
+// pass C swaps, kSwapIlp swaps per thread at a time.
+template <uint32_t kSelBlock>
 __global__ __launch_bounds__(kSelBlock) void k_bvh_select(const uint4* nodes, const uint32_t* acc, const float* kx,
                                                          const float* ky, const float* kz, float* kv, uint32_t* ord,
                                                          uint32_t* lpos, uint32_t* rpos, float* refBox) {
-    __shared__ uint32_t sw[kSelBlock / 64 + 1];
+    constexpr uint32_t kSelWaves = kSelBlock / 64;
+    constexpr uint32_t kSelChunk = kSelBlock * kSelItems;
+    __shared__ uint32_t sw[kSelWaves + 1];
+    __shared__ uint32_t sTot[2][kSelWaves];
+    __shared__ uint32_t sLK, sK;
     __shared__ float sP;
-    __shared__ float sK[kSelSeq];
-    __shared__ uint32_t sI[kSelSeq];
+    __shared__ float sK32[kSelSeq];
+    __shared__ uint32_t sI32[kSelSeq];
     const uint4 nd = nodes[blockIdx.x];
-    const uint32_t first = nd.x, count = nd.y, tid = threadIdx.x;
+    const uint32_t first = nd.x, count = nd.y, tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
     const uint32_t* a = acc + 6 * (size_t)nd.w;
     const F3 mn = mk3(fromOrd(a[0]), fromOrd(a[1]), fromOrd(a[2]));
     const F3 mx = mk3(fromOrd(a[3]), fromOrd(a[4]), fromOrd(a[5]));
@@ -193,16 +211,16 @@ __global__ __launch_bounds__(kSelBlock) void k_bvh_select(const uint4* nodes, co
         if (len <= kSelSeq) {
             // the rest on one thread, in LDS (introselect is translation
             // invariant: positions relative to lo)
-            for (uint32_t p = tid; p < len; p += kSelBlock) {
-                sK[p] = kv[lo + p];
-                sI[p] = ord[lo + p];
+            if (tid < len) {
+                sK32[tid] = kv[lo + tid];
+                sI32[tid] = ord[lo + tid];
             }
             __syncthreads();
-            if (tid == 0) sel::selectFrom(sK, sI, 0, nth - lo, len, depth);
+            if (tid == 0) sel::selectFrom(sK32, sI32, 0, nth - lo, len, depth);
             __syncthreads();
-            for (uint32_t p = tid; p < len; p += kSelBlock) {
-                kv[lo + p] = sK[p];
-                ord[lo + p] = sI[p];
+            if (tid < len) {
+                kv[lo + tid] = sK32[tid];
+                ord[lo + tid] = sI32[tid];
             }
             break;
         }
@@ -214,28 +232,47 @@ __global__ __launch_bounds__(kSelBlock) void k_bvh_select(const uint4* nodes, co
         if (tid == 0) {
             sel::medianToFirst(kv, ord, lo, lo + 1, lo + len / 2, hi - 1);
             sP = kv[lo];
+            sLK = hi;
+            sK = 0;
         }
         __syncthreads();
         const float P = sP;
-        // Hoare's partition of [lo + 1, hi) around P, by ranks.
+        // pass A
         uint32_t cnt = 0;
-        for (uint32_t p = lo + 1 + tid; p < hi; p += kSelBlock) cnt += !(P < kv[p]) ? 1u : 0u;
-        const uint32_t nLE = blockSum(cnt, sw);
-        uint32_t carryG = 0, carryL = 0, kacc = 0;
-        for (uint32_t base = lo + 1; base < hi; base += kSelChunk) {
+        {
+            uint32_t p = lo + 1 + tid;
+            for (; p + 3 * kSelBlock < hi; p += 4 * kSelBlock) {
+                const float k0 = kv[p], k1 = kv[p + kSelBlock], k2 = kv[p + 2 * kSelBlock], k3 = kv[p + 3 * kSelBlock];
+                cnt += (!(P < k0) ? 1u : 0u) + (!(P < k1) ? 1u : 0u) + (!(P < k2) ? 1u : 0u) + (!(P < k3) ? 1u : 0u);
+            }
+            for (; p < hi; p += kSelBlock) cnt += !(P < kv[p]) ? 1u : 0u;
+        }
+        const uint32_t nLE = blockSum<kSelBlock>(cnt, sw);
+        // pass B
+        uint32_t carryG = 0, carryL = 0, kacc = 0, lk = hi, par = 0;
+        for (uint32_t base = lo + 1; base < hi; base += kSelChunk, par ^= 1u) {
             const uint32_t b0 = base + tid * kSelItems;
             float k[kSelItems];
             uint32_t g = 0, l = 0;
 #pragma unroll
             for (uint32_t j = 0; j < kSelItems; ++j) {
-                k[j] = b0 + j < hi ? kv[b0 + j] : 0.0f;
-                if (b0 + j < hi) {
-                    g += !(k[j] < P) ? 1u : 0u;
-                    l += !(P < k[j]) ? 1u : 0u;
-                }
+                const bool in = b0 + j < hi;
+                k[j] = in ? kv[b0 + j] : 0.0f;
+                g += (in && !(k[j] < P)) ? 1u : 0u;
+                l += (in && !(P < k[j])) ? 1u : 0u;
             }
-            uint32_t tot;
-            const uint32_t pre = blockExclusiveScan(g | (l << 16), tot, sw);
+            const uint32_t x = g | (l << 16);
+            const uint32_t incl = waveInclusiveScan(x);
+            if (lane == 63u) sTot[par][wave] = incl;
+            __syncthreads();
+            uint32_t before = 0, tot = 0;
+#pragma unroll
+            for (uint32_t w = 0; w < kSelWaves; ++w) {
+                const uint32_t t = sTot[par][w];
+                before += w < wave ? t : 0u;
+                tot += t;
+            }
+            const uint32_t pre = before + incl - x;
             uint32_t G = carryG + (pre & 0xFFFFu), Lc = carryL + (pre >> 16);
 #pragma unroll
             for (uint32_t j = 0; j < kSelItems; ++j) {
@@ -244,20 +281,52 @@ __global__ __launch_bounds__(kSelBlock) void k_bvh_select(const uint4* nodes, co
                 if (le) ++Lc;
                 const uint32_t E = nLE - Lc;  // keys <= P right of this position
                 if (ge) {
-                    lpos[first + G] = b0 + j;
-                    if (E > G) ++kacc;
-                    ++G;
+                    if (E > G) {
+                        lpos[first + G] = b0 + j;
+                        ++kacc;
+                    } else {
+                        lk = min(lk, b0 + j);
+                    }
                 }
-                if (le) rpos[first + E] = b0 + j;
+                if (le && G > E) rpos[first + E] = b0 + j;
+                if (ge) ++G;
             }
             carryG += tot & 0xFFFFu;
             carryL += tot >> 16;
         }
-        const uint32_t K = blockSum(kacc, sw);  // also orders the lpos / rpos writes before their reads
-        const uint32_t nGE = carryG;
-        for (uint32_t m = tid; m < K; m += kSelBlock) sel::swapAt(kv, ord, lpos[first + m], rpos[first + m]);
-        uint32_t cut = K < nGE ? lpos[first + K] : hi;
+        if (lk < hi) atomicMin(&sLK, lk);
+        if (kacc) atomicAdd(&sK, kacc);
+        __syncthreads();  // also orders the lpos / rpos writes before their reads
+        const uint32_t K = sK;
+        uint32_t cut = sLK;
         if (K > 0) cut = min(cut, rpos[first + K - 1]);
+        // pass C
+        for (uint32_t m0 = tid * kSwapIlp; m0 < K; m0 += kSelBlock * kSwapIlp) {
+            uint32_t pa[kSwapIlp], pb[kSwapIlp], ia[kSwapIlp], ib[kSwapIlp];
+            float ka[kSwapIlp], kb[kSwapIlp];
+#pragma unroll
+            for (uint32_t j = 0; j < kSwapIlp; ++j)
+                if (m0 + j < K) {
+                    pa[j] = lpos[first + m0 + j];
+                    pb[j] = rpos[first + m0 + j];
+                }
+#pragma unroll
+            for (uint32_t j = 0; j < kSwapIlp; ++j)
+                if (m0 + j < K) {
+                    ka[j] = kv[pa[j]];
+                    kb[j] = kv[pb[j]];
+                    ia[j] = ord[pa[j]];
+                    ib[j] = ord[pb[j]];
+                }
+#pragma unroll
+            for (uint32_t j = 0; j < kSwapIlp; ++j)
+                if (m0 + j < K) {
+                    kv[pa[j]] = kb[j];
+                    kv[pb[j]] = ka[j];
+                    ord[pa[j]] = ib[j];
+                    ord[pb[j]] = ia[j];
+                }
+        }
         __syncthreads();
         if (cut <= nth)
             lo = cut;
@@ -349,10 +418,16 @@ void launchBvhFit(hipStream_t s, const uint4* chunks, uint32_t nChunks, const ui
     if (nChunks) k_bvh_fit<<<nChunks, kFitBlock, 0, s>>>(chunks, ord, boxes, acc);
 }
 
-void launchBvhSelect(hipStream_t s, const uint4* nodes, uint32_t nNodes, const uint32_t* acc, const float* kx,
-                     const float* ky, const float* kz, float* kv, uint32_t* ord, uint32_t* lpos, uint32_t* rpos,
-                     float* refBox) {
-    if (nNodes) k_bvh_select<<<nNodes, kSelBlock, 0, s>>>(nodes, acc, kx, ky, kz, kv, ord, lpos, rpos, refBox);
+void launchBvhSelect(hipStream_t s, const uint4* nodes, uint32_t nNodes, uint32_t count, const uint32_t* acc,
+                     const float* kx, const float* ky, const float* kz, float* kv, uint32_t* ord, uint32_t* lpos,
+                     uint32_t* rpos, float* refBox) {
+    if (!nNodes) return;
+    if (count > 32768)
+        k_bvh_select<1024><<<nNodes, 1024, 0, s>>>(nodes, acc, kx, ky, kz, kv, ord, lpos, rpos, refBox);
+    else if (count > 2048)
+        k_bvh_select<256><<<nNodes, 256, 0, s>>>(nodes, acc, kx, ky, kz, kv, ord, lpos, rpos, refBox);
+    else
+        k_bvh_select<64><<<nNodes, 64, 0, s>>>(nodes, acc, kx, ky, kz, kv, ord, lpos, rpos, refBox);
 }
 
 void launchBvhSubtrees(hipStream_t s, const uint4* roots, uint32_t nRoots, const uint32_t* internalOf, const float* kx,

@@ -304,9 +304,10 @@ void launchBvhPrims(hipStream_t s, const HostPoint* pts, uint32_t n, float* kx, 
 void launchBvhInitFit(hipStream_t s, uint32_t* acc, uint32_t nNodes);
 void launchBvhFit(hipStream_t s, const uint4* chunks, uint32_t nChunks, const uint32_t* ord, const float* boxes,
                   uint32_t* acc);
-void launchBvhSelect(hipStream_t s, const uint4* nodes, uint32_t nNodes, const uint32_t* acc, const float* kx,
-                     const float* ky, const float* kz, float* kv, uint32_t* ord, uint32_t* lpos, uint32_t* rpos,
-                     float* refBox);
+// count: the level's largest node (picks the workgroup size)
+void launchBvhSelect(hipStream_t s, const uint4* nodes, uint32_t nNodes, uint32_t count, const uint32_t* acc,
+                     const float* kx, const float* ky, const float* kz, float* kv, uint32_t* ord, uint32_t* lpos,
+                     uint32_t* rpos, float* refBox);
 void launchBvhSubtrees(hipStream_t s, const uint4* roots, uint32_t nRoots, const uint32_t* internalOf, const float* kx,
                        const float* ky, const float* kz, const float* boxes, float* kv, uint32_t* ord, float* refBox);
 void launchBvhWide(hipStream_t s, const uint4* wide, uint32_t nWide, const float* refBox, uint32_t base,

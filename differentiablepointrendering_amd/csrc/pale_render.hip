@@ -103,7 +103,7 @@ F3 ld3h(const float* p) { return mk3(p[0], p[1], p[2]); }
 // order of the surfels and, from them, the surfel records and wide nodes.
 class DeviceCloudBuild final : public CloudBuild {
    public:
-    static constexpr uint32_t kSubtree = 64;      // subtrees this small: one thread each
+    static constexpr uint32_t kSubtree = 32;      // subtrees this small: one thread each
     static constexpr uint32_t kFitChunk = 16384;  // surfels per fit workgroup
 
     hipStream_t stream = nullptr;
@@ -112,21 +112,20 @@ class DeviceCloudBuild final : public CloudBuild {
     DevMem* shade = nullptr;
     DevMem* nodes = nullptr;
 
-    bool build(const std::vector<HostPoint>& pts, uint32_t wideBase, std::vector<RefNode>& out,
-               std::vector<uint32_t>& perm, uint32_t& depthOut) override {
-        const uint32_t n = (uint32_t)pts.size();
-        if (n < std::max<uint32_t>(minCount, kMaxLeafPrims + 1)) return false;
-        if (n != shapeN) shape(n);
-        // stage the points in pinned memory (host threads), then one copy
+    HostPoint* staging(uint32_t n) override {
+        if (n < std::max<uint32_t>(minCount, kMaxLeafPrims + 1)) return nullptr;
         pinned(stage, (size_t)n * sizeof(HostPoint));
-        HostPoint* hp = stage.as<HostPoint>();
+        return stage.as<HostPoint>();
+    }
+
+    bool build(const HostPoint* pts, uint32_t n, uint32_t wideBase, std::vector<RefNode>& blas,
+               std::vector<uint32_t>& perm, uint32_t& depthOut) override {
+        if (pts != stage.as<HostPoint>() || (size_t)n * sizeof(HostPoint) > stage.bytes)
+            throw StatusError(PALE_ERR_INVALID_ARG, "device BVH build: points not in its staging memory");
+        if (n != shapeN) shape(n);
         constexpr size_t kChunk = 1u << 15;
-        parallelFor((n + kChunk - 1) / kChunk, [&](size_t c) {
-            const size_t a = c * kChunk, b = std::min<size_t>(n, a + kChunk);
-            std::memcpy(hp + a, pts.data() + a, (b - a) * sizeof(HostPoint));
-        });
         dPts.reserve((size_t)n * sizeof(HostPoint));
-        HIPCHECK(hipMemcpyAsync(dPts.p, hp, (size_t)n * sizeof(HostPoint), hipMemcpyHostToDevice, stream));
+        HIPCHECK(hipMemcpyAsync(dPts.p, pts, (size_t)n * sizeof(HostPoint), hipMemcpyHostToDevice, stream));
         for (DevMem* m : {&kx, &ky, &kz, &kv}) m->reserve((size_t)n * sizeof(float));
         for (DevMem* m : {&ord, &lpos, &rpos}) m->reserve((size_t)n * sizeof(uint32_t));
         boxes.reserve((size_t)n * 6 * sizeof(float));
@@ -144,7 +143,7 @@ class DeviceCloudBuild final : public CloudBuild {
             launchBvhFit(stream, dChunks.as<uint4>() + sh.levelChunk[d], sh.levelChunk[d + 1] - sh.levelChunk[d],
                          ord.as<uint32_t>(), boxes.as<float>(), acc.as<uint32_t>());
             launchBvhSelect(stream, dBig.as<uint4>() + sh.levelBig[d], sh.levelBig[d + 1] - sh.levelBig[d],
-                            acc.as<uint32_t>(), kx.as<float>(), ky.as<float>(), kz.as<float>(), kv.as<float>(),
+                            levelMax[d], acc.as<uint32_t>(), kx.as<float>(), ky.as<float>(), kz.as<float>(), kv.as<float>(),
                             ord.as<uint32_t>(), lpos.as<uint32_t>(), rpos.as<uint32_t>(), refBox.as<float>());
         }
         launchBvhSubtrees(stream, dRoots.as<uint4>(), (uint32_t)sh.roots.size(), dInternal.as<uint32_t>(),
@@ -165,19 +164,20 @@ class DeviceCloudBuild final : public CloudBuild {
         uint32_t fl;
         std::memcpy(&fl, hb + permB + boxB, sizeof(fl));
         if (fl) return false;  // non-finite or -0 box coordinates: the host builds this scene
-        out = sh.topo;
         perm.resize(n);
         const uint32_t* hperm = reinterpret_cast<const uint32_t*>(hb);
         const float* hbox = reinterpret_cast<const float*>(hb + permB);
-        parallelFor((n + kChunk - 1) / kChunk, [&](size_t c) {
-            const size_t a = c * kChunk, b = std::min<size_t>(n, a + kChunk);
-            std::memcpy(perm.data() + a, hperm + a, (b - a) * sizeof(uint32_t));
+        const size_t base = blas.size(), nn = sh.topo.size();
+        blas.resize(base + nn);
+        RefNode* out = blas.data() + base;
+        parallelFor((std::max<size_t>(n, nn) + kChunk - 1) / kChunk, [&](size_t c) {
+            const size_t a = c * kChunk;
+            if (a < n) std::memcpy(perm.data() + a, hperm + a, (std::min<size_t>(n, a + kChunk) - a) * sizeof(uint32_t));
+            for (size_t i = a; i < std::min(nn, a + kChunk); ++i) {
+                const float* b = hbox + 6 * i;
+                out[i] = RefNode{mk3(b[0], b[1], b[2]), mk3(b[3], b[4], b[5]), sh.topo[i].leftFirst, sh.topo[i].count};
+            }
         });
-        for (size_t i = 0; i < out.size(); ++i) {
-            const float* b = hbox + 6 * i;
-            out[i].mn = mk3(b[0], b[1], b[2]);
-            out[i].mx = mk3(b[3], b[4], b[5]);
-        }
         depthOut = sh.depth;
         return true;
     }
@@ -186,6 +186,7 @@ class DeviceCloudBuild final : public CloudBuild {
     // shape of the tree for shapeN surfels
     uint32_t shapeN = kInvalid;
     CloudShape sh;
+    std::vector<uint32_t> levelMax;  // largest node of each level
     DevMem dBig, dChunks, dRoots, dInternal, dWide;
     // per build
     DevMem dPts, kx, ky, kz, kv, ord, lpos, rpos, boxes, refBox, acc, flags;
@@ -201,6 +202,9 @@ class DeviceCloudBuild final : public CloudBuild {
 
     void shape(uint32_t n) {
         sh = cloudShape(n, kSubtree, kFitChunk);
+        levelMax.assign(sh.levelBig.size(), 0);
+        for (size_t d = 0; d + 1 < sh.levelBig.size(); ++d)
+            for (uint32_t b = sh.levelBig[d]; b < sh.levelBig[d + 1]; ++b) levelMax[d] = std::max(levelMax[d], sh.big[b].y);
         auto put = [&](DevMem& m, const auto& v) {
             m.reserve(std::max<size_t>(1, v.size()) * sizeof(v[0]));
             if (!v.empty()) HIPCHECK(hipMemcpyAsync(m.p, v.data(), v.size() * sizeof(v[0]), hipMemcpyHostToDevice, stream));
@@ -514,8 +518,8 @@ struct pale_handle {
         cloudBuild.shade = &shade;
         cloudBuild.nodes = &nodes;
         cloudBuild.minCount = S.bvh_build == 1 ? 0u : 65536u;
-        bp = buildScene(meshes, asset, S.bvh_build == 2 ? nullptr : &cloudBuild);
-        img = makeDeviceImage(bp, meshes);
+        buildScene(meshes, asset, bp, S.bvh_build == 2 ? nullptr : &cloudBuild);
+        makeDeviceImage(bp, meshes, img);
         uploadScene();
         setScene();
         stats.bvh_on_device = bp.cloudOnDevice ? 1u : 0u;
@@ -2256,30 +2260,35 @@ int pale_apply_points(pale_handle* h, const pale_points_desc* points) {
             throw StatusError(PALE_ERR_SIZE,
                               "apply_point_optimization expects consistent point count; use densification API for "
                               "adding/removing points.");
-        const std::vector<AssetPoint> in = toAsset(*points);
+        const pale_points_desc& p = *points;
+        if (!p.position || !p.tangent_u || !p.tangent_v || !p.scale || !p.albedo || !p.opacity || !p.beta)
+            throw StatusError(PALE_ERR_INVALID_ARG, "points: missing array");
         h->syncHostPoints();
-        for (size_t i = 0; i < in.size(); ++i) {
-            HostPoint& q = h->bp.points[i];
-            q.p = in[i].p;
-            q.tu = in[i].tu;
-            q.tv = in[i].tv;
-            q.su = in[i].su;
-            q.sv = in[i].sv;
-            q.albedo = in[i].albedo;
-            q.opacity = in[i].opacity;
-            q.beta = in[i].beta;
-            // Mirror into the asset (shape is not mirrored, as in the
-            // reference, pale_bindings.cpp:1025-1040).
-            AssetPoint& a = h->asset[i];
-            a.p = q.p;
-            a.tu = q.tu;
-            a.tv = q.tv;
-            a.su = q.su;
-            a.sv = q.sv;
-            a.albedo = q.albedo;
-            a.opacity = q.opacity;
-            a.beta = q.beta;
-        }
+        constexpr size_t kChunk = 1u << 14;
+        parallelFor((p.count + kChunk - 1) / kChunk, [&](size_t c) {
+            for (size_t i = c * kChunk; i < std::min<size_t>(p.count, (c + 1) * kChunk); ++i) {
+                HostPoint& q = h->bp.points[i];
+                q.p = ld3h(p.position + 3 * i);
+                q.tu = ld3h(p.tangent_u + 3 * i);
+                q.tv = ld3h(p.tangent_v + 3 * i);
+                q.su = p.scale[2 * i];
+                q.sv = p.scale[2 * i + 1];
+                q.albedo = ld3h(p.albedo + 3 * i);
+                q.opacity = p.opacity[i];
+                q.beta = p.beta[i];
+                // Mirror into the asset (shape is not mirrored, as in the
+                // reference, pale_bindings.cpp:1025-1040).
+                AssetPoint& a = h->asset[i];
+                a.p = q.p;
+                a.tu = q.tu;
+                a.tv = q.tv;
+                a.su = q.su;
+                a.sv = q.sv;
+                a.albedo = q.albedo;
+                a.opacity = q.opacity;
+                a.beta = q.beta;
+            }
+        });
         refreshSurfels(h->bp.points, h->img);
         upload(h->stream, h->geo, h->img.surfGeo);
         upload(h->stream, h->shade, h->img.surfShade);

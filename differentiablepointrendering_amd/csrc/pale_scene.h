@@ -140,15 +140,20 @@ struct BuildProducts {
     // The surfel BLAS was built on the device, which already holds the
     // surfel records and the cloud's wide nodes (after the meshes' ones).
     bool cloudOnDevice = false;
+    std::vector<HostPoint> collected;  // scratch of a host build: the points in asset order
 };
 
-// A device build of the surfel BLAS.  It returns the reference nodes and the
-// permutation (perm[slot] = index in pts), equal to the host build's, and the
-// tree depth, and leaves the surfel records and the wide nodes (the cloud's
-// first at wide index wideBase) on the device; false = build on the host.
+// A device build of the surfel BLAS.  staging(n) is where the scene build
+// collects the n points (pinned memory), or null when the device build does
+// not take a cloud of n points.  build() appends the reference nodes to
+// `blas` and fills the permutation (perm[slot] = index in pts), both equal to
+// the host build's, and the tree depth, and leaves the surfel records and
+// the wide nodes (the cloud's first at wide index wideBase) on the device;
+// false (nothing appended) = build on the host.
 struct CloudBuild {
     virtual ~CloudBuild() = default;
-    virtual bool build(const std::vector<HostPoint>& pts, uint32_t wideBase, std::vector<RefNode>& nodes,
+    virtual HostPoint* staging(uint32_t n) = 0;
+    virtual bool build(const HostPoint* pts, uint32_t n, uint32_t wideBase, std::vector<RefNode>& blas,
                        std::vector<uint32_t>& perm, uint32_t& depth) = 0;
 };
 
@@ -172,10 +177,10 @@ struct CloudShape {
 };
 CloudShape cloudShape(uint32_t n, uint32_t subtree, uint32_t fitChunk);
 
-// Builds everything from the meshes and the asset points (the surfel BLAS
-// with `cloud` when given and it accepts the scene).
-BuildProducts buildScene(const std::vector<HostMesh>& meshes, const std::vector<AssetPoint>& asset,
-                         CloudBuild* cloud = nullptr);
+// Builds everything from the meshes and the asset points into bp (the
+// surfel BLAS with `cloud` when given and it accepts the scene).
+void buildScene(const std::vector<HostMesh>& meshes, const std::vector<AssetPoint>& asset, BuildProducts& bp,
+                CloudBuild* cloud = nullptr);
 
 // Device-layout images of the build products (host memory).
 struct DeviceImage {
@@ -195,7 +200,9 @@ struct DeviceImage {
     uint32_t meshWide = 0;
 };
 
-DeviceImage makeDeviceImage(const BuildProducts& bp, const std::vector<HostMesh>& meshes);
+// Rebuilt in place: the surfel records keep their memory (after a device
+// build they are left as they are, not current, until refreshSurfels).
+void makeDeviceImage(const BuildProducts& bp, const std::vector<HostMesh>& meshes, DeviceImage& img);
 
 // Refresh only the surfel records of an image (apply_point_optimization).
 void refreshSurfels(const std::vector<HostPoint>& pts, DeviceImage& img);

@@ -329,9 +329,22 @@ void cameraFocal(HostCamera& c) {
     c.fx = c.fy * (W / H);
 }
 
-BuildProducts buildScene(const std::vector<HostMesh>& meshes, const std::vector<AssetPoint>& asset,
-                         CloudBuild* cloud) {
-    BuildProducts bp;
+void buildScene(const std::vector<HostMesh>& meshes, const std::vector<AssetPoint>& asset, BuildProducts& bp,
+                CloudBuild* cloud) {
+    // The products of the last build are rebuilt in place: the large arrays
+    // keep their memory (a rebuild of the same size writes into pages that
+    // are already mapped).
+    bp.verts.clear();
+    bp.tris.clear();
+    bp.meshRanges.clear();
+    bp.instances.clear();
+    bp.blas.clear();
+    bp.blasRanges.clear();
+    bp.tlas.clear();
+    bp.lights.clear();
+    bp.emTris.clear();
+    bp.blasMaxDepth = bp.tlasMaxDepth = 0;
+    bp.cloudOnDevice = false;
     // collectGeometry (SceneBuild.cpp:72-127)
     const float oneThird = 1.0f / 3.0f;
     std::vector<uint32_t> vertBase;
@@ -356,8 +369,15 @@ BuildProducts buildScene(const std::vector<HostMesh>& meshes, const std::vector<
         std::copy(meshes[i].w2o, meshes[i].w2o + 16, in.w2o);
         bp.instances.push_back(in);
     }
-    // collectPointCloudGeometry (SceneBuild.cpp:181-227)
-    std::vector<HostPoint> pts(asset.size());
+    // collectPointCloudGeometry (SceneBuild.cpp:181-227), into the device
+    // build's staging memory when it takes the scene
+    const uint32_t n = (uint32_t)asset.size();
+    HostPoint* pts = cloud ? cloud->staging(n) : nullptr;
+    const bool staged = pts != nullptr;
+    if (!staged) {
+        bp.collected.resize(n);
+        pts = bp.collected.data();
+    }
     parallelFor((asset.size() + 16383) / 16384, [&](size_t c) {
       for (size_t i = c * 16384; i < std::min(asset.size(), (c + 1) * 16384); ++i) {
         const AssetPoint& a = asset[i];
@@ -452,15 +472,16 @@ BuildProducts buildScene(const std::vector<HostMesh>& meshes, const std::vector<
     }
     uint32_t cloudDepth = 0;
     {
-        std::vector<RefNode> nodes;
         const size_t chunk = 1u << 14;
         uint32_t wideBase = 0;
         for (const Range& r : bp.blasRanges)
             for (uint32_t i = 0; i < r.count; ++i)
                 wideBase += (bp.blas[r.first + i].count == 0 && r.count > 1) ? 1u : 0u;
-        if (cloud && cloud->build(pts, wideBase, nodes, bp.perm, cloudDepth)) {
+        const uint32_t first = (uint32_t)bp.blas.size();
+        if (staged && cloud->build(pts, n, wideBase, bp.blas, bp.perm, cloudDepth)) {
             bp.cloudOnDevice = true;
         } else {
+            std::vector<RefNode> nodes;
             // Each surfel as one record (centroid, box, index), so that the
             // selections and the box fits stream through memory instead of
             // gathering keys and boxes by index.
@@ -469,29 +490,28 @@ BuildProducts buildScene(const std::vector<HostMesh>& meshes, const std::vector<
                 uint32_t idx;
                 Box box;
             };
-            std::vector<Prim> prims(pts.size());
-            parallelFor((pts.size() + chunk - 1) / chunk, [&](size_t c) {
-                for (size_t i = c * chunk; i < std::min(pts.size(), (c + 1) * chunk); ++i)
+            std::vector<Prim> prims(n);
+            parallelFor((n + chunk - 1) / chunk, [&](size_t c) {
+                for (size_t i = c * chunk; i < std::min<size_t>(n, (c + 1) * chunk); ++i)
                     prims[i] = Prim{{pts[i].p.x, pts[i].p.y, pts[i].p.z}, (uint32_t)i, surfelBox(pts[i])};
             });
             auto ms = medianSplit(
                 nodes, prims, [](const Prim& e) { return e.box; }, [](const Prim& e, int a) { return e.key[a]; },
                 kMaxLeafPrims);
-            ms.build((uint32_t)pts.size());
-            bp.perm.resize(pts.size());
-            parallelFor((pts.size() + chunk - 1) / chunk, [&](size_t c) {
-                for (size_t k = c * chunk; k < std::min(pts.size(), (c + 1) * chunk); ++k) bp.perm[k] = prims[k].idx;
+            ms.build(n);
+            bp.perm.resize(n);
+            parallelFor((n + chunk - 1) / chunk, [&](size_t c) {
+                for (size_t k = c * chunk; k < std::min<size_t>(n, (c + 1) * chunk); ++k) bp.perm[k] = prims[k].idx;
             });
             cloudDepth = maxDepth(nodes, 0, (uint32_t)nodes.size());
+            bp.blas.insert(bp.blas.end(), nodes.begin(), nodes.end());
         }
-        bp.points.resize(pts.size());
-        parallelFor((pts.size() + chunk - 1) / chunk, [&](size_t c) {
-            for (size_t k = c * chunk; k < std::min(pts.size(), (c + 1) * chunk); ++k) bp.points[k] = pts[bp.perm[k]];
+        bp.points.resize(n);
+        parallelFor((n + chunk - 1) / chunk, [&](size_t c) {
+            for (size_t k = c * chunk; k < std::min<size_t>(n, (c + 1) * chunk); ++k) bp.points[k] = pts[bp.perm[k]];
         });
-        const uint32_t first = (uint32_t)bp.blas.size();
-        bp.blas.insert(bp.blas.end(), nodes.begin(), nodes.end());
         const uint32_t bi = (uint32_t)bp.blasRanges.size();
-        bp.blasRanges.push_back(Range{first, (uint32_t)nodes.size()});
+        bp.blasRanges.push_back(Range{first, (uint32_t)bp.blas.size() - first});
         for (HostInstance& in : bp.instances) in.blas = (in.type == kGeoMesh) ? meshToBlas[in.geom] : bi;
     }
     // the surfel BLAS's depth comes with its build
@@ -562,7 +582,6 @@ BuildProducts buildScene(const std::vector<HostMesh>& meshes, const std::vector<
     };
     if (!boxes.empty()) build(0, (uint32_t)boxes.size(), 0);
     bp.tlasMaxDepth = tdepth;
-    return bp;
 }
 
 void refreshSurfels(const std::vector<HostPoint>& pts, DeviceImage& img) {
@@ -577,9 +596,15 @@ void refreshSurfels(const std::vector<HostPoint>& pts, DeviceImage& img) {
     });
 }
 
-DeviceImage makeDeviceImage(const BuildProducts& bp, const std::vector<HostMesh>& meshes) {
-    DeviceImage img;
+void makeDeviceImage(const BuildProducts& bp, const std::vector<HostMesh>& meshes, DeviceImage& img) {
     if (bp.points.size() >= (1u << 29)) throw std::runtime_error("more than 2^29 surfels");
+    img.nodes.clear();
+    img.tris.clear();
+    img.instances.clear();
+    img.tlas.clear();
+    img.materials.clear();
+    img.lights.clear();
+    img.emTris.clear();
     if (!bp.cloudOnDevice) refreshSurfels(bp.points, img);
     img.tris.resize(bp.tris.size());
     for (size_t i = 0; i < bp.tris.size(); ++i) {
@@ -659,7 +684,6 @@ DeviceImage makeDeviceImage(const BuildProducts& bp, const std::vector<HostMesh>
     // Near-far depth-first traversal keeps at most one pending sibling per
     // level: depth + 1 entries per BLAS, tlasDepth + 1 below it.
     img.stackDepth = (bp.tlasMaxDepth + 1) + (bp.blasMaxDepth + 1);
-    return img;
 }
 
 }  // namespace pale
