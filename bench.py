@@ -121,7 +121,7 @@ def parse():
                          "that scales rocprof FETCH_SIZE / WRITE_SIZE to bytes")
     ap.add_argument("--adjoint-decorrelate", action="store_true",
                     help="plain form: adjoint sample k seeds from 42 + k (with --decorrelate: every sample independent)")
-    ap.add_argument("--form", choices=("plain", "independent", "batched", "served", "training"), default=None,
+    ap.add_argument("--form", choices=("plain", "independent", "batched", "served", "training", "bvh"), default=None,
                     help="run only this form (plain: the headline step; independent: the same step with every "
                          "gather pass and adjoint sample independently seeded)")
     ap.add_argument("--marker", action="store_true",
@@ -569,6 +569,36 @@ def training_leg(a):
             "bvh_rebuilt_on": "GPU" if on_device else "host", "host_threads": os.cpu_count()}
 
 
+def bvh_leg(a):
+    """rebuild_bvh of the plain workload's 1,048,576 surfels after
+    apply_point_optimization of the current parameters (SceneBuild.cpp:642-
+    772 each time), `warmup` untimed then `steps` timed: wall time of the
+    whole scene build and upload, and where the surfel BVH was built."""
+    from differentiablepointrendering_amd import scene
+    from differentiablepointrendering_amd.pale import Renderer
+
+    spec = scene.config_scene(1)
+    r = Renderer.from_spec(spec, dict(spec.settings, photons=20_000))
+    step = {k: v for k, v in r.get_point_parameters().items() if k != "shape"}
+    ms = []
+    for i in range(a.warmup + a.steps):
+        if i == a.warmup and a.marker:
+            marker()
+        r.apply_point_optimization(step)
+        t = time.perf_counter()
+        r.rebuild_bvh()
+        dt = (time.perf_counter() - t) * 1e3
+        if i >= a.warmup:
+            ms.append(dt)
+        step = {k: v for k, v in r.get_point_parameters().items() if k != "shape"}
+        print(f"[bench] rebuild {i}: {dt:.1f} ms", file=sys.stderr, flush=True)
+    on_device = bool(r.stats()["bvh_on_device"])
+    r.close()
+    return {"workload": f"{len(spec.points['position'])} clustered surfels: rebuild_bvh after "
+                        f"apply_point_optimization", "rebuilds": len(ms), "ms": round(float(np.mean(ms)), 2),
+            "ms_min": round(float(np.min(ms)), 2), "bvh_built_on": "GPU" if on_device else "host"}
+
+
 def served_hbm():
     """HBM rate of the served form while the device is busy, from the counted
     traffic of every kernel of a served run (its summary in profiles/:
@@ -999,6 +1029,9 @@ def main():
         return 0
     if a.form == "training":
         print(json.dumps({"training_iteration": training_leg(a)}))
+        return 0
+    if a.form == "bvh":
+        print(json.dumps({"bvh_rebuild": bvh_leg(a)}))
         return 0
 
     from differentiablepointrendering_amd import scene

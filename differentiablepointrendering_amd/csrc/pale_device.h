@@ -93,8 +93,10 @@ __device__ __forceinline__ bool slab(F3 o, F3 inv, float mnx, float mny, float m
 // accesses compile to LDS instructions; the host pass only parses the code.
 #if defined(__HIP_DEVICE_COMPILE__)
 typedef __attribute__((address_space(3))) uint2 LdsUint2;
+typedef __attribute__((address_space(3))) uint64_t LdsU64;
 #else
 typedef uint2 LdsUint2;
+typedef uint64_t LdsU64;
 #endif
 
 struct LaneStack {
@@ -116,11 +118,15 @@ struct LaneStack {
             ++sp;
         }
     }
-    // The LDS read is unconditional (an LDS instruction, not a generic one
-    // that could reach either memory); a spilled entry is then read again
-    // from the spill area.
+    // The LDS read is unconditional and a relaxed atomic load, so that it
+    // stays an LDS instruction: two plain loads would be merged into one
+    // generic (flat) load of a selected address, which goes through the
+    // vector-memory path even when it reads LDS.  A spilled entry is then
+    // read again from the spill area.
     __device__ __forceinline__ uint2 get(uint32_t k) const {
-        uint2 v = base[min(k, inLds - 1u) * stride];
+        const LdsU64* p = reinterpret_cast<const LdsU64*>(base + min(k, inLds - 1u) * stride);
+        const uint64_t bits = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        uint2 v = make_uint2((uint32_t)bits, (uint32_t)(bits >> 32));
         if (k >= inLds) v = spill[(k - inLds) * spillStride];
         return v;
     }
@@ -431,8 +437,40 @@ __device__ bool traverseMesh(const DevScene& S, const DevInstance& I, F3 o, F3 d
 
 // intersectScene, FirstHit mode (IntersectionKernels.h:954-1097).  The TLAS
 // stack shares the lane's LDS stack below the BLAS entries.
+// The scene's top level (TLAS nodes and instances), read where it lies in
+// global memory (TopGlobal) or from a copy in LDS that a workgroup makes when
+// it starts (TopLds: every lane of a wave reads the same few nodes and
+// instances, which the LDS broadcasts without using the vector-memory path).
+struct TopGlobal {
+    const TlasNode* tlas;
+    const DevInstance* inst;
+};
+constexpr uint32_t kTopLdsTlas = 15, kTopLdsInst = 8;
+struct TopLds {
+    TlasNode tlas[kTopLdsTlas];
+    DevInstance inst[kTopLdsInst];
+};
+__host__ __device__ inline bool topFitsLds(const DevScene& S) {
+    return S.nTlas <= kTopLdsTlas && S.nInst <= kTopLdsInst;
+}
+// Copy of the top level into LDS by the whole workgroup (then a barrier).
+__device__ inline void loadTopLds(const DevScene& S, TopLds& t) {
+    const uint32_t nt = S.nTlas * (uint32_t)(sizeof(TlasNode) / 4), ni = S.nInst * (uint32_t)(sizeof(DevInstance) / 4);
+    const uint32_t* gt = reinterpret_cast<const uint32_t*>(S.tlas);
+    const uint32_t* gi = reinterpret_cast<const uint32_t*>(S.inst);
+    uint32_t* lt = reinterpret_cast<uint32_t*>(t.tlas);
+    uint32_t* li = reinterpret_cast<uint32_t*>(t.inst);
+    for (uint32_t k = threadIdx.x; k < nt; k += blockDim.x) lt[k] = gt[k];
+    for (uint32_t k = threadIdx.x; k < ni; k += blockDim.x) li[k] = gi[k];
+    __syncthreads();
+}
+
 template <bool kCount, int kLeafWords = 4>
-__device__ bool intersectScene(const DevScene& S, F3 o, F3 d, const LaneStack& st, Hit& h, Work& w) {
+__device__ bool intersectScene(const DevScene& S, F3 o, F3 d, const LaneStack& st, Hit& h, Work& w);
+
+template <bool kCount, int kLeafWords, typename Top>
+__device__ bool intersectSceneTop(const DevScene& S, const Top& top, F3 o, F3 d, const LaneStack& st, Hit& h,
+                                  Work& w) {
     h.inst = kInvalid;
     if (S.nTlas == 0) return false;
     const F3 inv = safeInvDir(d);
@@ -442,7 +480,7 @@ __device__ bool intersectScene(const DevScene& S, F3 o, F3 d, const LaneStack& s
     st.push(sp, 0u, 0.0f);
     while (sp > 0) {
         const uint32_t ni = st.get(--sp).x;
-        const TlasNode nd = S.tlas[ni];
+        const TlasNode nd = top.tlas[ni];
         if (kCount) {
             w.pops++;
             w.aabb++;
@@ -450,7 +488,7 @@ __device__ bool intersectScene(const DevScene& S, F3 o, F3 d, const LaneStack& s
         float te;
         if (!slab(o, inv, nd.mnx, nd.mny, nd.mnz, nd.mxx, nd.mxy, nd.mxz, bestW, 0.0f, te)) continue;
         if (nd.count == 0) {
-            const TlasNode L = S.tlas[nd.left], R = S.tlas[nd.right];
+            const TlasNode L = top.tlas[nd.left], R = top.tlas[nd.right];
             if (kCount) w.aabb += 2;
             float tl = __builtin_huge_valf(), tr = __builtin_huge_valf();
             const bool hl = slab(o, inv, L.mnx, L.mny, L.mnz, L.mxx, L.mxy, L.mxz, bestW, 0.0f, tl);
@@ -471,7 +509,7 @@ __device__ bool intersectScene(const DevScene& S, F3 o, F3 d, const LaneStack& s
             continue;
         }
         const uint32_t ii = nd.left;
-        const DevInstance& I = S.inst[ii];
+        const DevInstance& I = top.inst[ii];
         if (I.empty) continue;
         // toObjectSpace (KernelHelpers.h:217-227)
         float hv[4];
@@ -501,6 +539,11 @@ __device__ bool intersectScene(const DevScene& S, F3 o, F3 d, const LaneStack& s
         }
     }
     return found;
+}
+
+template <bool kCount, int kLeafWords>
+__device__ bool intersectScene(const DevScene& S, F3 o, F3 d, const LaneStack& st, Hit& h, Work& w) {
+    return intersectSceneTop<kCount, kLeafWords>(S, TopGlobal{S.tlas, S.inst}, o, d, st, h, w);
 }
 
 // Gradient of alpha_geom (intersectSurfel, KernelHelpers.h:637-669) with

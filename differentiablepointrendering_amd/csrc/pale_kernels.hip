@@ -718,11 +718,13 @@ struct BatchMap {
 // Slots a wave of the layer walk claims at a time from its region.
 constexpr uint32_t kWalkUnit = 64;
 
-template <bool kCount, typename Map>
+template <bool kCount, typename Map, bool kTopLds>
 __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_cam_trace(
     DevScene S, DevState* st, Map map, PixelArrays px, EntryArrays ent, uint32_t cap, uint32_t* cursor,
     uint2* spill, uint32_t inLds) {
     extern __shared__ uint2 s_stack[];
+    __shared__ TopLds sTop;
+    if (kTopLds) loadTopLds(S, sTop);
     const LaneStack stk = walkStack(s_stack, S.stackDepth, spill, inLds);
     const uint32_t lane = laneId();
     const uint32_t slots = map.slots();
@@ -797,8 +799,10 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(4))
         Work wl{};
         if (active) {
             Hit h;
-            if (intersectScene<kCount, 2>(S, o, d, stk, h, wl)) {
-                const DevInstance& I = S.inst[h.inst];
+            const bool hit = kTopLds ? intersectSceneTop<kCount, 2>(S, sTop, o, d, stk, h, wl)
+                                     : intersectSceneTop<kCount, 2>(S, TopGlobal{S.tlas, S.inst}, o, d, stk, h, wl);
+            if (hit) {
+                const DevInstance& I = kTopLds ? sTop.inst[h.inst] : S.inst[h.inst];
                 want = true;
                 if (I.type == kGeoCloud) {
                     const SurfelShade sh = S.shade[h.prim];
@@ -1731,7 +1735,7 @@ void launchBuildGrid(hipStream_t s, const GridParams& g, DevState* st, uint32_t 
 
 // Persistent grid of the camera trace: as many workgroups as fit on the device.
 // Renderers on several host threads share the cache.
-template <bool kCount, typename Map>
+template <bool kCount, typename Map, bool kTopLds>
 static uint32_t camTraceGrid(size_t lds) {
     static std::mutex lock;
     static uint32_t cached[2] = {0, 0};
@@ -1742,7 +1746,7 @@ static uint32_t camTraceGrid(size_t lds) {
     int dev = 0, cus = 0, perCu = 0;
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, k_cam_trace<kCount, Map>, kTraceBlock, lds);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, k_cam_trace<kCount, Map, kTopLds>, kTraceBlock, lds);
     cached[k] = (uint32_t)std::max(1, cus) * (uint32_t)std::max(1, perCu);
     cachedLds[k] = lds;
     return cached[k];
@@ -1771,8 +1775,11 @@ static size_t walkLdsBytes(const DevScene& S, const uint2* spill, uint32_t inLds
 size_t walkSpillBytes(const DevScene& S, uint32_t inLds) {
     if (S.stackDepth <= inLds) return 0;
     const size_t lds = (size_t)inLds * sizeof(uint2) * kTraceBlock;
-    const uint32_t grid = std::max(std::max(camTraceGrid<false, BatchMap>(lds), camTraceGrid<true, BatchMap>(lds)),
-                                   std::max(camTraceGrid<false, ImageMap>(lds), camTraceGrid<true, ImageMap>(lds)));
+    const uint32_t grid =
+        std::max(std::max(std::max(camTraceGrid<false, BatchMap, false>(lds), camTraceGrid<true, BatchMap, false>(lds)),
+                          std::max(camTraceGrid<false, ImageMap, false>(lds), camTraceGrid<true, ImageMap, false>(lds))),
+                 std::max(std::max(camTraceGrid<false, BatchMap, true>(lds), camTraceGrid<true, BatchMap, true>(lds)),
+                          std::max(camTraceGrid<false, ImageMap, true>(lds), camTraceGrid<true, ImageMap, true>(lds))));
     return (size_t)(S.stackDepth - inLds) * grid * kTraceBlock * sizeof(uint2);
 }
 
@@ -1783,12 +1790,21 @@ void launchCamTrace(hipStream_t s, const DevScene& S, DevState* st, uint32_t npx
     if (S.stackDepth <= inLds) spill = nullptr;
     const size_t lds = walkLdsBytes(S, spill, inLds);
     const ImageMap map{width, npx / width, (width + 7u) / 8u};
+    const bool top = topFitsLds(S);
     if (count) {
-        const uint32_t g = min(camTraceGrid<true, ImageMap>(lds), blocks(npx, kTraceBlock));
-        k_cam_trace<true><<<g, kTraceBlock, lds, s>>>(S, st, map, px, ent, cap, cursor, spill, inLds);
+        const uint32_t g = min(top ? camTraceGrid<true, ImageMap, true>(lds) : camTraceGrid<true, ImageMap, false>(lds),
+                               blocks(npx, kTraceBlock));
+        if (top)
+            k_cam_trace<true, ImageMap, true><<<g, kTraceBlock, lds, s>>>(S, st, map, px, ent, cap, cursor, spill, inLds);
+        else
+            k_cam_trace<true, ImageMap, false><<<g, kTraceBlock, lds, s>>>(S, st, map, px, ent, cap, cursor, spill, inLds);
     } else {
-        const uint32_t g = min(camTraceGrid<false, ImageMap>(lds), blocks(npx, kTraceBlock));
-        k_cam_trace<false><<<g, kTraceBlock, lds, s>>>(S, st, map, px, ent, cap, cursor, spill, inLds);
+        const uint32_t g = min(top ? camTraceGrid<false, ImageMap, true>(lds) : camTraceGrid<false, ImageMap, false>(lds),
+                               blocks(npx, kTraceBlock));
+        if (top)
+            k_cam_trace<false, ImageMap, true><<<g, kTraceBlock, lds, s>>>(S, st, map, px, ent, cap, cursor, spill, inLds);
+        else
+            k_cam_trace<false, ImageMap, false><<<g, kTraceBlock, lds, s>>>(S, st, map, px, ent, cap, cursor, spill, inLds);
     }
 }
 
@@ -1804,14 +1820,21 @@ void launchCamTraceBatch(hipStream_t s, const DevScene& S, DevState* st, BatchDe
     if (S.stackDepth <= inLds) spill = nullptr;
     const size_t lds = walkLdsBytes(S, spill, inLds);
     const BatchMap map{b.tiles, b.segs, b.hdr};
+    const bool top = topFitsLds(S);
     if (count) {
-        const uint32_t full = camTraceGrid<true, BatchMap>(lds);
+        const uint32_t full = top ? camTraceGrid<true, BatchMap, true>(lds) : camTraceGrid<true, BatchMap, false>(lds);
         const uint32_t g = min(max(1u, (uint32_t)((float)full * gridFraction)), blocks(maxSlots, kTraceBlock));
-        k_cam_trace<true><<<g, kTraceBlock, lds, s>>>(S, st, map, px, ent, cap, cursor, spill, inLds);
+        if (top)
+            k_cam_trace<true, BatchMap, true><<<g, kTraceBlock, lds, s>>>(S, st, map, px, ent, cap, cursor, spill, inLds);
+        else
+            k_cam_trace<true, BatchMap, false><<<g, kTraceBlock, lds, s>>>(S, st, map, px, ent, cap, cursor, spill, inLds);
     } else {
-        const uint32_t full = camTraceGrid<false, BatchMap>(lds);
+        const uint32_t full = top ? camTraceGrid<false, BatchMap, true>(lds) : camTraceGrid<false, BatchMap, false>(lds);
         const uint32_t g = min(max(1u, (uint32_t)((float)full * gridFraction)), blocks(maxSlots, kTraceBlock));
-        k_cam_trace<false><<<g, kTraceBlock, lds, s>>>(S, st, map, px, ent, cap, cursor, spill, inLds);
+        if (top)
+            k_cam_trace<false, BatchMap, true><<<g, kTraceBlock, lds, s>>>(S, st, map, px, ent, cap, cursor, spill, inLds);
+        else
+            k_cam_trace<false, BatchMap, false><<<g, kTraceBlock, lds, s>>>(S, st, map, px, ent, cap, cursor, spill, inLds);
     }
 }
 
