@@ -177,8 +177,8 @@ def _read_csvs(dirs, suffix):
 
 
 def _short(name: str) -> str:
-    """Kernel name without template arguments and parameter list."""
-    name = name.split("(")[0].split("<")[0].strip()
+    """Kernel name without namespaces, template arguments and parameter list."""
+    name = name.replace("(anonymous namespace)::", "").split("(")[0].split("<")[0].strip()
     return name.split()[-1].split("::")[-1] if name else name
 
 
