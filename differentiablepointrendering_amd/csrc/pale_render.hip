@@ -2042,9 +2042,10 @@ int pale_apply_points_device(pale_handle* h, uint32_t n, const float* position, 
             throw StatusError(PALE_ERR_SIZE,
                               "apply_point_optimization expects consistent point count; use densification API for "
                               "adding/removing points.");
-        // Host fields that the device arrays do not carry (shape, alpha_r)
-        // must be current before the mirror is marked stale again.
-        h->syncHostPoints();
+        // The host mirror is left stale (refreshed from rawPts when host code
+        // next reads it): the fields the device arrays do not carry (shape,
+        // alpha_r, alpha_t) are never changed here, and rawPts always holds
+        // the last applied parameters, so applies in a row need no download.
         h->rawPts.reserve((size_t)n * 16 * sizeof(float));
         launchApplyPoints(h->stream, h->geo.as<SurfelGeo>(), h->shade.as<SurfelShade>(), h->rawPts.as<float>(),
                           DevPointArrays{position, tangent_u, tangent_v, scale, albedo, opacity, beta}, n);

@@ -208,7 +208,8 @@ def test_render_surfels_frozen_map_finite_differences():
 def test_apply_points_device_equals_host_apply():
     """pale_apply_points_device writes the records the host apply writes: the
     same images bit for bit, the same parameters read back, and the same BVH
-    after a rebuild (the host mirror is refreshed from the device)."""
+    after a rebuild (the host mirror is refreshed from the device, also after
+    two device applies in a row)."""
     import torch
 
     spec = scene.config_scene(0, n=4096, width=64, height=64)
@@ -225,6 +226,10 @@ def test_apply_points_device_equals_host_apply():
     q["opacity"] = rng.uniform(0.1, 0.9, p["opacity"].shape).astype(np.float32)
     q["beta"] = rng.uniform(-1, 1, p["beta"].shape).astype(np.float32)
     b.apply_point_optimization(q)
+    # two device applies in a row (the host mirror stays stale in between):
+    # the second one is what counts
+    z = {k: (v * np.float32(0.5)).astype(np.float32) for k, v in q.items() if k != "shape"}
+    a.apply_point_optimization_device({k: torch.from_numpy(v).cuda() for k, v in z.items()})
     a.apply_point_optimization_device({k: torch.from_numpy(v).cuda() for k, v in q.items() if k != "shape"})
     ia = a.render_forward()["camera1_raw"]
     ib = b.render_forward()["camera1_raw"]
