@@ -302,6 +302,8 @@ struct pale_handle {
     uint32_t adjCap = 0, poolCap = 0;
     DevMem pend0, pend1, pend2, pend3, pend4, evStg[6], evCmp[6], evCell, evSorted, evFlags, dloss, gradOpacity,
         gradFull;
+    // Host gradient outputs: the split arrays on the device, then pinned.
+    DevMem gradStage, gradHost;
     bool adjFull = false;  // event rays allocated (full gradients or debug images)
     // Camera gather: pixel state, entries of one round, their cell sort.
     uint32_t pxCap = 0, entryCap = 0;
@@ -1870,24 +1872,47 @@ static const size_t kGradWidth[8] = {3, 3, 3, 2, 3, 1, 1, 1};
 
 static void copyGradsHost(pale_handle* h, float* const* grads) {
     const size_t N = h->bp.points.size();
-    if (!grads) return;
-    std::vector<float> full;
-    if (h->S.full_gradients && N) {
-        full.resize(N * 16);
-        HIPCHECK(hipMemcpy(full.data(), h->gradFull.p, N * 16 * sizeof(float), hipMemcpyDeviceToHost));
-    }
-    const int first[8] = {0, 3, 6, 9, 11, -1, 15, -1};  // columns of the N x 16 accumulator
+    if (!grads || !N) return;
+    // The arrays are split on the device (as for device outputs), come back
+    // in one copy into pinned staging memory, and are copied into the
+    // caller's arrays by host threads.  Staging layout: position, tangent_u,
+    // tangent_v, scale, albedo, opacity, beta, one after another.
+    size_t off[8], total = 0;
     for (int g = 0; g < 8; ++g) {
-        if (!grads[g]) continue;
-        if (g == 5) {
-            if (N) HIPCHECK(hipMemcpy(grads[g], h->gradOpacity.p, N * sizeof(float), hipMemcpyDeviceToHost));
-        } else if (!full.empty() && first[g] >= 0) {
-            for (size_t i = 0; i < N; ++i)
-                for (size_t k = 0; k < kGradWidth[g]; ++k) grads[g][i * kGradWidth[g] + k] = full[i * 16 + first[g] + k];
-        } else {
-            std::memset(grads[g], 0, N * kGradWidth[g] * sizeof(float));
-        }
+        off[g] = total;
+        if (g != 7) total += N * kGradWidth[g];
     }
+    h->gradStage.reserve(total * sizeof(float));
+    float* d = h->gradStage.as<float>();
+    const bool full = h->S.full_gradients != 0;
+    if (full)
+        launchSplitGradients(h->stream, h->gradFull.as<float>(), (uint32_t)N, d + off[0], d + off[1], d + off[2],
+                             d + off[3], d + off[4], d + off[6]);
+    HIPCHECK(hipMemcpyAsync(d + off[5], h->gradOpacity.p, N * sizeof(float), hipMemcpyDeviceToDevice, h->stream));
+    if (!h->gradHost.p || h->gradHost.bytes < total * sizeof(float)) {
+        h->gradHost.release();
+        HIPCHECK(hipHostMalloc(&h->gradHost.p, total * sizeof(float), hipHostMallocDefault));
+        h->gradHost.bytes = total * sizeof(float);
+        h->gradHost.pinned = true;
+    }
+    float* st = h->gradHost.as<float>();
+    // without full gradients only the opacity column is computed
+    const size_t from = full ? 0 : off[5], to = full ? total : off[6];
+    HIPCHECK(hipMemcpyAsync(st + from, d + from, (to - from) * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+    HIPCHECK(hipStreamSynchronize(h->stream));
+    HIPCHECK(hipGetLastError());
+    constexpr size_t kChunk = 1u << 15;
+    parallelFor((N + kChunk - 1) / kChunk, [&](size_t c) {
+        const size_t a = c * kChunk, b = std::min(N, a + kChunk);
+        for (int g = 0; g < 8; ++g) {
+            if (!grads[g]) continue;
+            const size_t w = kGradWidth[g];
+            if (g == 5 || (full && g != 7))
+                std::memcpy(grads[g] + a * w, st + off[g] + a * w, (b - a) * w * sizeof(float));
+            else
+                std::memset(grads[g] + a * w, 0, (b - a) * w * sizeof(float));
+        }
+    });
 }
 
 static void copyGradsDevice(pale_handle* h, float* const* grads_dev) {
