@@ -886,21 +886,28 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(4))
     if (kCount) flushWork(w, st, kWorkGather);
 }
 
-// Runs of equal keys among consecutive lanes of a wave: the lane that heads
-// this lane's run and the run's length.  Every lane of the wave calls it.
-struct KeyRun {
-    uint32_t head, len;
+// The lanes of a wave that hold the same key, wherever they sit: the lowest
+// of them (the group's head), their number and this lane's rank among them.
+// One pass per distinct key of the wave; every lane of the wave calls it.
+struct KeyGroup {
+    uint32_t head, len, rank;
 };
-__device__ __forceinline__ KeyRun keyRun(uint32_t key) {
+__device__ __forceinline__ KeyGroup keyGroup(uint32_t key) {
     const uint32_t lane = laneId();
-    const uint32_t next = __shfl_down(key, 1, 64);
-    const uint32_t prev = __shfl_up(key, 1, 64);
-    const unsigned long long heads = __ballot(lane == 0u || prev != key);
-    const unsigned long long ends = __ballot(lane == 63u || next != key);
-    const unsigned long long upTo = lane == 63u ? ~0ull : ((2ull << lane) - 1ull);
-    const uint32_t head = 63u - (uint32_t)__clzll((long long)(heads & upTo));
-    const uint32_t end = lane + (uint32_t)__ffsll((long long)(ends >> lane)) - 1u;
-    return KeyRun{head, end - head + 1u};
+    unsigned long long todo = __ballot(1);
+    KeyGroup g{lane, 1u, 0u};
+    while (todo) {
+        const uint32_t leader = (uint32_t)__ffsll((long long)todo) - 1u;
+        const uint32_t lk = __shfl(key, (int)leader, 64);
+        const unsigned long long peers = __ballot(key == lk);
+        if (key == lk) {
+            g.head = leader;
+            g.len = (uint32_t)__popcll(peers);
+            g.rank = (uint32_t)__popcll(peers & ((1ull << lane) - 1ull));
+        }
+        todo &= ~peers;
+    }
+    return g;
 }
 
 constexpr uint32_t kNoCell = 0xFFFFFFFFu;
@@ -909,8 +916,8 @@ constexpr uint32_t kNoCell = 0xFFFFFFFFu;
 // per-cell count of the gather entries (the cell of worldToCellClamped,
 // KernelHelpers.h:920-934).  An entry that gathers nothing (an emitter) has
 // its contribution w * k written here (the gather's E is one).  Neighbouring
-// entries come from neighbouring lanes of the layer walk, so equal cells
-// arrive in runs: one atomic per run.
+// entries come from neighbouring lanes of the layer walk, so a wave holds
+// few distinct cells: one atomic per distinct cell of the wave.
 // The entry kernels stride over the entries with a bounded grid: the count
 // is known only on the device and the capacity is far above it.  The loop is
 // uniform over the block, so whole waves take part in the run detection.
@@ -938,25 +945,25 @@ __global__ __launch_bounds__(256) void k_entry_key(GridParams g, const DevState*
             }
             ent.cell[i] = c;
         }
-        const KeyRun r = keyRun(c);
+        const KeyGroup r = keyGroup(c);
         if (laneId() == r.head && c != kNoCell) atomicAdd(&cellCount[c], r.len);
     }
 }
 
-// Counting-sort scatter of the gather entries; leaves cellCount zeroed.  A
-// run of equal cells takes its places with one atomic.
+// Counting-sort scatter of the gather entries; leaves cellCount zeroed.  The
+// lanes of a wave with the same cell take their places with one atomic.
 __global__ __launch_bounds__(256) void k_entry_scatter(const DevState* st, uint32_t cap, const uint32_t* cellStart,
                                                        uint32_t* cellCount, EntryArrays ent) {
     const uint32_t n = min(st->entries, cap);
     for (uint32_t i0 = blockIdx.x * blockDim.x; i0 < n; i0 += gridDim.x * blockDim.x) {
         const uint32_t i = i0 + threadIdx.x;
         const uint32_t c = i < n ? ent.cell[i] : kNoCell;
-        const KeyRun r = keyRun(c);
+        const KeyGroup r = keyGroup(c);
         const uint32_t lane = laneId();
         uint32_t base = 0;
         if (lane == r.head && c != kNoCell) base = cellStart[c] + atomicSub(&cellCount[c], r.len) - r.len;
         base = __shfl(base, (int)r.head, 64);
-        if (c != kNoCell) ent.sorted[base + (lane - r.head)] = i;
+        if (c != kNoCell) ent.sorted[base + r.rank] = i;
     }
 }
 
@@ -1365,7 +1372,7 @@ __global__ __launch_bounds__(256) void k_event_key(GridParams g, const DevState*
             c = (uint32_t)((z * g.ry + y) * g.rx + x);
             cell[i] = c;
         }
-        const KeyRun r = keyRun(c);
+        const KeyGroup r = keyGroup(c);
         if (laneId() == r.head && c != kNoCell) atomicAdd(&cellCount[c], r.len);
     }
 }
@@ -1380,12 +1387,12 @@ __global__ __launch_bounds__(256) void k_event_scatter(const DevState* st, uint3
     for (uint32_t i0 = blockIdx.x * blockDim.x; i0 < n; i0 += gridDim.x * blockDim.x) {
         const uint32_t i = i0 + threadIdx.x;
         const uint32_t c = i < n ? cell[i] : kNoCell;
-        const KeyRun r = keyRun(c);
+        const KeyGroup r = keyGroup(c);
         const uint32_t lane = laneId();
         uint32_t base = 0;
         if (lane == r.head && c != kNoCell) base = cellStart[c] + atomicSub(&cellCount[c], r.len) - r.len;
         base = __shfl(base, (int)r.head, 64);
-        if (c != kNoCell) sorted[base + (lane - r.head)] = i;
+        if (c != kNoCell) sorted[base + r.rank] = i;
     }
 }
 
