@@ -184,7 +184,8 @@ __global__ __launch_bounds__(kFitBlock) void k_bvh_fit(const uint4* chunks, cons
 template <uint32_t kSelBlock>
 __global__ __launch_bounds__(kSelBlock) void k_bvh_select(const uint4* nodes, const uint32_t* acc, const float* kx,
                                                          const float* ky, const float* kz, float* kv, uint32_t* ord,
-                                                         uint32_t* lpos, uint32_t* rpos, float* refBox) {
+                                                         uint32_t* lpos, uint32_t* rpos, float* refBox,
+                                                         const BvhWideState* resume) {
     constexpr uint32_t kSelWaves = kSelBlock / 64;
     constexpr uint32_t kSelChunk = kSelBlock * kSelItems;
     __shared__ uint32_t sw[kSelWaves + 1];
@@ -195,17 +196,25 @@ __global__ __launch_bounds__(kSelBlock) void k_bvh_select(const uint4* nodes, co
     __shared__ uint32_t sI32[kSelSeq];
     const uint4 nd = nodes[blockIdx.x];
     const uint32_t first = nd.x, count = nd.y, tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
-    const uint32_t* a = acc + 6 * (size_t)nd.w;
-    const F3 mn = mk3(fromOrd(a[0]), fromOrd(a[1]), fromOrd(a[2]));
-    const F3 mx = mk3(fromOrd(a[3]), fromOrd(a[4]), fromOrd(a[5]));
-    if (tid < 6) refBox[6 * (size_t)nd.z + tid] = tid < 3 ? (&mn.x)[tid] : (&mx.x)[tid - 3];
-    const float* key = keysOf(kx, ky, kz, longestAxis(mn, mx));
-    for (uint32_t p = first + tid; p < first + count; p += kSelBlock) kv[p] = key[ord[p]];
-    __syncthreads();
-
     uint32_t lo = first, hi = first + count;
     const uint32_t nth = first + (count >> 1);
     int depth = 2 * sel::floorLog2(count);
+    if (resume) {
+        // the state the workgroups of k_wsel_* left (box, keys and the first
+        // introselect steps done)
+        const BvhWideState w = resume[blockIdx.x];
+        lo = w.lo;
+        hi = w.hi;
+        depth = (int)w.depth;
+    } else {
+        const uint32_t* a = acc + 6 * (size_t)nd.w;
+        const F3 mn = mk3(fromOrd(a[0]), fromOrd(a[1]), fromOrd(a[2]));
+        const F3 mx = mk3(fromOrd(a[3]), fromOrd(a[4]), fromOrd(a[5]));
+        if (tid < 6) refBox[6 * (size_t)nd.z + tid] = tid < 3 ? (&mn.x)[tid] : (&mx.x)[tid - 3];
+        const float* key = keysOf(kx, ky, kz, longestAxis(mn, mx));
+        for (uint32_t p = first + tid; p < first + count; p += kSelBlock) kv[p] = key[ord[p]];
+        __syncthreads();
+    }
     while (true) {
         const uint32_t len = hi - lo;
         if (len <= kSelSeq) {
@@ -335,6 +344,186 @@ __global__ __launch_bounds__(kSelBlock) void k_bvh_select(const uint4* nodes, co
     }
 }
 
+// The introselect steps of large nodes spread over many workgroups (levels
+// whose nodes exceed kWideMin surfels, where one workgroup per node would
+// leave most of the chip idle).  The node's range is cut into tiles of
+// kWideTile positions; per step: k_wsel_pivot (one thread per node: the
+// median of three moved to the front), k_wsel_count (keys >= P and <= P
+// per tile), k_wsel_scan (their prefixes over the node's tiles),
+// k_wsel_rank (each tile ranks its positions and records the swap partners,
+// as k_bvh_select does over the whole range), k_wsel_swap, k_wsel_cut (the
+// new range).  A node whose range has come down to kWideMin, or whose depth
+// limit is reached, stops; k_bvh_select then continues from its state.
+// tiles[t] = {node (index in the level), first, end, tile index in node};
+// tileOf[w] = {first tile, tiles} of node w.
+constexpr uint32_t kWideBlock = 256;
+constexpr uint32_t kWideChunk = kWideBlock * kSelItems;
+
+__global__ __launch_bounds__(kWideBlock) void k_wsel_init(const uint4* nodes, const uint4* tiles, const uint32_t* acc,
+                                                         const float* kx, const float* ky, const float* kz,
+                                                         float* kv, const uint32_t* ord, float* refBox,
+                                                         BvhWideState* ws) {
+    const uint4 t = tiles[blockIdx.x];
+    const uint4 nd = nodes[t.x];
+    const uint32_t* a = acc + 6 * (size_t)nd.w;
+    const F3 mn = mk3(fromOrd(a[0]), fromOrd(a[1]), fromOrd(a[2]));
+    const F3 mx = mk3(fromOrd(a[3]), fromOrd(a[4]), fromOrd(a[5]));
+    const float* key = keysOf(kx, ky, kz, longestAxis(mn, mx));
+    for (uint32_t p = t.y + threadIdx.x; p < t.z; p += kWideBlock) kv[p] = key[ord[p]];
+    if (t.w == 0) {
+        if (threadIdx.x < 6) refBox[6 * (size_t)nd.z + threadIdx.x] = threadIdx.x < 3 ? (&mn.x)[threadIdx.x]
+                                                                                         : (&mx.x)[threadIdx.x - 3];
+        if (threadIdx.x == 0) {
+            BvhWideState w{};
+            w.lo = nd.x;
+            w.hi = nd.x + nd.y;
+            w.depth = (uint32_t)(2 * sel::floorLog2(nd.y));
+            w.active = 1u;
+            ws[t.x] = w;
+        }
+    }
+}
+
+__global__ __launch_bounds__(64) void k_wsel_pivot(const uint4* nodes, uint32_t nNodes, float* kv, uint32_t* ord,
+                                                  BvhWideState* ws, uint32_t wideMin) {
+    const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= nNodes) return;
+    BvhWideState s = ws[w];
+    if (!s.active) return;
+    if (s.hi - s.lo <= wideMin || s.depth == 0) {
+        ws[w].active = 0u;
+        return;
+    }
+    --s.depth;
+    sel::medianToFirst(kv, ord, s.lo, s.lo + 1, s.lo + (s.hi - s.lo) / 2, s.hi - 1);
+    s.P = kv[s.lo];
+    s.K = 0u;
+    s.LK = s.hi;
+    ws[w] = s;
+}
+
+__global__ __launch_bounds__(kWideBlock) void k_wsel_count(const uint4* tiles, const float* kv,
+                                                          const BvhWideState* ws, uint2* tileCnt) {
+    __shared__ uint32_t sw[kWideBlock / 64 + 1];
+    const uint4 t = tiles[blockIdx.x];
+    const BvhWideState s = ws[t.x];
+    if (!s.active) return;
+    const uint32_t a = max(t.y, s.lo + 1), b = min(t.z, s.hi);
+    uint32_t g = 0, l = 0;
+    for (uint32_t p = a + threadIdx.x; p < b; p += kWideBlock) {
+        const float k = kv[p];
+        g += !(k < s.P) ? 1u : 0u;
+        l += !(s.P < k) ? 1u : 0u;
+    }
+    const uint32_t tot = blockSum<kWideBlock>(g | (l << 16), sw);  // a tile holds at most kWideTile < 2^16 keys
+    if (threadIdx.x == 0) tileCnt[blockIdx.x] = make_uint2(tot & 0xFFFFu, tot >> 16);
+}
+
+__global__ __launch_bounds__(64) void k_wsel_scan(const uint2* tileOf, uint32_t nNodes, const uint2* tileCnt,
+                                                 uint2* tilePre, BvhWideState* ws) {
+    const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= nNodes || !ws[w].active) return;
+    const uint2 tr = tileOf[w];
+    uint32_t g = 0, l = 0;
+    for (uint32_t t = tr.x; t < tr.x + tr.y; ++t) {
+        tilePre[t] = make_uint2(g, l);
+        const uint2 c = tileCnt[t];
+        g += c.x;
+        l += c.y;
+    }
+    ws[w].nLE = l;
+}
+
+__global__ __launch_bounds__(kWideBlock) void k_wsel_rank(const uint4* tiles, const uint4* nodes, const float* kv,
+                                                         BvhWideState* ws, const uint2* tilePre, uint32_t* lpos,
+                                                         uint32_t* rpos) {
+    __shared__ uint32_t sTot[2][kWideBlock / 64];
+    const uint4 t = tiles[blockIdx.x];
+    const BvhWideState s = ws[t.x];
+    if (!s.active) return;
+    const uint32_t first = nodes[t.x].x, lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint32_t a = max(t.y, s.lo + 1), b = min(t.z, s.hi);
+    const float P = s.P;
+    const uint32_t nLE = s.nLE;
+    const uint2 pre0 = tilePre[blockIdx.x];
+    uint32_t carryG = pre0.x, carryL = pre0.y, kacc = 0, lk = 0xFFFFFFFFu, par = 0;
+    for (uint32_t base = a; base < b; base += kWideChunk, par ^= 1u) {
+        const uint32_t b0 = base + threadIdx.x * kSelItems;
+        float k[kSelItems];
+        uint32_t g = 0, l = 0;
+#pragma unroll
+        for (uint32_t j = 0; j < kSelItems; ++j) {
+            const bool in = b0 + j < b;
+            k[j] = in ? kv[b0 + j] : 0.0f;
+            g += (in && !(k[j] < P)) ? 1u : 0u;
+            l += (in && !(P < k[j])) ? 1u : 0u;
+        }
+        const uint32_t x = g | (l << 16);
+        const uint32_t incl = waveInclusiveScan(x);
+        if (lane == 63u) sTot[par][wave] = incl;
+        __syncthreads();
+        uint32_t before = 0, tot = 0;
+#pragma unroll
+        for (uint32_t q = 0; q < kWideBlock / 64; ++q) {
+            const uint32_t v = sTot[par][q];
+            before += q < wave ? v : 0u;
+            tot += v;
+        }
+        const uint32_t pre = before + incl - x;
+        uint32_t G = carryG + (pre & 0xFFFFu), Lc = carryL + (pre >> 16);
+#pragma unroll
+        for (uint32_t j = 0; j < kSelItems; ++j) {
+            if (b0 + j >= b) break;
+            const bool ge = !(k[j] < P), le = !(P < k[j]);
+            if (le) ++Lc;
+            const uint32_t E = nLE - Lc;
+            if (ge) {
+                if (E > G) {
+                    lpos[first + G] = b0 + j;
+                    ++kacc;
+                } else {
+                    lk = min(lk, b0 + j);
+                }
+            }
+            if (le && G > E) rpos[first + E] = b0 + j;
+            if (ge) ++G;
+        }
+        carryG += tot & 0xFFFFu;
+        carryL += tot >> 16;
+    }
+    if (lk != 0xFFFFFFFFu) atomicMin(&ws[t.x].LK, lk);
+    if (kacc) atomicAdd(&ws[t.x].K, kacc);
+}
+
+__global__ __launch_bounds__(kWideBlock) void k_wsel_swap(const uint4* tiles, const uint4* nodes,
+                                                         const BvhWideState* ws, float* kv, uint32_t* ord,
+                                                         const uint32_t* lpos, const uint32_t* rpos,
+                                                         uint32_t tileSize) {
+    const uint4 t = tiles[blockIdx.x];
+    const BvhWideState s = ws[t.x];
+    if (!s.active) return;
+    const uint32_t first = nodes[t.x].x;
+    const uint32_t m0 = t.w * tileSize, m1 = min(m0 + tileSize, s.K);
+    for (uint32_t m = m0 + threadIdx.x; m < m1; m += kWideBlock) sel::swapAt(kv, ord, lpos[first + m], rpos[first + m]);
+}
+
+__global__ __launch_bounds__(64) void k_wsel_cut(const uint4* nodes, uint32_t nNodes, BvhWideState* ws,
+                                                const uint32_t* rpos) {
+    const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= nNodes) return;
+    BvhWideState s = ws[w];
+    if (!s.active) return;
+    const uint4 nd = nodes[w];
+    const uint32_t nth = nd.x + (nd.y >> 1);
+    uint32_t cut = s.LK;
+    if (s.K > 0) cut = min(cut, rpos[nd.x + s.K - 1]);
+    if (cut <= nth)
+        s.lo = cut;
+    else
+        s.hi = cut;
+    ws[w] = s;
+}
+
 // roots[k] = {first, count, ref, preorder rank among internal nodes}: one
 // thread builds the whole subtree below a node of at most kSubtree
 // surfels.  The children of the internal node of rank r are the pair at
@@ -420,14 +609,31 @@ void launchBvhFit(hipStream_t s, const uint4* chunks, uint32_t nChunks, const ui
 
 void launchBvhSelect(hipStream_t s, const uint4* nodes, uint32_t nNodes, uint32_t count, const uint32_t* acc,
                      const float* kx, const float* ky, const float* kz, float* kv, uint32_t* ord, uint32_t* lpos,
-                     uint32_t* rpos, float* refBox) {
+                     uint32_t* rpos, float* refBox, const BvhWideState* resume) {
     if (!nNodes) return;
     if (count > 32768)
-        k_bvh_select<1024><<<nNodes, 1024, 0, s>>>(nodes, acc, kx, ky, kz, kv, ord, lpos, rpos, refBox);
+        k_bvh_select<1024><<<nNodes, 1024, 0, s>>>(nodes, acc, kx, ky, kz, kv, ord, lpos, rpos, refBox, resume);
     else if (count > 2048)
-        k_bvh_select<256><<<nNodes, 256, 0, s>>>(nodes, acc, kx, ky, kz, kv, ord, lpos, rpos, refBox);
+        k_bvh_select<256><<<nNodes, 256, 0, s>>>(nodes, acc, kx, ky, kz, kv, ord, lpos, rpos, refBox, resume);
     else
-        k_bvh_select<64><<<nNodes, 64, 0, s>>>(nodes, acc, kx, ky, kz, kv, ord, lpos, rpos, refBox);
+        k_bvh_select<64><<<nNodes, 64, 0, s>>>(nodes, acc, kx, ky, kz, kv, ord, lpos, rpos, refBox, resume);
+}
+
+void launchBvhWideSteps(hipStream_t s, const uint4* nodes, uint32_t nNodes, const uint4* tiles, uint32_t nTiles,
+                        const uint2* tileOf, uint32_t tileSize, uint32_t steps, uint32_t wideMin, const uint32_t* acc,
+                        const float* kx, const float* ky, const float* kz, float* kv, uint32_t* ord, uint32_t* lpos,
+                        uint32_t* rpos, float* refBox, BvhWideState* ws, uint2* tileCnt, uint2* tilePre) {
+    if (!nNodes || !nTiles) return;
+    const uint32_t nb = nblocks(nNodes, 64);
+    k_wsel_init<<<nTiles, kWideBlock, 0, s>>>(nodes, tiles, acc, kx, ky, kz, kv, ord, refBox, ws);
+    for (uint32_t k = 0; k < steps; ++k) {
+        k_wsel_pivot<<<nb, 64, 0, s>>>(nodes, nNodes, kv, ord, ws, wideMin);
+        k_wsel_count<<<nTiles, kWideBlock, 0, s>>>(tiles, kv, ws, tileCnt);
+        k_wsel_scan<<<nb, 64, 0, s>>>(tileOf, nNodes, tileCnt, tilePre, ws);
+        k_wsel_rank<<<nTiles, kWideBlock, 0, s>>>(tiles, nodes, kv, ws, tilePre, lpos, rpos);
+        k_wsel_swap<<<nTiles, kWideBlock, 0, s>>>(tiles, nodes, ws, kv, ord, lpos, rpos, tileSize);
+        k_wsel_cut<<<nb, 64, 0, s>>>(nodes, nNodes, ws, rpos);
+    }
 }
 
 void launchBvhSubtrees(hipStream_t s, const uint4* roots, uint32_t nRoots, const uint32_t* internalOf, const float* kx,

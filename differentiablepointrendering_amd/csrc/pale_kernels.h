@@ -304,10 +304,25 @@ void launchBvhPrims(hipStream_t s, const HostPoint* pts, uint32_t n, float* kx, 
 void launchBvhInitFit(hipStream_t s, uint32_t* acc, uint32_t nNodes);
 void launchBvhFit(hipStream_t s, const uint4* chunks, uint32_t nChunks, const uint32_t* ord, const float* boxes,
                   uint32_t* acc);
-// count: the level's largest node (picks the workgroup size)
+// State of a large node's introselect between the multi-workgroup steps:
+// the current range and depth limit, the step's pivot, its counts and its
+// cut candidates.
+struct BvhWideState {
+    uint32_t lo, hi, depth, active;
+    float P;
+    uint32_t nLE, K, LK;
+};
+// count: the level's largest node (picks the workgroup size); resume: the
+// state each node's introselect continues from (null: from the start)
 void launchBvhSelect(hipStream_t s, const uint4* nodes, uint32_t nNodes, uint32_t count, const uint32_t* acc,
                      const float* kx, const float* ky, const float* kz, float* kv, uint32_t* ord, uint32_t* lpos,
-                     uint32_t* rpos, float* refBox);
+                     uint32_t* rpos, float* refBox, const BvhWideState* resume);
+// `steps` introselect steps of a level's nodes over many workgroups (ranges
+// cut into tiles of tileSize positions); a node stops at wideMin positions
+void launchBvhWideSteps(hipStream_t s, const uint4* nodes, uint32_t nNodes, const uint4* tiles, uint32_t nTiles,
+                        const uint2* tileOf, uint32_t tileSize, uint32_t steps, uint32_t wideMin, const uint32_t* acc,
+                        const float* kx, const float* ky, const float* kz, float* kv, uint32_t* ord, uint32_t* lpos,
+                        uint32_t* rpos, float* refBox, BvhWideState* ws, uint2* tileCnt, uint2* tilePre);
 void launchBvhSubtrees(hipStream_t s, const uint4* roots, uint32_t nRoots, const uint32_t* internalOf, const float* kx,
                        const float* ky, const float* kz, const float* boxes, float* kv, uint32_t* ord, float* refBox);
 void launchBvhWide(hipStream_t s, const uint4* wide, uint32_t nWide, const float* refBox, uint32_t base,
