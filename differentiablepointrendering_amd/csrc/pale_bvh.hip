@@ -170,6 +170,38 @@ __global__ __launch_bounds__(kFitBlock) void k_bvh_fit(const uint4* chunks, cons
     }
 }
 
+// Swaps m0 <= m < m1 exchange positions lp[m] and rp[m] (keys and
+// indices); each thread takes kSwapIlp swaps at a time, loads before stores.
+template <uint32_t kBlock>
+__device__ void swapPairs(float* kv, uint32_t* ord, const uint32_t* lp, const uint32_t* rp, uint32_t m0, uint32_t m1) {
+    for (uint32_t m = m0 + threadIdx.x * kSwapIlp; m < m1; m += kBlock * kSwapIlp) {
+        uint32_t pa[kSwapIlp], pb[kSwapIlp], ia[kSwapIlp], ib[kSwapIlp];
+        float ka[kSwapIlp], kb[kSwapIlp];
+#pragma unroll
+        for (uint32_t j = 0; j < kSwapIlp; ++j)
+            if (m + j < m1) {
+                pa[j] = lp[m + j];
+                pb[j] = rp[m + j];
+            }
+#pragma unroll
+        for (uint32_t j = 0; j < kSwapIlp; ++j)
+            if (m + j < m1) {
+                ka[j] = kv[pa[j]];
+                kb[j] = kv[pb[j]];
+                ia[j] = ord[pa[j]];
+                ib[j] = ord[pb[j]];
+            }
+#pragma unroll
+        for (uint32_t j = 0; j < kSwapIlp; ++j)
+            if (m + j < m1) {
+                kv[pa[j]] = kb[j];
+                kv[pb[j]] = ka[j];
+                ord[pa[j]] = ib[j];
+                ord[pb[j]] = ia[j];
+            }
+    }
+}
+
 // nodes[k] = {first, count, ref, slot in acc}: one workgroup per node.
 //
 // One introselect step's partition of [lo + 1, hi) around P, by ranks
@@ -310,32 +342,7 @@ __global__ __launch_bounds__(kSelBlock) void k_bvh_select(const uint4* nodes, co
         uint32_t cut = sLK;
         if (K > 0) cut = min(cut, rpos[first + K - 1]);
         // pass C
-        for (uint32_t m0 = tid * kSwapIlp; m0 < K; m0 += kSelBlock * kSwapIlp) {
-            uint32_t pa[kSwapIlp], pb[kSwapIlp], ia[kSwapIlp], ib[kSwapIlp];
-            float ka[kSwapIlp], kb[kSwapIlp];
-#pragma unroll
-            for (uint32_t j = 0; j < kSwapIlp; ++j)
-                if (m0 + j < K) {
-                    pa[j] = lpos[first + m0 + j];
-                    pb[j] = rpos[first + m0 + j];
-                }
-#pragma unroll
-            for (uint32_t j = 0; j < kSwapIlp; ++j)
-                if (m0 + j < K) {
-                    ka[j] = kv[pa[j]];
-                    kb[j] = kv[pb[j]];
-                    ia[j] = ord[pa[j]];
-                    ib[j] = ord[pb[j]];
-                }
-#pragma unroll
-            for (uint32_t j = 0; j < kSwapIlp; ++j)
-                if (m0 + j < K) {
-                    kv[pa[j]] = kb[j];
-                    kv[pb[j]] = ka[j];
-                    ord[pa[j]] = ib[j];
-                    ord[pb[j]] = ia[j];
-                }
-        }
+        swapPairs<kSelBlock>(kv, ord, lpos + first, rpos + first, 0, K);
         __syncthreads();
         if (cut <= nth)
             lo = cut;
@@ -345,19 +352,20 @@ __global__ __launch_bounds__(kSelBlock) void k_bvh_select(const uint4* nodes, co
 }
 
 // The introselect steps of large nodes spread over many workgroups (levels
-// whose nodes exceed kWideMin surfels, where one workgroup per node would
+// whose nodes exceed wideMin surfels, where one workgroup per node would
 // leave most of the chip idle).  The node's range is cut into tiles of
-// kWideTile positions; per step: k_wsel_pivot (one thread per node: the
-// median of three moved to the front), k_wsel_count (keys >= P and <= P
-// per tile), k_wsel_scan (their prefixes over the node's tiles),
-// k_wsel_rank (each tile ranks its positions and records the swap partners,
-// as k_bvh_select does over the whole range), k_wsel_swap, k_wsel_cut (the
-// new range).  A node whose range has come down to kWideMin, or whose depth
-// limit is reached, stops; k_bvh_select then continues from its state.
-// tiles[t] = {node (index in the level), first, end, tile index in node};
-// tileOf[w] = {first tile, tiles} of node w.
+// tileSize positions; per step: k_wsel_next (one thread per node: the cut
+// of the step before, then the median of three moved to the front as the
+// pivot), k_wsel_count (keys >= P and <= P per tile), k_wsel_rank (each
+// tile adds up the counts of the node's earlier tiles, ranks its positions
+// and records the swap partners, as k_bvh_select does over the whole
+// range), k_wsel_swap.  A node whose range has come down to wideMin, or
+// whose depth limit is reached, stops; k_bvh_select then continues from its
+// state.  tiles[t] = {node (index in the level), first, end, tile index in
+// node}; a node's tiles are consecutive.
 constexpr uint32_t kWideBlock = 256;
 constexpr uint32_t kWideChunk = kWideBlock * kSelItems;
+constexpr uint32_t kSwapSplit = 4;  // swap workgroups per tile
 
 __global__ __launch_bounds__(kWideBlock) void k_wsel_init(const uint4* nodes, const uint4* tiles, const uint32_t* acc,
                                                          const float* kx, const float* ky, const float* kz,
@@ -384,21 +392,34 @@ __global__ __launch_bounds__(kWideBlock) void k_wsel_init(const uint4* nodes, co
     }
 }
 
-__global__ __launch_bounds__(64) void k_wsel_pivot(const uint4* nodes, uint32_t nNodes, float* kv, uint32_t* ord,
-                                                  BvhWideState* ws, uint32_t wideMin) {
+__global__ __launch_bounds__(64) void k_wsel_next(const uint4* nodes, uint32_t nNodes, float* kv, uint32_t* ord,
+                                                 BvhWideState* ws, const uint32_t* rpos, uint32_t wideMin,
+                                                 uint32_t cut, uint32_t pivot) {
     const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
     if (w >= nNodes) return;
     BvhWideState s = ws[w];
     if (!s.active) return;
-    if (s.hi - s.lo <= wideMin || s.depth == 0) {
-        ws[w].active = 0u;
-        return;
+    if (cut) {
+        const uint4 nd = nodes[w];
+        const uint32_t nth = nd.x + (nd.y >> 1);
+        uint32_t c = s.LK;
+        if (s.K > 0) c = min(c, rpos[nd.x + s.K - 1]);
+        if (c <= nth)
+            s.lo = c;
+        else
+            s.hi = c;
     }
-    --s.depth;
-    sel::medianToFirst(kv, ord, s.lo, s.lo + 1, s.lo + (s.hi - s.lo) / 2, s.hi - 1);
-    s.P = kv[s.lo];
-    s.K = 0u;
-    s.LK = s.hi;
+    if (pivot) {
+        if (s.hi - s.lo <= wideMin || s.depth == 0) {
+            s.active = 0u;
+        } else {
+            --s.depth;
+            sel::medianToFirst(kv, ord, s.lo, s.lo + 1, s.lo + (s.hi - s.lo) / 2, s.hi - 1);
+            s.P = kv[s.lo];
+            s.K = 0u;
+            s.LK = s.hi;
+        }
+    }
     ws[w] = s;
 }
 
@@ -409,44 +430,69 @@ __global__ __launch_bounds__(kWideBlock) void k_wsel_count(const uint4* tiles, c
     const BvhWideState s = ws[t.x];
     if (!s.active) return;
     const uint32_t a = max(t.y, s.lo + 1), b = min(t.z, s.hi);
-    uint32_t g = 0, l = 0;
-    for (uint32_t p = a + threadIdx.x; p < b; p += kWideBlock) {
-        const float k = kv[p];
-        g += !(k < s.P) ? 1u : 0u;
-        l += !(s.P < k) ? 1u : 0u;
+    const float P = s.P;
+    uint32_t g = 0, l = 0, p = a + threadIdx.x;
+    for (; p + 7 * kWideBlock < b; p += 8 * kWideBlock) {
+        float k[8];
+#pragma unroll
+        for (uint32_t j = 0; j < 8; ++j) k[j] = kv[p + j * kWideBlock];
+#pragma unroll
+        for (uint32_t j = 0; j < 8; ++j) {
+            g += !(k[j] < P) ? 1u : 0u;
+            l += !(P < k[j]) ? 1u : 0u;
+        }
     }
-    const uint32_t tot = blockSum<kWideBlock>(g | (l << 16), sw);  // a tile holds at most kWideTile < 2^16 keys
+    for (; p < b; p += kWideBlock) {
+        const float k = kv[p];
+        g += !(k < P) ? 1u : 0u;
+        l += !(P < k) ? 1u : 0u;
+    }
+    const uint32_t tot = blockSum<kWideBlock>(g | (l << 16), sw);  // a tile holds fewer than 2^16 keys
     if (threadIdx.x == 0) tileCnt[blockIdx.x] = make_uint2(tot & 0xFFFFu, tot >> 16);
 }
 
-__global__ __launch_bounds__(64) void k_wsel_scan(const uint2* tileOf, uint32_t nNodes, const uint2* tileCnt,
-                                                 uint2* tilePre, BvhWideState* ws) {
-    const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
-    if (w >= nNodes || !ws[w].active) return;
-    const uint2 tr = tileOf[w];
-    uint32_t g = 0, l = 0;
-    for (uint32_t t = tr.x; t < tr.x + tr.y; ++t) {
-        tilePre[t] = make_uint2(g, l);
-        const uint2 c = tileCnt[t];
-        g += c.x;
-        l += c.y;
-    }
-    ws[w].nLE = l;
-}
-
 __global__ __launch_bounds__(kWideBlock) void k_wsel_rank(const uint4* tiles, const uint4* nodes, const float* kv,
-                                                         BvhWideState* ws, const uint2* tilePre, uint32_t* lpos,
-                                                         uint32_t* rpos) {
+                                                         BvhWideState* ws, const uint2* tileCnt, uint32_t tileSize,
+                                                         uint32_t* lpos, uint32_t* rpos) {
     __shared__ uint32_t sTot[2][kWideBlock / 64];
+    __shared__ uint32_t sRed[3][kWideBlock / 64];
     const uint4 t = tiles[blockIdx.x];
     const BvhWideState s = ws[t.x];
     if (!s.active) return;
-    const uint32_t first = nodes[t.x].x, lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint4 nd = nodes[t.x];
+    const uint32_t first = nd.x, lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    // keys >= P and <= P in the node's tiles before this one, and keys <= P
+    // in all of them
+    uint32_t pg = 0, pl = 0, al = 0;
+    {
+        const uint32_t t0 = blockIdx.x - t.w, nt = (nd.y + tileSize - 1) / tileSize;
+        for (uint32_t i = threadIdx.x; i < nt; i += kWideBlock) {
+            const uint2 c = tileCnt[t0 + i];
+            pg += i < t.w ? c.x : 0u;
+            pl += i < t.w ? c.y : 0u;
+            al += c.y;
+        }
+        pg = waveSum(pg);
+        pl = waveSum(pl);
+        al = waveSum(al);
+        if (lane == 0) {
+            sRed[0][wave] = pg;
+            sRed[1][wave] = pl;
+            sRed[2][wave] = al;
+        }
+        __syncthreads();
+        pg = pl = al = 0;
+#pragma unroll
+        for (uint32_t q = 0; q < kWideBlock / 64; ++q) {
+            pg += sRed[0][q];
+            pl += sRed[1][q];
+            al += sRed[2][q];
+        }
+    }
     const uint32_t a = max(t.y, s.lo + 1), b = min(t.z, s.hi);
     const float P = s.P;
-    const uint32_t nLE = s.nLE;
-    const uint2 pre0 = tilePre[blockIdx.x];
-    uint32_t carryG = pre0.x, carryL = pre0.y, kacc = 0, lk = 0xFFFFFFFFu, par = 0;
+    const uint32_t nLE = al;
+    uint32_t carryG = pg, carryL = pl, kacc = 0, lk = 0xFFFFFFFFu, par = 0;
     for (uint32_t base = a; base < b; base += kWideChunk, par ^= 1u) {
         const uint32_t b0 = base + threadIdx.x * kSelItems;
         float k[kSelItems];
@@ -495,33 +541,20 @@ __global__ __launch_bounds__(kWideBlock) void k_wsel_rank(const uint4* tiles, co
     if (kacc) atomicAdd(&ws[t.x].K, kacc);
 }
 
+// kSwapSplit workgroups per tile; the swaps of a node are split over its
+// tiles (a node has more tile positions than swaps)
 __global__ __launch_bounds__(kWideBlock) void k_wsel_swap(const uint4* tiles, const uint4* nodes,
                                                          const BvhWideState* ws, float* kv, uint32_t* ord,
                                                          const uint32_t* lpos, const uint32_t* rpos,
                                                          uint32_t tileSize) {
-    const uint4 t = tiles[blockIdx.x];
+    const uint4 t = tiles[blockIdx.x / kSwapSplit];
     const BvhWideState s = ws[t.x];
     if (!s.active) return;
-    const uint32_t first = nodes[t.x].x;
-    const uint32_t m0 = t.w * tileSize, m1 = min(m0 + tileSize, s.K);
-    for (uint32_t m = m0 + threadIdx.x; m < m1; m += kWideBlock) sel::swapAt(kv, ord, lpos[first + m], rpos[first + m]);
-}
-
-__global__ __launch_bounds__(64) void k_wsel_cut(const uint4* nodes, uint32_t nNodes, BvhWideState* ws,
-                                                const uint32_t* rpos) {
-    const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
-    if (w >= nNodes) return;
-    BvhWideState s = ws[w];
-    if (!s.active) return;
-    const uint4 nd = nodes[w];
-    const uint32_t nth = nd.x + (nd.y >> 1);
-    uint32_t cut = s.LK;
-    if (s.K > 0) cut = min(cut, rpos[nd.x + s.K - 1]);
-    if (cut <= nth)
-        s.lo = cut;
-    else
-        s.hi = cut;
-    ws[w] = s;
+    const uint32_t first = nodes[t.x].x, part = (tileSize + kSwapSplit - 1) / kSwapSplit;
+    const uint32_t tileEnd = (t.w + 1) * tileSize;
+    const uint32_t m0 = t.w * tileSize + (blockIdx.x % kSwapSplit) * part;
+    const uint32_t m1 = min(min(m0 + part, tileEnd), s.K);
+    swapPairs<kWideBlock>(kv, ord, lpos + first, rpos + first, m0, m1);
 }
 
 // roots[k] = {first, count, ref, preorder rank among internal nodes}: one
@@ -620,20 +653,19 @@ void launchBvhSelect(hipStream_t s, const uint4* nodes, uint32_t nNodes, uint32_
 }
 
 void launchBvhWideSteps(hipStream_t s, const uint4* nodes, uint32_t nNodes, const uint4* tiles, uint32_t nTiles,
-                        const uint2* tileOf, uint32_t tileSize, uint32_t steps, uint32_t wideMin, const uint32_t* acc,
-                        const float* kx, const float* ky, const float* kz, float* kv, uint32_t* ord, uint32_t* lpos,
-                        uint32_t* rpos, float* refBox, BvhWideState* ws, uint2* tileCnt, uint2* tilePre) {
+                        uint32_t tileSize, uint32_t steps, uint32_t wideMin, const uint32_t* acc, const float* kx,
+                        const float* ky, const float* kz, float* kv, uint32_t* ord, uint32_t* lpos, uint32_t* rpos,
+                        float* refBox, BvhWideState* ws, uint2* tileCnt) {
     if (!nNodes || !nTiles) return;
     const uint32_t nb = nblocks(nNodes, 64);
     k_wsel_init<<<nTiles, kWideBlock, 0, s>>>(nodes, tiles, acc, kx, ky, kz, kv, ord, refBox, ws);
     for (uint32_t k = 0; k < steps; ++k) {
-        k_wsel_pivot<<<nb, 64, 0, s>>>(nodes, nNodes, kv, ord, ws, wideMin);
+        k_wsel_next<<<nb, 64, 0, s>>>(nodes, nNodes, kv, ord, ws, rpos, wideMin, k > 0 ? 1u : 0u, 1u);
         k_wsel_count<<<nTiles, kWideBlock, 0, s>>>(tiles, kv, ws, tileCnt);
-        k_wsel_scan<<<nb, 64, 0, s>>>(tileOf, nNodes, tileCnt, tilePre, ws);
-        k_wsel_rank<<<nTiles, kWideBlock, 0, s>>>(tiles, nodes, kv, ws, tilePre, lpos, rpos);
-        k_wsel_swap<<<nTiles, kWideBlock, 0, s>>>(tiles, nodes, ws, kv, ord, lpos, rpos, tileSize);
-        k_wsel_cut<<<nb, 64, 0, s>>>(nodes, nNodes, ws, rpos);
+        k_wsel_rank<<<nTiles, kWideBlock, 0, s>>>(tiles, nodes, kv, ws, tileCnt, tileSize, lpos, rpos);
+        k_wsel_swap<<<nTiles * kSwapSplit, kWideBlock, 0, s>>>(tiles, nodes, ws, kv, ord, lpos, rpos, tileSize);
     }
+    if (steps) k_wsel_next<<<nb, 64, 0, s>>>(nodes, nNodes, kv, ord, ws, rpos, wideMin, 1u, 0u);
 }
 
 void launchBvhSubtrees(hipStream_t s, const uint4* roots, uint32_t nRoots, const uint32_t* internalOf, const float* kx,

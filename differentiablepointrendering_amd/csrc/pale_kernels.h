@@ -318,11 +318,12 @@ void launchBvhSelect(hipStream_t s, const uint4* nodes, uint32_t nNodes, uint32_
                      const float* kx, const float* ky, const float* kz, float* kv, uint32_t* ord, uint32_t* lpos,
                      uint32_t* rpos, float* refBox, const BvhWideState* resume);
 // `steps` introselect steps of a level's nodes over many workgroups (ranges
-// cut into tiles of tileSize positions); a node stops at wideMin positions
+// cut into tiles of tileSize positions, tiles[t] = {node, first, end, tile
+// index in node}); a node stops at wideMin positions
 void launchBvhWideSteps(hipStream_t s, const uint4* nodes, uint32_t nNodes, const uint4* tiles, uint32_t nTiles,
-                        const uint2* tileOf, uint32_t tileSize, uint32_t steps, uint32_t wideMin, const uint32_t* acc,
-                        const float* kx, const float* ky, const float* kz, float* kv, uint32_t* ord, uint32_t* lpos,
-                        uint32_t* rpos, float* refBox, BvhWideState* ws, uint2* tileCnt, uint2* tilePre);
+                        uint32_t tileSize, uint32_t steps, uint32_t wideMin, const uint32_t* acc, const float* kx,
+                        const float* ky, const float* kz, float* kv, uint32_t* ord, uint32_t* lpos, uint32_t* rpos,
+                        float* refBox, BvhWideState* ws, uint2* tileCnt);
 void launchBvhSubtrees(hipStream_t s, const uint4* roots, uint32_t nRoots, const uint32_t* internalOf, const float* kx,
                        const float* ky, const float* kz, const float* boxes, float* kv, uint32_t* ord, float* refBox);
 void launchBvhWide(hipStream_t s, const uint4* wide, uint32_t nWide, const float* refBox, uint32_t base,

@@ -151,10 +151,10 @@ class DeviceCloudBuild final : public CloudBuild {
                 uint32_t steps = 3u;  // log2(count / kWideMin) + 3: halving steps, and some to spare
                 for (uint32_t r = levelMax[d] / kWideMin; r > 1u; r >>= 1) ++steps;
                 launchBvhWideSteps(stream, lvl, nNodes, dTiles.as<uint4>() + levelTile[d], levelTile[d + 1] - levelTile[d],
-                                   dTileOf.as<uint2>() + sh.levelBig[d], kWideTile, steps, kWideMin, acc.as<uint32_t>(),
-                                   kx.as<float>(), ky.as<float>(), kz.as<float>(), kv.as<float>(), ord.as<uint32_t>(),
-                                   lpos.as<uint32_t>(), rpos.as<uint32_t>(), refBox.as<float>(),
-                                   wstate.as<BvhWideState>(), tileCnt.as<uint2>(), tilePre.as<uint2>());
+                                   kWideTile, steps, kWideMin, acc.as<uint32_t>(), kx.as<float>(), ky.as<float>(),
+                                   kz.as<float>(), kv.as<float>(), ord.as<uint32_t>(), lpos.as<uint32_t>(),
+                                   rpos.as<uint32_t>(), refBox.as<float>(), wstate.as<BvhWideState>(),
+                                   tileCnt.as<uint2>());
                 resume = wstate.as<BvhWideState>();
             }
             launchBvhSelect(stream, lvl, nNodes, levelMax[d], acc.as<uint32_t>(), kx.as<float>(), ky.as<float>(),
@@ -203,9 +203,9 @@ class DeviceCloudBuild final : public CloudBuild {
     CloudShape sh;
     std::vector<uint32_t> levelMax;   // largest node of each level
     std::vector<uint32_t> levelTile;  // tiles of the multi-workgroup steps, level by level
-    DevMem dBig, dChunks, dRoots, dInternal, dWide, dTiles, dTileOf;
+    DevMem dBig, dChunks, dRoots, dInternal, dWide, dTiles;
     // per build
-    DevMem dPts, kx, ky, kz, kv, ord, lpos, rpos, boxes, refBox, acc, flags, wstate, tileCnt, tilePre;
+    DevMem dPts, kx, ky, kz, kv, ord, lpos, rpos, boxes, refBox, acc, flags, wstate, tileCnt;
     DevMem stage, back;  // pinned host staging
 
     static void pinned(DevMem& m, size_t b) {
@@ -222,10 +222,9 @@ class DeviceCloudBuild final : public CloudBuild {
         for (size_t d = 0; d + 1 < sh.levelBig.size(); ++d)
             for (uint32_t b = sh.levelBig[d]; b < sh.levelBig[d + 1]; ++b) levelMax[d] = std::max(levelMax[d], sh.big[b].y);
         // Tiles of the levels whose nodes take multi-workgroup steps:
-        // {node index in the level, first, end, tile index in the node}, and
-        // per big node {its first tile (in the level), its tiles}.
+        // {node index in the level, first, end, tile index in the node}, a
+        // node's tiles one after the other.
         std::vector<uint4> tiles;
-        std::vector<uint2> tileOf(sh.big.size(), make_uint2(0, 0));
         levelTile.assign(sh.levelBig.size(), 0);
         size_t maxNodes = 1, maxTiles = 1;
         for (size_t d = 0; d + 1 < sh.levelBig.size(); ++d) {
@@ -235,18 +234,15 @@ class DeviceCloudBuild final : public CloudBuild {
             const uint32_t t0 = (uint32_t)tiles.size();
             for (uint32_t b = sh.levelBig[d]; b < sh.levelBig[d + 1]; ++b) {
                 const uint4 nd = sh.big[b];
-                const uint32_t first = (uint32_t)tiles.size() - t0;
                 uint32_t k = 0;
                 for (uint32_t a = nd.x; a < nd.x + nd.y; a += kWideTile, ++k)
                     tiles.push_back(make_uint4(b - sh.levelBig[d], a, std::min(a + kWideTile, nd.x + nd.y), k));
-                tileOf[b] = make_uint2(first, k);
             }
             maxTiles = std::max<size_t>(maxTiles, tiles.size() - t0);
         }
         levelTile.back() = (uint32_t)tiles.size();
         wstate.reserve(maxNodes * sizeof(BvhWideState));
         tileCnt.reserve(maxTiles * sizeof(uint2));
-        tilePre.reserve(maxTiles * sizeof(uint2));
         auto put = [&](DevMem& m, const auto& v) {
             m.reserve(std::max<size_t>(1, v.size()) * sizeof(v[0]));
             if (!v.empty()) HIPCHECK(hipMemcpyAsync(m.p, v.data(), v.size() * sizeof(v[0]), hipMemcpyHostToDevice, stream));
@@ -257,7 +253,6 @@ class DeviceCloudBuild final : public CloudBuild {
         put(dInternal, sh.internalOf);
         put(dWide, sh.wide);
         put(dTiles, tiles);
-        put(dTileOf, tileOf);
         HIPCHECK(hipStreamSynchronize(stream));
         shapeN = n;
     }
