@@ -231,7 +231,16 @@ __device__ bool traverseCloud(const DevScene& S, const DevInstance& I, F3 o, F3 
 // unrelated rays (photon and adjoint bounces); 2 = the plane words only, the
 // tangents of a surfel loaded when its plane hit is in range, which suits
 // the camera walk (fewer bytes per lane when neighbouring rays diverge).
-// Same tests in the same order either way.
+// Same tests in the same order either way.  With kLeafWords = 2 (the camera
+// walk, whose waves hold neighbouring pixels, and with shared seeds the
+// same ray in several passes), a node that every lane of the wave is at is
+// read with scalar loads (in the plain workload 57 % of the wave-wide node
+// reads with shared seeds, 31 % with independent samples); for the photon
+// and adjoint queues the added scalar registers cost more than it saves.
+
+// Node and surfel records read through the constant address space (scalar
+// loads when the address is the same in every lane).
+typedef __attribute__((address_space(4))) const WideNode ConstWideNode;
 
 // The same traversal without the counters, and without a stack round trip
 // for the nearer child: the reference pushes the farther child, then the
@@ -301,9 +310,27 @@ __device__ bool traverseCloudFast(const DevScene& S, const DevInstance& I, F3 o,
                 bestR2 = r2;
             }
         } else {
-            const float4* nd4 = reinterpret_cast<const float4*>(S.nodes + ref);
-            const float4 a = nd4[0], b = nd4[1], c = nd4[2];
-            const uint2 refs = reinterpret_cast<const uint2*>(S.nodes + ref)[6];  // left, right (not the padding)
+            float4 a, b, c;
+            uint2 refs;
+            const uint32_t r0 = __builtin_amdgcn_readfirstlane(ref);
+            if (kLeafWords == 2 && __builtin_amdgcn_ballot_w64(ref != r0) == 0ull) {
+                // Every lane of the wave here is at the same node: read it
+                // with scalar loads, off the vector-memory path.  (The empty
+                // asm keeps the compiler from folding this read into the
+                // vector one below.)
+                __asm__ volatile("");
+                const ConstWideNode* cn = (const ConstWideNode*)(S.nodes + r0);
+                a = make_float4(cn->lminx, cn->lminy, cn->lminz, cn->lmaxx);
+                b = make_float4(cn->lmaxy, cn->lmaxz, cn->rminx, cn->rminy);
+                c = make_float4(cn->rminz, cn->rmaxx, cn->rmaxy, cn->rmaxz);
+                refs = make_uint2(cn->left, cn->right);
+            } else {
+                const float4* nd4 = reinterpret_cast<const float4*>(S.nodes + ref);
+                a = nd4[0];
+                b = nd4[1];
+                c = nd4[2];
+                refs = reinterpret_cast<const uint2*>(S.nodes + ref)[6];  // left, right (not the padding)
+            }
             float tl = __builtin_huge_valf(), tr = __builtin_huge_valf();
             const bool hl = slab(o, inv, a.x, a.y, a.z, a.w, b.x, b.y, best, 1e-6f, tl);
             const bool hr = slab(o, inv, b.z, b.w, c.x, c.y, c.z, c.w, best, 1e-6f, tr);

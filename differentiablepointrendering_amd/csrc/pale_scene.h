@@ -3,7 +3,11 @@
 
 #include <algorithm>
 #include <atomic>
+#include <condition_variable>
 #include <cstdint>
+#include <functional>
+#include <mutex>
+#include <unistd.h>
 #include <string>
 #include <thread>
 #include <vector>
@@ -209,15 +213,98 @@ void refreshSurfels(const std::vector<HostPoint>& pts, DeviceImage& img);
 
 void cameraFocal(HostCamera& c);
 
-// f(k) for k < n on up to 16 threads (in any order).
+// Host worker threads kept for the life of the process (up to 15; the
+// calling thread is the 16th), so that a parallel loop does not pay for
+// creating and joining its threads.  One loop runs on them at a time.
+class HostPool {
+   public:
+    static HostPool& instance() {
+        static HostPool pool;
+        return pool;
+    }
+    // f(k) for k < n on the workers and the calling thread; false (nothing
+    // run) when another thread's loop holds the workers.
+    bool tryRun(size_t n, const std::function<void(size_t)>& f) {
+        if (getpid() != pid) return false;  // a forked child has none of the workers
+        std::unique_lock<std::mutex> owner(runMu, std::try_to_lock);
+        if (!owner.owns_lock()) return false;
+        {
+            std::lock_guard<std::mutex> g(mu);
+            job = &f;
+            count = n;
+            next.store(0);
+            pending = workers.size();
+            ++gen;
+        }
+        cv.notify_all();
+        drain();
+        std::unique_lock<std::mutex> g(mu);
+        doneCv.wait(g, [&] { return pending == 0; });
+        job = nullptr;
+        return true;
+    }
+    size_t threads() const { return workers.size() + 1; }
+    static bool& inLoop() {
+        thread_local bool flag = false;
+        return flag;
+    }
+
+   private:
+    HostPool() : pid(getpid()) {
+        const size_t hw = std::max<unsigned>(1u, std::thread::hardware_concurrency());
+        const size_t nt = std::min<size_t>(hw, 16);
+        for (size_t t = 1; t < nt; ++t) workers.emplace_back([this] { loop(); });
+    }
+    ~HostPool() {
+        {
+            std::lock_guard<std::mutex> g(mu);
+            stop = true;
+        }
+        cv.notify_all();
+        for (auto& t : workers) t.join();
+    }
+    void drain() {
+        inLoop() = true;
+        for (size_t k; (k = next.fetch_add(1)) < count;) (*job)(k);
+        inLoop() = false;
+    }
+    void loop() {
+        uint64_t seen = 0;
+        std::unique_lock<std::mutex> g(mu);
+        while (true) {
+            cv.wait(g, [&] { return stop || gen != seen; });
+            if (stop) return;
+            seen = gen;
+            g.unlock();
+            drain();
+            g.lock();
+            if (--pending == 0) doneCv.notify_one();
+        }
+    }
+    std::mutex runMu, mu;
+    std::condition_variable cv, doneCv;
+    std::vector<std::thread> workers;
+    const std::function<void(size_t)>* job = nullptr;
+    std::atomic<size_t> next{0};
+    size_t count = 0, pending = 0;
+    uint64_t gen = 0;
+    bool stop = false;
+    pid_t pid;
+};
+
+// f(k) for k < n on up to 16 threads (in any order): the pool's, or, when a
+// loop of another thread holds the pool, threads of its own; a loop started
+// inside a loop's task runs on the calling thread.
 template <typename F>
 inline void parallelFor(size_t n, F&& f) {
-    const size_t hw = std::max<unsigned>(1u, std::thread::hardware_concurrency());
-    const size_t nt = std::min<size_t>(std::min<size_t>(hw, 16), n);
-    if (nt <= 1) {
+    if (n <= 1 || HostPool::inLoop()) {
         for (size_t k = 0; k < n; ++k) f(k);
         return;
     }
+    const std::function<void(size_t)> fn = [&](size_t k) { f(k); };
+    if (HostPool::instance().tryRun(n, fn)) return;
+    const size_t hw = std::max<unsigned>(1u, std::thread::hardware_concurrency());
+    const size_t nt = std::min<size_t>(std::min<size_t>(hw, 16), n);
     std::atomic<size_t> next{0};
     std::vector<std::thread> th;
     for (size_t t = 0; t < nt; ++t)
