@@ -242,6 +242,13 @@ __device__ bool traverseCloud(const DevScene& S, const DevInstance& I, F3 o, F3 
 // loads when the address is the same in every lane).
 typedef __attribute__((address_space(4))) const WideNode ConstWideNode;
 
+// The closest hit's alpha, (1 - r^2)^(4 e^beta) in binary64, as a call: its
+// 64-bit constants then stay inside the call instead of being hoisted into
+// registers that live across the traversal loop (inlined, they pushed the
+// camera walk into scratch spills; as a call the walk has none, and the
+// walk, photon and adjoint traces run 2-10 % faster).
+__device__ __attribute__((noinline)) float hitAlpha(float oneMinusR2, float e4b) { return pow_r(oneMinusR2, e4b); }
+
 // The same traversal without the counters, and without a stack round trip
 // for the nearer child: the reference pushes the farther child, then the
 // nearer, and pops the nearer at once with an unchanged best, so descending
@@ -368,7 +375,7 @@ __device__ bool traverseCloudFast(const DevScene& S, const DevInstance& I, F3 o,
     out.t = best;
     out.prim = bestIdx;
     out.pos = bestX;
-    out.alpha = pow_r(1.0f - bestR2, S.geo[bestIdx].e4b);
+    out.alpha = hitAlpha(1.0f - bestR2, S.geo[bestIdx].e4b);
     return true;
 }
 

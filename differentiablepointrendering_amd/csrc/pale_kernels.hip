@@ -1162,8 +1162,11 @@ __global__ __launch_bounds__(64) void k_group_lanes(const uint32_t* segFirst, co
 }
 
 // launchAdjointIntersectKernel (AdjointKernels.cpp:87-386) for one queue.
+// Capped at 128 VGPRs (4 waves per SIMD): it needs 125 without spills since
+// the hit's alpha is a call (uncapped it took 129, and 3 waves per SIMD ran
+// 5-8 % slower).
 template <bool kCount>
-__global__ __launch_bounds__(kTraceBlock) void k_adj_trace(DevScene S, DevState* st, uint32_t bounceIndex,
+__global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_adj_trace(DevScene S, DevState* st, uint32_t bounceIndex,
                                                            RayQueue in, RayQueue out, uint8_t* extFlags,
                                                            EventArrays ev, uint8_t* evFlags, PendingArrays pend,
                                                            uint32_t pendCap, TraceParams tp) {
