@@ -768,6 +768,103 @@ __device__ __forceinline__ bool rowSpan(const DevGrid& g, F3 q, float r2, int cy
     return ax <= bx;
 }
 
+typedef __attribute__((address_space(4))) const float4 ConstF4;
+typedef __attribute__((address_space(4))) const float ConstF;
+typedef __attribute__((address_space(4))) const uint32_t ConstU32;
+
+__device__ __forceinline__ float waveMinF(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
+    return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v)));
+}
+__device__ __forceinline__ float waveMaxF(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v)));
+}
+
+// rowSpan for every point of the box [qlo, qhi]: the row is skipped only
+// when its extent lies farther than r from the whole box, and the x range
+// covers the chords of all its points (every bound and margin is monotone in
+// the point, so the span contains each point's own span).
+__device__ __forceinline__ bool rowSpanBox(const DevGrid& g, F3 qlo, F3 qhi, float r2, int cy, int cz, int lx,
+                                           int hx, int& ax, int& bx) {
+    auto axisGap = [](float ql, float qh, float o, float c, int i, int res, float m) {
+        const float lo = (i == 0) ? -INFINITY : o + (float)i * c - m;
+        const float hi = (i == res - 1) ? INFINITY : o + (float)(i + 1) * c + m;
+        return fmaxf(fmaxf(lo - qh, ql - hi), 0.0f);
+    };
+    const float my = 1e-3f * g.cy + 2e-6f * (fmaxf(fabsf(qlo.y), fabsf(qhi.y)) + fabsf(g.oy) + g.radius);
+    const float mz = 1e-3f * g.cz + 2e-6f * (fmaxf(fabsf(qlo.z), fabsf(qhi.z)) + fabsf(g.oz) + g.radius);
+    const float dy = axisGap(qlo.y, qhi.y, g.oy, g.cy, cy, g.ry, my);
+    const float dz = axisGap(qlo.z, qhi.z, g.oz, g.cz, cz, g.rz, mz);
+    const float room = r2 * 1.001f - (dy * dy + dz * dz);
+    if (!(room >= 0.0f)) return false;
+    const float hw = sqrtf(room);
+    const float mx = 1e-3f * g.cx + 2e-6f * (fmaxf(fabsf(qlo.x), fabsf(qhi.x)) + fabsf(g.ox) + g.radius);
+    const float fa = floorf((qlo.x - hw - mx - g.ox) / g.cx);
+    const float fb = floorf((qhi.x + hw + mx - g.ox) / g.cx);
+    ax = (fa > (float)lx) ? (int)fa : lx;
+    bx = (fb < (float)hx) ? (int)fb : hx;
+    return ax <= bx;
+}
+
+// The gather of a full wave whose points share one cell box and lie close
+// together (qlo..qhi, within kWaveSpread of the radius on every axis): the
+// rows and photons are the same for every lane, so they are walked once for
+// the wave with scalar loads, and each lane applies its own tests to every
+// photon of the rows' spans for the whole wave.  A lane adds exactly the
+// photons of its own box that pass its tests, in the same order as the
+// per-lane walk, so the sums are the same to the bit.
+constexpr float kWaveSpread = 0.5f;
+__device__ F3 gatherWave(const DevGrid& g, F3 q, F3 n, F3 qlo, F3 qhi, float r2, float invArea, int lx, int ly,
+                         int lz, int hx, int hy, int hz) {
+    F3 E = mk3(0, 0, 0);
+    auto addPhoton = [&](const float4& pp, const float4& dp, float pz) {
+        const F3 dd = mk3(pp.x, pp.y, pp.z) - q;
+        const float cosine = dot(n, -mk3(dp.x, dp.y, dp.z));
+        if (cosine < 0.0f) return;
+        const float d2 = dot(dd, dd);
+        if (d2 > r2) return;
+        E = E + mk3(pp.w, dp.w, pz) * invArea;
+    };
+    const ConstF4* pos = (const ConstF4*)g.posPow;
+    const ConstF4* dir = (const ConstF4*)g.dirPow;
+    const ConstF* pzs = (const ConstF*)g.powZ;
+    const ConstU32* cs = (const ConstU32*)g.cellStart;
+    for (int cz = lz; cz <= hz; ++cz) {
+        for (int cy = ly; cy <= hy; ++cy) {
+            int ax, bx;
+            if (!rowSpanBox(g, qlo, qhi, r2, cy, cz, lx, hx, ax, bx)) continue;
+            ax = __builtin_amdgcn_readfirstlane(ax);
+            bx = __builtin_amdgcn_readfirstlane(bx);
+            const uint32_t row = (uint32_t)((cz * g.ry + cy) * g.rx);
+            __asm__ volatile("");
+            uint32_t j = cs[row + ax];
+            const uint32_t b = cs[row + bx + 1];
+            for (; j + 4 <= b; j += 4) {
+                const float4 p0 = make_float4(pos[j].x, pos[j].y, pos[j].z, pos[j].w);
+                const float4 p1 = make_float4(pos[j + 1].x, pos[j + 1].y, pos[j + 1].z, pos[j + 1].w);
+                const float4 p2 = make_float4(pos[j + 2].x, pos[j + 2].y, pos[j + 2].z, pos[j + 2].w);
+                const float4 p3 = make_float4(pos[j + 3].x, pos[j + 3].y, pos[j + 3].z, pos[j + 3].w);
+                const float4 d0 = make_float4(dir[j].x, dir[j].y, dir[j].z, dir[j].w);
+                const float4 d1 = make_float4(dir[j + 1].x, dir[j + 1].y, dir[j + 1].z, dir[j + 1].w);
+                const float4 d2 = make_float4(dir[j + 2].x, dir[j + 2].y, dir[j + 2].z, dir[j + 2].w);
+                const float4 d3 = make_float4(dir[j + 3].x, dir[j + 3].y, dir[j + 3].z, dir[j + 3].w);
+                const float z0 = pzs[j], z1 = pzs[j + 1], z2 = pzs[j + 2], z3 = pzs[j + 3];
+                addPhoton(p0, d0, z0);
+                addPhoton(p1, d1, z1);
+                addPhoton(p2, d2, z2);
+                addPhoton(p3, d3, z3);
+            }
+            for (; j < b; ++j)
+                addPhoton(make_float4(pos[j].x, pos[j].y, pos[j].z, pos[j].w),
+                          make_float4(dir[j].x, dir[j].y, dir[j].z, dir[j].w), pzs[j]);
+        }
+    }
+    return E;
+}
+
 // gatherDiffuseIrradianceAtPoint (KernelHelpers.h:990-1037).  Photons are
 // stored sorted by (cell, deposit index), so the cells of one grid row form
 // one contiguous range; summation order equals the reference's cell-by-cell,
@@ -775,10 +872,13 @@ __device__ __forceinline__ bool rowSpan(const DevGrid& g, F3 q, float r2, int cy
 // scanned (rowSpan).  Memory-level parallelism: the row bounds of a whole
 // z-slice are loaded before any photon of that slice, and photons are loaded
 // four at a time (added in order).  Counting mode counts the reference's
-// cells and photons (the whole box), not the scanned ones.
+// cells and photons (the whole box), not the scanned ones.  kWave (the
+// camera gather, whose waves hold cell-sorted points, with shared seeds
+// often the same point several times): a wave whose points share a cell box
+// and lie close together gathers through gatherWave.
 constexpr int kSliceRows = 6;  // cells per axis: 4 + 1, +1 for rounding
 
-template <bool kCount>
+template <bool kCount, bool kWave = false>
 __device__ F3 gatherIrradiance(const DevGrid& g, F3 q, F3 n, Work& w) {
     if (kCount) w.gathers++;
     if (!g.valid) return mk3(0, 0, 0);
@@ -789,6 +889,23 @@ __device__ F3 gatherIrradiance(const DevGrid& g, F3 q, F3 n, Work& w) {
     worldToCell(g, q - mk3(r, r, r), lx, ly, lz);
     worldToCell(g, q + mk3(r, r, r), hx, hy, hz);
     F3 E = mk3(0, 0, 0);
+    bool waveDone = false;
+    if (kWave && !kCount && __builtin_amdgcn_ballot_w64(true) == ~0ull) {
+        const int ulx = __builtin_amdgcn_readfirstlane(lx), uly = __builtin_amdgcn_readfirstlane(ly),
+                  ulz = __builtin_amdgcn_readfirstlane(lz), uhx = __builtin_amdgcn_readfirstlane(hx),
+                  uhy = __builtin_amdgcn_readfirstlane(hy), uhz = __builtin_amdgcn_readfirstlane(hz);
+        if (__builtin_amdgcn_ballot_w64(lx != ulx || ly != uly || lz != ulz || hx != uhx || hy != uhy || hz != uhz) ==
+            0ull) {
+            const F3 qlo = mk3(waveMinF(q.x), waveMinF(q.y), waveMinF(q.z));
+            const F3 qhi = mk3(waveMaxF(q.x), waveMaxF(q.y), waveMaxF(q.z));
+            const F3 ext = qhi - qlo;
+            if (fmaxf(fmaxf(ext.x, ext.y), ext.z) <= kWaveSpread * r) {
+                E = gatherWave(g, q, n, qlo, qhi, r2, invArea, ulx, uly, ulz, uhx, uhy, uhz);
+                waveDone = true;
+            }
+        }
+    }
+    if (!waveDone) {
     auto addPhoton = [&](const float4& pp, const float4& dp, float pz) {
         const F3 dd = mk3(pp.x, pp.y, pp.z) - q;
         // dot(n, -dir) == dot(-n, dir) exactly (negation is exact).
@@ -834,6 +951,7 @@ __device__ F3 gatherIrradiance(const DevGrid& g, F3 q, F3 n, Work& w) {
                 for (; j < b; ++j) addPhoton(g.posPow[j], g.dirPow[j], g.powZ[j]);
             }
         }
+    }
     }
     const uint32_t T = *g.attempted;
     const uint32_t D = min(T, g.capacity);

@@ -983,7 +983,7 @@ __global__ __launch_bounds__(256) void k_entry_gather(DevGrid G, DevState* st, c
         const uint32_t e = ent.sorted[i];
         const float4 a = ent.qa[e], b = ent.qb[e], c = ent.comp[e];
         const uint32_t pos = ent.pos[e];
-        const F3 E = gatherIrradiance<kCount>(G, mk3(a.x, a.y, a.z), mk3(a.w, b.x, b.y), w);
+        const F3 E = gatherIrradiance<kCount, true>(G, mk3(a.x, a.y, a.z), mk3(a.w, b.x, b.y), w);
         const F3 Lo = E * mk3(c.x, c.y, c.z);
         ent.contrib[pos] = f4(c.w * Lo, 0.0f);
     }
