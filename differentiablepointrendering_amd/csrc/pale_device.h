@@ -231,12 +231,12 @@ __device__ bool traverseCloud(const DevScene& S, const DevInstance& I, F3 o, F3 
 // unrelated rays (photon and adjoint bounces); 2 = the plane words only, the
 // tangents of a surfel loaded when its plane hit is in range, which suits
 // the camera walk (fewer bytes per lane when neighbouring rays diverge).
-// Same tests in the same order either way.  With kLeafWords = 2 (the camera
-// walk, whose waves hold neighbouring pixels, and with shared seeds the
-// same ray in several passes), a node that every lane of the wave is at is
-// read with scalar loads (in the plain workload 57 % of the wave-wide node
-// reads with shared seeds, 31 % with independent samples); for the photon
-// and adjoint queues the added scalar registers cost more than it saves.
+// Same tests in the same order either way.  A node that every lane of the
+// wave is at is read with scalar loads: in the camera walk, whose waves hold
+// neighbouring pixels (and with shared seeds the same ray in several
+// passes), 57 % of the wave-wide node reads of the plain workload with
+// shared seeds, 31 % with independent samples; fewer in the photon and
+// adjoint queues, where it still saves 3-4 %.
 
 // Node and surfel records read through the constant address space (scalar
 // loads when the address is the same in every lane).
@@ -320,7 +320,7 @@ __device__ bool traverseCloudFast(const DevScene& S, const DevInstance& I, F3 o,
             float4 a, b, c;
             uint2 refs;
             const uint32_t r0 = __builtin_amdgcn_readfirstlane(ref);
-            if (kLeafWords == 2 && __builtin_amdgcn_ballot_w64(ref != r0) == 0ull) {
+            if (__builtin_amdgcn_ballot_w64(ref != r0) == 0ull) {
                 // Every lane of the wave here is at the same node: read it
                 // with scalar loads, off the vector-memory path.  (The empty
                 // asm keeps the compiler from folding this read into the
