@@ -5,6 +5,7 @@
 #include <atomic>
 #include <condition_variable>
 #include <cstdint>
+#include <exception>
 #include <functional>
 #include <mutex>
 #include <unistd.h>
@@ -223,7 +224,9 @@ class HostPool {
         return pool;
     }
     // f(k) for k < n on the workers and the calling thread; false (nothing
-    // run) when another thread's loop holds the workers.
+    // run) when another thread's loop holds the workers.  An exception from
+    // f stops the loop (items not yet started are skipped) and is rethrown
+    // here once every worker has finished.
     bool tryRun(size_t n, const std::function<void(size_t)>& f) {
         if (getpid() != pid) return false;  // a forked child has none of the workers
         std::unique_lock<std::mutex> owner(runMu, std::try_to_lock);
@@ -234,6 +237,7 @@ class HostPool {
             count = n;
             next.store(0);
             pending = workers.size();
+            error = nullptr;
             ++gen;
         }
         cv.notify_all();
@@ -241,6 +245,10 @@ class HostPool {
         std::unique_lock<std::mutex> g(mu);
         doneCv.wait(g, [&] { return pending == 0; });
         job = nullptr;
+        const std::exception_ptr e = error;
+        error = nullptr;
+        g.unlock();
+        if (e) std::rethrow_exception(e);
         return true;
     }
     size_t threads() const { return workers.size() + 1; }
@@ -265,7 +273,15 @@ class HostPool {
     }
     void drain() {
         inLoop() = true;
-        for (size_t k; (k = next.fetch_add(1)) < count;) (*job)(k);
+        for (size_t k; (k = next.fetch_add(1)) < count;) {
+            try {
+                (*job)(k);
+            } catch (...) {
+                std::lock_guard<std::mutex> g(mu);
+                if (!error) error = std::current_exception();
+                next.store(count);
+            }
+        }
         inLoop() = false;
     }
     void loop() {
@@ -287,6 +303,7 @@ class HostPool {
     const std::function<void(size_t)>* job = nullptr;
     std::atomic<size_t> next{0};
     size_t count = 0, pending = 0;
+    std::exception_ptr error;
     uint64_t gen = 0;
     bool stop = false;
     pid_t pid;
@@ -306,12 +323,23 @@ inline void parallelFor(size_t n, F&& f) {
     const size_t hw = std::max<unsigned>(1u, std::thread::hardware_concurrency());
     const size_t nt = std::min<size_t>(std::min<size_t>(hw, 16), n);
     std::atomic<size_t> next{0};
+    std::exception_ptr error;
+    std::mutex errorMu;
     std::vector<std::thread> th;
     for (size_t t = 0; t < nt; ++t)
         th.emplace_back([&] {
-            for (size_t k; (k = next.fetch_add(1)) < n;) f(k);
+            for (size_t k; (k = next.fetch_add(1)) < n;) {
+                try {
+                    f(k);
+                } catch (...) {
+                    std::lock_guard<std::mutex> g(errorMu);
+                    if (!error) error = std::current_exception();
+                    next.store(n);
+                }
+            }
         });
     for (auto& x : th) x.join();
+    if (error) std::rethrow_exception(error);
 }
 
 
