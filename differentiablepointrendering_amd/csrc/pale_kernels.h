@@ -175,6 +175,11 @@ struct TraceParams {
     // offset (batched: BatchSeg::pixOffset).
     uint32_t debug;
     uint32_t debugBase;
+    // The traversal stack: with spill null the whole stack of every lane is
+    // in LDS; else its first inLds entries, the rest in this per-lane area
+    // (stackDepth - inLds entries for each of the launch's lanes).
+    uint2* spill;
+    uint32_t inLds;
 };
 
 // Photon-map reservoir (depositPhotonSurface past capacity): per queue item

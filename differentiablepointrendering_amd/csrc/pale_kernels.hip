@@ -190,7 +190,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_photon_trace(DevScene S, DevSta
                                                               RayQueue out, uint8_t* extFlags, PhotonArrays dep,
                                                               uint8_t* depFlags, TraceParams tp, Reservoir rs) {
     extern __shared__ uint2 s_stack[];
-    const LaneStack stk = laneStack(s_stack, S.stackDepth);
+    const LaneStack stk = walkStack(s_stack, S.stackDepth, tp.spill, tp.inLds);
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t n = st->live ? st->qCount : 0u;
     const bool active = i < n;
@@ -1171,7 +1171,7 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(4))
                                                            EventArrays ev, uint8_t* evFlags, PendingArrays pend,
                                                            uint32_t pendCap, TraceParams tp) {
     extern __shared__ uint2 s_stack[];
-    const LaneStack stk = laneStack(s_stack, S.stackDepth);
+    const LaneStack stk = walkStack(s_stack, S.stackDepth, tp.spill, tp.inLds);
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t n = st->qCount;
     uint32_t i = t;
@@ -1683,11 +1683,11 @@ void launchPhotonTrace(hipStream_t s, const DevScene& S, DevState* st, uint32_t 
         k_tile_scan<<<1, 1024, 0, s>>>(tiles, qCount, 0u, maxRays, tiles + nt, 0xFFFFFFFFu);
         k_resv_rank<<<nt, kScanThreads, 0, s>>>(depFlags, st, tiles, rs.attempt);
     }
+    const size_t lds = tp.spill ? (size_t)tp.inLds * sizeof(uint2) * kTraceBlock : stackBytes(S);
     if (count)
-        k_photon_trace<true><<<g, kTraceBlock, stackBytes(S), s>>>(S, st, in, staged, extFlags, dep, depFlags, tp, rs);
+        k_photon_trace<true><<<g, kTraceBlock, lds, s>>>(S, st, in, staged, extFlags, dep, depFlags, tp, rs);
     else
-        k_photon_trace<false><<<g, kTraceBlock, stackBytes(S), s>>>(S, st, in, staged, extFlags, dep, depFlags, tp,
-                                                                    rs);
+        k_photon_trace<false><<<g, kTraceBlock, lds, s>>>(S, st, in, staged, extFlags, dep, depFlags, tp, rs);
 }
 void launchReservoirWrite(hipStream_t s, DevState* st, uint32_t maxRays, Reservoir rs, PhotonArrays dep,
                           PhotonArrays raw) {
@@ -1905,12 +1905,13 @@ void launchAdjTrace(hipStream_t s, const DevScene& S, DevState* st, uint32_t max
                     TraceParams tp, bool count) {
     if (!maxRays) return;
     const dim3 g(blocks(maxRays, kTraceBlock));
+    const size_t lds = tp.spill ? (size_t)tp.inLds * sizeof(uint2) * kTraceBlock : stackBytes(S);
     if (count)
-        k_adj_trace<true><<<g, kTraceBlock, stackBytes(S), s>>>(S, st, bounce, in, staged, extFlags, ev, evFlags,
-                                                                pend, maxRays, tp);
+        k_adj_trace<true><<<g, kTraceBlock, lds, s>>>(S, st, bounce, in, staged, extFlags, ev, evFlags, pend,
+                                                      maxRays, tp);
     else
-        k_adj_trace<false><<<g, kTraceBlock, stackBytes(S), s>>>(S, st, bounce, in, staged, extFlags, ev, evFlags,
-                                                                 pend, maxRays, tp);
+        k_adj_trace<false><<<g, kTraceBlock, lds, s>>>(S, st, bounce, in, staged, extFlags, ev, evFlags, pend,
+                                                       maxRays, tp);
 }
 void launchEventSort(hipStream_t s, const GridParams& g, DevState* st, uint32_t poolCap, EventArrays ev,
                      uint32_t* cell, uint32_t* sorted, uint32_t* cellCount, uint32_t* cellStart, uint32_t* tiles,

@@ -331,6 +331,27 @@ struct pale_handle {
     DevMem walkSpill;  // layer-walk stack entries beyond the LDS part (deep BVHs)
     uint2* walkSpillPtr() const { return walkSpill.p ? walkSpill.as<uint2>() : nullptr; }
     uint32_t walkLds() const { return tune.walk_lds_entries ? tune.walk_lds_entries : kWalkLdsEntries; }
+    // The photon and adjoint traces keep kTraceLdsEntries stack entries per
+    // lane in LDS and the rest in traceSpill: 16 KB of LDS per workgroup
+    // instead of the whole stack's (23 KB at 1M surfels), so that LDS allows
+    // the 4 waves per SIMD their registers allow.
+    static constexpr uint32_t kTraceLdsEntries = 16;
+    DevMem traceSpill;
+    size_t traceSpillBytes(uint32_t rays) const {
+        if (img.stackDepth <= kTraceLdsEntries || !rays) return 0;
+        const size_t lanes = (size_t)((rays + kTraceBlock - 1) / kTraceBlock) * kTraceBlock;
+        return (size_t)(img.stackDepth - kTraceLdsEntries) * lanes * sizeof(uint2);
+    }
+    // t with the split stack when the spill area (sized with the queues)
+    // covers a launch of `rays` lanes; else the whole stack in LDS.
+    TraceParams withTraceSpill(TraceParams t, uint32_t rays) const {
+        const size_t need = traceSpillBytes(rays);
+        if (need && traceSpill.p && traceSpill.bytes >= need) {
+            t.spill = traceSpill.as<uint2>();
+            t.inLds = kTraceLdsEntries;
+        }
+        return t;
+    }
     // The spill area of the layer walk for the current scene and tuning
     // (before any launch: a captured graph holds its address).
     void ensureWalkSpill() {
@@ -667,6 +688,7 @@ struct pale_handle {
         a(counts, 16 * sizeof(uint32_t));
         a(resvAttempt, (size_t)cap * sizeof(uint32_t));
         a(resvSlot, (size_t)cap * sizeof(uint32_t));
+        if (const size_t sb = traceSpillBytes(cap)) a(traceSpill, sb);
         if (!A::kDry) queueCap = cap;
     }
     void ensureQueues(uint32_t cap) {
@@ -1002,8 +1024,8 @@ struct pale_handle {
                 launchSeedNext(stream, st, seedTable.as<uint32_t>(), tlen, (int)b);
                 const size_t te = tBegin(kTPhoton);
                 launchPhotonTrace(stream, scene, st, P, rq(in), rq(stg), extFlags.as<uint8_t>(),
-                                  pa(depStg[0], depStg[1], depStg[2]), depFlags.as<uint8_t>(), tp(), rs,
-                                  tiles.as<uint32_t>(), countWork);
+                                  pa(depStg[0], depStg[1], depStg[2]), depFlags.as<uint8_t>(),
+                                  withTraceSpill(tp(), P), rs, tiles.as<uint32_t>(), countWork);
                 tEnd(te);
                 launchCompactRays(stream, extFlags.as<uint8_t>(), qCount, P, tiles.as<uint32_t>(), extTotal, rq(stg),
                                   rq(out), queueCap);
@@ -1551,7 +1573,7 @@ struct pale_handle {
                                bGroups.as<uint4>(), tpb.nGroups, bGroupSegs.as<uint32_t>());
                 const size_t ta = tBegin(kTAdjTrace);
                 launchAdjTrace(stream, scene, st, nr, k, rq(in), rq(stg), extFlags.as<uint8_t>(), ea(evStg),
-                               evFlags.as<uint8_t>(), pend, tpb, countWork);
+                               evFlags.as<uint8_t>(), pend, withTraceSpill(tpb, nr), countWork);
                 tEnd(ta);
                 launchCompactEvents(stream, evFlags.as<uint8_t>(), qCount, nr, tiles.as<uint32_t>(), st, ea(evStg),
                                     ea(evCmp), poolCap);
@@ -1619,7 +1641,7 @@ struct pale_handle {
                     TraceParams tpc = tp(invSpp, nullptr, nullptr, adjointSeed(spp));
                     tpc.debugBase = (uint32_t)(off[k] / 3);
                     launchAdjTrace(stream, scene, st, npx, b, rq(in), rq(stg), extFlags.as<uint8_t>(), ea(evStg),
-                                   evFlags.as<uint8_t>(), pend, tpc, countWork);
+                                   evFlags.as<uint8_t>(), pend, withTraceSpill(tpc, npx), countWork);
                     tEnd(ta);
                     launchCompactEvents(stream, evFlags.as<uint8_t>(), qCount, npx, tiles.as<uint32_t>(), st,
                                         ea(evStg), ea(evCmp), poolCap);
